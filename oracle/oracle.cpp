@@ -1277,7 +1277,11 @@ ora_sys *ora_create(const ora_params *p, const ora_model *m, int32_t seed,
       double ni = norm_inv(n23);
       double nat_factor = ni * (ni * (ni * 1.0));
       double nat_chir = dot(n12, cross(n23, n34)) * nat_factor;
-      s->chir.push_back(ora_sys::chir_t{i1, i2, i3, i4, nat_factor, nat_chir});
+      /* The reference emplaces (.., nat_factor, nat_chir) into a struct whose
+       * fields are declared (.., nat_chir, nat_factor)
+       * (state.cpp:384, cg/include/cg/chir/chiral_quad.h:12-13), so the
+       * evaluated quad carries the two values swapped. */
+      s->chir.push_back(ora_sys::chir_t{i1, i2, i3, i4, nat_chir, nat_factor});
     }
 
   /* state::setup_tether (state.cpp:389-402): length from orig positions;
@@ -1511,6 +1515,83 @@ int32_t ora_list(const ora_sys *s, int32_t kind, int32_t cap, int32_t *i1,
       put(k, s->qa_contacts[k].i1, s->qa_contacts[k].i2, s->qa_contacts[k].sat,
           s->qa_contacts[k].vacant ? 255 : s->qa_contacts[k].type % 255);
     break;
+  }
+  return size;
+}
+int32_t ora_static_list(const ora_sys *s, int32_t kind, int32_t cap,
+                        int32_t *idx, double *v0, double *v1, int32_t *ity) {
+  int32_t size = 0;
+  auto put = [&](int k, int a, int b, int c, int d, double x, double y, int t) {
+    if (k >= cap) return;
+    if (idx) {
+      idx[4 * k] = a;
+      idx[4 * k + 1] = b;
+      idx[4 * k + 2] = c;
+      idx[4 * k + 3] = d;
+    }
+    if (v0) v0[k] = x;
+    if (v1) v1[k] = y;
+    if (ity) ity[k] = t;
+  };
+  switch (kind) {
+  case 0:
+    size = (int)s->tethers.size();
+    for (int k = 0; k < size; ++k)
+      put(k, s->tethers[k].i1, s->tethers[k].i2, -1, -1, s->tethers[k].nat, 0, 0);
+    break;
+  case 1:
+    size = (int)s->lrep.size();
+    for (int k = 0; k < size; ++k)
+      put(k, s->lrep[k].first, s->lrep[k].second, -1, -1, 0, 0, 0);
+    break;
+  case 2:
+    size = (int)s->chir.size();
+    for (int k = 0; k < size; ++k) {
+      auto const &q = s->chir[k];
+      put(k, q.i1, q.i2, q.i3, q.i4, q.nat_factor, q.nat_chir, 0);
+    }
+    break;
+  case 3:
+  case 4: {
+    auto const &L = kind == 3 ? s->nat_ang : s->heur_ang;
+    size = (int)L.size();
+    for (int k = 0; k < size; ++k)
+      put(k, L[k].i1, L[k].i2, L[k].i3, -1, L[k].theta, 0, L[k].type);
+    break;
+  }
+  case 5:
+  case 6: {
+    auto const &L = kind == 5 ? s->nat_dih : s->heur_dih;
+    size = (int)L.size();
+    for (int k = 0; k < size; ++k)
+      put(k, L[k].i1, L[k].i2, L[k].i3, L[k].i4, L[k].phi, 0, L[k].type);
+    break;
+  }
+  case 7:
+    size = (int)s->all_nc.size();
+    for (int k = 0; k < size; ++k)
+      put(k, s->all_nc[k].i1, s->all_nc[k].i2, -1, -1, s->all_nc[k].nat_dist, 0,
+          s->all_nc[k].type);
+    break;
+  }
+  return size;
+}
+int32_t ora_qa_contacts(const ora_sys *s, int32_t cap, int32_t *ij,
+                        double *orig, int32_t *type, double *sat, int8_t *sd,
+                        uint8_t *vac) {
+  int32_t size = (int32_t)s->qa_contacts.size();
+  for (int k = 0; k < size && k < cap; ++k) {
+    auto const &c = s->qa_contacts[k];
+    ij[2 * k] = c.i1;
+    ij[2 * k + 1] = c.i2;
+    orig[k] = c.orig_dist;
+    type[k] = c.type;
+    sat[k] = c.sat;
+    for (int a = 0; a < 4; ++a) {
+      sd[8 * k + a] = c.d1.v[a];
+      sd[8 * k + 4 + a] = c.d2.v[a];
+    }
+    vac[k] = c.vacant;
   }
   return size;
 }

@@ -189,6 +189,17 @@ void ora_set_noise_scale(ora_sys *s, double scale); /* 0 turns noise off */
  * Writes i1,i2 (and a per-kind value) for up to cap items, returns size. */
 int32_t ora_list(const ora_sys *s, int32_t kind, int32_t cap, int32_t *i1,
                  int32_t *i2, double *val, uint8_t *flag);
+/* Static lists as built by the state setup. kind: 0 tethers (v0 nat_dist),
+ * 1 local_rep pairs, 2 chiral quads (v0 nat_factor field, v1 nat_chir field,
+ * as stored), 3 native angles (v0 theta), 4 heuristic angles (ity type),
+ * 5 native dihedrals (v0 phi), 6 heuristic dihedrals (ity type),
+ * 7 all native contacts (v0 nat_dist, ity type). idx holds 4 per item. */
+int32_t ora_static_list(const ora_sys *s, int32_t kind, int32_t cap,
+                        int32_t *idx, double *v0, double *v1, int32_t *ity);
+/* QA contacts with every field (sync diffs as 8 int8 per contact). */
+int32_t ora_qa_contacts(const ora_sys *s, int32_t cap, int32_t *ij,
+                        double *orig, int32_t *type, double *sat, int8_t *sd,
+                        uint8_t *vac);
 /* Native contacts (all): active flags and change times. */
 void ora_nat_cont_state(const ora_sys *s, uint8_t *active, double *change_t);
 int32_t ora_num_changed(const ora_sys *s);

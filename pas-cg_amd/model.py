@@ -117,9 +117,10 @@ def _compact_saw(n: int, rng: np.random.Generator, bond: float, excl: float,
                  trials: int = 24) -> np.ndarray:
     """Self-avoiding chain with fixed bond length grown towards its own centre
     of mass (a Rosenbluth-style biased growth): among `trials` random bond
-    directions whose bend angle is at least 60 degrees, the one closest to the
-    running centroid that keeps every |i-j| >= 3 pair farther than `excl` is
-    taken. Restarts from scratch if the chain traps itself."""
+    directions giving a C-alpha bond angle between 85 and 150 degrees (so that
+    i, i+2 stay beyond 5 A), the one closest to the running centroid that keeps
+    every |i-j| >= 3 pair farther than `excl` is taken. Restarts from scratch
+    if the chain traps itself."""
     for _attempt in range(200):
         pos = np.zeros((n, 3))
         ok = True
@@ -130,8 +131,8 @@ def _compact_saw(n: int, rng: np.random.Generator, bond: float, excl: float,
             if i >= 2:
                 prev = pos[i - 1] - pos[i - 2]
                 prev /= np.linalg.norm(prev)
-                keep = dirs @ prev > -0.5 + 1e-9  # bond angle >= 60 deg ... 180
-                keep &= dirs @ prev < 0.95
+                c = dirs @ prev  # = -cos(bond angle)
+                keep = (c > -0.0872) & (c < 0.866)
                 dirs = dirs[keep]
             if len(dirs) == 0:
                 ok = False
@@ -182,9 +183,11 @@ def synthetic_protein(n: int, seed: int, native_cutoff_A: float = 7.5,
     return empty_tips(m)
 
 
-def replicate(m: dict, copies: int, box_side: float, seed: int) -> dict:
+def replicate(m: dict, copies: int, box_side: float, seed: int,
+              min_sep: float = 4.56 * ANGSTROM) -> dict:
     """`copies` randomly placed and rotated copies of one chain in a cubic
-    periodic box of side `box_side` (each copy is its own chain)."""
+    periodic box of side `box_side` (each copy is its own chain), no two
+    residues of different copies closer than `min_sep`."""
     rng = np.random.Generator(np.random.PCG64(seed))
     n = m["n"]
     out = {k: [] for k in ("pos", "atype", "chain", "seq", "teth", "ang", "ang_has",
@@ -197,8 +200,19 @@ def replicate(m: dict, copies: int, box_side: float, seed: int) -> dict:
         R = np.array([[a*a+b*b-cc*cc-d*d, 2*(b*cc-a*d), 2*(b*d+a*cc)],
                       [2*(b*cc+a*d), a*a-b*b+cc*cc-d*d, 2*(cc*d-a*b)],
                       [2*(b*d-a*cc), 2*(cc*d+a*b), a*a-b*b-cc*cc+d*d]])
-        shift = rng.uniform(-box_side / 2, box_side / 2, size=3)
-        out["pos"].append(m["pos"] @ R.T + shift)
+        placed = np.concatenate(out["pos"]) if out["pos"] else np.zeros((0, 3))
+        for _try in range(2000):
+            shift = rng.uniform(-box_side / 2, box_side / 2, size=3)
+            cand = m["pos"] @ R.T + shift
+            if len(placed) == 0:
+                break
+            d = cand[:, None, :] - placed[None, :, :]
+            d -= box_side * np.round(d / box_side)
+            if np.sqrt((d * d).sum(-1)).min() > min_sep:
+                break
+        else:
+            raise RuntimeError("box too crowded to place the copies")
+        out["pos"].append(cand)
         off = c * n
         out["atype"].append(m["atype"])
         out["chain"].append(np.full(n, c, np.int32))

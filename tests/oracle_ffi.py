@@ -139,6 +139,13 @@ def lib():
             getattr(L, f).restype = I
             getattr(L, f).argtypes = [P]
         L.ora_sync.argtypes = [P, C.POINTER(C.c_int8)]
+        L.ora_static_list.restype = I
+        L.ora_static_list.argtypes = [P, I, I, C.POINTER(I), C.POINTER(D), C.POINTER(D),
+                                      C.POINTER(I)]
+        L.ora_qa_contacts.restype = I
+        L.ora_qa_contacts.argtypes = [P, I, C.POINTER(I), C.POINTER(D), C.POINTER(I),
+                                      C.POINTER(D), C.POINTER(C.c_int8),
+                                      C.POINTER(C.c_uint8)]
         L.ora_rng_new.restype = P
         L.ora_rng_new.argtypes = [I]
         L.ora_rng_free.argtypes = [P]
@@ -296,6 +303,28 @@ class Oracle:
         self.L.ora_list(self.h, kind, size, _ptr(i1, I), _ptr(i2, I), _ptr(val, D),
                         _ptr(flag, C.c_uint8))
         return i1, i2, val, flag
+
+    def static_list(self, kind: int):
+        size = self.L.ora_static_list(self.h, kind, 0, None, None, None, None)
+        idx = np.zeros((size, 4), np.int32)
+        v0 = np.zeros(size)
+        v1 = np.zeros(size)
+        ity = np.zeros(size, np.int32)
+        self.L.ora_static_list(self.h, kind, size, _ptr(idx, I), _ptr(v0, D), _ptr(v1, D),
+                               _ptr(ity, I))
+        return idx, v0, v1, ity
+
+    def qa_contacts(self):
+        size = self.L.ora_qa_contacts(self.h, 0, None, None, None, None, None, None)
+        ij = np.zeros((size, 2), np.int32)
+        orig = np.zeros(size)
+        ty = np.zeros(size, np.int32)
+        sat = np.zeros(size)
+        sd = np.zeros((size, 8), np.int8)
+        vac = np.zeros(size, np.uint8)
+        self.L.ora_qa_contacts(self.h, size, _ptr(ij, I), _ptr(orig, D), _ptr(ty, I),
+                               _ptr(sat, D), _ptr(sd, C.c_int8), _ptr(vac, C.c_uint8))
+        return ij, orig, ty, sat, sd, vac
 
     def nat_cont_state(self, k: int):
         a = np.zeros(k, np.uint8)
