@@ -1,0 +1,268 @@
+/*
+ * pcg.h -- C ABI of the MI355X-native per-step loop of the PAS CG model.
+ *
+ * Drop-in boundary for the reference's per-step loop
+ * (cg/src/cg/simul/thread_main.cpp:373-536). Every entry point returns an int
+ * status (PCG_OK or a negative PCG_E* code); the message of the last failure
+ * is available from pcg_last_error(). No exception crosses the ABI. Buffers
+ * are caller-owned on the host side and context-owned on the device side.
+ * A context (or batch, or server) is bound to one HIP stream of one device
+ * and is not re-entrant; separate objects may be used from separate threads.
+ *
+ * Units are the reference's internal units (cg/include/cg/utils/units.h):
+ * length 1 = 5 A, time 1 = tau, energy 1 = eps. Arrays of 3-vectors are
+ * xyz-interleaved on the host side ([3n]).
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PCG_OK = 0,
+  PCG_E_ARG = -1,      /* invalid argument */
+  PCG_E_HIP = -2,      /* HIP runtime failure */
+  PCG_E_NOMEM = -3,    /* device allocation or budget refused */
+  PCG_E_OVERFLOW = -4, /* a list outgrew its capacity */
+  PCG_E_STATE = -5,    /* call not valid in the current state */
+  PCG_E_NOTFOUND = -6
+};
+
+/* Force terms in the reference's evaluation order
+ * (cg/src/cg/simul/thread_setup.cpp:27-48). */
+enum pcg_term {
+  PCG_CHIR = 0,
+  PCG_TETHER,
+  PCG_LREP,
+  PCG_NAT_CONT,
+  PCG_PAULI,
+  PCG_CONST_DH,
+  PCG_REL_DH,
+  PCG_QA_CAND,
+  PCG_QA_CONT,
+  PCG_PID,
+  PCG_NAT_ANG,
+  PCG_HEUR_ANG,
+  PCG_HEUR_DIH,
+  PCG_CND,
+  PCG_SND,
+  PCG_VEL_AFM,
+  PCG_NUM_TERMS
+};
+
+/* Force-field parameters (cg/default.yml sections, internal units). The
+ * field meaning matches the parameters bound in thread::setup_<term>
+ * (cg/src/cg/simul/thread_setup.cpp:122-720). */
+typedef struct pcg_params {
+  double rep_cutoff;      /* general.repulsive cutoff */
+  double counting_factor; /* general.counting factor */
+  double nl_cutoff;       /* neighbor list.cutoff; rebuild uses max(this, dh_cutoff) */
+  double nl_pad;          /* neighbor list.pad */
+  int32_t tether_on;
+  double tether_H1, tether_H2;
+  int32_t lrep_on;
+  double lrep_depth;
+  int32_t chir_on;
+  double e_chi;
+  int32_t nat_ang_on;
+  double CBA;
+  int32_t heur_ang_on;
+  double heur_ang_poly[7][9];
+  int32_t nat_dih_on;
+  int32_t nat_dih_variant; /* 0 complex, 1 simple */
+  double CDA, CDB, CDH;
+  int32_t heur_dih_on;
+  double heur_dih_coeffs[6][9];
+  int32_t pauli_on;
+  double pauli_depth;
+  int32_t nat_cont_on;
+  double nat_cont_depth;
+  int32_t nc_ss_kind; /* 0 none, 1 harmonic, 2 lj */
+  double nc_ss_H1, nc_ss_H2, nc_ss_nat_r, nc_ss_depth, nc_ss_r_min;
+  int32_t dh_on;
+  int32_t dh_variant; /* 0 constant, 1 relative */
+  double dh_screen_dist, dh_cutoff, dh_perm;
+  int32_t qa_on;
+  int32_t qa_include4;
+  double qa_phase_dur, qa_formation_tol;
+  double qa_min_cos_hr, qa_min_cos_hh, qa_max_cos_nr;
+  double qa_lj_depth[445], qa_lj_r_low[445], qa_lj_r_high[445];
+  int32_t pid_on;
+  int32_t pid_include4;
+  int32_t pid_lambda_variant; /* 0 cosine, 1 algebraic */
+  double pid_bbp_alpha, pid_bbp_psi0, pid_bbm_alpha, pid_bbm_psi0;
+  double pid_ss_alpha, pid_ss_psi0;
+  double pid_bbp_lj[3], pid_bbm_lj[3];
+  double pid_ss_depth[441], pid_ss_r_low[441], pid_ss_r_high[441];
+  int32_t afm_on;
+  double afm_H1, afm_H2;
+  int32_t lang_on;
+  double gamma, temperature, dt;
+  double mass[21];
+  double charge[21];
+  int32_t ptype[21];
+  int32_t base_sync[21][4];
+} pcg_params;
+
+/* One protein system after state::compile_model
+ * (cg/src/cg/simul/state.cpp:144-249); see the reference model
+ * cg/include/cg/input/model.h:268-329. */
+typedef struct pcg_model {
+  int32_t n;
+  const double *pos;      /* [3n] starting positions */
+  const double *orig_pos; /* [3n] positions the tether lengths come from */
+  const uint8_t *atype;
+  const int32_t *chain_idx, *seq_idx;
+  int32_t n_teth;
+  const int32_t *teth; /* [2k] */
+  int32_t n_ang;
+  const int32_t *ang; /* [3k] */
+  const uint8_t *ang_has;
+  const double *ang_theta;
+  int32_t n_dih;
+  const int32_t *dih; /* [4k] */
+  const uint8_t *dih_has;
+  const double *dih_phi;
+  int32_t n_cont;
+  const int32_t *cont; /* [2k] */
+  const double *cont_len;
+  const int32_t *cont_type; /* 5 = SSBOND */
+  double cell[3];
+  int32_t n_tips;
+  const int32_t *tip_res;
+  const double *tip_orig, *tip_vel;
+} pcg_model;
+
+enum { PCG_MIXED = 0, PCG_FP64 = 1 };               /* force arithmetic */
+enum { PCG_NOISE_PHILOX = 0, PCG_NOISE_RAN2 = 1 }; /* Langevin noise source */
+
+typedef struct pcg_options {
+  int32_t precision;  /* PCG_MIXED: FP32 forces, FP64 integration */
+  int32_t noise;      /* PHILOX (device, counter-based) or RAN2 (reference stream) */
+  int32_t seed;       /* ran2 seed (initial velocities; noise in RAN2 mode) */
+  uint64_t philox_seed;
+  double temperature; /* <0: take params.temperature */
+  int32_t device;
+} pcg_options;
+
+const char *pcg_last_error(void); /* thread-local message of the last failure */
+int pcg_device_count(int *count);
+
+/* ---- plain single-system context (the drop-in path) ------------------- */
+typedef struct pcg_ctx pcg_ctx;
+
+/* state::traj_init (state.cpp:83-112) + thread::init_kernels: initial
+ * velocities from ran2(seed) (state.cpp:270-335), every per-term list. */
+int pcg_create(const pcg_params *p, const pcg_model *m, const pcg_options *o,
+               pcg_ctx **out);
+int pcg_destroy(pcg_ctx *c);
+
+/* thread::advance_by_step (thread_main.cpp:373-377) repeated nsteps times */
+int pcg_step(pcg_ctx *c, int32_t nsteps);
+
+/* The pieces of one step, for per-call parity (thread_main.cpp:379-536). */
+int pcg_nl_verify(pcg_ctx *c, int32_t *invalid);     /* nl::verify */
+int pcg_fix_nl(pcg_ctx *c);                          /* legacy_update + updates */
+int pcg_prepare_nh(pcg_ctx *c);                      /* qa::prepare_nh */
+int pcg_eval_forces(pcg_ctx *c);                     /* thread::eval_forces */
+/* one term over items [from, to) of its list -- sliceable_task::for_slice
+ * (cg/include/cg/simul/runtime.h:17-21). F (3n, may be NULL) receives the
+ * term's forces alone, V its energy. to < 0 means the whole list. */
+int pcg_eval_term(pcg_ctx *c, int32_t term, int32_t from, int32_t to,
+                  double *F, double *V);
+int pcg_term_size(pcg_ctx *c, int32_t term, int32_t *size); /* total_size() */
+int pcg_qa_finish(pcg_ctx *c);                       /* qa::finish_processing */
+int pcg_lang_step(pcg_ctx *c);                       /* lang::step */
+
+/* State access. which: 0..5 Gear y0..y5 (y0 = positions), 6 forces,
+ * 7 velocities. */
+int pcg_get(pcg_ctx *c, int32_t which, double *out);
+int pcg_set(pcg_ctx *c, int32_t which, const double *in);
+int pcg_get_energy(pcg_ctx *c, double *V, double *term_V /*[PCG_NUM_TERMS]*/);
+int pcg_get_time(pcg_ctx *c, double *t, int64_t *step);
+int pcg_set_time(pcg_ctx *c, double t);
+int pcg_set_noise_scale(pcg_ctx *c, double scale); /* 0: no noise */
+/* Lists. kind: 0 NL, 1 current native contacts, 2 pauli, 3 dh, 4 pid,
+ * 5 qa free pairs (flag = vacant), 6 qa contacts (val = saturation,
+ * flag = vacant). Returns the size; fills up to cap items. */
+int pcg_get_list(pcg_ctx *c, int32_t kind, int32_t cap, int32_t *size,
+                 int32_t *i1, int32_t *i2, double *val, uint8_t *flag);
+int pcg_nat_cont_state(pcg_ctx *c, uint8_t *active, double *change_t,
+                       int32_t *num_changed);
+int pcg_qa_state(pcg_ctx *c, int8_t *sync /*[4n]*/, int32_t *num_contacts);
+int pcg_nl_rebuilds(pcg_ctx *c, int32_t *count);
+/* HIP stream the context launches on (hipStream_t as void*) */
+int pcg_stream(pcg_ctx *c, void **stream);
+/* Capture one full step as a hipGraph and replay it for later pcg_step
+ * calls (steps without a NL rebuild); 0 turns it off. */
+int pcg_use_graph(pcg_ctx *c, int32_t on);
+
+/* ---- batched form: many systems, one persistent launch ---------------- */
+typedef struct pcg_batch pcg_batch;
+typedef struct pcg_batch_options {
+  int32_t precision;
+  uint64_t philox_seed; /* per-system streams: (seed, system index) */
+  int32_t block;        /* threads per workgroup (0: autotune) */
+  int32_t steps_per_launch; /* steps fused in one persistent launch (0: auto) */
+  int32_t use_graph;    /* replay the launch sequence as a hipGraph */
+  int32_t device;
+} pcg_batch_options;
+
+/* temps[i] < 0 takes params.temperature; seeds: ran2 seed of each system's
+ * initial velocities. */
+int pcg_batch_create(const pcg_params *p, int32_t nsys, const pcg_model *models,
+                     const double *temps, const int32_t *seeds,
+                     const pcg_batch_options *o, pcg_batch **out);
+int pcg_batch_destroy(pcg_batch *b);
+int pcg_batch_run(pcg_batch *b, int32_t nsteps);
+int pcg_batch_sync(pcg_batch *b);
+int pcg_batch_set_noise_scale(pcg_batch *b, double scale); /* 0: no noise */
+int pcg_batch_get(pcg_batch *b, int32_t sys, int32_t which, double *out);
+int pcg_batch_energy(pcg_batch *b, int32_t sys, double *V, double *term_V);
+int pcg_batch_info(pcg_batch *b, int32_t *block, int32_t *steps_per_launch,
+                   int64_t *device_bytes);
+/* HIP events around the last pcg_batch_run's launches, milliseconds */
+int pcg_batch_last_ms(pcg_batch *b, double *ms, int32_t *launches);
+
+/* ---- served form: continuous batching of simulation jobs -------------- */
+typedef struct pcg_server pcg_server;
+typedef struct pcg_server_options {
+  int32_t precision;
+  int64_t hbm_budget_bytes; /* admission limit for resident job state */
+  int32_t max_resident;     /* slots */
+  int32_t chunk_steps;      /* steps per scheduling round */
+  uint64_t philox_seed;
+  int32_t device;
+} pcg_server_options;
+
+enum { PCG_JOB_QUEUED = 0, PCG_JOB_RUNNING = 1, PCG_JOB_DONE = 2,
+       PCG_JOB_CANCELLED = 3, PCG_JOB_FAILED = 4 };
+
+int pcg_server_create(const pcg_params *p, const pcg_server_options *o,
+                      pcg_server **out);
+int pcg_server_destroy(pcg_server *s);
+/* queue a job: model, temperature (<0: params), ran2 seed, step count;
+ * pulling jobs carry their tips in the model */
+int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature,
+                      int32_t seed, int64_t nsteps, int64_t *job_id);
+/* one scheduling round: admit by budget, run chunk_steps for every resident
+ * job in one launch, retire finished jobs. Returns the number of jobs still
+ * queued or running in *active. */
+int pcg_server_round(pcg_server *s, int32_t *active);
+int pcg_server_status(pcg_server *s, int64_t job, int32_t *status,
+                      int64_t *steps_done);
+int pcg_server_result(pcg_server *s, int64_t job, double *pos /*[3n]*/,
+                      double *V);
+int pcg_server_cancel(pcg_server *s, int64_t job);
+/* wall-clock seconds (steady clock) of submission, admission and end */
+int pcg_server_job_times(pcg_server *s, int64_t job, double *submit, double *start,
+                         double *end);
+int pcg_server_stats(pcg_server *s, int64_t *resident_bytes, int32_t *resident,
+                     int32_t *queued);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,375 @@
+// pcg_batch.hip -- the persistent batched form: one workgroup owns one
+// system and runs many whole steps of thread::advance_by_step
+// (cg/src/cg/simul/thread_main.cpp:373-536) inside a single launch.
+//
+// Positions (FP64) and forces (R) of the system stay in LDS across steps;
+// force terms accumulate with LDS atomics; the NL rebuild, the ordered list
+// partition and the QA finish run inside the workgroup; the Gear-5 state is
+// streamed from/to HBM (it is L2/MALL-resident for the batch sizes used).
+// Systems are independent, so workgroups never synchronise with each other.
+#include "pcg_launch.hpp"
+#include "pcg_stages.hpp"
+
+namespace pcg {
+
+struct GeoL {
+  const double *x, *y, *z;
+  int res0;
+  __device__ __forceinline__ v3<double> r(int i) const {
+    int k = i - res0;
+    return v3<double>{x[k], y[k], z[k]};
+  }
+};
+
+template <typename RF> struct AccL {
+  RF *fx, *fy, *fz;
+  int res0;
+  template <typename T> __device__ __forceinline__ void add(int i, v3<T> f) {
+    int k = i - res0;
+    atomicAdd(fx + k, (RF)f.x);
+    atomicAdd(fy + k, (RF)f.y);
+    atomicAdd(fz + k, (RF)f.z);
+  }
+};
+
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  int lane = threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// In-workgroup exclusive scan of the n row counts stored at cnt[0..n);
+// returns the total (all threads).
+__device__ int block_exscan(int *cnt, int n) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int base = 0; base < n; base += blockDim.x) {
+    int i = base + threadIdx.x;
+    int v = i < n ? cnt[i] : 0;
+    int x = wave_incl_scan(v);
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int off = carry;
+    for (int k = 0; k < w; ++k) off += wsum[k];
+    if (i < n) cnt[i] = off + x - v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int k = 0; k < nw; ++k) carry += wsum[k];
+    __syncthreads();
+  }
+  int tot = carry;
+  __syncthreads();
+  return tot;
+}
+
+__device__ __forceinline__ double block_max(double v) {
+  __shared__ double part[16];
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  double m = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, part[k]);
+  __syncthreads();
+  return m;
+}
+
+// ordered ranks for the partition, any block size up to 1024
+__device__ __forceinline__ void block_ranks5(int cls, int valid, int counts[5], int ranks[5]) {
+  __shared__ int wc[16][5];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned long long lt = (1ull << lane) - 1ull;
+  int mine[5];
+  for (int c = 0; c < 5; ++c) {
+    unsigned long long m = __ballot(valid && (cls & (1 << c)));
+    mine[c] = __popcll(m & lt);
+    if (lane == 0) wc[w][c] = __popcll(m);
+  }
+  __syncthreads();
+  for (int c = 0; c < 5; ++c) {
+    int off = 0, tot = 0;
+    for (int k = 0; k < nw; ++k) {
+      if (k < w) off += wc[k][c];
+      tot += wc[k][c];
+    }
+    ranks[c] = off + mine[c];
+    counts[c] = tot;
+  }
+  __syncthreads();
+}
+
+
+// NL rebuild + ordered partition inside the workgroup.
+__device__ void wg_fix_nl(const DevBatch &b, const SysDesc &sd, SysState &st, const GeoL &g) {
+  const double req = b.P->nl_eff_cutoff + b.P->p.nl_pad;
+  int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int end = sd.res0 + sd.n;
+  int *cnt = b.row_cnt + sd.res0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int row = sd.res0 + w; row < end; row += nw) {
+      int out = pass ? cnt[row - sd.res0] : 0, c = 0;
+      for (int j0 = row + 1; j0 < end; j0 += 64) {
+        int j = j0 + lane;
+        double dist = 0.0;
+        bool keep = false;
+        if (j < end) {
+          bool same = b.chain[row] == b.chain[j];
+          int dseq = abs(b.seq[row] - b.seq[j]);
+          if (!(same && dseq < 3)) {
+            v3<double> dd = xwrapd(xsub(g.r(j), g.r(row)), sd);
+            dist = xnorm3(dd);
+            keep = dist < req;
+          }
+        }
+        unsigned long long m = __ballot(keep);
+        if (pass && keep) {
+          int pos = out + c + __popcll(m & ((1ull << lane) - 1ull));
+          if (pos < sd.nlcap) {
+            int slot = sd.nl0 + pos;
+            b.nl_i[0][slot] = row;
+            b.nl_i[1][slot] = j;
+            b.nl_d[slot] = dist;
+          }
+        }
+        c += __popcll(m);
+      }
+      if (!pass && lane == 0) cnt[row - sd.res0] = c;
+    }
+    __syncthreads();
+    if (!pass) {
+      int tot = block_exscan(cnt, sd.n);
+      if (threadIdx.x == 0) {
+        st.nl_n = tot < sd.nlcap ? tot : sd.nlcap;
+        if (tot > sd.nlcap) st.overflow = 1;
+      }
+      __syncthreads();
+    }
+  }
+  // snapshot (legacy_update.cpp:80-89)
+  for (int k = threadIdx.x; k < sd.n; k += blockDim.x) {
+    b.orig_r[0][sd.res0 + k] = g.x[k];
+    b.orig_r[1][sd.res0 + k] = g.y[k];
+    b.orig_r[2][sd.res0 + k] = g.z[k];
+  }
+  if (threadIdx.x == 0) {
+    st.orig_pad = b.P->p.nl_pad;
+    for (int c = 0; c < 3; ++c) st.orig_cell[c] = sd.cell[c];
+  }
+  __syncthreads();
+  // ordered partition: chunks in order, running offsets
+  __shared__ int run[5];
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 5; ++c) run[c] = 0;
+  __syncthreads();
+  const pcg_params &p = b.P->p;
+  int nl_n = st.nl_n;
+  for (int base = 0; base < nl_n; base += blockDim.x) {
+    int k = base + threadIdx.x;
+    int valid = k < nl_n;
+    int slot = sd.nl0 + k;
+    int cls = valid ? classify(b, sd, st, slot) : 0;
+    int counts[5], ranks[5];
+    block_ranks5(cls, valid, counts, ranks);
+    if (valid && cls) {
+      int i1 = b.nl_i[0][slot], i2 = b.nl_i[1][slot];
+      double d = b.nl_d[slot];
+      if (cls & 1) b.ncur[sd.nc0 + run[0] + ranks[0]] = find_contact(b, sd, i1, i2);
+      if (cls & 2) {
+        int pos = run[1] + ranks[1];
+        if (pos < sd.qfcap) {
+          int q = sd.qf0 + pos;
+          b.qf_i[0][q] = i1;
+          b.qf_i[1][q] = i2;
+          b.qf_d[q] = d;
+          b.qf_vac[q] = 0;
+        }
+      }
+      if (cls & 4) {
+        int q = sd.nl0 + run[2] + ranks[2];
+        b.pid_i[0][q] = i1;
+        b.pid_i[1][q] = i2;
+        b.pid_d[q] = d;
+        b.pid_t[q] = (int16_t)(b.atype[i1] * 21 + b.atype[i2]);
+      }
+      if (cls & 8) {
+        int q = sd.nl0 + run[3] + ranks[3];
+        b.dh_i[0][q] = i1;
+        b.dh_i[1][q] = i2;
+        b.dh_q[q] = p.charge[b.atype[i1]] * p.charge[b.atype[i2]];
+      }
+      if (cls & 16) {
+        int q = sd.nl0 + run[4] + ranks[4];
+        b.pauli_i[0][q] = i1;
+        b.pauli_i[1][q] = i2;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int c = 0; c < 5; ++c) run[c] += counts[c];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    st.nc_n = run[0];
+    st.qf_n = run[1] < sd.qfcap ? run[1] : sd.qfcap;
+    if (run[1] > sd.qfcap) st.overflow = 1;
+    st.pid_n = run[2];
+    st.dh_n = run[3];
+    st.pauli_n = run[4];
+    st.nl_invalid = 0;
+    st.rebuilds += 1;
+  }
+  __syncthreads();
+}
+
+template <typename R, typename RF>
+__device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
+  const SysDesc &sd = b.sys[s];
+  SysState &st = b.st[s];
+  if (!st.active) return;
+  const DevParams &P = *b.P;
+  const int n = sd.n, r0 = sd.res0;
+  extern __shared__ __align__(16) unsigned char lds[];
+  double *rx = (double *)lds, *ry = rx + n, *rz = ry + n;
+  RF *fx = (RF *)(rz + n), *fy = fx + n, *fz = fy + n;
+  __shared__ double vacc[NT];
+  __shared__ int sizes[NT + 1];
+  __shared__ int flag_rebuild, stop;
+
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    rx[k] = b.y[0][0][r0 + k];
+    ry[k] = b.y[0][1][r0 + k];
+    rz[k] = b.y[0][2][r0 + k];
+  }
+  GeoL g{rx, ry, rz, r0};
+  AccL<RF> acc{fx, fy, fz, r0};
+  __syncthreads();
+
+  for (int it = 0; it < nsteps; ++it) {
+    bool want_v = (it == nsteps - 1);
+    if (threadIdx.x == 0) {
+      stop = st.step >= sd.steps_target;
+      // nl::verify (cg/src/cg/nl/verify.cpp:3-34)
+      if (!stop && P.nl_required) {
+        if (st.verify_first) {
+          st.verify_first = 0;
+          st.nl_invalid = 1;
+        } else {
+          double maxd = __longlong_as_double((long long)st.max_disp_bits);
+          double box = fabs(sd.cell[0] - st.orig_cell[0]) + fabs(sd.cell[1] - st.orig_cell[1]) +
+                       fabs(sd.cell[2] - st.orig_cell[2]);
+          st.nl_invalid = __dadd_rn(__dmul_rn(2.0, maxd), box) > st.orig_pad ? 1 : 0;
+        }
+      }
+      flag_rebuild = st.nl_invalid;
+      for (int k = 0; k < NT; ++k) vacc[k] = 0.0;
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      fx[k] = 0;
+      fy[k] = 0;
+      fz[k] = 0;
+    }
+    __syncthreads();
+    if (stop) break;
+    if (P.p.qa_on)
+      for (int k = threadIdx.x; k < n; k += blockDim.x) prepare_nh_item(b, g, r0 + k);
+    if (flag_rebuild) wg_fix_nl(b, sd, st, g);
+    if (threadIdx.x == 0) {
+      int acc_n = 0;
+      for (int t = 0; t < NT; ++t) {
+        sizes[t] = acc_n;
+        acc_n += term_on(P, t) ? term_size(sd, st, t) : 0;
+      }
+      sizes[NT] = acc_n;
+    }
+    __syncthreads();
+    // every term's items in one flat index space, term order preserved
+    int total = sizes[NT];
+    int t = 0;
+    for (int gidx = threadIdx.x; gidx < total; gidx += blockDim.x) {
+      while (gidx >= sizes[t + 1]) ++t;
+      double v = eval_item<R>(b, sd, st, g, acc, t, gidx - sizes[t]);
+      if (want_v && v != 0.0) atomicAdd(&vacc[t], v);
+    }
+    __syncthreads();
+    if (P.p.qa_on) qa_finish_block(b, sd, st);
+    // lang::step (cg/src/cg/langevin/step.cpp:23-77)
+    double maxd = 0.0;
+    if (P.p.lang_on) {
+      for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        int i = r0 + k;
+        double nz[3];
+        philox_normals(sd.philox_key, st.step, k, nz);
+        double r[3];
+        gear_update(b, sd, st, i, (double)fx[k], (double)fy[k], (double)fz[k], nz, r);
+        rx[k] = r[0];
+        ry[k] = r[1];
+        rz[k] = r[2];
+        maxd = fmax(maxd, disp_from_orig(b, sd, i, r));
+      }
+    }
+    maxd = block_max(maxd);
+    if (threadIdx.x == 0) {
+      st.max_disp_bits = (unsigned long long)__double_as_longlong(maxd);
+      if (want_v) {
+        double v = 0.0;
+        for (int k = 0; k < NT; ++k) {
+          st.V[k] = vacc[k];
+          v += vacc[k];
+        }
+        st.Vtot = v;
+      }
+      if (P.p.lang_on) {
+        st.true_t += P.dt;
+        st.t = st.true_t;
+        st.step += 1;
+      }
+    }
+    __syncthreads();
+  }
+  // forces of the last evaluation back to HBM
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    ((RF *)b.F[0])[r0 + k] = fx[k];
+    ((RF *)b.F[1])[r0 + k] = fy[k];
+    ((RF *)b.F[2])[r0 + k] = fz[k];
+  }
+}
+
+template <typename R, typename RF>
+__global__ __launch_bounds__(1024) void k_persistent(DevBatch b, int nsteps) {
+  persistent_body<R, RF>(b, blockIdx.x, nsteps);
+}
+
+// served form: one resident job (a one-system batch) per workgroup
+template <typename R, typename RF>
+__global__ __launch_bounds__(1024) void k_persistent_jobs(const DevBatch *jobs, int nsteps) {
+  persistent_body<R, RF>(jobs[blockIdx.x], 0, nsteps);
+}
+
+void launch_persistent_jobs(const DevBatch *jobs, int njobs, int prec, int block, int nsteps,
+                            size_t lds_bytes, hipStream_t s) {
+  if (prec == PCG_FP64)
+    hipLaunchKernelGGL((k_persistent_jobs<double, double>), dim3(njobs), dim3(block), lds_bytes,
+                       s, jobs, nsteps);
+  else
+    hipLaunchKernelGGL((k_persistent_jobs<float, float>), dim3(njobs), dim3(block), lds_bytes, s,
+                       jobs, nsteps);
+}
+
+void launch_persistent(const DevBatch &b, int prec, int block, int nsteps, size_t lds_bytes,
+                       hipStream_t s) {
+  if (prec == PCG_FP64)
+    hipLaunchKernelGGL((k_persistent<double, double>), dim3(b.nsys), dim3(block), lds_bytes, s, b,
+                       nsteps);
+  else
+    hipLaunchKernelGGL((k_persistent<float, float>), dim3(b.nsys), dim3(block), lds_bytes, s, b,
+                       nsteps);
+}
+
+} // namespace pcg

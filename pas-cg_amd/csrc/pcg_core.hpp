@@ -1,0 +1,142 @@
+// pcg_core.hpp -- device data layout shared by the plain, batched and served
+// forms of the per-step loop.
+//
+// Every form works on a *batch*: systems concatenated residue-wise, each with
+// a SysDesc naming its slices of the static lists and its capacity segments
+// of the dynamic lists (NL, pauli, dh, pid, QA). The plain drop-in context is
+// a batch of one. All arrays are structure-of-arrays; residue indices in
+// lists are global (batch) indices.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pcg.h"
+
+namespace pcg {
+
+constexpr int NT = PCG_NUM_TERMS;
+constexpr double C216_INV = 0.89089871814033927; // cg/include/cg/utils/math.h:6
+
+struct SysDesc {
+  int res0, n;
+  int teth0, nteth, lrep0, nlrep, chir0, nchir;
+  int nang0, nnang, hang0, nhang, ndih0, nndih, hdih0, nhdih;
+  int nc0, nnc, tip0, ntip;
+  int nl0, nlcap;   // NL and the NL-sized lists (pauli, dh, pid, QA candidates)
+  int qf0, qfcap;   // QA free pairs
+  int qc0, qccap;   // QA contacts and removed indices
+  int pad0;
+  double cell[3], cell_inv[3];
+  double temperature, const2; // const2 = sqrt(2 kB T gamma dt) dt (step.cpp:7-17)
+  uint64_t philox_key;
+  int64_t steps_target;       // batched/served: steps this system still owes
+};
+
+// Mutable per-system scalars.
+struct SysState {
+  int nl_n, nc_n, pauli_n, dh_n, pid_n, qf_n, qc_n, cand_n, rem_n, num_qa;
+  int nl_invalid, verify_first, num_changed, overflow;
+  int rebuilds, active;
+  int64_t step;
+  double t, true_t, orig_pad;
+  double orig_cell[3];
+  double V[NT];
+  double Vtot;
+  unsigned long long max_disp_bits; // FP64 max displacement since rebuild (as bits)
+};
+
+// Force-field parameters on the device (a copy of pcg_params plus derived
+// constants).
+struct DevParams {
+  pcg_params p;
+  double nl_eff_cutoff;      // max(nl.cutoff, dh.cutoff), thread_setup.cpp:368
+  double V_factor, screen_inv;
+  double gamma2, dt, dt_inv, deltsq;
+  double qa_max_req, qa_factor, qa_sat_speed;
+  int pauli_on;              // pauli.enabled && !qa.enabled (state.cpp:473)
+  int nl_required;
+};
+
+// Device arrays of a batch (pointers into one allocation).
+struct DevBatch {
+  int nsys, ntot;
+  const SysDesc *sys;
+  SysState *st;
+  const DevParams *P;
+  // residues
+  double *y[6][3];            // Gear y0..y5 (y0 = positions)
+  double *vel[3];
+  void *F[3];                 // float or double forces (precision)
+  double *orig_r[3];          // positions at the last NL rebuild
+  double *nh[6];              // QA n and h vectors (FP64)
+  const uint8_t *atype;
+  const int *chain, *seq, *prev, *next, *sysid, *local;
+  int8_t *sync;               // [4 ntot]
+  double *noise[3];           // RAN2 mode: host-supplied noise
+  // static lists
+  const int *teth_i[2]; const double *teth_nat;
+  const int *lrep_i[2];
+  const int *chir_i[4]; const double *chir_f0, *chir_f1; // stored fields (nat_factor, nat_chir)
+  const int *nang_i[3]; const double *nang_th;
+  const int *hang_i[3]; const uint8_t *hang_t;
+  const int *ndih_i[4]; const double *ndih_ph;
+  const int *hdih_i[4]; const uint8_t *hdih_t;
+  const int *nc_i[2]; const double *nc_nat; const uint8_t *nc_type;
+  uint8_t *nc_active; double *nc_change_t;
+  const int *tip_res; const double *tip_orig[3], *tip_vel[3];
+  // dynamic lists
+  int *nl_i[2]; double *nl_d; uint8_t *nl_cls;  // class bits of the partition
+  int *ncur;                                      // current native contacts (index into all)
+  int *pauli_i[2];
+  int *dh_i[2]; double *dh_q;
+  int *pid_i[2]; double *pid_d; int16_t *pid_t;
+  int *qf_i[2]; double *qf_d; uint8_t *qf_vac;
+  int *qc_i[2]; double *qc_d; int16_t *qc_t; double *qc_sat; int8_t *qc_sd; uint8_t *qc_vac;
+  int *cand_i[2]; double *cand_od, *cand_dist; int *cand_idx; int16_t *cand_t; int8_t *cand_sd;
+  int *cand_perm;
+  int *rem;
+  int *row_cnt;               // NL rebuild scratch: per-residue row counts / offsets
+  int *blk_cnt;               // partition scratch: per-chunk counts
+};
+
+// ------------------------------------------------------------- helpers ----
+template <typename R> struct v3 { R x, y, z; };
+template <typename R> __device__ __forceinline__ v3<R> mk3(R a, R b, R c) { return v3<R>{a, b, c}; }
+template <typename R> __device__ __forceinline__ v3<R> operator+(v3<R> a, v3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename R> __device__ __forceinline__ v3<R> operator-(v3<R> a, v3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename R> __device__ __forceinline__ v3<R> operator-(v3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <typename R> __device__ __forceinline__ v3<R> operator*(v3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <typename R> __device__ __forceinline__ v3<R> operator*(R s, v3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename R> __device__ __forceinline__ v3<R> operator/(v3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <typename R> __device__ __forceinline__ R dot(v3<R> a, v3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename R> __device__ __forceinline__ R nsq(v3<R> a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+template <typename R> __device__ __forceinline__ v3<R> cross(v3<R> a, v3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float rsq(float x) { return 1.0f / sqrtf(x); }
+__device__ __forceinline__ double rsq(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float sq(float x) { return sqrtf(x); }
+__device__ __forceinline__ double sq(double x) { return sqrt(x); }
+template <typename R> __device__ __forceinline__ R clampR(R v, R lo, R hi) { return v < lo ? lo : (v < hi ? v : hi); }
+template <typename R> __device__ __forceinline__ R ipow6(R s) { R s2 = s * s; return s2 * s2 * s2; }
+__device__ __forceinline__ float acosc(float x) { return acosf(fminf(fmaxf(x, -1.0f), 1.0f)); }
+__device__ __forceinline__ double acosc(double x) { return acos(fmin(fmax(x, -1.0), 1.0)); }
+__device__ __forceinline__ float sinR(float x) { return sinf(x); }
+__device__ __forceinline__ double sinR(double x) { return sin(x); }
+__device__ __forceinline__ float cosR(float x) { return cosf(x); }
+__device__ __forceinline__ double cosR(double x) { return cos(x); }
+__device__ __forceinline__ float expR(float x) { return expf(x); }
+__device__ __forceinline__ double expR(double x) { return exp(x); }
+
+// Exact FP64 geometry for the decisions that must match the CPU path bit
+// for bit (NL predicate, list filters, contact-state transitions, QA
+// criteria, displacement check): no contraction, IEEE sqrt and division,
+// the reference's operand order (vec3_ops_expr.h:6-33, pbc.h:37-49).
+__device__ __forceinline__ double xwrap1(double v, double cell, double cinv) {
+  return __dsub_rn(v, __dmul_rn(cell, round(__dmul_rn(v, cinv))));
+}
+__device__ __forceinline__ double xnorm(double x, double y, double z) {
+  return __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z)));
+}
+
+} // namespace pcg

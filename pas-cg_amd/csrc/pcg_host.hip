@@ -1,0 +1,1592 @@
+// pcg_host.hip -- host side of the C ABI declared in include/pcg.h.
+//
+// Builds a batch (one or many systems) the way state::traj_init and
+// thread::init_kernels build the reference's state (cg/src/cg/simul/state.cpp:
+// 83-112, 144-568; thread_setup.cpp:51-720), lays it out in one device
+// allocation, and drives the stage kernels (plain context), the persistent
+// kernel (batch) or continuous rounds of it (server).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pcg_launch.hpp"
+
+using namespace pcg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(PCG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+// Numerical Recipes ran2 (L'Ecuyer with Bays-Durham shuffle) and its
+// cos-branch Box-Muller, the generator of the reference's COMPAT_MODE build
+// (cg/include/cg/utils/random/nr.h:24-91, cg/CMakeLists.txt:67-69).
+struct Ran2 {
+  int iy = 0, idum = -448, idum2 = 123456789;
+  int iv[32] = {0};
+  explicit Ran2(int seed = 0) {
+    draw(seed);
+    draw(seed);
+  }
+  double draw(int iseed = 0) {
+    const int im1 = 2147483563, im2 = 2147483399, imm1 = im1 - 1, ia1 = 40014, ia2 = 40692,
+              iq1 = 53668, iq2 = 52774, ir1 = 12211, ir2 = 3791, ntab = 32,
+              ndiv = 1 + imm1 / ntab;
+    const double eps = 1.2e-7f, rnmx = 1.f - eps, am = 1.f / (float)im1;
+    int k, j;
+    if (iseed != 0) idum = -iseed;
+    if (idum <= 0) {
+      idum2 = idum = std::max(-idum, 1);
+      for (j = ntab + 7; j >= 0; --j) {
+        k = idum / iq1;
+        idum = ia1 * (idum - k * iq1) - k * ir1;
+        if (idum < 0) idum += im1;
+        if (j < ntab) iv[j] = idum;
+      }
+      iy = iv[0];
+    }
+    k = idum / iq1;
+    idum = ia1 * (idum - k * iq1) - k * ir1;
+    if (idum < 0) idum += im1;
+    k = idum2 / iq2;
+    idum2 = ia2 * (idum2 - k * iq2) - k * ir2;
+    if (idum2 < 0) idum2 += im2;
+    j = iy / ndiv;
+    iy = iv[j] - idum2;
+    iv[j] = idum;
+    if (iy < 1) iy += imm1;
+    return std::min(am * iy, rnmx);
+  }
+  double normal() {
+    double r1 = draw(), r2 = draw();
+    return std::sqrt(-2.0 * std::log(r1)) * std::cos((2.0 * M_PI) * r2);
+  }
+};
+
+uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// ----------------------------------------------------------- host batch ----
+struct HostSys {
+  int n = 0;
+  std::vector<double> pos, orig;
+  std::vector<uint8_t> atype;
+  std::vector<int> chain, seq;
+  std::vector<int> teth[2];
+  std::vector<double> teth_nat;
+  std::vector<int> lrep[2];
+  std::vector<int> chir[4];
+  std::vector<double> chir_f0, chir_f1;
+  std::vector<int> nang[3];
+  std::vector<double> nang_th;
+  std::vector<int> hang[3];
+  std::vector<uint8_t> hang_t;
+  std::vector<int> ndih[4];
+  std::vector<double> ndih_ph;
+  std::vector<int> hdih[4];
+  std::vector<uint8_t> hdih_t;
+  std::vector<int> nc[2];
+  std::vector<double> nc_nat;
+  std::vector<uint8_t> nc_type, nc_active;
+  std::vector<int> tip_res;
+  std::vector<double> tip_orig[3], tip_vel[3];
+  std::vector<double> y1;  // initial velocities * dt
+  double temperature = 0.0;
+  int nlcap = 0, qfcap = 0, qccap = 0;
+};
+
+double norm3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+int heur_type(int a2, int a3) { // aa_heur_pair, cg/src/cg/angles/aa_heur_pair.cpp:11-27
+  int o1 = a2 == 7 ? 0 : (a2 == 14 ? 1 : 2);
+  int o2 = a3 == 7 ? 0 : (a3 == 14 ? 1 : 2);
+  return 3 * o1 + o2;
+}
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// state::setup_* of cg/src/cg/simul/state.cpp for one system.
+int build_system(const pcg_params &p, const pcg_model &m, double temperature, int seed,
+                 HostSys &h) {
+  int n = m.n;
+  if (n <= 0 || !m.pos || !m.atype || !m.chain_idx || !m.seq_idx)
+    return fail(PCG_E_ARG, "model needs n > 0, pos, atype, chain_idx, seq_idx");
+  h.n = n;
+  h.pos.assign(m.pos, m.pos + 3 * n);
+  h.orig.assign(m.orig_pos ? m.orig_pos : m.pos, (m.orig_pos ? m.orig_pos : m.pos) + 3 * n);
+  h.atype.assign(m.atype, m.atype + n);
+  h.chain.assign(m.chain_idx, m.chain_idx + n);
+  h.seq.assign(m.seq_idx, m.seq_idx + n);
+  for (int i = 0; i < n; ++i)
+    if (h.atype[i] > 20) return fail(PCG_E_ARG, "residue type out of range");
+  h.temperature = temperature < 0 ? p.temperature : temperature;
+  auto P = [&](int i) { return &h.pos[3 * i]; };
+  auto chk = [&](int i) { return i >= 0 && i < n; };
+
+  // initial velocities from ran2 (state::setup_langevin, state.cpp:270-335)
+  h.y1.assign(3 * n, 0.0);
+  if (p.lang_on) {
+    Ran2 g(seed);
+    double sum[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+      double xx = 2.0 * (g.draw() - 0.5), yy = 2.0 * (g.draw() - 0.5), zz = 2.0 * (g.draw() - 0.5);
+      double xyz = 1.0 / std::sqrt(xx * xx + yy * yy + zz * zz);
+      h.y1[3 * i] = xx * xyz;
+      h.y1[3 * i + 1] = yy * xyz;
+      h.y1[3 * i + 2] = zz * xyz;
+      for (int c = 0; c < 3; ++c) sum[c] += h.y1[3 * i + c];
+    }
+    double x = 0.0;
+    for (int i = 0; i < n; ++i) {
+      for (int c = 0; c < 3; ++c) h.y1[3 * i + c] -= sum[c] / (double)n;
+      double *v = &h.y1[3 * i];
+      x = x + v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    }
+    double dt = p.dt, delsq = dt * dt;
+    double heat = std::sqrt(delsq * 3.0 * n * h.temperature / x);
+    for (double &v : h.y1) v *= heat;
+  }
+  // local repulsion: (i1, i3) of every angle (state.cpp:354-364)
+  for (int a = 0; a < m.n_ang; ++a) {
+    int i1 = m.ang[3 * a], i2 = m.ang[3 * a + 1], i3 = m.ang[3 * a + 2];
+    if (!chk(i1) || !chk(i2) || !chk(i3)) return fail(PCG_E_ARG, "angle index out of range");
+    if (p.lrep_on) {
+      h.lrep[0].push_back(i1);
+      h.lrep[1].push_back(i3);
+    }
+    if (m.ang_has && m.ang_has[a]) {
+      if (p.nat_ang_on) {
+        h.nang[0].push_back(i1);
+        h.nang[1].push_back(i2);
+        h.nang[2].push_back(i3);
+        h.nang_th.push_back(m.ang_theta[a]);
+      }
+    } else if (p.heur_ang_on) {
+      h.hang[0].push_back(i1);
+      h.hang[1].push_back(i2);
+      h.hang[2].push_back(i3);
+      h.hang_t.push_back((uint8_t)heur_type(h.atype[i2], h.atype[i3]));
+    }
+  }
+  for (int d = 0; d < m.n_dih; ++d) {
+    int q[4] = {m.dih[4 * d], m.dih[4 * d + 1], m.dih[4 * d + 2], m.dih[4 * d + 3]};
+    for (int k = 0; k < 4; ++k)
+      if (!chk(q[k])) return fail(PCG_E_ARG, "dihedral index out of range");
+    if (p.chir_on) {
+      // state::setup_chir (state.cpp:366-387); the reference emplaces
+      // (nat_factor, nat_chir) into fields declared (nat_chir, nat_factor)
+      // (cg/include/cg/chir/chiral_quad.h:12-13), so the stored quad carries
+      // them swapped: f0 (read as nat_factor) = nat_chir, f1 = nat_factor.
+      double n12[3], n23[3], n34[3];
+      for (int c = 0; c < 3; ++c) {
+        n12[c] = P(q[1])[c] - P(q[0])[c];
+        n23[c] = P(q[2])[c] - P(q[1])[c];
+        n34[c] = P(q[3])[c] - P(q[2])[c];
+      }
+      double ni = 1.0 / std::sqrt(n23[0] * n23[0] + n23[1] * n23[1] + n23[2] * n23[2]);
+      double nat_factor = ni * (ni * (ni * 1.0));
+      double cx = n23[1] * n34[2] - n23[2] * n34[1], cy = n23[2] * n34[0] - n23[0] * n34[2],
+             cz = n23[0] * n34[1] - n23[1] * n34[0];
+      double nat_chir = (n12[0] * cx + n12[1] * cy + n12[2] * cz) * nat_factor;
+      for (int k = 0; k < 4; ++k) h.chir[k].push_back(q[k]);
+      h.chir_f0.push_back(nat_chir);
+      h.chir_f1.push_back(nat_factor);
+    }
+    if (m.dih_has && m.dih_has[d]) {
+      if (p.nat_dih_on) {
+        for (int k = 0; k < 4; ++k) h.ndih[k].push_back(q[k]);
+        h.ndih_ph.push_back(m.dih_phi[d]);
+      }
+    } else if (p.heur_dih_on) {
+      for (int k = 0; k < 4; ++k) h.hdih[k].push_back(q[k]);
+      h.hdih_t.push_back((uint8_t)heur_type(h.atype[q[1]], h.atype[q[2]]));
+    }
+  }
+  // tethers: length from the orig positions (state.cpp:389-402)
+  if (p.tether_on)
+    for (int k = 0; k < m.n_teth; ++k) {
+      int i1 = m.teth[2 * k], i2 = m.teth[2 * k + 1];
+      if (!chk(i1) || !chk(i2)) return fail(PCG_E_ARG, "tether index out of range");
+      double d[3];
+      for (int c = 0; c < 3; ++c) d[c] = h.orig[3 * i2 + c] - h.orig[3 * i1 + c];
+      h.teth[0].push_back(i1);
+      h.teth[1].push_back(i2);
+      h.teth_nat.push_back(norm3(d));
+    }
+  // native contacts (state.cpp:480-502), ordered by residue indices
+  if (p.nat_cont_on && m.n_cont > 0) {
+    std::vector<int> ord(m.n_cont);
+    std::iota(ord.begin(), ord.end(), 0);
+    auto key = [&](int k) {
+      int a = m.cont[2 * k], b = m.cont[2 * k + 1];
+      return a < b ? std::make_pair(a, b) : std::make_pair(b, a);
+    };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key(a) < key(b); });
+    for (int k : ord) {
+      auto pr = key(k);
+      if (!chk(pr.first) || !chk(pr.second)) return fail(PCG_E_ARG, "contact index out of range");
+      double nat = m.cont_len[k];
+      double d[3];
+      for (int c = 0; c < 3; ++c) d[c] = P(pr.second)[c] - P(pr.first)[c];
+      h.nc[0].push_back(pr.first);
+      h.nc[1].push_back(pr.second);
+      h.nc_nat.push_back(nat);
+      h.nc_type.push_back((uint8_t)(m.cont_type ? m.cont_type[k] : 0));
+      h.nc_active.push_back(norm3(d) < p.counting_factor * C216_INV * nat ? 1 : 0);
+    }
+  }
+  for (int k = 0; k < m.n_tips; ++k) {
+    if (!chk(m.tip_res[k])) return fail(PCG_E_ARG, "tip residue out of range");
+    h.tip_res.push_back(m.tip_res[k]);
+    for (int c = 0; c < 3; ++c) {
+      h.tip_orig[c].push_back(m.tip_orig[3 * k + c]);
+      h.tip_vel[c].push_back(m.tip_vel[3 * k + c]);
+    }
+  }
+  long long pairs = (long long)n * (n - 1) / 2;
+  long long cap = std::min<long long>(pairs, 1200LL * n);
+  h.nlcap = round_up((int)std::max<long long>(cap, 1), BLK);
+  h.qccap = 12 * n + 64;
+  h.qfcap = h.nlcap + h.qccap;
+  return PCG_OK;
+}
+
+void fill_dev_params(const pcg_params &p, DevParams &d) {
+  d.p = p;
+  d.nl_eff_cutoff = std::max(p.nl_cutoff, p.dh_cutoff);
+  d.V_factor = 1.0 / (4.0 * M_PI * p.dh_perm);
+  d.screen_inv = 1.0 / p.dh_screen_dist;
+  d.dt = p.dt;
+  d.dt_inv = 1.0 / p.dt;
+  d.gamma2 = p.gamma * d.dt_inv;
+  d.deltsq = (p.dt * p.dt) / 2.0;
+  double mx = 0.0;
+  for (int k = 0; k < 445; ++k) mx = std::max(mx, p.qa_lj_r_high[k]);
+  d.qa_max_req = mx;
+  d.qa_factor = p.counting_factor * C216_INV;
+  d.qa_sat_speed = 1.0 / p.qa_phase_dur;
+  d.pauli_on = p.pauli_on && !p.qa_on;
+  d.nl_required = d.pauli_on || p.dh_on || p.qa_on || p.pid_on;
+}
+
+double const2_of(const pcg_params &p, double T) {
+  double c = 2.0 * 1.0 * T * p.gamma * p.dt; // step.cpp:101-102
+  return std::sqrt(c) * p.dt;
+}
+
+// One device allocation holding every array of a batch.
+struct Arena {
+  std::vector<std::pair<void **, size_t>> req;
+  size_t total = 0;
+  void *base = nullptr;
+  template <typename T> void add(T **ptr, size_t count) {
+    req.push_back({(void **)ptr, count * sizeof(T)});
+    total += (count * sizeof(T) + 255) / 256 * 256;
+  }
+  int alloc() {
+    if (hipMalloc(&base, total ? total : 256) != hipSuccess)
+      return fail(PCG_E_NOMEM, "hipMalloc of " + std::to_string(total) + " bytes failed");
+    size_t off = 0;
+    for (auto &r : req) {
+      *r.first = (char *)base + off;
+      off += (r.second + 255) / 256 * 256;
+    }
+    return PCG_OK;
+  }
+};
+
+} // namespace
+
+// A device batch: the systems, their arena and the launch configuration.
+struct DeviceBatch {
+  int prec = PCG_MIXED;
+  int noise = PCG_NOISE_PHILOX;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevBatch d{};
+  DevParams hp{};
+  DevParams *dP = nullptr;
+  std::vector<SysDesc> sys;
+  std::vector<HostSys> hs;
+  Arena arena;
+  int max_nlcap = 0, max_n = 0, max_items = 0;
+  size_t bytes = 0;
+  std::vector<Ran2> ran2;  // per system, RAN2 noise mode
+  std::vector<double> noise_h;
+  double *noise_pinned = nullptr;
+  bool own_stream = true;
+
+  ~DeviceBatch() {
+    if (arena.base) (void)hipFree(arena.base);
+    if (noise_pinned) (void)hipHostFree(noise_pinned);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const double *temps,
+               const int32_t *seeds, int prec, uint64_t philox_seed, int device,
+               std::unique_ptr<DeviceBatch> &out, hipStream_t use_stream = nullptr,
+               size_t *dry_run_bytes = nullptr) {
+  if (nsys <= 0 || !models) return fail(PCG_E_ARG, "no systems");
+  if (prec != PCG_MIXED && prec != PCG_FP64) return fail(PCG_E_ARG, "unknown precision");
+  auto B = std::make_unique<DeviceBatch>();
+  B->prec = prec;
+  B->device = device;
+  HIPCHK(hipSetDevice(device));
+  if (use_stream) {
+    B->stream = use_stream;
+    B->own_stream = false;
+  } else if (!dry_run_bytes) {
+    HIPCHK(hipStreamCreate(&B->stream));
+  }
+  fill_dev_params(p, B->hp);
+  B->hs.resize(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    int rc = build_system(p, models[s], temps ? temps[s] : -1.0, seeds ? seeds[s] : 448, B->hs[s]);
+    if (rc) return rc;
+  }
+  // offsets
+  long long ntot = 0, nteth = 0, nlrep = 0, nchir = 0, nnang = 0, nhang = 0, nndih = 0,
+            nhdih = 0, nnc = 0, ntip = 0, nl = 0, qf = 0, qc = 0;
+  B->sys.resize(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    HostSys &h = B->hs[s];
+    SysDesc &d = B->sys[s];
+    std::memset(&d, 0, sizeof(d));
+    d.res0 = (int)ntot;
+    d.n = h.n;
+    d.teth0 = (int)nteth, d.nteth = (int)h.teth_nat.size();
+    d.lrep0 = (int)nlrep, d.nlrep = (int)h.lrep[0].size();
+    d.chir0 = (int)nchir, d.nchir = (int)h.chir_f0.size();
+    d.nang0 = (int)nnang, d.nnang = (int)h.nang_th.size();
+    d.hang0 = (int)nhang, d.nhang = (int)h.hang_t.size();
+    d.ndih0 = (int)nndih, d.nndih = (int)h.ndih_ph.size();
+    d.hdih0 = (int)nhdih, d.nhdih = (int)h.hdih_t.size();
+    d.nc0 = (int)nnc, d.nnc = (int)h.nc_nat.size();
+    d.tip0 = (int)ntip, d.ntip = (int)h.tip_res.size();
+    d.nl0 = (int)nl, d.nlcap = h.nlcap;
+    d.qf0 = (int)qf, d.qfcap = h.qfcap;
+    d.qc0 = (int)qc, d.qccap = h.qccap;
+    const double *cell = models[s].cell;
+    for (int c = 0; c < 3; ++c) {
+      d.cell[c] = cell[c];
+      d.cell_inv[c] = cell[c] != 0.0 ? 1.0 / cell[c] : 0.0;
+    }
+    d.temperature = h.temperature;
+    d.const2 = const2_of(p, h.temperature);
+    d.philox_key = mix64(philox_seed ^ mix64((uint64_t)s + 1));
+    d.steps_target = 0;
+    ntot += h.n;
+    nteth += d.nteth, nlrep += d.nlrep, nchir += d.nchir, nnang += d.nnang, nhang += d.nhang;
+    nndih += d.nndih, nhdih += d.nhdih, nnc += d.nnc, ntip += d.ntip;
+    nl += h.nlcap, qf += h.qfcap, qc += h.qccap;
+    B->max_nlcap = std::max(B->max_nlcap, h.nlcap);
+    B->max_n = std::max(B->max_n, h.n);
+    int items = std::max({d.nteth, d.nlrep, d.nchir, d.nnang, d.nhang, d.nndih, d.nhdih, d.nnc,
+                          h.nlcap, h.qfcap});
+    B->max_items = std::max(B->max_items, items);
+  }
+  if (nl > INT32_MAX / 2 || qf > INT32_MAX / 2) return fail(PCG_E_NOMEM, "batch too large");
+  DevBatch &D = B->d;
+  D.nsys = nsys;
+  D.ntot = (int)ntot;
+  Arena &A = B->arena;
+  size_t fsz = prec == PCG_FP64 ? sizeof(double) : sizeof(float);
+  SysDesc *dsys;
+  SysState *dst;
+  A.add(&dsys, nsys);
+  A.add(&dst, nsys);
+  A.add(&B->dP, 1);
+  for (int k = 0; k < 6; ++k)
+    for (int c = 0; c < 3; ++c) A.add(&D.y[k][c], ntot);
+  for (int c = 0; c < 3; ++c) {
+    A.add(&D.vel[c], ntot);
+    A.add((char **)&D.F[c], ntot * fsz);
+    A.add(&D.orig_r[c], ntot);
+    A.add(&D.noise[c], ntot);
+  }
+  for (int c = 0; c < 6; ++c) A.add(&D.nh[c], ntot);
+  uint8_t *atype;
+  int *chain, *seq, *prev, *next, *sysid, *local;
+  A.add(&atype, ntot);
+  A.add(&chain, ntot);
+  A.add(&seq, ntot);
+  A.add(&prev, ntot);
+  A.add(&next, ntot);
+  A.add(&sysid, ntot);
+  A.add(&local, ntot);
+  A.add(&D.sync, 4 * ntot);
+  A.add(&D.row_cnt, ntot + 1);
+  int *teth_i[2], *lrep_i[2], *chir_i[4], *nang_i[3], *hang_i[3], *ndih_i[4], *hdih_i[4], *nc_i[2];
+  double *teth_nat, *chir_f0, *chir_f1, *nang_th, *ndih_ph, *nc_nat;
+  uint8_t *hang_t, *hdih_t, *nc_type;
+  int *tip_res;
+  double *tip_orig[3], *tip_vel[3];
+  for (int k = 0; k < 2; ++k) A.add(&teth_i[k], std::max(nteth, 1LL));
+  A.add(&teth_nat, std::max(nteth, 1LL));
+  for (int k = 0; k < 2; ++k) A.add(&lrep_i[k], std::max(nlrep, 1LL));
+  for (int k = 0; k < 4; ++k) A.add(&chir_i[k], std::max(nchir, 1LL));
+  A.add(&chir_f0, std::max(nchir, 1LL));
+  A.add(&chir_f1, std::max(nchir, 1LL));
+  for (int k = 0; k < 3; ++k) A.add(&nang_i[k], std::max(nnang, 1LL));
+  A.add(&nang_th, std::max(nnang, 1LL));
+  for (int k = 0; k < 3; ++k) A.add(&hang_i[k], std::max(nhang, 1LL));
+  A.add(&hang_t, std::max(nhang, 1LL));
+  for (int k = 0; k < 4; ++k) A.add(&ndih_i[k], std::max(nndih, 1LL));
+  A.add(&ndih_ph, std::max(nndih, 1LL));
+  for (int k = 0; k < 4; ++k) A.add(&hdih_i[k], std::max(nhdih, 1LL));
+  A.add(&hdih_t, std::max(nhdih, 1LL));
+  for (int k = 0; k < 2; ++k) A.add(&nc_i[k], std::max(nnc, 1LL));
+  A.add(&nc_nat, std::max(nnc, 1LL));
+  A.add(&nc_type, std::max(nnc, 1LL));
+  A.add(&D.nc_active, std::max(nnc, 1LL));
+  A.add(&D.nc_change_t, std::max(nnc, 1LL));
+  A.add(&D.ncur, std::max(nnc, 1LL));
+  A.add(&tip_res, std::max(ntip, 1LL));
+  for (int c = 0; c < 3; ++c) {
+    A.add(&tip_orig[c], std::max(ntip, 1LL));
+    A.add(&tip_vel[c], std::max(ntip, 1LL));
+  }
+  for (int k = 0; k < 2; ++k) {
+    A.add(&D.nl_i[k], nl);
+    A.add(&D.pauli_i[k], nl);
+    A.add(&D.dh_i[k], nl);
+    A.add(&D.pid_i[k], nl);
+    A.add(&D.cand_i[k], nl);
+    A.add(&D.qf_i[k], qf);
+    A.add(&D.qc_i[k], qc);
+  }
+  A.add(&D.nl_d, nl);
+  A.add(&D.nl_cls, nl);
+  A.add(&D.dh_q, nl);
+  A.add(&D.pid_d, nl);
+  A.add(&D.pid_t, nl);
+  A.add(&D.cand_od, nl);
+  A.add(&D.cand_dist, nl);
+  A.add(&D.cand_idx, nl);
+  A.add(&D.cand_t, nl);
+  A.add(&D.cand_sd, 8 * nl);
+  A.add(&D.cand_perm, nl);
+  A.add(&D.qf_d, qf);
+  A.add(&D.qf_vac, qf);
+  A.add(&D.qc_d, qc);
+  A.add(&D.qc_t, qc);
+  A.add(&D.qc_sat, qc);
+  A.add(&D.qc_sd, 8 * qc);
+  A.add(&D.qc_vac, qc);
+  A.add(&D.rem, qc);
+  A.add(&D.blk_cnt, (nl / BLK + 1) * 5);
+  if (dry_run_bytes) {
+    *dry_run_bytes = A.total;
+    return PCG_OK;
+  }
+  int rc = A.alloc();
+  if (rc) return rc;
+  B->bytes = A.total;
+  D.sys = dsys;
+  D.st = dst;
+  D.P = B->dP;
+  D.atype = atype, D.chain = chain, D.seq = seq, D.prev = prev, D.next = next;
+  D.sysid = sysid, D.local = local;
+  for (int k = 0; k < 2; ++k) D.teth_i[k] = teth_i[k], D.lrep_i[k] = lrep_i[k], D.nc_i[k] = nc_i[k];
+  for (int k = 0; k < 4; ++k) D.chir_i[k] = chir_i[k], D.ndih_i[k] = ndih_i[k], D.hdih_i[k] = hdih_i[k];
+  for (int k = 0; k < 3; ++k) D.nang_i[k] = nang_i[k], D.hang_i[k] = hang_i[k];
+  D.teth_nat = teth_nat, D.chir_f0 = chir_f0, D.chir_f1 = chir_f1, D.nang_th = nang_th;
+  D.ndih_ph = ndih_ph, D.nc_nat = nc_nat, D.hang_t = hang_t, D.hdih_t = hdih_t, D.nc_type = nc_type;
+  D.tip_res = tip_res;
+  for (int c = 0; c < 3; ++c) D.tip_orig[c] = tip_orig[c], D.tip_vel[c] = tip_vel[c];
+
+  // host staging, then one copy per array
+  std::vector<SysState> st(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    std::memset(&st[s], 0, sizeof(SysState));
+    st[s].nl_invalid = 1;
+    st[s].verify_first = 1;
+    st[s].active = 1;
+  }
+  auto up = [&](void *dst_, const void *src, size_t bytes) -> int {
+    if (bytes) HIPCHK(hipMemcpyAsync(dst_, src, bytes, hipMemcpyHostToDevice, B->stream));
+    return PCG_OK;
+  };
+  std::vector<double> col(ntot);
+  std::vector<int> ci(ntot);
+  std::vector<uint8_t> cu(ntot);
+  std::vector<int8_t> sync(4 * ntot);
+  std::vector<double> ybuf[6][3], vel[3];
+  for (int k = 0; k < 6; ++k)
+    for (int c = 0; c < 3; ++c) ybuf[k][c].assign(ntot, 0.0);
+  std::vector<int> vchain(ntot), vseq(ntot), vprev(ntot), vnext(ntot), vsys(ntot), vloc(ntot);
+  std::vector<uint8_t> vat(ntot);
+  for (int s = 0; s < nsys; ++s) {
+    HostSys &h = B->hs[s];
+    int r0 = B->sys[s].res0;
+    for (int i = 0; i < h.n; ++i) {
+      int g = r0 + i;
+      for (int c = 0; c < 3; ++c) {
+        ybuf[0][c][g] = h.pos[3 * i + c];
+        ybuf[1][c][g] = h.y1[3 * i + c];
+      }
+      vat[g] = h.atype[i];
+      vchain[g] = h.chain[i];
+      vseq[g] = h.seq[i];
+      vprev[g] = (i > 0 && h.chain[i - 1] == h.chain[i]) ? g - 1 : -1;
+      vnext[g] = (i + 1 < h.n && h.chain[i + 1] == h.chain[i]) ? g + 1 : -1;
+      vsys[g] = s;
+      vloc[g] = i;
+      for (int k = 0; k < 4; ++k) sync[4 * g + k] = (int8_t)p.base_sync[h.atype[i]][k];
+    }
+  }
+  for (int k = 0; k < 6; ++k)
+    for (int c = 0; c < 3; ++c)
+      if ((rc = up(D.y[k][c], ybuf[k][c].data(), ntot * sizeof(double)))) return rc;
+  if ((rc = up(atype, vat.data(), ntot))) return rc;
+  if ((rc = up(chain, vchain.data(), ntot * 4))) return rc;
+  if ((rc = up(seq, vseq.data(), ntot * 4))) return rc;
+  if ((rc = up(prev, vprev.data(), ntot * 4))) return rc;
+  if ((rc = up(next, vnext.data(), ntot * 4))) return rc;
+  if ((rc = up(sysid, vsys.data(), ntot * 4))) return rc;
+  if ((rc = up(local, vloc.data(), ntot * 4))) return rc;
+  if ((rc = up(D.sync, sync.data(), 4 * ntot))) return rc;
+  // static lists, concatenated with residue indices shifted to the batch
+  auto cat_i = [&](int *dst_, auto getv, long long total) -> int {
+    std::vector<int> v;
+    v.reserve(total);
+    for (int s = 0; s < nsys; ++s) {
+      const std::vector<int> &x = getv(B->hs[s]);
+      for (int a : x) v.push_back(a + B->sys[s].res0);
+    }
+    return up(dst_, v.data(), v.size() * 4);
+  };
+  auto cat_d = [&](double *dst_, auto getv) -> int {
+    std::vector<double> v;
+    for (int s = 0; s < nsys; ++s) {
+      const std::vector<double> &x = getv(B->hs[s]);
+      v.insert(v.end(), x.begin(), x.end());
+    }
+    return up(dst_, v.data(), v.size() * 8);
+  };
+  auto cat_u = [&](uint8_t *dst_, auto getv) -> int {
+    std::vector<uint8_t> v;
+    for (int s = 0; s < nsys; ++s) {
+      const std::vector<uint8_t> &x = getv(B->hs[s]);
+      v.insert(v.end(), x.begin(), x.end());
+    }
+    return up(dst_, v.data(), v.size());
+  };
+#define CATI(dst_, field) \
+  if ((rc = cat_i(dst_, [&](HostSys &h) -> const std::vector<int> & { return h.field; }, 0))) return rc
+#define CATD(dst_, field) \
+  if ((rc = cat_d(dst_, [&](HostSys &h) -> const std::vector<double> & { return h.field; }))) return rc
+#define CATU(dst_, field) \
+  if ((rc = cat_u(dst_, [&](HostSys &h) -> const std::vector<uint8_t> & { return h.field; }))) return rc
+  for (int k = 0; k < 2; ++k) {
+    CATI(teth_i[k], teth[k]);
+    CATI(lrep_i[k], lrep[k]);
+    CATI(nc_i[k], nc[k]);
+  }
+  for (int k = 0; k < 4; ++k) {
+    CATI(chir_i[k], chir[k]);
+    CATI(ndih_i[k], ndih[k]);
+    CATI(hdih_i[k], hdih[k]);
+  }
+  for (int k = 0; k < 3; ++k) {
+    CATI(nang_i[k], nang[k]);
+    CATI(hang_i[k], hang[k]);
+  }
+  CATI(tip_res, tip_res);
+  CATD(teth_nat, teth_nat);
+  CATD(chir_f0, chir_f0);
+  CATD(chir_f1, chir_f1);
+  CATD(nang_th, nang_th);
+  CATD(ndih_ph, ndih_ph);
+  CATD(nc_nat, nc_nat);
+  for (int c = 0; c < 3; ++c) {
+    CATD(tip_orig[c], tip_orig[c]);
+    CATD(tip_vel[c], tip_vel[c]);
+  }
+  CATU(hang_t, hang_t);
+  CATU(hdih_t, hdih_t);
+  CATU(nc_type, nc_type);
+  CATU(D.nc_active, nc_active);
+  {
+    std::vector<double> ct(std::max(nnc, 1LL), -1.0);
+    if ((rc = up(D.nc_change_t, ct.data(), ct.size() * 8))) return rc;
+  }
+#undef CATI
+#undef CATD
+#undef CATU
+  if ((rc = up(dsys, B->sys.data(), nsys * sizeof(SysDesc)))) return rc;
+  if ((rc = up(dst, st.data(), nsys * sizeof(SysState)))) return rc;
+  if ((rc = up(B->dP, &B->hp, sizeof(DevParams)))) return rc;
+  // forces start at zero so that the first Langevin step (t == 0) reads them
+  std::vector<char> zero(ntot * fsz, 0);
+  for (int c = 0; c < 3; ++c)
+    if ((rc = up(D.F[c], zero.data(), zero.size()))) return rc;
+  HIPCHK(hipStreamSynchronize(B->stream));
+  out = std::move(B);
+  return PCG_OK;
+}
+
+int read_state(DeviceBatch &B, int s, SysState &st) {
+  HIPCHK(hipMemcpyAsync(&st, B.d.st + s, sizeof(SysState), hipMemcpyDeviceToHost, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+int write_state(DeviceBatch &B, int s, const SysState &st) {
+  HIPCHK(hipMemcpyAsync(B.d.st + s, &st, sizeof(SysState), hipMemcpyHostToDevice, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+// copies 3 device columns of system s into xyz-interleaved host memory
+int get_cols(DeviceBatch &B, int s, double *const cols[3], double *out) {
+  const SysDesc &d = B.sys[s];
+  std::vector<double> tmp(3 * (size_t)d.n);
+  for (int c = 0; c < 3; ++c)
+    HIPCHK(hipMemcpyAsync(tmp.data() + c * d.n, cols[c] + d.res0, d.n * 8, hipMemcpyDeviceToHost,
+                          B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  for (int i = 0; i < d.n; ++i)
+    for (int c = 0; c < 3; ++c) out[3 * i + c] = tmp[c * d.n + i];
+  return PCG_OK;
+}
+int get_forces(DeviceBatch &B, int s, double *out) {
+  const SysDesc &d = B.sys[s];
+  size_t fsz = B.prec == PCG_FP64 ? 8 : 4;
+  std::vector<char> tmp(3 * fsz * d.n);
+  for (int c = 0; c < 3; ++c)
+    HIPCHK(hipMemcpyAsync(tmp.data() + c * fsz * d.n, (char *)B.d.F[c] + fsz * d.res0, fsz * d.n,
+                          hipMemcpyDeviceToHost, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  for (int i = 0; i < d.n; ++i)
+    for (int c = 0; c < 3; ++c)
+      out[3 * i + c] = B.prec == PCG_FP64 ? ((double *)tmp.data())[c * d.n + i]
+                                          : (double)((float *)tmp.data())[c * d.n + i];
+  return PCG_OK;
+}
+
+// Reference-order noise for RAN2 mode: all x, then all y, then all z per
+// system (cg/src/cg/langevin/step.cpp:119-127).
+int upload_ran2_noise(DeviceBatch &B) {
+  int ntot = B.d.ntot;
+  if (!B.noise_pinned) HIPCHK(hipHostMalloc((void **)&B.noise_pinned, 3 * ntot * sizeof(double)));
+  for (size_t s = 0; s < B.sys.size(); ++s) {
+    const SysDesc &d = B.sys[s];
+    for (int c = 0; c < 3; ++c)
+      for (int i = 0; i < d.n; ++i) B.noise_pinned[c * ntot + d.res0 + i] = B.ran2[s].normal();
+  }
+  for (int c = 0; c < 3; ++c)
+    HIPCHK(hipMemcpyAsync(B.d.noise[c], B.noise_pinned + c * ntot, ntot * 8, hipMemcpyHostToDevice,
+                          B.stream));
+  // the pinned buffer is rewritten next step: wait for the copy
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+int check_overflow(DeviceBatch &B) {
+  for (size_t s = 0; s < B.sys.size(); ++s) {
+    SysState st;
+    int rc = read_state(B, (int)s, st);
+    if (rc) return rc;
+    if (st.overflow)
+      return fail(PCG_E_OVERFLOW, "a neighbour or QA list of system " + std::to_string(s) +
+                                      " outgrew its capacity");
+  }
+  return PCG_OK;
+}
+
+} // namespace
+
+// ------------------------------------------------------------------ ctx ----
+struct pcg_ctx {
+  std::unique_ptr<DeviceBatch> B;
+  bool graph_on = false;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int64_t steps_since_check = 0;
+  ~pcg_ctx() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+
+void enqueue_step(DeviceBatch &B) {
+  const DevBatch &d = B.d;
+  const pcg_params &p = B.hp.p;
+  launch_begin(d, B.stream);
+  launch_zero_forces(d, B.prec, B.stream);
+  if (p.qa_on) launch_prepare_nh(d, B.stream);
+  launch_fix_nl(d, B.max_nlcap, B.stream);
+  launch_terms(d, B.prec, B.max_items, -1, 0, -1, B.stream);
+  if (p.qa_on) launch_qa_finish(d, B.stream);
+  launch_lang(d, B.prec, B.noise, B.stream);
+  launch_tick(d, 1, B.stream);
+}
+
+} // namespace
+
+extern "C" {
+
+const char *pcg_last_error(void) { return g_err.c_str(); }
+
+int pcg_device_count(int *count) {
+  if (!count) return fail(PCG_E_ARG, "null count");
+  HIPCHK(hipGetDeviceCount(count));
+  return PCG_OK;
+}
+
+int pcg_create(const pcg_params *p, const pcg_model *m, const pcg_options *o, pcg_ctx **out) {
+  if (!p || !m || !out) return fail(PCG_E_ARG, "null argument");
+  pcg_options opt{};
+  if (o) opt = *o;
+  double T = opt.temperature > 0 ? opt.temperature : -1.0;
+  int32_t seed = opt.seed;
+  auto c = std::make_unique<pcg_ctx>();
+  int rc = make_batch(*p, 1, m, &T, &seed, opt.precision, opt.philox_seed, opt.device, c->B);
+  if (rc) return rc;
+  c->B->noise = opt.noise;
+  if (opt.noise == PCG_NOISE_RAN2) {
+    // the noise stream continues the one the initial velocities came from
+    Ran2 g(seed);
+    if (p->lang_on)
+      for (int k = 0; k < 3 * m->n; ++k) g.draw();
+    c->B->ran2.push_back(g);
+  }
+  *out = c.release();
+  return PCG_OK;
+}
+
+int pcg_destroy(pcg_ctx *c) {
+  delete c;
+  return PCG_OK;
+}
+
+int pcg_step(pcg_ctx *c, int32_t nsteps) {
+  if (!c || nsteps < 0) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  HIPCHK(hipSetDevice(B.device));
+  for (int k = 0; k < nsteps; ++k) {
+    int rc;
+    if (B.noise == PCG_NOISE_RAN2 && (rc = upload_ran2_noise(B))) return rc;
+    if (c->graph_on) {
+      if (!c->gexec) {
+        HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(B);
+        HIPCHK(hipStreamEndCapture(B.stream, &c->graph));
+        HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(c->gexec, B.stream));
+    } else {
+      enqueue_step(B);
+    }
+  }
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return check_overflow(B);
+}
+
+int pcg_use_graph(pcg_ctx *c, int32_t on) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  c->graph_on = on != 0;
+  return PCG_OK;
+}
+
+int pcg_nl_verify(pcg_ctx *c, int32_t *invalid) {
+  if (!c || !invalid) return fail(PCG_E_ARG, "null argument");
+  DeviceBatch &B = *c->B;
+  launch_begin(B.d, B.stream);
+  SysState st;
+  int rc = read_state(B, 0, st);
+  if (rc) return rc;
+  *invalid = st.nl_invalid;
+  return PCG_OK;
+}
+
+int pcg_fix_nl(pcg_ctx *c) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  DeviceBatch &B = *c->B;
+  SysState st;
+  int rc = read_state(B, 0, st);
+  if (rc) return rc;
+  st.nl_invalid = 1;
+  st.verify_first = 0;
+  if ((rc = write_state(B, 0, st))) return rc;
+  launch_fix_nl(B.d, B.max_nlcap, B.stream);
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return check_overflow(B);
+}
+
+int pcg_prepare_nh(pcg_ctx *c) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  launch_prepare_nh(c->B->d, c->B->stream);
+  HIPCHK(hipStreamSynchronize(c->B->stream));
+  return PCG_OK;
+}
+
+int pcg_eval_forces(pcg_ctx *c) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  DeviceBatch &B = *c->B;
+  SysState st;
+  int rc = read_state(B, 0, st);
+  if (rc) return rc;
+  for (int k = 0; k < NT; ++k) st.V[k] = 0.0;
+  if ((rc = write_state(B, 0, st))) return rc;
+  launch_zero_forces(B.d, B.prec, B.stream);
+  launch_terms(B.d, B.prec, B.max_items, -1, 0, -1, B.stream);
+  launch_tick(B.d, 0, B.stream);
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return check_overflow(B);
+}
+
+int pcg_term_size(pcg_ctx *c, int32_t term, int32_t *size) {
+  if (!c || !size || term < 0 || term >= NT) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  SysState st;
+  int rc = read_state(B, 0, st);
+  if (rc) return rc;
+  const SysDesc &s = B.sys[0];
+  int v = 0;
+  switch (term) {
+  case PCG_CHIR: v = s.nchir; break;
+  case PCG_TETHER: v = s.nteth; break;
+  case PCG_LREP: v = s.nlrep; break;
+  case PCG_NAT_CONT: v = st.nc_n; break;
+  case PCG_PAULI: v = st.pauli_n; break;
+  case PCG_CONST_DH: case PCG_REL_DH: v = st.dh_n; break;
+  case PCG_QA_CAND: v = st.qf_n; break;
+  case PCG_QA_CONT: v = std::min(st.qc_n, st.qf_n); break;
+  case PCG_PID: v = st.pid_n; break;
+  case PCG_NAT_ANG: v = s.nnang; break;
+  case PCG_HEUR_ANG: v = s.nhang; break;
+  case PCG_HEUR_DIH: v = s.nhdih; break;
+  case PCG_CND: case PCG_SND: v = s.nndih; break;
+  case PCG_VEL_AFM: v = s.ntip; break;
+  }
+  *size = v;
+  return PCG_OK;
+}
+
+int pcg_eval_term(pcg_ctx *c, int32_t term, int32_t from, int32_t to, double *F, double *V) {
+  if (!c || term < 0 || term >= NT) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  SysState st;
+  int rc = read_state(B, 0, st);
+  if (rc) return rc;
+  for (int k = 0; k < NT; ++k) st.V[k] = 0.0;
+  if ((rc = write_state(B, 0, st))) return rc;
+  launch_zero_forces(B.d, B.prec, B.stream);
+  launch_terms(B.d, B.prec, B.max_items, term, from, to, B.stream);
+  HIPCHK(hipStreamSynchronize(B.stream));
+  if ((rc = read_state(B, 0, st))) return rc;
+  if (V) *V = st.V[term];
+  if (F && (rc = get_forces(B, 0, F))) return rc;
+  return check_overflow(B);
+}
+
+int pcg_qa_finish(pcg_ctx *c) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  launch_qa_finish(c->B->d, c->B->stream);
+  HIPCHK(hipStreamSynchronize(c->B->stream));
+  return check_overflow(*c->B);
+}
+
+int pcg_lang_step(pcg_ctx *c) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  DeviceBatch &B = *c->B;
+  int rc;
+  if (B.noise == PCG_NOISE_RAN2 && (rc = upload_ran2_noise(B))) return rc;
+  launch_lang(B.d, B.prec, B.noise, B.stream);
+  launch_tick(B.d, 1, B.stream);
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+int pcg_get(pcg_ctx *c, int32_t which, double *out) {
+  if (!c || !out || which < 0 || which > 7) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  if (which <= 5) return get_cols(B, 0, B.d.y[which], out);
+  if (which == 6) return get_forces(B, 0, out);
+  return get_cols(B, 0, B.d.vel, out);
+}
+
+int pcg_set(pcg_ctx *c, int32_t which, const double *in) {
+  if (!c || !in || which < 0 || which > 5) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  const SysDesc &d = B.sys[0];
+  std::vector<double> tmp(3 * (size_t)d.n);
+  for (int i = 0; i < d.n; ++i)
+    for (int c2 = 0; c2 < 3; ++c2) tmp[c2 * d.n + i] = in[3 * i + c2];
+  for (int c2 = 0; c2 < 3; ++c2)
+    HIPCHK(hipMemcpyAsync(B.d.y[which][c2] + d.res0, tmp.data() + c2 * d.n, d.n * 8,
+                          hipMemcpyHostToDevice, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+int pcg_get_energy(pcg_ctx *c, double *V, double *term_V) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  SysState st;
+  int rc = read_state(*c->B, 0, st);
+  if (rc) return rc;
+  if (V) *V = st.Vtot;
+  if (term_V)
+    for (int k = 0; k < NT; ++k) term_V[k] = st.V[k];
+  return PCG_OK;
+}
+
+int pcg_get_time(pcg_ctx *c, double *t, int64_t *step) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  SysState st;
+  int rc = read_state(*c->B, 0, st);
+  if (rc) return rc;
+  if (t) *t = st.t;
+  if (step) *step = st.step;
+  return PCG_OK;
+}
+
+int pcg_set_time(pcg_ctx *c, double t) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  SysState st;
+  int rc = read_state(*c->B, 0, st);
+  if (rc) return rc;
+  st.t = st.true_t = t;
+  return write_state(*c->B, 0, st);
+}
+
+int pcg_set_noise_scale(pcg_ctx *c, double scale) {
+  if (!c || scale < 0) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  B.sys[0].const2 = const2_of(B.hp.p, B.sys[0].temperature) * scale;
+  HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), sizeof(SysDesc), hipMemcpyHostToDevice,
+                        B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+int pcg_get_list(pcg_ctx *c, int32_t kind, int32_t cap, int32_t *size, int32_t *i1, int32_t *i2,
+                 double *val, uint8_t *flag) {
+  if (!c || !size || kind < 0 || kind > 6) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  SysState st;
+  int rc = read_state(B, 0, st);
+  if (rc) return rc;
+  const SysDesc &d = B.sys[0];
+  const DevBatch &D = B.d;
+  int n = 0, off = 0;
+  const int *a = nullptr, *b2 = nullptr;
+  const double *v = nullptr;
+  const uint8_t *f = nullptr;
+  switch (kind) {
+  case 0: n = st.nl_n, off = d.nl0, a = D.nl_i[0], b2 = D.nl_i[1], v = D.nl_d; break;
+  case 1: n = st.nc_n, off = d.nc0; break;
+  case 2: n = st.pauli_n, off = d.nl0, a = D.pauli_i[0], b2 = D.pauli_i[1]; break;
+  case 3: n = st.dh_n, off = d.nl0, a = D.dh_i[0], b2 = D.dh_i[1], v = D.dh_q; break;
+  case 4: n = st.pid_n, off = d.nl0, a = D.pid_i[0], b2 = D.pid_i[1], v = D.pid_d; break;
+  case 5: n = st.qf_n, off = d.qf0, a = D.qf_i[0], b2 = D.qf_i[1], v = D.qf_d, f = D.qf_vac; break;
+  case 6: n = st.qc_n, off = d.qc0, a = D.qc_i[0], b2 = D.qc_i[1], v = D.qc_sat, f = D.qc_vac; break;
+  }
+  *size = n;
+  int m = std::min(n, cap);
+  if (m <= 0) return PCG_OK;
+  std::vector<int> ta(m), tb(m);
+  std::vector<double> tv(m, 0.0);
+  std::vector<uint8_t> tf(m, 0);
+  if (kind == 1) {
+    std::vector<int> idx(m);
+    HIPCHK(hipMemcpy(idx.data(), D.ncur + off, m * 4, hipMemcpyDeviceToHost));
+    std::vector<int> A(d.nnc), Bv(d.nnc);
+    std::vector<double> nat(d.nnc);
+    std::vector<uint8_t> act(d.nnc);
+    HIPCHK(hipMemcpy(A.data(), D.nc_i[0] + d.nc0, d.nnc * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(Bv.data(), D.nc_i[1] + d.nc0, d.nnc * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nat.data(), D.nc_nat + d.nc0, d.nnc * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(act.data(), D.nc_active + d.nc0, d.nnc, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) {
+      int q = idx[k] - d.nc0;
+      ta[k] = A[q] - d.res0;
+      tb[k] = Bv[q] - d.res0;
+      tv[k] = nat[q];
+      tf[k] = act[q];
+    }
+  } else {
+    HIPCHK(hipMemcpy(ta.data(), a + off, m * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tb.data(), b2 + off, m * 4, hipMemcpyDeviceToHost));
+    if (v) HIPCHK(hipMemcpy(tv.data(), v + off, m * 8, hipMemcpyDeviceToHost));
+    if (f) HIPCHK(hipMemcpy(tf.data(), f + off, m, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) {
+      ta[k] -= d.res0;
+      tb[k] -= d.res0;
+    }
+  }
+  if (i1) std::memcpy(i1, ta.data(), m * 4);
+  if (i2) std::memcpy(i2, tb.data(), m * 4);
+  if (val) std::memcpy(val, tv.data(), m * 8);
+  if (flag) std::memcpy(flag, tf.data(), m);
+  return PCG_OK;
+}
+
+int pcg_nat_cont_state(pcg_ctx *c, uint8_t *active, double *change_t, int32_t *num_changed) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  DeviceBatch &B = *c->B;
+  const SysDesc &d = B.sys[0];
+  if (active && d.nnc)
+    HIPCHK(hipMemcpy(active, B.d.nc_active + d.nc0, d.nnc, hipMemcpyDeviceToHost));
+  if (change_t && d.nnc)
+    HIPCHK(hipMemcpy(change_t, B.d.nc_change_t + d.nc0, d.nnc * 8, hipMemcpyDeviceToHost));
+  if (num_changed) {
+    SysState st;
+    int rc = read_state(B, 0, st);
+    if (rc) return rc;
+    *num_changed = st.num_changed;
+  }
+  return PCG_OK;
+}
+
+int pcg_qa_state(pcg_ctx *c, int8_t *sync, int32_t *num_contacts) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  DeviceBatch &B = *c->B;
+  const SysDesc &d = B.sys[0];
+  if (sync) HIPCHK(hipMemcpy(sync, B.d.sync + 4 * d.res0, 4 * d.n, hipMemcpyDeviceToHost));
+  if (num_contacts) {
+    SysState st;
+    int rc = read_state(B, 0, st);
+    if (rc) return rc;
+    *num_contacts = st.num_qa;
+  }
+  return PCG_OK;
+}
+
+int pcg_nl_rebuilds(pcg_ctx *c, int32_t *count) {
+  if (!c || !count) return fail(PCG_E_ARG, "null argument");
+  SysState st;
+  int rc = read_state(*c->B, 0, st);
+  if (rc) return rc;
+  *count = st.rebuilds;
+  return PCG_OK;
+}
+
+int pcg_stream(pcg_ctx *c, void **stream) {
+  if (!c || !stream) return fail(PCG_E_ARG, "null argument");
+  *stream = (void *)c->B->stream;
+  return PCG_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- batch ----
+struct pcg_batch {
+  std::unique_ptr<DeviceBatch> B;
+  int block = 0, spl = 0;  // threads per workgroup, steps per launch
+  bool persistent = true;
+  bool use_graph = false;
+  size_t lds = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int last_launches = 0;
+  bool tuned = false;
+  ~pcg_batch() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
+
+namespace {
+
+size_t lds_for(const DeviceBatch &B) {
+  size_t fsz = B.prec == PCG_FP64 ? 8 : 4;
+  return (size_t)B.max_n * (3 * 8 + 3 * fsz) + 64;
+}
+
+// steps owed by every system = current + nsteps
+int set_targets(DeviceBatch &B, int64_t nsteps) {
+  std::vector<SysState> st(B.sys.size());
+  HIPCHK(hipMemcpyAsync(st.data(), B.d.st, st.size() * sizeof(SysState), hipMemcpyDeviceToHost,
+                        B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  for (size_t s = 0; s < B.sys.size(); ++s) B.sys[s].steps_target = st[s].step + nsteps;
+  HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), B.sys.size() * sizeof(SysDesc),
+                        hipMemcpyHostToDevice, B.stream));
+  return PCG_OK;
+}
+
+int run_persistent(pcg_batch *b, int nsteps) {
+  DeviceBatch &B = *b->B;
+  int rc = set_targets(B, nsteps);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(b->ev0, B.stream));
+  int launches = 0;
+  for (int done = 0; done < nsteps; done += b->spl) {
+    int k = std::min(b->spl, nsteps - done);
+    launch_persistent(B.d, B.prec, b->block, k, b->lds, B.stream);
+    ++launches;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->ev1, B.stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  b->last_launches = launches;
+  return PCG_OK;
+}
+
+int run_stages(pcg_batch *b, int nsteps) {
+  DeviceBatch &B = *b->B;
+  HIPCHK(hipEventRecord(b->ev0, B.stream));
+  for (int k = 0; k < nsteps; ++k) enqueue_step(B);
+  HIPCHK(hipEventRecord(b->ev1, B.stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  b->last_launches = nsteps * 8;
+  return PCG_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pcg_batch_create(const pcg_params *p, int32_t nsys, const pcg_model *models,
+                     const double *temps, const int32_t *seeds, const pcg_batch_options *o,
+                     pcg_batch **out) {
+  if (!p || !models || !out || nsys <= 0) return fail(PCG_E_ARG, "bad arguments");
+  pcg_batch_options opt{};
+  if (o) opt = *o;
+  auto b = std::make_unique<pcg_batch>();
+  int rc = make_batch(*p, nsys, models, temps, seeds, opt.precision, opt.philox_seed, opt.device,
+                      b->B);
+  if (rc) return rc;
+  DeviceBatch &B = *b->B;
+  b->lds = lds_for(B);
+  // systems whose positions and forces do not fit in LDS take the stage kernels
+  b->persistent = b->lds <= 150 * 1024;
+  b->block = opt.block;
+  if (b->block <= 0) {
+    int want = 64;
+    while (want < 1024 && want < B.max_n * 2) want *= 2;
+    b->block = std::min(std::max(want, 128), 512);
+  }
+  b->spl = opt.steps_per_launch > 0 ? opt.steps_per_launch : 100;
+  b->use_graph = opt.use_graph != 0;
+  HIPCHK(hipEventCreate(&b->ev0));
+  HIPCHK(hipEventCreate(&b->ev1));
+  *out = b.release();
+  return PCG_OK;
+}
+
+int pcg_batch_destroy(pcg_batch *b) {
+  delete b;
+  return PCG_OK;
+}
+
+int pcg_batch_run(pcg_batch *b, int32_t nsteps) {
+  if (!b || nsteps < 0) return fail(PCG_E_ARG, "bad arguments");
+  HIPCHK(hipSetDevice(b->B->device));
+  int rc = b->persistent ? run_persistent(b, nsteps) : run_stages(b, nsteps);
+  if (rc) return rc;
+  return check_overflow(*b->B);
+}
+
+int pcg_batch_sync(pcg_batch *b) {
+  if (!b) return fail(PCG_E_ARG, "null batch");
+  HIPCHK(hipStreamSynchronize(b->B->stream));
+  return PCG_OK;
+}
+
+int pcg_batch_set_noise_scale(pcg_batch *b, double scale) {
+  if (!b || scale < 0) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *b->B;
+  for (size_t s = 0; s < B.sys.size(); ++s)
+    B.sys[s].const2 = const2_of(B.hp.p, B.sys[s].temperature) * scale;
+  HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), B.sys.size() * sizeof(SysDesc),
+                        hipMemcpyHostToDevice, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+int pcg_batch_get(pcg_batch *b, int32_t sys, int32_t which, double *out) {
+  if (!b || !out || sys < 0 || sys >= (int)b->B->sys.size() || which < 0 || which > 7)
+    return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *b->B;
+  if (which <= 5) return get_cols(B, sys, B.d.y[which], out);
+  if (which == 6) return get_forces(B, sys, out);
+  return get_cols(B, sys, B.d.vel, out);
+}
+
+int pcg_batch_energy(pcg_batch *b, int32_t sys, double *V, double *term_V) {
+  if (!b || sys < 0 || sys >= (int)b->B->sys.size()) return fail(PCG_E_ARG, "bad arguments");
+  SysState st;
+  int rc = read_state(*b->B, sys, st);
+  if (rc) return rc;
+  if (V) *V = st.Vtot;
+  if (term_V)
+    for (int k = 0; k < NT; ++k) term_V[k] = st.V[k];
+  return PCG_OK;
+}
+
+int pcg_batch_info(pcg_batch *b, int32_t *block, int32_t *spl, int64_t *bytes) {
+  if (!b) return fail(PCG_E_ARG, "null batch");
+  if (block) *block = b->persistent ? b->block : 0;
+  if (spl) *spl = b->spl;
+  if (bytes) *bytes = (int64_t)b->B->bytes;
+  return PCG_OK;
+}
+
+int pcg_batch_last_ms(pcg_batch *b, double *ms, int32_t *launches) {
+  if (!b) return fail(PCG_E_ARG, "null batch");
+  if (ms) *ms = b->last_ms;
+  if (launches) *launches = b->last_launches;
+  return PCG_OK;
+}
+
+} // extern "C"
+
+// --------------------------------------------------------------- server ----
+// Continuous batching: every job owns a one-system arena sized to it, which
+// is what admission charges against the HBM budget. A round launches one
+// persistent kernel over the descriptors of all resident jobs; jobs finish,
+// fail or are cancelled independently and their slots refill from the queue
+// in the next round.
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct OwnedModel {
+  int n = 0;
+  std::vector<double> pos, orig, ang_theta, dih_phi, cont_len, tip_orig, tip_vel;
+  std::vector<uint8_t> atype, ang_has, dih_has;
+  std::vector<int32_t> chain, seq, teth, ang, dih, cont, cont_type, tip_res;
+  double cell[3] = {0, 0, 0};
+  pcg_model view() const {
+    pcg_model m{};
+    m.n = n;
+    m.pos = pos.data();
+    m.orig_pos = orig.data();
+    m.atype = atype.data();
+    m.chain_idx = chain.data();
+    m.seq_idx = seq.data();
+    m.n_teth = (int)teth.size() / 2;
+    m.teth = teth.data();
+    m.n_ang = (int)ang.size() / 3;
+    m.ang = ang.data();
+    m.ang_has = ang_has.data();
+    m.ang_theta = ang_theta.data();
+    m.n_dih = (int)dih.size() / 4;
+    m.dih = dih.data();
+    m.dih_has = dih_has.data();
+    m.dih_phi = dih_phi.data();
+    m.n_cont = (int)cont.size() / 2;
+    m.cont = cont.data();
+    m.cont_len = cont_len.data();
+    m.cont_type = cont_type.data();
+    for (int c = 0; c < 3; ++c) m.cell[c] = cell[c];
+    m.n_tips = (int)tip_res.size();
+    m.tip_res = tip_res.data();
+    m.tip_orig = tip_orig.data();
+    m.tip_vel = tip_vel.data();
+    return m;
+  }
+};
+
+template <typename T> std::vector<T> cp(const T *p, size_t n) {
+  return p ? std::vector<T>(p, p + n) : std::vector<T>(n, T{});
+}
+
+OwnedModel own(const pcg_model &m) {
+  OwnedModel o;
+  o.n = m.n;
+  o.pos = cp(m.pos, 3 * (size_t)m.n);
+  o.orig = m.orig_pos ? cp(m.orig_pos, 3 * (size_t)m.n) : o.pos;
+  o.atype = cp(m.atype, m.n);
+  o.chain = cp(m.chain_idx, m.n);
+  o.seq = cp(m.seq_idx, m.n);
+  o.teth = cp(m.teth, 2 * (size_t)m.n_teth);
+  o.ang = cp(m.ang, 3 * (size_t)m.n_ang);
+  o.ang_has = cp(m.ang_has, m.n_ang);
+  o.ang_theta = cp(m.ang_theta, m.n_ang);
+  o.dih = cp(m.dih, 4 * (size_t)m.n_dih);
+  o.dih_has = cp(m.dih_has, m.n_dih);
+  o.dih_phi = cp(m.dih_phi, m.n_dih);
+  o.cont = cp(m.cont, 2 * (size_t)m.n_cont);
+  o.cont_len = cp(m.cont_len, m.n_cont);
+  o.cont_type = cp(m.cont_type, m.n_cont);
+  o.tip_res = cp(m.tip_res, m.n_tips);
+  o.tip_orig = cp(m.tip_orig, 3 * (size_t)m.n_tips);
+  o.tip_vel = cp(m.tip_vel, 3 * (size_t)m.n_tips);
+  for (int c = 0; c < 3; ++c) o.cell[c] = m.cell[c];
+  return o;
+}
+
+struct Job {
+  int64_t id = 0;
+  OwnedModel model;
+  double T = -1;
+  int32_t seed = 448;
+  int64_t nsteps = 0, done = 0;
+  int status = PCG_JOB_QUEUED;
+  size_t bytes = 0;
+  std::unique_ptr<DeviceBatch> B;
+  std::vector<double> result;
+  double V = 0;
+  double t_submit = 0, t_start = 0, t_end = 0;
+};
+
+} // namespace
+
+struct pcg_server {
+  pcg_params p{};
+  pcg_server_options o{};
+  hipStream_t stream = nullptr;
+  std::map<int64_t, std::unique_ptr<Job>> jobs;
+  std::deque<int64_t> queue;
+  std::vector<int64_t> resident;
+  int64_t next_id = 1;
+  size_t resident_bytes = 0;
+  DevBatch *d_jobs = nullptr;
+  int d_jobs_cap = 0;
+  ~pcg_server() {
+    jobs.clear();
+    if (d_jobs) (void)hipFree(d_jobs);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int retire(pcg_server *s, Job &j, int status) {
+  if (j.B) {
+    SysState st;
+    if (status == PCG_JOB_DONE) {
+      j.result.assign(3 * (size_t)j.model.n, 0.0);
+      int rc = get_cols(*j.B, 0, j.B->d.y[0], j.result.data());
+      if (rc) return rc;
+      if ((rc = read_state(*j.B, 0, st))) return rc;
+      j.V = st.Vtot;
+      j.done = st.step;
+    }
+    s->resident_bytes -= j.bytes;
+    j.B.reset();
+  }
+  j.status = status;
+  j.t_end = now_s();
+  return PCG_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pcg_server_create(const pcg_params *p, const pcg_server_options *o, pcg_server **out) {
+  if (!p || !out) return fail(PCG_E_ARG, "null argument");
+  auto s = std::make_unique<pcg_server>();
+  s->p = *p;
+  if (o) s->o = *o;
+  if (s->o.max_resident <= 0) s->o.max_resident = 1024;
+  if (s->o.chunk_steps <= 0) s->o.chunk_steps = 1000;
+  if (s->o.hbm_budget_bytes <= 0) s->o.hbm_budget_bytes = (int64_t)64 << 30;
+  HIPCHK(hipSetDevice(s->o.device));
+  HIPCHK(hipStreamCreate(&s->stream));
+  *out = s.release();
+  return PCG_OK;
+}
+
+int pcg_server_destroy(pcg_server *s) {
+  delete s;
+  return PCG_OK;
+}
+
+int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature, int32_t seed,
+                      int64_t nsteps, int64_t *job_id) {
+  if (!s || !m || nsteps < 0 || m->n <= 0) return fail(PCG_E_ARG, "bad arguments");
+  auto j = std::make_unique<Job>();
+  j->id = s->next_id++;
+  j->model = own(*m);
+  j->T = temperature;
+  j->seed = seed;
+  j->nsteps = nsteps;
+  j->t_submit = now_s();
+  // size check up front: a job that can never fit is refused
+  std::unique_ptr<DeviceBatch> probe;
+  pcg_model v = j->model.view();
+  size_t bytes = 0;
+  int rc = make_batch(s->p, 1, &v, &j->T, &j->seed, s->o.precision, 0, s->o.device, probe,
+                      s->stream, &bytes);
+  if (rc) return rc;
+  if ((int64_t)bytes > s->o.hbm_budget_bytes)
+    return fail(PCG_E_NOMEM, "job needs " + std::to_string(bytes) + " bytes, over the budget");
+  j->bytes = bytes;
+  if (job_id) *job_id = j->id;
+  s->queue.push_back(j->id);
+  s->jobs[j->id] = std::move(j);
+  return PCG_OK;
+}
+
+int pcg_server_round(pcg_server *s, int32_t *active) {
+  if (!s) return fail(PCG_E_ARG, "null server");
+  HIPCHK(hipSetDevice(s->o.device));
+  // admission: first fit in arrival order
+  for (auto it = s->queue.begin(); it != s->queue.end() && (int)s->resident.size() < s->o.max_resident;) {
+    Job &j = *s->jobs[*it];
+    if (j.status == PCG_JOB_CANCELLED) {
+      it = s->queue.erase(it);
+      continue;
+    }
+    if ((int64_t)(s->resident_bytes + j.bytes) > s->o.hbm_budget_bytes) {
+      ++it;
+      continue;
+    }
+    pcg_model v = j.model.view();
+    uint64_t key = mix64(s->o.philox_seed ^ mix64((uint64_t)(uint32_t)j.seed));
+    int rc = make_batch(s->p, 1, &v, &j.T, &j.seed, s->o.precision, key, s->o.device, j.B,
+                        s->stream);
+    if (rc == PCG_E_NOMEM) {
+      ++it;
+      continue;
+    }
+    if (rc) return rc;
+    j.status = PCG_JOB_RUNNING;
+    j.t_start = now_s();
+    s->resident_bytes += j.bytes;
+    s->resident.push_back(j.id);
+    it = s->queue.erase(it);
+  }
+  int nres = (int)s->resident.size();
+  if (nres > 0) {
+    // per-job step budget for this round
+    std::vector<DevBatch> descs(nres);
+    int max_n = 0;
+    for (int k = 0; k < nres; ++k) {
+      Job &j = *s->jobs[s->resident[k]];
+      DeviceBatch &B = *j.B;
+      B.sys[0].steps_target = std::min<int64_t>(j.done + s->o.chunk_steps, j.nsteps);
+      HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), sizeof(SysDesc), hipMemcpyHostToDevice,
+                            s->stream));
+      descs[k] = B.d;
+      max_n = std::max(max_n, B.max_n);
+    }
+    if (nres > s->d_jobs_cap) {
+      if (s->d_jobs) HIPCHK(hipFree(s->d_jobs));
+      s->d_jobs_cap = std::max(nres, 2 * s->d_jobs_cap);
+      HIPCHK(hipMalloc(&s->d_jobs, s->d_jobs_cap * sizeof(DevBatch)));
+    }
+    HIPCHK(hipMemcpyAsync(s->d_jobs, descs.data(), nres * sizeof(DevBatch), hipMemcpyHostToDevice,
+                          s->stream));
+    size_t fsz = s->o.precision == PCG_FP64 ? 8 : 4;
+    size_t lds = (size_t)max_n * (24 + 3 * fsz) + 64;
+    if (lds > 150 * 1024) return fail(PCG_E_ARG, "job too large for the persistent kernel");
+    int block = 64;
+    while (block < 512 && block < 2 * max_n) block *= 2;
+    block = std::max(block, 128);
+    launch_persistent_jobs(s->d_jobs, nres, s->o.precision, block, s->o.chunk_steps, lds,
+                           s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    // retire finished / failed jobs
+    std::vector<int64_t> keep;
+    for (int64_t id : s->resident) {
+      Job &j = *s->jobs[id];
+      SysState st;
+      int rc = read_state(*j.B, 0, st);
+      if (rc) return rc;
+      j.done = st.step;
+      bool bad = st.overflow || !std::isfinite(st.Vtot);
+      if (bad) {
+        if ((rc = retire(s, j, PCG_JOB_FAILED))) return rc;
+      } else if (j.done >= j.nsteps) {
+        if ((rc = retire(s, j, PCG_JOB_DONE))) return rc;
+      } else {
+        keep.push_back(id);
+      }
+    }
+    s->resident.swap(keep);
+  }
+  if (active) *active = (int)(s->resident.size() + s->queue.size());
+  return PCG_OK;
+}
+
+int pcg_server_status(pcg_server *s, int64_t job, int32_t *status, int64_t *steps_done) {
+  if (!s) return fail(PCG_E_ARG, "null server");
+  auto it = s->jobs.find(job);
+  if (it == s->jobs.end()) return fail(PCG_E_NOTFOUND, "no such job");
+  if (status) *status = it->second->status;
+  if (steps_done) *steps_done = it->second->done;
+  return PCG_OK;
+}
+
+int pcg_server_result(pcg_server *s, int64_t job, double *pos, double *V) {
+  if (!s) return fail(PCG_E_ARG, "null server");
+  auto it = s->jobs.find(job);
+  if (it == s->jobs.end()) return fail(PCG_E_NOTFOUND, "no such job");
+  Job &j = *it->second;
+  if (j.status != PCG_JOB_DONE) return fail(PCG_E_STATE, "job not finished");
+  if (pos) std::memcpy(pos, j.result.data(), j.result.size() * sizeof(double));
+  if (V) *V = j.V;
+  return PCG_OK;
+}
+
+int pcg_server_cancel(pcg_server *s, int64_t job) {
+  if (!s) return fail(PCG_E_ARG, "null server");
+  auto it = s->jobs.find(job);
+  if (it == s->jobs.end()) return fail(PCG_E_NOTFOUND, "no such job");
+  Job &j = *it->second;
+  if (j.status == PCG_JOB_QUEUED) {
+    j.status = PCG_JOB_CANCELLED;
+    j.t_end = now_s();
+  } else if (j.status == PCG_JOB_RUNNING) {
+    int rc = retire(s, j, PCG_JOB_CANCELLED);
+    if (rc) return rc;
+    s->resident.erase(std::remove(s->resident.begin(), s->resident.end(), job), s->resident.end());
+  }
+  return PCG_OK;
+}
+
+int pcg_server_stats(pcg_server *s, int64_t *resident_bytes, int32_t *resident, int32_t *queued) {
+  if (!s) return fail(PCG_E_ARG, "null server");
+  if (resident_bytes) *resident_bytes = (int64_t)s->resident_bytes;
+  if (resident) *resident = (int)s->resident.size();
+  if (queued) *queued = (int)s->queue.size();
+  return PCG_OK;
+}
+
+int pcg_server_job_times(pcg_server *s, int64_t job, double *submit, double *start, double *end) {
+  if (!s) return fail(PCG_E_ARG, "null server");
+  auto it = s->jobs.find(job);
+  if (it == s->jobs.end()) return fail(PCG_E_NOTFOUND, "no such job");
+  if (submit) *submit = it->second->t_submit;
+  if (start) *start = it->second->t_start;
+  if (end) *end = it->second->t_end;
+  return PCG_OK;
+}
+
+} // extern "C"

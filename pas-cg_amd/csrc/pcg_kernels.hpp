@@ -1,0 +1,150 @@
+// pcg_kernels.hpp -- grid-wide kernels of the per-step loop over a batch:
+// one launch per stage, items of every system in one grid (blockIdx.y is the
+// system). Used by the plain drop-in context (a batch of one) and for systems
+// too large for the persistent workgroup-per-system kernel.
+#pragma once
+#include "pcg_terms.hpp"
+
+namespace pcg {
+
+constexpr int BLK = 256;
+
+struct GeoG {
+  const double *x, *y, *z;
+  __device__ __forceinline__ v3<double> r(int i) const { return v3<double>{x[i], y[i], z[i]}; }
+};
+
+template <typename RF> struct AccG {
+  RF *fx, *fy, *fz;
+  template <typename T> __device__ __forceinline__ void add(int i, v3<T> f) {
+    atomicAdd(fx + i, (RF)f.x);
+    atomicAdd(fy + i, (RF)f.y);
+    atomicAdd(fz + i, (RF)f.z);
+  }
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Sum of v over the block, valid in thread 0.
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double part[BLK / 64];
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += part[k];
+  return s;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based.
+__device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+// three standard normals for (system key, step, residue): Box-Muller
+__device__ __forceinline__ void philox_normals(uint64_t key, int64_t step, int res, double out[3]) {
+  uint32_t c[4] = {(uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)res, 0x5043u};
+  philox4x32(c, (uint32_t)key, (uint32_t)(key >> 32));
+  const double inv = 2.3283064365386963e-10; // 2^-32
+  double u1 = ((double)c[0] + 1.0) * inv, u2 = (double)c[1] * inv;
+  double u3 = ((double)c[2] + 1.0) * inv, u4 = (double)c[3] * inv;
+  double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+  double s1, c1, s2, c2;
+  sincos(2.0 * M_PI * u2, &s1, &c1);
+  sincos(2.0 * M_PI * u4, &s2, &c2);
+  out[0] = r1 * c1;
+  out[1] = r1 * s1;
+  out[2] = r2 * c2;
+  (void)s2;
+}
+
+// ------------------------------------------------------------ per term ----
+__device__ __forceinline__ int term_size(const SysDesc &s, const SysState &st, int term) {
+  switch (term) {
+  case PCG_CHIR: return s.nchir;
+  case PCG_TETHER: return s.nteth;
+  case PCG_LREP: return s.nlrep;
+  case PCG_NAT_CONT: return st.nc_n;
+  case PCG_PAULI: return st.pauli_n;
+  case PCG_CONST_DH: return st.dh_n;
+  case PCG_REL_DH: return st.dh_n;
+  case PCG_QA_CAND: return st.qf_n;
+  case PCG_QA_CONT: return st.qc_n < st.qf_n ? st.qc_n : st.qf_n;
+  case PCG_PID: return st.pid_n;
+  case PCG_NAT_ANG: return s.nnang;
+  case PCG_HEUR_ANG: return s.nhang;
+  case PCG_HEUR_DIH: return s.nhdih;
+  case PCG_CND: return s.nndih;
+  case PCG_SND: return s.nndih;
+  case PCG_VEL_AFM: return s.ntip;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ bool term_on(const DevParams &P, int term) {
+  const pcg_params &p = P.p;
+  switch (term) {
+  case PCG_CHIR: return p.chir_on;
+  case PCG_TETHER: return p.tether_on;
+  case PCG_LREP: return p.lrep_on;
+  case PCG_NAT_CONT: return p.nat_cont_on;
+  case PCG_PAULI: return P.pauli_on;
+  case PCG_CONST_DH: return p.dh_on && p.dh_variant == 0;
+  case PCG_REL_DH: return p.dh_on && p.dh_variant == 1;
+  case PCG_QA_CAND: return p.qa_on;
+  case PCG_QA_CONT: return p.qa_on;
+  case PCG_PID: return p.pid_on;
+  case PCG_NAT_ANG: return p.nat_ang_on;
+  case PCG_HEUR_ANG: return p.heur_ang_on;
+  case PCG_HEUR_DIH: return p.heur_dih_on;
+  case PCG_CND: return p.nat_dih_on && p.nat_dih_variant == 0;
+  case PCG_SND: return p.nat_dih_on && p.nat_dih_variant == 1;
+  case PCG_VEL_AFM: return p.afm_on;
+  }
+  return false;
+}
+
+// One item k (local to the system's list) of one term. Pair terms run in R;
+// the angular terms, tethers, local repulsion, chirality, PID and the AFM
+// tips in FP64 (acos near 0/pi and the heuristic-angle polynomial lose too
+// much in FP32).
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double eval_item(const DevBatch &b, const SysDesc &s, SysState &st,
+                                            const G &g, A &acc, int term, int k) {
+  switch (term) {
+  case PCG_CHIR: return chir_item<double>(b, g, acc, s.chir0 + k);
+  case PCG_TETHER: return tether_item<double>(b, g, acc, s.teth0 + k);
+  case PCG_LREP: return lrep_item<double>(b, g, acc, s.lrep0 + k);
+  case PCG_NAT_CONT: return nat_cont_item<R>(b, s, st, g, acc, s.nc0 + k, st.t);
+  case PCG_PAULI: return pauli_item<R>(b, s, g, acc, s.nl0 + k);
+  case PCG_CONST_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 0);
+  case PCG_REL_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 1);
+  case PCG_QA_CAND: return qa_cand_item<R>(b, s, st, g, acc, k);
+  case PCG_QA_CONT: return qa_cont_item<R>(b, s, st, g, acc, k);
+  case PCG_PID: return pid_item<double>(b, s, g, acc, s.nl0 + k);
+  case PCG_NAT_ANG: return nat_ang_item<double>(b, g, acc, s.nang0 + k);
+  case PCG_HEUR_ANG: return heur_ang_item<double>(b, g, acc, s.hang0 + k);
+  case PCG_HEUR_DIH: return heur_dih_item<double>(b, g, acc, s.hdih0 + k);
+  case PCG_CND: return nat_dih_item<double>(b, g, acc, s.ndih0 + k, 0);
+  case PCG_SND: return nat_dih_item<double>(b, g, acc, s.ndih0 + k, 1);
+  case PCG_VEL_AFM: return vel_afm_item<double>(b, g, acc, s.tip0 + k, st.t);
+  }
+  return 0.0;
+}
+
+} // namespace pcg

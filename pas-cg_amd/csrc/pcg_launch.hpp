@@ -1,0 +1,21 @@
+// pcg_launch.hpp -- launchers of the grid-wide stage kernels (pcg_kernels.hip)
+// and of the persistent batched kernel (pcg_batch.hip).
+#pragma once
+#include "pcg_kernels.hpp"
+
+namespace pcg {
+void launch_begin(const DevBatch &b, hipStream_t s);
+void launch_zero_forces(const DevBatch &b, int prec, hipStream_t s);
+void launch_prepare_nh(const DevBatch &b, hipStream_t s);
+void launch_fix_nl(const DevBatch &b, int max_nlcap, hipStream_t s);
+void launch_terms(const DevBatch &b, int prec, int max_items, int only_term, int from, int to,
+                  hipStream_t s);
+void launch_qa_finish(const DevBatch &b, hipStream_t s);
+void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s);
+void launch_tick(const DevBatch &b, int advance, hipStream_t s);
+// persistent form: every system of the batch advances nsteps in one launch
+void launch_persistent(const DevBatch &b, int prec, int block, int nsteps, size_t lds_bytes,
+                       hipStream_t s);
+void launch_persistent_jobs(const DevBatch *jobs, int njobs, int prec, int block, int nsteps,
+                            size_t lds_bytes, hipStream_t s);
+} // namespace pcg

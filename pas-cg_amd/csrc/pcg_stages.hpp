@@ -1,0 +1,210 @@
+// pcg_stages.hpp -- device stages shared by the grid-wide kernels and the
+// persistent workgroup-per-system kernel: NL pair classification for the
+// ordered list updates, the QA finish, and the Gear-5 update.
+#pragma once
+#include "pcg_kernels.hpp"
+
+namespace pcg {
+
+enum { CL_NC = 1, CL_QA = 2, CL_PID = 4, CL_DH = 8, CL_PAULI = 16 };
+
+__device__ inline int find_contact(const DevBatch &b, const SysDesc &s, int i1, int i2) {
+  int lo = s.nc0, hi = s.nc0 + s.nnc;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    int a = b.nc_i[0][mid], c = b.nc_i[1][mid];
+    if (a < i1 || (a == i1 && c < i2)) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < s.nc0 + s.nnc && b.nc_i[0][lo] == i1 && b.nc_i[1][lo] == i2) return lo;
+  return -1;
+}
+
+__device__ inline int classify(const DevBatch &b, const SysDesc &s, const SysState &st,
+                                        int slot) {
+  const DevParams &P = *b.P;
+  const pcg_params &p = P.p;
+  int i1 = b.nl_i[0][slot], i2 = b.nl_i[1][slot];
+  double d = b.nl_d[slot];
+  double pad = st.orig_pad;
+  bool four = b.chain[i1] == b.chain[i2] && abs(b.seq[i1] - b.seq[i2]) == 4;
+  if (p.nat_cont_on && s.nnc > 0 && d < __dadd_rn(p.nl_cutoff, pad) && find_contact(b, s, i1, i2) >= 0)
+    return CL_NC;
+  if (p.qa_on && !(!p.qa_include4 && four) && d < __dadd_rn(p.nl_cutoff, pad)) return CL_QA;
+  int cls = 0;
+  if (p.pid_on && !(!p.pid_include4 && four) && d < __dadd_rn(p.nl_cutoff, pad)) cls |= CL_PID;
+  if (p.dh_on) {
+    double q = p.charge[b.atype[i1]] * p.charge[b.atype[i2]];
+    if (q != 0.0 && __ddiv_rn(1.0, d) > __ddiv_rn(1.0, __dadd_rn(p.dh_cutoff, pad))) cls |= CL_DH;
+  }
+  if (P.pauli_on && d < __dadd_rn(p.rep_cutoff, pad)) cls |= CL_PAULI;
+  return cls;
+}
+
+// qa::finish_processing (cg/src/cg/qa/finish_processing.cpp:5-74), one block
+// per system: removals in contact order, candidates sorted by (distance,
+// free-pair index), sequential acceptance against the sync budgets, and
+// vect::set::compress, which moves items but not vacancy flags
+// (cg/include/cg/vect/ind/set.h:62-71).
+__device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, SysState &st) {
+  int rem_n = st.rem_n < sd.qccap ? st.rem_n : sd.qccap;
+  int cand_n = st.cand_n < sd.nlcap ? st.cand_n : sd.nlcap;
+  // rank of every candidate by (dist, free-pair index) -> permutation
+  for (int c = threadIdx.x; c < cand_n; c += blockDim.x) {
+    int sc = sd.nl0 + c;
+    double dc = b.cand_dist[sc];
+    int ic = b.cand_idx[sc];
+    int rank = 0;
+    for (int o = 0; o < cand_n; ++o) {
+      int so = sd.nl0 + o;
+      double dd = b.cand_dist[so];
+      rank += (dd < dc || (dd == dc && b.cand_idx[so] < ic)) ? 1 : 0;
+    }
+    b.cand_perm[sd.nl0 + rank] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // removed contacts in increasing index order
+    int *rem = b.rem + sd.qc0;
+    for (int a = 1; a < rem_n; ++a) {
+      int v = rem[a], j = a - 1;
+      while (j >= 0 && rem[j] > v) {
+        rem[j + 1] = rem[j];
+        --j;
+      }
+      rem[j + 1] = v;
+    }
+    for (int a = 0; a < rem_n; ++a) {
+      int q = sd.qc0 + rem[a];
+      int i1 = b.qc_i[0][q], i2 = b.qc_i[1][q];
+      for (int c = 0; c < 4; ++c) {
+        b.sync[4 * i1 + c] = (int8_t)(b.sync[4 * i1 + c] + b.qc_sd[8 * q + c]);
+        b.sync[4 * i2 + c] = (int8_t)(b.sync[4 * i2 + c] + b.qc_sd[8 * q + 4 + c]);
+      }
+      if (st.qf_n < sd.qfcap) {
+        int f = sd.qf0 + st.qf_n;
+        b.qf_i[0][f] = i1;
+        b.qf_i[1][f] = i2;
+        b.qf_d[f] = b.qc_d[q];
+        b.qf_vac[f] = 0;
+        st.qf_n += 1;
+      } else {
+        st.overflow = 1;
+      }
+      b.qc_vac[q] = 1;
+      st.num_qa -= 1;
+    }
+    // sequential acceptance
+    for (int a = 0; a < cand_n; ++a) {
+      int sc = sd.nl0 + b.cand_perm[sd.nl0 + a];
+      int i1 = b.cand_i[0][sc], i2 = b.cand_i[1][sc];
+      bool ok = true;
+      int8_t n1[4], n2[4];
+      for (int c = 0; c < 4; ++c) {
+        n1[c] = (int8_t)(b.sync[4 * i1 + c] - b.cand_sd[8 * sc + c]);
+        n2[c] = (int8_t)(b.sync[4 * i2 + c] - b.cand_sd[8 * sc + 4 + c]);
+      }
+      for (int c = 0; c < 4; ++c) ok = ok && n1[c] >= 0;
+      if (!ok) continue;
+      for (int c = 0; c < 4; ++c) ok = ok && n2[c] >= 0;
+      if (!ok) continue;
+      if (st.qc_n >= sd.qccap) {
+        st.overflow = 1;
+        continue;
+      }
+      int q = sd.qc0 + st.qc_n;
+      b.qc_i[0][q] = i1;
+      b.qc_i[1][q] = i2;
+      b.qc_d[q] = b.cand_od[sc];
+      b.qc_t[q] = b.cand_t[sc];
+      b.qc_sat[q] = 0.0;
+      for (int c = 0; c < 8; ++c) b.qc_sd[8 * q + c] = b.cand_sd[8 * sc + c];
+      b.qc_vac[q] = 0;
+      st.qc_n += 1;
+      st.num_qa += 1;
+      for (int c = 0; c < 4; ++c) {
+        b.sync[4 * i1 + c] = n1[c];
+        b.sync[4 * i2 + c] = n2[c];
+      }
+      b.qf_vac[sd.qf0 + b.cand_idx[sc]] = 1;
+    }
+    // compress
+    if (2 * st.num_qa < st.qc_n) {
+      int slot = 0;
+      for (int cur = 0; cur < st.qc_n; ++cur) {
+        int qc = sd.qc0 + cur;
+        if (b.qc_vac[qc]) continue;
+        int qs = sd.qc0 + slot;
+        if (qs != qc) {
+          b.qc_i[0][qs] = b.qc_i[0][qc];
+          b.qc_i[1][qs] = b.qc_i[1][qc];
+          b.qc_d[qs] = b.qc_d[qc];
+          b.qc_t[qs] = b.qc_t[qc];
+          b.qc_sat[qs] = b.qc_sat[qc];
+          for (int c = 0; c < 8; ++c) b.qc_sd[8 * qs + c] = b.qc_sd[8 * qc + c];
+          // the vacancy flag of the destination slot stays as it was
+        }
+        ++slot;
+      }
+      st.qc_n = slot;
+    }
+    st.cand_n = 0;
+    st.rem_n = 0;
+  }
+  __syncthreads();
+}
+
+// -------------------------------------------------------------- Langevin ----
+// lang::step::omp_async (cg/src/cg/langevin/step.cpp:23-77): FP64 Gear-5
+// predictor-corrector with the reference's float-computed coefficients
+// (cg/include/cg/langevin/step.h:10-12). The displacement from the last NL
+// rebuild is reduced here for the next step's verify.
+__device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd, const SysState &st,
+                                            int i, double Fx, double Fy, double Fz,
+                                            const double nz[3], double out_r[3]) {
+  const DevParams &P = *b.P;
+  const double f02 = (double)(3.0f / 16.0f), f12 = (double)(251.0f / 360.0f),
+               f32 = (double)(11.0f / 18.0f), f42 = (double)(1.0f / 6.0f),
+               f52 = (double)(1.0f / 60.0f);
+  int aa = b.atype[i];
+  double minv = 1.0 / P.p.mass[aa];
+  double gam2 = P.gamma2 * minv, con2 = sd.const2 * sqrt(minv);
+  double F[3] = {Fx, Fy, Fz};
+  for (int c = 0; c < 3; ++c) {
+    double y0 = b.y[0][c][i], y1 = b.y[1][c][i], y2 = b.y[2][c][i], y3 = b.y[3][c][i],
+           y4 = b.y[4][c][i], y5 = b.y[5][c][i];
+    if (st.t == 0) y2 = F[c] * P.deltsq;
+    double f = F[c] - gam2 * y1;
+    y1 = y1 + con2 * nz[c];
+    double err = y2 - P.deltsq * f;
+    y0 -= f02 * err;
+    y1 -= f12 * err;
+    y2 -= 1.0 * err;
+    y3 -= f32 * err;
+    y4 -= f42 * err;
+    y5 -= f52 * err;
+    y0 += y1 + y2 + y3 + y4 + y5;
+    y1 += 2.0 * y2 + 3.0 * y3 + 4.0 * y4 + 5.0 * y5;
+    y2 += 3.0 * y3 + 6.0 * y4 + 10.0 * y5;
+    y3 += 4.0 * y4 + 10.0 * y5;
+    y4 += 5.0 * y5;
+    b.y[0][c][i] = y0;
+    b.y[1][c][i] = y1;
+    b.y[2][c][i] = y2;
+    b.y[3][c][i] = y3;
+    b.y[4][c][i] = y4;
+    b.y[5][c][i] = y5;
+    b.vel[c][i] = y1 * P.dt_inv;
+    out_r[c] = y0;
+  }
+}
+
+__device__ __forceinline__ double disp_from_orig(const DevBatch &b, const SysDesc &sd, int i,
+                                                 const double r[3]) {
+  v3<double> d{__dsub_rn(b.orig_r[0][i], r[0]), __dsub_rn(b.orig_r[1][i], r[1]),
+               __dsub_rn(b.orig_r[2][i], r[2])};
+  return xnorm3(xwrapd(d, sd));
+}
+
+
+} // namespace pcg

@@ -1,0 +1,562 @@
+// pcg_terms.hpp -- per-item physics of every force term, as device functions.
+//
+// Each function evaluates one list item of one term: it reads positions
+// through a geometry accessor G (global arrays for the per-term kernels, LDS
+// for the persistent batched kernel), pushes forces through an accumulator A
+// (global or LDS atomics) and returns the item's energy in FP64. R is the
+// force arithmetic type (float in the mixed mode, double in FP64 mode);
+// coordinate differences are formed in FP64 and then rounded to R.
+// Decisions that must agree with the CPU path bit for bit (cutoffs of the
+// contact-state transitions, QA criteria, saturation) use the exact FP64
+// helpers of pcg_core.hpp.
+#pragma once
+#include "pcg_core.hpp"
+
+namespace pcg {
+
+template <typename R> __device__ __forceinline__ v3<R> cast3(v3<double> d) {
+  return v3<R>{(R)d.x, (R)d.y, (R)d.z};
+}
+
+__device__ __forceinline__ v3<double> wrapd(v3<double> d, const SysDesc &s) {
+  d.x -= s.cell[0] * round(d.x * s.cell_inv[0]);
+  d.y -= s.cell[1] * round(d.y * s.cell_inv[1]);
+  d.z -= s.cell[2] * round(d.z * s.cell_inv[2]);
+  return d;
+}
+__device__ __forceinline__ v3<double> xwrapd(v3<double> d, const SysDesc &s) {
+  return v3<double>{xwrap1(d.x, s.cell[0], s.cell_inv[0]), xwrap1(d.y, s.cell[1], s.cell_inv[1]),
+                    xwrap1(d.z, s.cell[2], s.cell_inv[2])};
+}
+__device__ __forceinline__ v3<double> xsub(v3<double> a, v3<double> b) {
+  return v3<double>{__dsub_rn(a.x, b.x), __dsub_rn(a.y, b.y), __dsub_rn(a.z, b.z)};
+}
+__device__ __forceinline__ double xnorm3(v3<double> d) { return xnorm(d.x, d.y, d.z); }
+
+// Base potentials (cg/include/cg/base_forces/{harmonic,lj,shifted_lj,sink_lj}.h)
+template <typename R> __device__ __forceinline__ void harmonicR(R H1, R H2, R nat, R r, R &V, R &dV) {
+  R dr = r - nat, dr2 = dr * dr;
+  V = (R)0.5 * dr2 * (H1 + H2 * dr2);
+  dV = clampR(dr * (H1 + (R)2 * H2 * dr2), (R)-1e3, (R)1e3);
+}
+template <typename R> __device__ __forceinline__ void ljR(R depth, R r_min, R r_inv, R &V, R &dV) {
+  R s = r_inv * r_min, s6 = ipow6(s), s12 = s6 * s6;
+  V = depth * (s12 - (R)2 * s6);
+  dV = (R)12 * depth * r_inv * (s6 - s12);
+}
+template <typename R> __device__ __forceinline__ void shifted_ljR(R depth, R r_min, R r_inv, R &V, R &dV) {
+  R s = r_inv * r_min, s6 = ipow6(s), s12 = s6 * s6;
+  V = depth * (s12 - (R)2 * s6 + (R)1);
+  dV = clampR((R)12 * depth * r_inv * (s6 - s12), (R)-1e3, (R)1e3);
+}
+template <typename R> __device__ __forceinline__ void sink_ljR(R depth, R r_low, R r_high, R r, R r_inv, R &V, R &dV) {
+  R r_eff = r < r_low ? r_low : (r < r_high ? r : r_high);
+  R s = r_inv * r_eff, s6 = ipow6(s), s12 = s6 * s6;
+  V = depth * (s12 - (R)2 * s6);
+  dV = (R)12 * depth * r_inv * (s6 - s12);
+}
+
+// ---------------------------------------------------------------- terms ----
+
+// tether::eval_forces::iter (cg/src/cg/tether/eval_forces.cpp:18-33)
+template <typename R, typename G, typename A>
+__device__ double tether_item(const DevBatch &b, const G &g, A &acc, int k) {
+  const pcg_params &p = b.P->p;
+  int i1 = b.teth_i[0][k], i2 = b.teth_i[1][k];
+  v3<R> r12 = cast3<R>(g.r(i2) - g.r(i1));
+  R rn = sq(nsq(r12)), V, dV;
+  harmonicR<R>((R)p.tether_H1, (R)p.tether_H2, (R)b.teth_nat[k], rn, V, dV);
+  v3<R> f = (dV / rn) * r12;
+  acc.add(i1, f);
+  acc.add(i2, -f);
+  return (double)V;
+}
+
+// local_rep::eval_forces::iter (cg/src/cg/local_rep/eval_forces.cpp:11-28)
+template <typename R, typename G, typename A>
+__device__ double lrep_item(const DevBatch &b, const G &g, A &acc, int k) {
+  const pcg_params &p = b.P->p;
+  int i1 = b.lrep_i[0][k], i2 = b.lrep_i[1][k];
+  v3<R> r12 = cast3<R>(g.r(i2) - g.r(i1));
+  R rn = rsq(nsq(r12)), cut = (R)p.rep_cutoff;
+  if ((R)1 > rn * cut) return 0.0;
+  R V, dV;
+  shifted_ljR<R>((R)p.lrep_depth, cut, rn, V, dV);
+  v3<R> f = (dV * rn) * r12;
+  acc.add(i1, f);
+  acc.add(i2, -f);
+  return (double)V;
+}
+
+// pauli::eval_forces::iter (cg/src/cg/pauli/eval_forces.cpp:12-30)
+template <typename R, typename G, typename A>
+__device__ double pauli_item(const DevBatch &b, const SysDesc &s, const G &g, A &acc, int k) {
+  const pcg_params &p = b.P->p;
+  int i1 = b.pauli_i[0][k], i2 = b.pauli_i[1][k];
+  v3<R> r12 = cast3<R>(wrapd(g.r(i2) - g.r(i1), s));
+  R rn = rsq(nsq(r12)), cut = (R)p.rep_cutoff;
+  if ((R)1 > rn * cut) return 0.0;
+  R V, dV;
+  shifted_ljR<R>((R)p.pauli_depth, cut, rn, V, dV);
+  v3<R> f = (dV * rn) * r12;
+  acc.add(i1, f);
+  acc.add(i2, -f);
+  return (double)V;
+}
+
+// nat_cont::eval_forces::iter (cg/src/cg/nat_cont/eval_forces.cpp:11-50).
+// k indexes the current list, which holds indices of the native contacts;
+// active/change_t live once, in the list of all contacts.
+template <typename R, typename G, typename A>
+__device__ double nat_cont_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
+                                A &acc, int k, double t) {
+  const pcg_params &p = b.P->p;
+  int c = b.ncur[k];
+  int i1 = b.nc_i[0][c], i2 = b.nc_i[1][c];
+  double nat = b.nc_nat[c];
+  v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
+  double rnx = xnorm3(d);
+  if (rnx >= p.nl_cutoff) return 0.0;
+  v3<R> r12 = cast3<R>(d);
+  R rn = (R)rnx, V, dV;
+  if (p.nc_ss_kind != 0 && b.nc_type[c] == 5) {
+    // disulfide_force::operator() receives the distance as its inverse-distance
+    // argument (nat_cont/eval_forces.cpp:27, disulfide.h:12-20)
+    if (p.nc_ss_kind == 1)
+      harmonicR<R>((R)p.nc_ss_H1, (R)p.nc_ss_H2, (R)p.nc_ss_nat_r, (R)1 / rn, V, dV);
+    else
+      ljR<R>((R)p.nc_ss_depth, (R)p.nc_ss_r_min, rn, V, dV);
+  } else {
+    ljR<R>((R)p.nat_cont_depth, (R)nat, (R)1 / rn, V, dV);
+  }
+  dV = clampR(dV, (R)-1e3, (R)1e3);
+  v3<R> f = (dV / rn) * r12;
+  acc.add(i1, f);
+  acc.add(i2, -f);
+  // state transition, exact FP64
+  bool now = rnx <= __dmul_rn(__dmul_rn(nat, C216_INV), p.counting_factor);
+  if ((uint8_t)now != b.nc_active[c]) {
+    b.nc_active[c] = (uint8_t)now;
+    if (b.nc_change_t[c] < 0) {
+      b.nc_change_t[c] = t;
+      atomicAdd(&st.num_changed, 1);
+    }
+  }
+  return (double)V;
+}
+
+// const_dh / rel_dh ::iter (cg/src/cg/dh/const.cpp:18-40, rel.cpp:17-40)
+template <typename R, typename G, typename A>
+__device__ double dh_item(const DevBatch &b, const SysDesc &s, const G &g, A &acc, int k, int variant) {
+  const pcg_params &p = b.P->p;
+  int i1 = b.dh_i[0][k], i2 = b.dh_i[1][k];
+  v3<double> d = wrapd(g.r(i2) - g.r(i1), s);
+  double rnd = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+  if (rnd > p.dh_cutoff) return 0.0;
+  v3<R> r12 = cast3<R>(d);
+  R rn = (R)rnd, rinv = (R)1 / rn, sinv = (R)b.P->screen_inv;
+  R Vij, dV;
+  if (variant == 0) {
+    Vij = (R)b.P->V_factor * (R)b.dh_q[k] * expR(-rn * sinv) * rinv;
+    dV = -Vij * (sinv + rinv);
+  } else {
+    Vij = (R)b.P->V_factor * (R)b.dh_q[k] * expR(-rn * sinv) * rinv * rinv;
+    dV = -Vij * (sinv + (R)2 * rinv);
+  }
+  v3<R> f = (dV * rinv) * r12;
+  acc.add(i1, f);
+  acc.add(i2, -f);
+  return (double)Vij;
+}
+
+// chir::eval_forces::iter (cg/src/cg/chir/eval_forces.cpp:10-31); f0/f1 are
+// the stored nat_factor/nat_chir fields of the quad.
+template <typename R, typename G, typename A>
+__device__ double chir_item(const DevBatch &b, const G &g, A &acc, int k) {
+  const pcg_params &p = b.P->p;
+  int i1 = b.chir_i[0][k], i2 = b.chir_i[1][k], i3 = b.chir_i[2][k], i4 = b.chir_i[3][k];
+  v3<double> r1 = g.r(i1), r2 = g.r(i2), r3 = g.r(i3), r4 = g.r(i4);
+  v3<R> r12 = cast3<R>(r2 - r1), r23 = cast3<R>(r3 - r2), r34 = cast3<R>(r4 - r3);
+  v3<R> x12_23 = cross(r12, r23), x12_34 = cross(r12, r34), x23_34 = cross(r23, r34);
+  R f0 = (R)b.chir_f0[k], f1 = (R)b.chir_f1[k], e = (R)p.e_chi;
+  R cd = dot(r12, x23_34) * f0 - f1;
+  R f = e * cd * f0;
+  acc.add(i1, f * x12_23);
+  acc.add(i2, -(f * (x12_34 + x23_34)));
+  acc.add(i3, f * (x12_23 + x12_34));
+  acc.add(i4, -(f * x12_23));
+  return (double)((R)0.5 * e * cd * cd);
+}
+
+// angle geometry of nat_ang / heur_ang (angles/*/eval_forces.cpp:10-38)
+template <typename R, typename G>
+__device__ __forceinline__ R angle_geom(const G &g, int i1, int i2, int i3, v3<R> &d1, v3<R> &d2, v3<R> &d3) {
+  v3<R> r12 = cast3<R>(g.r(i2) - g.r(i1)), r23 = cast3<R>(g.r(i3) - g.r(i2));
+  v3<R> x = cross(r12, r23);
+  R r12_rn = rsq(nsq(r12)), r23_rn = rsq(nsq(r23));
+  v3<R> c1 = cross(r12, x), c3 = cross(r23, x);
+  d1 = (rsq(nsq(c1)) * r12_rn) * c1;
+  d3 = (rsq(nsq(c3)) * r23_rn) * c3;
+  d2 = -d1 - d3;
+  R cos_t = clampR(-dot(r12, r23) * r12_rn * r23_rn, (R)-1, (R)1);
+  if (cos_t == (R)0) d1 = d2 = d3 = v3<R>{0, 0, 0};
+  return acosc(cos_t);
+}
+
+template <typename R, typename G, typename A>
+__device__ double nat_ang_item(const DevBatch &b, const G &g, A &acc, int k) {
+  int i1 = b.nang_i[0][k], i2 = b.nang_i[1][k], i3 = b.nang_i[2][k];
+  v3<R> d1, d2, d3;
+  R th = angle_geom<R>(g, i1, i2, i3, d1, d2, d3);
+  R CBA = (R)b.P->p.CBA, dth = th - (R)b.nang_th[k], dV = (R)2 * CBA * dth;
+  acc.add(i1, -(dV * d1));
+  acc.add(i2, -(dV * d2));
+  acc.add(i3, -(dV * d3));
+  return (double)(CBA * dth * dth);
+}
+
+template <typename R, typename G, typename A>
+__device__ double heur_ang_item(const DevBatch &b, const G &g, A &acc, int k) {
+  int i1 = b.hang_i[0][k], i2 = b.hang_i[1][k], i3 = b.hang_i[2][k], ty = b.hang_t[k];
+  v3<R> d1, d2, d3;
+  R th = angle_geom<R>(g, i1, i2, i3, d1, d2, d3);
+  const double(*c)[9] = b.P->p.heur_ang_poly;
+  // the coefficients (~1e5) cancel to O(1): evaluated as the reference writes
+  // it, in the precision R (callers use FP64)
+  R t2 = th * th, t3 = t2 * th, t4 = t3 * th, t5 = t4 * th, t6 = t3 * t3;
+  R V = (R)c[0][ty] + (R)c[1][ty] * th + (R)c[2][ty] * t2 + (R)c[3][ty] * t3 +
+        (R)c[4][ty] * t4 + (R)c[5][ty] * t5 + (R)c[6][ty] * t6;
+  R dV = (R)c[1][ty] + (R)(2.0 * c[2][ty]) * th + (R)(3.0 * c[3][ty]) * t2 +
+         (R)(4.0 * c[4][ty]) * t3 + (R)(5.0 * c[5][ty]) * t4 + (R)(6.0 * c[6][ty]) * t5;
+  acc.add(i1, -(dV * d1));
+  acc.add(i2, -(dV * d2));
+  acc.add(i3, -(dV * d3));
+  return (double)V;
+}
+
+// cnd / snd ::iter (angles/nat_dih/complex.cpp:10-47, simple.cpp:10-42)
+template <typename R, typename G, typename A>
+__device__ double nat_dih_item(const DevBatch &b, const G &g, A &acc, int k, int variant) {
+  const pcg_params &p = b.P->p;
+  int i1 = b.ndih_i[0][k], i2 = b.ndih_i[1][k], i3 = b.ndih_i[2][k], i4 = b.ndih_i[3][k];
+  v3<double> q1 = g.r(i1), q2 = g.r(i2), q3 = g.r(i3), q4 = g.r(i4);
+  v3<R> r12 = cast3<R>(q2 - q1), r23 = cast3<R>(q3 - q2), r34 = cast3<R>(q4 - q3);
+  v3<R> x1 = cross(r12, r23), x2 = cross(r23, r34);
+  R x1rn = rsq(nsq(x1)), x2rn = rsq(nsq(x2));
+  v3<R> x1u = x1 * x1rn, x2u = x2 * x2rn;
+  R phi = acosc(dot(x1u, x2u));
+  if (dot(x1, r34) < (R)0) phi = -phi;
+  R diff = phi - (R)b.ndih_ph[k], V, dVdphi;
+  if (variant == 0) {
+    R CDA = (R)p.CDA, CDB = (R)p.CDB;
+    V = CDA * ((R)1 - cosR(diff)) + CDB * ((R)1 - cosR((R)3 * diff));
+    dVdphi = CDA * sinR(diff) + CDB * (R)3 * sinR((R)3 * diff);
+  } else {
+    V = (R)0.5 * (R)p.CDH * diff * diff;
+    dVdphi = (R)p.CDH * diff;
+  }
+  R r23n = sq(nsq(r23));
+  v3<R> dr1 = -(x1u * (r23n * x1rn)), dr4 = x2u * (r23n * x2rn);
+  v3<R> df = (-dr1 * dot(r12, r23) + dr4 * dot(r23, r34)) * ((R)1 / (r23n * r23n));
+  v3<R> dr2 = -dr1 + df, dr3 = -dr4 - df;
+  acc.add(i1, -(dVdphi * dr1));
+  acc.add(i2, -(dVdphi * dr2));
+  acc.add(i3, -(dVdphi * dr3));
+  acc.add(i4, -(dVdphi * dr4));
+  return (double)V;
+}
+
+// heur_dih::eval_forces::iter (angles/heur_dih/eval_forces.cpp:10-55)
+template <typename R, typename G, typename A>
+__device__ double heur_dih_item(const DevBatch &b, const G &g, A &acc, int k) {
+  int i1 = b.hdih_i[0][k], i2 = b.hdih_i[1][k], i3 = b.hdih_i[2][k], i4 = b.hdih_i[3][k];
+  int ty = b.hdih_t[k];
+  const double(*c)[9] = b.P->p.heur_dih_coeffs;
+  v3<double> q1 = g.r(i1), q2 = g.r(i2), q3 = g.r(i3), q4 = g.r(i4);
+  v3<R> r12 = cast3<R>(q2 - q1), r23 = cast3<R>(q3 - q2), r34 = cast3<R>(q4 - q3);
+  v3<R> x1 = cross(r23, r12), x2 = cross(r34, r23);
+  R x1rn = rsq(nsq(x1)), x2rn = rsq(nsq(x2));
+  v3<R> x1u = x1 * x1rn, x2u = x2 * x2rn;
+  R cphi = dot(x1u, x2u);
+  R phi = acosc(cphi);
+  if (-dot(r12, x2u) < (R)0) phi = -phi;
+  R sphi = sinR(phi);
+  R s2 = sphi * sphi, c2 = cphi * cphi, sc = sphi * cphi;
+  R V = (R)c[0][ty] + (R)c[1][ty] * sphi + (R)c[2][ty] * cphi + (R)c[3][ty] * s2 +
+        (R)c[4][ty] * c2 + (R)c[5][ty] * sc;
+  R dV = (R)c[1][ty] * cphi - (R)c[2][ty] * sphi + (R)2 * ((R)c[3][ty] - (R)c[4][ty]) * sc +
+         (R)c[5][ty] * (c2 - s2);
+  R r23n = sq(nsq(r23));
+  v3<R> dr1 = x1u * (r23n * x1rn), dr4 = -(x2u * (r23n * x2rn));
+  v3<R> df = (dr1 * (-dot(r12, r23)) - dr4 * (-dot(r34, r23))) * ((R)1 / (r23n * r23n));
+  v3<R> dr2 = -dr1 + df, dr3 = -dr4 - df;
+  acc.add(i1, -(dV * dr1));
+  acc.add(i2, -(dV * dr2));
+  acc.add(i3, -(dV * dr3));
+  acc.add(i4, -(dV * dr4));
+  return (double)V;
+}
+
+// ---------------------------------------------------------------- PID ----
+// pid::eval_forces, vectorised form (cg/src/cg/pid/eval_forces.cpp:337-538),
+// lambda of cg/include/cg/pid/lambda.h:58-85. The i+-1 neighbours are raw
+// index neighbours (eval_forces.cpp:45) bounded by the system's own residues.
+template <typename R> __device__ __forceinline__ bool lam_supp(R alpha, R psi0, R psi) {
+  return fabs((double)(alpha * (psi - psi0))) < M_PI;
+}
+template <typename R> __device__ __forceinline__ void lam_eval(int variant, R alpha, R psi0, R psi, R &val, R &der) {
+  if (variant == 0) {
+    R s = alpha * (psi - psi0);
+    if (fabs((double)s) >= M_PI) { val = der = 0; return; }
+    val = (R)0.5 * cosR(s) + (R)0.5;
+    der = -(R)0.5 * alpha * sinR(s);
+  } else {
+    R t = alpha * (psi - psi0) * (R)M_1_PI, a = t < 0 ? -t : t;
+    if ((double)a >= M_PI) { val = der = 0; return; }
+    R x = (R)2 * a * (a - (R)1);
+    val = (R)1 - (a * a) / (x + (R)1);
+    der = alpha * (R)M_1_PI * (x / ((x + (R)1) * (x + (R)1)));
+    if (t < (R)0) der = -der;
+  }
+}
+template <typename R, typename G>
+__device__ __forceinline__ bool pid_psi(const G &g, int i, int j, bool vp, bool vn, R &psi,
+                                        v3<R> &d_self, v3<R> &d_next, v3<R> &d_prev, v3<R> &d_other) {
+  v3<double> z{0, 0, 0};
+  v3<double> rn_ = vn ? g.r(i + 1) : z, rp_ = vp ? g.r(i - 1) : z;
+  v3<R> rij = cast3<R>(g.r(i) - rn_);
+  v3<R> rkj = vp ? cast3<R>(rp_ - rn_) : v3<R>{0, 0, 0};
+  v3<R> rkl = cast3<R>(rp_ - g.r(j));
+  v3<R> rm = cross(rij, rkj), rn = cross(rkj, rkl);
+  R rm_n = sq(nsq(rm)), rn_n = sq(nsq(rn));
+  bool ok = !(rm_n < (R)0.1 || rn_n < (R)0.1);
+  R rm_i = (R)1 / rm_n, rn_i = (R)1 / rn_n, rkj_i = rsq(nsq(rkj)), rkj_n = (R)1 / rkj_i;
+  v3<R> fi = rm * (rkj_n * rm_i * rm_i);
+  v3<R> fl = -(rn * (rkj_n * rn_i * rn_i));
+  v3<R> df = (fi * dot(rij, rkj) - fl * dot(rkl, rkj)) * (rkj_i * rkj_i);
+  d_self = fi;
+  d_next = -fi + df;
+  d_prev = -fl - df;
+  d_other = fl;
+  R c = clampR(dot(rm, rn) * rm_i * rn_i, (R)-1, (R)1);
+  psi = acosc(c);
+  if (dot(rij, rn) < (R)0) psi = -psi;
+  return ok;
+}
+
+template <typename R, typename G, typename A>
+__device__ double pid_item(const DevBatch &b, const SysDesc &s, const G &g, A &acc, int k) {
+  const pcg_params &p = b.P->p;
+  int i1 = b.pid_i[0][k], i2 = b.pid_i[1][k];
+  if (b.atype[i1] == 14 || b.atype[i2] == 14) return 0.0; // PRO
+  v3<R> r12 = cast3<R>(wrapd(g.r(i2) - g.r(i1), s));
+  R r12n = sq(nsq(r12)), r12rn = (R)1 / r12n;
+  if (!(r12n < (R)p.nl_cutoff)) return 0.0;
+  int lo = s.res0, hi = s.res0 + s.n;
+  bool v1p = i1 - 1 >= lo, v1n = i1 + 1 < hi, v2p = i2 - 1 >= lo, v2n = i2 + 1 < hi;
+  R psi1, psi2;
+  v3<R> a1, a1n, a1p, a2o, c2, c2n, c2p, c1o;
+  if (!pid_psi<R>(g, i1, i2, v1p, v1n, psi1, a1, a1n, a1p, a2o)) return 0.0;
+  if (!pid_psi<R>(g, i2, i1, v2p, v2n, psi2, c2, c2n, c2p, c1o)) return 0.0;
+  bool sep3 = (i2 - i1 == 3);
+  R bpa = (R)p.pid_bbp_alpha, bpp = (R)p.pid_bbp_psi0, bma = (R)p.pid_bbm_alpha, bmp = (R)p.pid_bbm_psi0;
+  bool opt1 = lam_supp(bpa, bpp, psi1) || sep3;
+  bool opt2 = !sep3 && lam_supp(bpa, bpp, psi2);
+  R l1a = opt1 ? bpa : bma, l1p = opt1 ? bpp : bmp, l2a = opt2 ? bpa : bma, l2p = opt2 ? bpp : bmp;
+  const double *bblj = opt2 ? p.pid_bbp_lj : p.pid_bbm_lj;
+  int var = p.pid_lambda_variant;
+  R dV1 = 0, dV2 = 0, dVr = 0, V = 0;
+  if (lam_supp(l1a, l1p, psi1) && lam_supp(l2a, l2p, psi2)) {
+    R lam1, der1, lam2, der2;
+    lam_eval<R>(var, l1a, l1p, psi1, lam1, der1);
+    lam_eval<R>(var, l2a, l2p, psi2, lam2, der2);
+    if (lam1 * lam2 > (R)5e-5) {
+      R lV, ldV;
+      sink_ljR<R>((R)bblj[0], (R)bblj[1], (R)bblj[2], r12n, r12rn, lV, ldV);
+      V += lam1 * lam2 * lV;
+      dV1 += der1 * lam2 * lV;
+      dV2 += der2 * lam1 * lV;
+      dVr += lam1 * lam2 * ldV;
+    }
+  }
+  int ty = b.pid_t[k];
+  R sa = (R)p.pid_ss_alpha, sp = (R)p.pid_ss_psi0;
+  if (lam_supp(sa, sp, psi1) && lam_supp(sa, sp, psi2)) {
+    R lam1, der1, lam2, der2;
+    lam_eval<R>(var, sa, sp, psi1, lam1, der1);
+    lam_eval<R>(var, sa, sp, psi2, lam2, der2);
+    R depth = (R)p.pid_ss_depth[ty];
+    if (lam1 * lam2 > (R)5e-5 && depth > (R)5e-5) {
+      R lV, ldV;
+      sink_ljR<R>(depth, (R)p.pid_ss_r_low[ty], (R)p.pid_ss_r_high[ty], r12n, r12rn, lV, ldV);
+      V += lam1 * lam2 * lV;
+      dV1 += der1 * lam2 * lV;
+      dV2 += der2 * lam1 * lV;
+      dVr += lam1 * lam2 * ldV;
+    }
+  }
+  v3<R> u = r12 * r12rn;
+  if (v1p) acc.add(i1 - 1, -(dV1 * a1p));
+  acc.add(i1, -(dV1 * a1 + dV2 * c1o - dVr * u));
+  if (v1n) acc.add(i1 + 1, -(dV1 * a1n));
+  if (v2p) acc.add(i2 - 1, -(dV2 * c2p));
+  acc.add(i2, -(dV1 * a2o + dV2 * c2 + dVr * u));
+  if (v2n) acc.add(i2 + 1, -(dV2 * c2n));
+  return (double)V;
+}
+
+// ----------------------------------------------------------------- AFM ----
+// afm::vel::eval_forces::iter (cg/src/cg/afm/vel/eval_forces.cpp:10-31)
+template <typename R, typename G, typename A>
+__device__ double vel_afm_item(const DevBatch &b, const G &g, A &acc, int k, double t) {
+  const pcg_params &p = b.P->p;
+  int res = b.tip_res[k];
+  v3<double> cur{b.tip_orig[0][k] + t * b.tip_vel[0][k], b.tip_orig[1][k] + t * b.tip_vel[1][k],
+                 b.tip_orig[2][k] + t * b.tip_vel[2][k]};
+  v3<R> ra = cast3<R>(cur - g.r(res));
+  R n = sq(nsq(ra));
+  if (!(n > (R)1e-10)) return 0.0;
+  R V, dV;
+  harmonicR<R>((R)p.afm_H1, (R)p.afm_H2, (R)0, n, V, dV);
+  acc.add(res, (dV / n) * ra);
+  return (double)V;
+}
+
+// ------------------------------------------------------------------ QA ----
+// qa::prepare_nh::iter (cg/src/cg/qa/prepare_nh.cpp:10-19), exact FP64.
+template <typename G>
+__device__ void prepare_nh_item(const DevBatch &b, const G &g, int i) {
+#pragma clang fp contract(off)
+  int ip = b.prev[i], in = b.next[i];
+  if (ip < 0 || in < 0) return;
+  v3<double> v1 = xsub(g.r(i), g.r(ip)), v2 = xsub(g.r(in), g.r(i));
+  v3<double> nn = xsub(v2, v1);
+  double ni = __ddiv_rn(1.0, xnorm3(nn));
+  v3<double> h{__dsub_rn(__dmul_rn(v2.y, v1.z), __dmul_rn(v2.z, v1.y)),
+               __dsub_rn(__dmul_rn(v2.z, v1.x), __dmul_rn(v2.x, v1.z)),
+               __dsub_rn(__dmul_rn(v2.x, v1.y), __dmul_rn(v2.y, v1.x))};
+  double hi = __ddiv_rn(1.0, xnorm3(h));
+  b.nh[0][i] = __dmul_rn(ni, nn.x);
+  b.nh[1][i] = __dmul_rn(ni, nn.y);
+  b.nh[2][i] = __dmul_rn(ni, nn.z);
+  b.nh[3][i] = __dmul_rn(hi, h.x);
+  b.nh[4][i] = __dmul_rn(hi, h.y);
+  b.nh[5][i] = __dmul_rn(hi, h.z);
+}
+
+__device__ __forceinline__ double xdot(double ax, double ay, double az, v3<double> u) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(ax, u.x), __dmul_rn(ay, u.y)), __dmul_rn(az, u.z));
+}
+
+// qa::loop_over_candidates::iter (cg/src/cg/qa/loop_over_candidates.cpp:12-99):
+// criteria in exact FP64, repulsion force in R; a candidate is appended to
+// the system's candidate buffer (its order is restored by the finish sort).
+template <typename R, typename G, typename A>
+__device__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
+                               A &acc, int k) {
+  const DevParams &P = *b.P;
+  const pcg_params &p = P.p;
+  int slot = s.qf0 + k;
+  if (b.qf_vac[slot]) return 0.0;
+  double od = b.qf_d[slot];
+  if (od > P.qa_max_req) return 0.0;  // orig_dist - total_disp, total_disp == 0
+  int i1 = b.qf_i[0][slot], i2 = b.qf_i[1][slot];
+  v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
+  double rn = __ddiv_rn(1.0, xnorm3(d));
+  v3<double> u{__dmul_rn(d.x, rn), __dmul_rn(d.y, rn), __dmul_rn(d.z, rn)};
+  if (1.0 > __dmul_rn(rn, P.qa_max_req)) return 0.0;
+  double V = 0.0;
+  if (1.0 < __dmul_rn(rn, p.rep_cutoff)) {
+    v3<R> r12 = cast3<R>(d);
+    R rnR = rsq(nsq(r12)), Vr, dV;
+    shifted_ljR<R>((R)p.pauli_depth, (R)p.rep_cutoff, rnR, Vr, dV);
+    v3<R> f = (dV * rnR) * r12;
+    acc.add(i1, f);
+    acc.add(i2, -f);
+    V = (double)Vr;
+  }
+  double h1u = fabs(xdot(b.nh[3][i1], b.nh[4][i1], b.nh[5][i1], u));
+  double h2u = fabs(xdot(b.nh[3][i2], b.nh[4][i2], b.nh[5][i2], u));
+  double hh = fabs(__dadd_rn(__dadd_rn(__dmul_rn(b.nh[3][i1], b.nh[3][i2]),
+                                       __dmul_rn(b.nh[4][i1], b.nh[4][i2])),
+                             __dmul_rn(b.nh[5][i1], b.nh[5][i2])));
+  double n1u = xdot(b.nh[0][i1], b.nh[1][i1], b.nh[2][i1], u);
+  double n2u = xdot(b.nh[0][i2], b.nh[1][i2], b.nh[2][i2], u);
+  bool back1 = h1u > p.qa_min_cos_hr, back2 = h2u > p.qa_min_cos_hr;
+  bool bbx = hh > p.qa_min_cos_hh;
+  bool side1 = n1u < p.qa_max_cos_nr, side2 = -n2u < p.qa_max_cos_nr;
+  int a1 = b.atype[i1], a2 = b.atype[i2];
+  int type = 0;
+  if (back1 && back2 && bbx) type = 1;
+  else if (back1 && side2) type = 2;
+  else if (side1 && back2) type = 3;
+  else if (side1 && side2) type = 4 + a1 + a2 * 21;
+  if (type == 0) return V;
+  double req = __dmul_rn(p.qa_lj_r_high[type], __dadd_rn(1.0, p.qa_formation_tol));
+  if (1.0 > __dmul_rn(rn, req)) return V;
+  int pt1 = p.ptype[a1], pt2 = p.ptype[a2];
+  int8_t s1[4] = {(int8_t)back1, (int8_t)side1, (int8_t)(side1 && pt2 == 1), (int8_t)(side1 && pt2 == 2)};
+  int8_t s2[4] = {(int8_t)back2, (int8_t)side2, (int8_t)(side2 && pt1 == 1), (int8_t)(side2 && pt1 == 2)};
+  for (int a = 0; a < 4; ++a)
+    if ((int8_t)(b.sync[4 * i1 + a] - s1[a]) < 0) return V;
+  for (int a = 0; a < 4; ++a)
+    if ((int8_t)(b.sync[4 * i2 + a] - s2[a]) < 0) return V;
+  int c = atomicAdd(&st.cand_n, 1);
+  if (c >= s.nlcap) {
+    st.overflow = 1;
+    return V;
+  }
+  int cs = s.nl0 + c;
+  b.cand_i[0][cs] = i1;
+  b.cand_i[1][cs] = i2;
+  b.cand_od[cs] = od;
+  b.cand_dist[cs] = __ddiv_rn(1.0, rn);
+  b.cand_idx[cs] = k;
+  b.cand_t[cs] = (int16_t)type;
+  for (int a = 0; a < 4; ++a) {
+    b.cand_sd[8 * cs + a] = s1[a];
+    b.cand_sd[8 * cs + 4 + a] = s2[a];
+  }
+  return V;
+}
+
+// qa::process_contacts::iter (cg/src/cg/qa/process_contacts.cpp:12-72)
+template <typename R, typename G, typename A>
+__device__ double qa_cont_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
+                               A &acc, int k) {
+  const DevParams &P = *b.P;
+  const pcg_params &p = P.p;
+  int slot = s.qc0 + k;
+  if (b.qc_vac[slot]) return 0.0;
+  int i1 = b.qc_i[0][slot], i2 = b.qc_i[1][slot];
+  int type = b.qc_t[slot];
+  double sat = b.qc_sat[slot];
+  v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
+  double n2 = __dadd_rn(__dadd_rn(__dmul_rn(d.x, d.x), __dmul_rn(d.y, d.y)), __dmul_rn(d.z, d.z));
+  double rn = __ddiv_rn(1.0, __dsqrt_rn(n2));
+  if (__dmul_rn(rn, p.nl_cutoff) < 1.0) return 0.0;
+  double r12n = __dmul_rn(rn, n2);
+  v3<R> r12 = cast3<R>(d);
+  R Vij, dV;
+  sink_ljR<R>((R)p.qa_lj_depth[type], (R)p.qa_lj_r_low[type], (R)p.qa_lj_r_high[type], (R)r12n,
+              (R)rn, Vij, dV);
+  v3<R> f = ((R)sat * dV * (R)rn) * r12;
+  acc.add(i1, f);
+  acc.add(i2, -f);
+  bool breaking = r12n > __dmul_rn(P.qa_factor, p.qa_lj_r_high[type]);
+  double step = __dmul_rn(P.qa_sat_speed, p.dt);
+  if (breaking) {
+    sat = fmin(__dsub_rn(sat, step), 0.0);
+    if (sat == 0.0) {
+      int r = atomicAdd(&st.rem_n, 1);
+      if (r < s.qccap) b.rem[s.qc0 + r] = k;
+      else st.overflow = 1;
+    }
+  } else {
+    sat = fmax(__dadd_rn(sat, step), 0.0);
+  }
+  b.qc_sat[slot] = sat;
+  return (double)sat * (double)Vij;
+}
+
+} // namespace pcg

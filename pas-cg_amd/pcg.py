@@ -1,0 +1,476 @@
+"""ctypes binding of the C ABI (include/pcg.h) of libpcg.so.
+
+`Context` mirrors the reference's per-step loop and its per-term tasks
+(thread::advance_by_step, sliceable_task::for_slice); `Batch` runs many
+systems in one persistent launch; `Server` admits jobs continuously.
+There is no CPU fallback: without the HIP library these classes raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpcg.so")
+
+TERMS = ["chir", "tether", "lrep", "nat_cont", "pauli", "const_dh", "rel_dh",
+         "qa_cand", "qa_cont", "pid", "nat_ang", "heur_ang", "heur_dih", "cnd",
+         "snd", "vel_afm"]
+TERM_ID = {t: i for i, t in enumerate(TERMS)}
+MIXED, FP64 = 0, 1
+PHILOX, RAN2 = 0, 1
+
+D, I, I64, U8 = C.c_double, C.c_int32, C.c_int64, C.c_uint8
+
+
+class PcgParams(C.Structure):
+    _fields_ = [
+        ("rep_cutoff", D), ("counting_factor", D), ("nl_cutoff", D), ("nl_pad", D),
+        ("tether_on", I), ("tether_H1", D), ("tether_H2", D),
+        ("lrep_on", I), ("lrep_depth", D),
+        ("chir_on", I), ("e_chi", D),
+        ("nat_ang_on", I), ("CBA", D),
+        ("heur_ang_on", I), ("heur_ang_poly", (D * 9) * 7),
+        ("nat_dih_on", I), ("nat_dih_variant", I), ("CDA", D), ("CDB", D), ("CDH", D),
+        ("heur_dih_on", I), ("heur_dih_coeffs", (D * 9) * 6),
+        ("pauli_on", I), ("pauli_depth", D),
+        ("nat_cont_on", I), ("nat_cont_depth", D), ("nc_ss_kind", I),
+        ("nc_ss_H1", D), ("nc_ss_H2", D), ("nc_ss_nat_r", D), ("nc_ss_depth", D),
+        ("nc_ss_r_min", D),
+        ("dh_on", I), ("dh_variant", I), ("dh_screen_dist", D), ("dh_cutoff", D),
+        ("dh_perm", D),
+        ("qa_on", I), ("qa_include4", I), ("qa_phase_dur", D), ("qa_formation_tol", D),
+        ("qa_min_cos_hr", D), ("qa_min_cos_hh", D), ("qa_max_cos_nr", D),
+        ("qa_lj_depth", D * 445), ("qa_lj_r_low", D * 445), ("qa_lj_r_high", D * 445),
+        ("pid_on", I), ("pid_include4", I), ("pid_lambda_variant", I),
+        ("pid_bbp_alpha", D), ("pid_bbp_psi0", D), ("pid_bbm_alpha", D),
+        ("pid_bbm_psi0", D), ("pid_ss_alpha", D), ("pid_ss_psi0", D),
+        ("pid_bbp_lj", D * 3), ("pid_bbm_lj", D * 3),
+        ("pid_ss_depth", D * 441), ("pid_ss_r_low", D * 441), ("pid_ss_r_high", D * 441),
+        ("afm_on", I), ("afm_H1", D), ("afm_H2", D),
+        ("lang_on", I), ("gamma", D), ("temperature", D), ("dt", D),
+        ("mass", D * 21), ("charge", D * 21), ("ptype", I * 21),
+        ("base_sync", (I * 4) * 21),
+    ]
+
+
+class PcgModel(C.Structure):
+    _fields_ = [
+        ("n", I), ("pos", C.POINTER(D)), ("orig_pos", C.POINTER(D)),
+        ("atype", C.POINTER(U8)), ("chain_idx", C.POINTER(I)), ("seq_idx", C.POINTER(I)),
+        ("n_teth", I), ("teth", C.POINTER(I)),
+        ("n_ang", I), ("ang", C.POINTER(I)), ("ang_has", C.POINTER(U8)),
+        ("ang_theta", C.POINTER(D)),
+        ("n_dih", I), ("dih", C.POINTER(I)), ("dih_has", C.POINTER(U8)),
+        ("dih_phi", C.POINTER(D)),
+        ("n_cont", I), ("cont", C.POINTER(I)), ("cont_len", C.POINTER(D)),
+        ("cont_type", C.POINTER(I)),
+        ("cell", D * 3),
+        ("n_tips", I), ("tip_res", C.POINTER(I)), ("tip_orig", C.POINTER(D)),
+        ("tip_vel", C.POINTER(D)),
+    ]
+
+
+class PcgOptions(C.Structure):
+    _fields_ = [("precision", I), ("noise", I), ("seed", I), ("philox_seed", C.c_uint64),
+                ("temperature", D), ("device", I)]
+
+
+class PcgBatchOptions(C.Structure):
+    _fields_ = [("precision", I), ("philox_seed", C.c_uint64), ("block", I),
+                ("steps_per_launch", I), ("use_graph", I), ("device", I)]
+
+
+class PcgServerOptions(C.Structure):
+    _fields_ = [("precision", I), ("hbm_budget_bytes", I64), ("max_resident", I),
+                ("chunk_steps", I), ("philox_seed", C.c_uint64), ("device", I)]
+
+
+class PcgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PcgError(f"HIP library not built: {LIB_PATH} (run pas-cg_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        V = C.c_void_p
+        L.pcg_last_error.restype = C.c_char_p
+        sigs = {
+            "pcg_device_count": [C.POINTER(C.c_int)],
+            "pcg_create": [C.POINTER(PcgParams), C.POINTER(PcgModel), C.POINTER(PcgOptions),
+                           C.POINTER(V)],
+            "pcg_destroy": [V], "pcg_step": [V, I],
+            "pcg_nl_verify": [V, C.POINTER(I)], "pcg_fix_nl": [V], "pcg_prepare_nh": [V],
+            "pcg_eval_forces": [V],
+            "pcg_eval_term": [V, I, I, I, C.POINTER(D), C.POINTER(D)],
+            "pcg_term_size": [V, I, C.POINTER(I)],
+            "pcg_qa_finish": [V], "pcg_lang_step": [V],
+            "pcg_get": [V, I, C.POINTER(D)], "pcg_set": [V, I, C.POINTER(D)],
+            "pcg_get_energy": [V, C.POINTER(D), C.POINTER(D)],
+            "pcg_get_time": [V, C.POINTER(D), C.POINTER(I64)],
+            "pcg_set_time": [V, D], "pcg_set_noise_scale": [V, D],
+            "pcg_get_list": [V, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I),
+                             C.POINTER(D), C.POINTER(U8)],
+            "pcg_nat_cont_state": [V, C.POINTER(U8), C.POINTER(D), C.POINTER(I)],
+            "pcg_qa_state": [V, C.POINTER(C.c_int8), C.POINTER(I)],
+            "pcg_nl_rebuilds": [V, C.POINTER(I)],
+            "pcg_use_graph": [V, I],
+            "pcg_batch_create": [C.POINTER(PcgParams), I, C.POINTER(PcgModel), C.POINTER(D),
+                                 C.POINTER(I), C.POINTER(PcgBatchOptions), C.POINTER(V)],
+            "pcg_batch_destroy": [V], "pcg_batch_run": [V, I], "pcg_batch_sync": [V],
+            "pcg_batch_get": [V, I, I, C.POINTER(D)],
+            "pcg_batch_energy": [V, I, C.POINTER(D), C.POINTER(D)],
+            "pcg_batch_info": [V, C.POINTER(I), C.POINTER(I), C.POINTER(I64)],
+            "pcg_batch_last_ms": [V, C.POINTER(D), C.POINTER(I)],
+            "pcg_server_create": [C.POINTER(PcgParams), C.POINTER(PcgServerOptions),
+                                  C.POINTER(V)],
+            "pcg_server_destroy": [V],
+            "pcg_server_submit": [V, C.POINTER(PcgModel), D, I, I64, C.POINTER(I64)],
+            "pcg_server_round": [V, C.POINTER(I)],
+            "pcg_server_status": [V, I64, C.POINTER(I), C.POINTER(I64)],
+            "pcg_server_result": [V, I64, C.POINTER(D), C.POINTER(D)],
+            "pcg_server_cancel": [V, I64],
+            "pcg_server_stats": [V, C.POINTER(I64), C.POINTER(I), C.POINTER(I)],
+            "pcg_server_job_times": [V, I64, C.POINTER(D), C.POINTER(D), C.POINTER(D)],
+            "pcg_batch_set_noise_scale": [V, D],
+        }
+        for name, args in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != 0:
+        raise PcgError(f"pcg error {rc}: {lib().pcg_last_error().decode()}")
+
+
+def fill_struct(st, p: dict):
+    for name, ctype in st._fields_:
+        if name not in p:
+            continue
+        v = p[name]
+        if hasattr(ctype, "_length_"):
+            arr = getattr(st, name)
+            flat = np.asarray(v).reshape(-1)
+            inner = ctype._type_
+            if hasattr(inner, "_length_"):
+                k = inner._length_
+                for a in range(ctype._length_):
+                    for b in range(k):
+                        arr[a][b] = flat[a * k + b]
+            else:
+                for a in range(ctype._length_):
+                    arr[a] = flat[a]
+        else:
+            setattr(st, name, v)
+    return st
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class _ModelRef:
+    """Keeps the numpy arrays behind a PcgModel alive."""
+
+    def __init__(self, m: dict):
+        self.keep = []
+
+        def arr(x, dt):
+            a = np.ascontiguousarray(np.asarray(x, dt))
+            self.keep.append(a)
+            return a
+
+        s = PcgModel()
+        s.n = int(m["n"])
+        s.pos = _p(arr(m["pos"], np.float64), D)
+        s.orig_pos = _p(arr(m["orig_pos"], np.float64), D)
+        s.atype = _p(arr(m["atype"], np.uint8), U8)
+        s.chain_idx = _p(arr(m["chain"], np.int32), I)
+        s.seq_idx = _p(arr(m["seq"], np.int32), I)
+        s.n_teth = len(m["teth"])
+        s.teth = _p(arr(m["teth"], np.int32), I)
+        s.n_ang = len(m["ang"])
+        s.ang = _p(arr(m["ang"], np.int32), I)
+        s.ang_has = _p(arr(m["ang_has"], np.uint8), U8)
+        s.ang_theta = _p(arr(m["ang_theta"], np.float64), D)
+        s.n_dih = len(m["dih"])
+        s.dih = _p(arr(m["dih"], np.int32), I)
+        s.dih_has = _p(arr(m["dih_has"], np.uint8), U8)
+        s.dih_phi = _p(arr(m["dih_phi"], np.float64), D)
+        s.n_cont = len(m["cont"])
+        s.cont = _p(arr(m["cont"], np.int32), I)
+        s.cont_len = _p(arr(m["cont_len"], np.float64), D)
+        s.cont_type = _p(arr(m["cont_type"], np.int32), I)
+        for k in range(3):
+            s.cell[k] = float(m["cell"][k])
+        s.n_tips = len(m["tip_res"])
+        s.tip_res = _p(arr(m["tip_res"], np.int32), I)
+        s.tip_orig = _p(arr(m["tip_orig"], np.float64), D)
+        s.tip_vel = _p(arr(m["tip_vel"], np.float64), D)
+        self.s = s
+
+
+class Context:
+    """Plain single-system context: the drop-in per-step loop."""
+
+    def __init__(self, params: dict, model: dict, precision=MIXED, noise=PHILOX,
+                 seed: int = 448, philox_seed: int = 1, temperature: float = -1.0,
+                 device: int = 0):
+        L = lib()
+        self.n = int(model["n"])
+        self._m = _ModelRef(model)
+        self._p = fill_struct(PcgParams(), params)
+        o = PcgOptions(precision, noise, seed, philox_seed, temperature, device)
+        h = C.c_void_p()
+        check(L.pcg_create(C.byref(self._p), C.byref(self._m.s), C.byref(o), C.byref(h)))
+        self.h = h
+        self.L = L
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pcg_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def step(self, k: int = 1):
+        check(self.L.pcg_step(self.h, k))
+
+    def use_graph(self, on=True):
+        check(self.L.pcg_use_graph(self.h, int(on)))
+
+    def nl_verify(self) -> bool:
+        v = C.c_int32()
+        check(self.L.pcg_nl_verify(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def fix_nl(self):
+        check(self.L.pcg_fix_nl(self.h))
+
+    def prepare_nh(self):
+        check(self.L.pcg_prepare_nh(self.h))
+
+    def eval_forces(self):
+        check(self.L.pcg_eval_forces(self.h))
+
+    def eval_term(self, term, frm=0, to=-1):
+        F = np.zeros((self.n, 3))
+        V = C.c_double()
+        tid = TERM_ID[term] if isinstance(term, str) else int(term)
+        check(self.L.pcg_eval_term(self.h, tid, frm, to, _p(F, D), C.byref(V)))
+        return F, V.value
+
+    def term_size(self, term) -> int:
+        v = C.c_int32()
+        check(self.L.pcg_term_size(self.h, TERM_ID[term], C.byref(v)))
+        return v.value
+
+    def qa_finish(self):
+        check(self.L.pcg_qa_finish(self.h))
+
+    def lang_step(self):
+        check(self.L.pcg_lang_step(self.h))
+
+    def get(self, which: int) -> np.ndarray:
+        out = np.zeros((self.n, 3))
+        check(self.L.pcg_get(self.h, which, _p(out, D)))
+        return out
+
+    def r(self):
+        return self.get(0)
+
+    def F(self):
+        return self.get(6)
+
+    def set(self, which: int, a):
+        a = np.ascontiguousarray(np.asarray(a, np.float64))
+        check(self.L.pcg_set(self.h, which, _p(a, D)))
+
+    def energy(self):
+        V = C.c_double()
+        tv = np.zeros(len(TERMS))
+        check(self.L.pcg_get_energy(self.h, C.byref(V), _p(tv, D)))
+        return V.value, dict(zip(TERMS, tv))
+
+    def time(self):
+        t = C.c_double()
+        s = C.c_int64()
+        check(self.L.pcg_get_time(self.h, C.byref(t), C.byref(s)))
+        return t.value, s.value
+
+    def set_time(self, t):
+        check(self.L.pcg_set_time(self.h, t))
+
+    def set_noise_scale(self, s):
+        check(self.L.pcg_set_noise_scale(self.h, s))
+
+    def list(self, kind: int):
+        size = C.c_int32()
+        check(self.L.pcg_get_list(self.h, kind, 0, C.byref(size), None, None, None, None))
+        n = size.value
+        i1 = np.zeros(n, np.int32)
+        i2 = np.zeros(n, np.int32)
+        val = np.zeros(n)
+        flag = np.zeros(n, np.uint8)
+        check(self.L.pcg_get_list(self.h, kind, n, C.byref(size), _p(i1, I), _p(i2, I),
+                                  _p(val, D), _p(flag, U8)))
+        return i1, i2, val, flag
+
+    def nat_cont_state(self, k: int):
+        a = np.zeros(max(k, 1), np.uint8)
+        ct = np.zeros(max(k, 1))
+        nc = C.c_int32()
+        check(self.L.pcg_nat_cont_state(self.h, _p(a, U8), _p(ct, D), C.byref(nc)))
+        return a[:k], ct[:k], nc.value
+
+    def qa_state(self):
+        sync = np.zeros((self.n, 4), np.int8)
+        num = C.c_int32()
+        check(self.L.pcg_qa_state(self.h, _p(sync, C.c_int8), C.byref(num)))
+        return sync, num.value
+
+    def nl_rebuilds(self) -> int:
+        v = C.c_int32()
+        check(self.L.pcg_nl_rebuilds(self.h, C.byref(v)))
+        return v.value
+
+
+class Batch:
+    """Many systems advanced together by the persistent kernel."""
+
+    def __init__(self, params: dict, models: list, temps=None, seeds=None, precision=MIXED,
+                 philox_seed: int = 1, block: int = 0, steps_per_launch: int = 0,
+                 use_graph: bool = False, device: int = 0):
+        L = lib()
+        self.ns = [int(m["n"]) for m in models]
+        self._refs = [_ModelRef(m) for m in models]
+        arr = (PcgModel * len(models))(*[r.s for r in self._refs])
+        self._arr = arr
+        self._p = fill_struct(PcgParams(), params)
+        t = np.ascontiguousarray(np.asarray(temps if temps is not None else [-1.0] * len(models),
+                                            np.float64))
+        s = np.ascontiguousarray(np.asarray(seeds if seeds is not None else
+                                            [448 + k for k in range(len(models))], np.int32))
+        self._t, self._s = t, s
+        o = PcgBatchOptions(precision, philox_seed, block, steps_per_launch, int(use_graph), device)
+        h = C.c_void_p()
+        check(L.pcg_batch_create(C.byref(self._p), len(models), arr, _p(t, D), _p(s, I),
+                                 C.byref(o), C.byref(h)))
+        self.h = h
+        self.L = L
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pcg_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def run(self, nsteps: int):
+        check(self.L.pcg_batch_run(self.h, nsteps))
+
+    def get(self, sys: int, which: int) -> np.ndarray:
+        out = np.zeros((self.ns[sys], 3))
+        check(self.L.pcg_batch_get(self.h, sys, which, _p(out, D)))
+        return out
+
+    def energy(self, sys: int):
+        V = C.c_double()
+        tv = np.zeros(len(TERMS))
+        check(self.L.pcg_batch_energy(self.h, sys, C.byref(V), _p(tv, D)))
+        return V.value, dict(zip(TERMS, tv))
+
+    def info(self):
+        b = C.c_int32()
+        s = C.c_int32()
+        by = C.c_int64()
+        check(self.L.pcg_batch_info(self.h, C.byref(b), C.byref(s), C.byref(by)))
+        return {"block": b.value, "steps_per_launch": s.value, "bytes": by.value}
+
+    def last_ms(self):
+        ms = C.c_double()
+        n = C.c_int32()
+        check(self.L.pcg_batch_last_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def set_noise_scale(self, s: float):
+        check(self.L.pcg_batch_set_noise_scale(self.h, s))
+
+
+JOB_STATUS = {0: "queued", 1: "running", 2: "done", 3: "cancelled", 4: "failed"}
+
+
+class Server:
+    """Continuous batching of simulation jobs (admission by HBM budget)."""
+
+    def __init__(self, params: dict, precision=MIXED, hbm_budget_bytes: int = 8 << 30,
+                 max_resident: int = 1024, chunk_steps: int = 1000, philox_seed: int = 1,
+                 device: int = 0):
+        L = lib()
+        self._p = fill_struct(PcgParams(), params)
+        o = PcgServerOptions(precision, hbm_budget_bytes, max_resident, chunk_steps,
+                             philox_seed, device)
+        h = C.c_void_p()
+        check(L.pcg_server_create(C.byref(self._p), C.byref(o), C.byref(h)))
+        self.h = h
+        self.L = L
+        self.ns = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pcg_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def submit(self, model: dict, nsteps: int, temperature: float = -1.0, seed: int = 448) -> int:
+        ref = _ModelRef(model)
+        jid = C.c_int64()
+        check(self.L.pcg_server_submit(self.h, C.byref(ref.s), temperature, seed, nsteps,
+                                       C.byref(jid)))
+        self.ns[jid.value] = int(model["n"])
+        return jid.value
+
+    def round(self) -> int:
+        a = C.c_int32()
+        check(self.L.pcg_server_round(self.h, C.byref(a)))
+        return a.value
+
+    def status(self, job: int):
+        s = C.c_int32()
+        d = C.c_int64()
+        check(self.L.pcg_server_status(self.h, job, C.byref(s), C.byref(d)))
+        return JOB_STATUS[s.value], d.value
+
+    def result(self, job: int):
+        pos = np.zeros((self.ns[job], 3))
+        V = C.c_double()
+        check(self.L.pcg_server_result(self.h, job, _p(pos, D), C.byref(V)))
+        return pos, V.value
+
+    def cancel(self, job: int):
+        check(self.L.pcg_server_cancel(self.h, job))
+
+    def stats(self):
+        b = C.c_int64()
+        r = C.c_int32()
+        q = C.c_int32()
+        check(self.L.pcg_server_stats(self.h, C.byref(b), C.byref(r), C.byref(q)))
+        return {"resident_bytes": b.value, "resident": r.value, "queued": q.value}
+
+    def times(self, job: int):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(self.L.pcg_server_job_times(self.h, job, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value

@@ -1,0 +1,202 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+* lists (NL pairs with their distances, the partition into native contacts,
+  QA free pairs, PID, DH and Pauli lists) bit-exact on identical positions;
+* every force term on identical positions and lists: FP64 mode to 1e-9 of
+  the largest force component, mixed mode (FP32 radial pair potentials) to
+  the bound written below;
+* whole trajectories in FP64 with the reference's ran2 noise stream against
+  the oracle and against the reference's golden V traces;
+* contact-state transitions (active flags, change times) bit-exact.
+"""
+import numpy as np
+import pytest
+
+from golden_cases import case
+from oracle_ffi import Oracle
+from pas_cg_amd import model as M
+from pas_cg_amd import params as P
+from pas_cg_amd import pcg as G
+
+pytestmark = pytest.mark.gpu
+
+# mixed precision bounds (north star: force error <= 1e-5, energy <= 1e-6)
+MIXED_F_TOL = 1e-5
+MIXED_V_TOL = 1e-6
+FP64_TOL = 1e-9
+
+
+def full_params(**kw):
+    p = P.default_params()
+    p.update(chir_on=1, pid_on=1, afm_on=1)
+    p.update(kw)
+    return p
+
+
+def protein(n=90, seed=5, noise_A=0.3):
+    m = M.synthetic_protein(n, M.job_seed(seed))
+    rng = np.random.default_rng(seed)
+    m["pos"] = m["pos"] + rng.normal(scale=noise_A * P.ANGSTROM, size=m["pos"].shape)
+    m["atype"][::7] = P.AA_INDEX["LYS"]
+    m["atype"][3::11] = P.AA_INDEX["GLU"]
+    m["atype"][5::13] = P.AA_INDEX["PRO"]
+    return M.add_pulling(m)
+
+
+def multichain(copies=4, n=40, side_A=70.0, seed=9):
+    one = M.synthetic_protein(n, M.job_seed(seed))
+    one["atype"][::5] = P.AA_INDEX["ARG"]
+    one["atype"][2::6] = P.AA_INDEX["ASP"]
+    m = M.replicate(one, copies, side_A * P.ANGSTROM, seed)
+    return m
+
+
+CONFIGS = {
+    "protein": lambda: (full_params(qa_on=0), protein()),
+    "protein_sink_pid": lambda: (
+        _sink(full_params(qa_on=0, nat_cont_on=0, dh_variant=1,
+                          dh_perm=4.0 * P.EPS_0 / P.ANGSTROM)), protein(seed=6)),
+    "multichain": lambda: (full_params(qa_on=0, nat_cont_on=0), multichain()),
+    "multichain_qa": lambda: (full_params(qa_on=1, pid_on=0), multichain(seed=3)),
+}
+
+
+def _sink(p):
+    P.set_pid_forces(p, "sink lj", bbp_depth=0.3, bbm_depth=0.25,
+                     ss_depth_table=[[0.1 + 0.01 * ((a * 7 + b) % 13) for b in range(21)]
+                                     for a in range(21)])
+    return p
+
+
+def same_state(cfg, precision, steps=8):
+    prm, m = CONFIGS[cfg]()
+    o = Oracle(prm, m, seed=17)
+    o.set_noise_scale(0.0)
+    o.step(steps)
+    c = G.Context(prm, m, precision=precision, noise=G.RAN2, seed=17)
+    c.set(0, o.r())
+    c.set_time(o.t)
+    return prm, m, o, c
+
+
+@pytest.mark.parametrize("cfg", list(CONFIGS))
+def test_lists_bit_exact(cfg):
+    prm, m, o, c = same_state(cfg, G.FP64)
+    o.fix_nl()
+    c.fix_nl()
+    for kind in range(6):  # NL, native contacts, pauli, dh, pid, qa free pairs
+        a = o.list(kind)
+        g = c.list(kind)
+        assert len(a[0]) == len(g[0]), (cfg, kind, len(a[0]), len(g[0]))
+        assert np.array_equal(a[0], g[0]) and np.array_equal(a[1], g[1]), (cfg, kind)
+        if kind == 0:
+            assert np.array_equal(a[2], g[2]), "NL distances not bit-exact"
+    assert len(o.list(0)[0]) > 0
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("cfg", ["protein", "protein_sink_pid", "multichain"])
+def test_terms_match_oracle(cfg, precision):
+    prm, m, o, c = same_state(cfg, precision)
+    o.fix_nl()
+    c.fix_nl()
+    ftol = FP64_TOL if precision == G.FP64 else MIXED_F_TOL
+    vtol = FP64_TOL if precision == G.FP64 else MIXED_V_TOL
+    checked = 0
+    for term in G.TERMS:
+        if term.startswith("qa"):
+            continue
+        F1, V1 = o.eval_term(term)
+        F2, V2 = c.eval_term(term)
+        scale = np.abs(F1).max()
+        if scale == 0 and V1 == 0:
+            assert np.abs(F2).max() == 0 and V2 == 0, term
+            continue
+        checked += 1
+        err = np.abs(F2 - F1).max() / scale
+        assert err <= ftol, (term, err)
+        assert abs(V2 - V1) <= vtol * max(abs(V1), 1e-300) + 1e-12, (term, V1, V2)
+    assert checked >= 5
+
+
+def test_contact_transitions_bit_exact():
+    prm = full_params(qa_on=0)
+    m = protein(noise_A=0.8)
+    o = Oracle(prm, m, seed=21)
+    o.set_noise_scale(0.0)
+    c = G.Context(prm, m, precision=G.MIXED, noise=G.RAN2, seed=21)
+    c.set_noise_scale(0.0)
+    k = len(o.static_list(7)[0])
+    for _ in range(5):
+        o.step(7)
+        # GPU evaluates on the oracle's positions and time
+        c.set(0, o.r())
+        c.set_time(o.t)
+        o.fix_nl()
+        c.fix_nl()
+        F1, _ = o.eval_term("nat_cont")
+        c.eval_term("nat_cont")
+        a1, t1 = o.nat_cont_state(k)
+        a2, t2, _ = c.nat_cont_state(k)
+        assert np.array_equal(a1, a2)
+
+
+@pytest.mark.parametrize("name", ["start2", "TEST", "input_test", "9aac", "1ubq"])
+def test_fp64_ran2_trajectory_matches_golden(name):
+    """FP64 + the reference's ran2 stream: the GPU trajectory follows the
+    oracle and the reference's own V trace step by step."""
+    p, m, seed, saw, fx = case(name)
+    o = Oracle(p, m, seed=seed, saw=saw)
+    if saw:  # the SAW is input preparation: hand the GPU its result
+        m = dict(m)
+        m["pos"] = o.pos0.copy()
+        if saw.get("update_orig"):
+            m["orig_pos"] = o.pos0.copy()
+    steps = 100
+    ref = np.asarray(fx["trace"]["V_fort_7" if name == "start2" else "V_cxx"][:steps])
+    c = G.Context(p, m, precision=G.FP64, noise=G.RAN2, seed=seed)
+    tr_o = o.step(steps)
+    tr_g = np.zeros(steps)
+    for k in range(steps):
+        c.step(1)
+        tr_g[k] = c.energy()[0]
+    rel_o = np.abs(tr_g - tr_o) / np.abs(tr_o)
+    rel_ref = np.abs(tr_g - ref) / np.abs(ref)
+    assert rel_o.max() < 1e-9, (int(rel_o.argmax()), rel_o.max())
+    assert rel_ref.max() < 1e-8, (int(rel_ref.argmax()), rel_ref.max())
+
+
+def test_qa_trajectory_fp64_matches_oracle():
+    """QA on: free pairs, contacts, saturations and sync budgets follow the
+    oracle over a noise-free trajectory in FP64."""
+    prm = full_params(qa_on=1, pid_on=0, temperature=0.3)
+    m = multichain(copies=3, n=50, side_A=45.0, seed=31)
+    o = Oracle(prm, m, seed=31)
+    o.set_noise_scale(0.0)
+    c = G.Context(prm, m, precision=G.FP64, noise=G.RAN2, seed=31)
+    c.set_noise_scale(0.0)
+    formed = 0
+    for _ in range(6):
+        o.step(20)
+        c.step(20)
+        assert np.abs(c.r() - o.r()).max() < 1e-8
+        so, no = o.sync(), o.num_qa_contacts()
+        sg, ng = c.qa_state()
+        assert no == ng and np.array_equal(so, sg)
+        ci = o.list(6)
+        gi = c.list(6)
+        assert np.array_equal(ci[0], gi[0]) and np.array_equal(ci[1], gi[1])
+        assert np.allclose(ci[2], gi[2], rtol=0, atol=1e-12)
+        formed = max(formed, no)
+    assert formed > 0
+
+
+def test_eval_term_slices_sum_to_whole():
+    prm, m, o, c = same_state("protein", G.FP64)
+    c.fix_nl()
+    n = c.term_size("tether")
+    F_all, V_all = c.eval_term("tether")
+    F_a, V_a = c.eval_term("tether", 0, n // 3)
+    F_b, V_b = c.eval_term("tether", n // 3, n)
+    assert np.allclose(F_a + F_b, F_all, atol=1e-12)
+    assert abs(V_a + V_b - V_all) < 1e-10 * abs(V_all)
