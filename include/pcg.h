@@ -146,6 +146,7 @@ typedef struct pcg_options {
   uint64_t philox_seed;
   double temperature; /* <0: take params.temperature */
   int32_t device;
+  int64_t ran2_skip;  /* ran2 draws consumed by input preparation (a SAW) */
 } pcg_options;
 
 const char *pcg_last_error(void); /* thread-local message of the last failure */

@@ -132,11 +132,14 @@ struct ran2_t {
                           am = 1.f / (float)im1;
   int iy = 0, idum = -448, idum2 = 123456789;
   int iv[ntab] = {0};
+  long long draws = 0; /* draws after seeding */
   explicit ran2_t(int seed = 0) {
     next(seed);
     next(seed);
+    draws = 0;
   }
   double next(int iseed = 0) {
+    ++draws;
     int k, j;
     if (iseed != 0) idum = -iseed;
     if (idum <= 0) {
@@ -236,6 +239,7 @@ struct ora_sys {
   pbc_t pbc, nl_orig_pbc;
   ran2_t gen;
   double noise_scale = 1.0;
+  long long saw_draws = 0;
 
   /* time */
   double t = 0.0, true_t = 0.0;
@@ -1196,6 +1200,7 @@ ora_sys *ora_create(const ora_params *p, const ora_model *m, int32_t seed,
   }
   if (saw && saw->perform) {
     morph_saw(s->gen, *saw, n, m->chain_idx, pos);
+    s->saw_draws = s->gen.draws;
     /* for sequence-file inputs orig_model follows the SAW (state.cpp:134-135) */
     if (saw->update_orig) orig = pos;
     if (pos_out)
@@ -1609,6 +1614,7 @@ void ora_sync(const ora_sys *s, int8_t *out) {
       out[4 * i + k] = s->sync.empty() ? 0 : s->sync[i].v[k];
 }
 int32_t ora_nl_rebuilds(const ora_sys *s) { return s->nl_rebuilds; }
+int64_t ora_saw_draws(const ora_sys *s) { return s->saw_draws; }
 
 struct ora_rng {
   ran2_t g;

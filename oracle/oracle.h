@@ -206,6 +206,7 @@ int32_t ora_num_changed(const ora_sys *s);
 int32_t ora_num_qa_contacts(const ora_sys *s);
 void ora_sync(const ora_sys *s, int8_t *sync /*[4n]*/);
 int32_t ora_nl_rebuilds(const ora_sys *s);
+int64_t ora_saw_draws(const ora_sys *s); /* ran2 draws the SAW consumed */
 
 /* The ran2 generator by itself (cg/include/cg/utils/random/nr.h). */
 typedef struct ora_rng ora_rng;

@@ -108,6 +108,7 @@ __device__ __forceinline__ void block_ranks5(int cls, int valid, int counts[5], 
 
 // NL rebuild + ordered partition inside the workgroup.
 __device__ void wg_fix_nl(const DevBatch &b, const SysDesc &sd, SysState &st, const GeoL &g) {
+#pragma clang fp contract(off)
   const double req = b.P->nl_eff_cutoff + b.P->p.nl_pad;
   int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int end = sd.res0 + sd.n;
@@ -230,6 +231,7 @@ __device__ void wg_fix_nl(const DevBatch &b, const SysDesc &sd, SysState &st, co
 
 template <typename R, typename RF>
 __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
+#pragma clang fp contract(off)
   const SysDesc &sd = b.sys[s];
   SysState &st = b.st[s];
   if (!st.active) return;

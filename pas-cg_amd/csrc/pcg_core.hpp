@@ -130,12 +130,16 @@ __device__ __forceinline__ double expR(double x) { return exp(x); }
 
 // Exact FP64 geometry for the decisions that must match the CPU path bit
 // for bit (NL predicate, list filters, contact-state transitions, QA
-// criteria, displacement check): no contraction, IEEE sqrt and division,
-// the reference's operand order (vec3_ops_expr.h:6-33, pbc.h:37-49).
+// criteria, displacement check): IEEE sqrt and division, the reference's
+// operand order (vec3_ops_expr.h:6-33, pbc.h:37-49), and no FMA contraction
+// (the *_rn intrinsics are plain operators once inlined, so every function
+// that must round each operation turns contraction off).
 __device__ __forceinline__ double xwrap1(double v, double cell, double cinv) {
+#pragma clang fp contract(off)
   return __dsub_rn(v, __dmul_rn(cell, round(__dmul_rn(v, cinv))));
 }
 __device__ __forceinline__ double xnorm(double x, double y, double z) {
+#pragma clang fp contract(off)
   return __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z)));
 }
 

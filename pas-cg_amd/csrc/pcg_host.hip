@@ -132,7 +132,7 @@ int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 // state::setup_* of cg/src/cg/simul/state.cpp for one system.
 int build_system(const pcg_params &p, const pcg_model &m, double temperature, int seed,
-                 HostSys &h) {
+                 HostSys &h, int64_t ran2_skip = 0) {
   int n = m.n;
   if (n <= 0 || !m.pos || !m.atype || !m.chain_idx || !m.seq_idx)
     return fail(PCG_E_ARG, "model needs n > 0, pos, atype, chain_idx, seq_idx");
@@ -152,6 +152,7 @@ int build_system(const pcg_params &p, const pcg_model &m, double temperature, in
   h.y1.assign(3 * n, 0.0);
   if (p.lang_on) {
     Ran2 g(seed);
+    for (int64_t k = 0; k < ran2_skip; ++k) g.draw();
     double sum[3] = {0, 0, 0};
     for (int i = 0; i < n; ++i) {
       double xx = 2.0 * (g.draw() - 0.5), yy = 2.0 * (g.draw() - 0.5), zz = 2.0 * (g.draw() - 0.5);
@@ -308,9 +309,13 @@ struct Arena {
     req.push_back({(void **)ptr, count * sizeof(T)});
     total += (count * sizeof(T) + 255) / 256 * 256;
   }
-  int alloc() {
+  int alloc(hipStream_t stream) {
     if (hipMalloc(&base, total ? total : 256) != hipSuccess)
       return fail(PCG_E_NOMEM, "hipMalloc of " + std::to_string(total) + " bytes failed");
+    // every array starts at zero, as the reference's vectors do (e.g. the QA
+    // n/h vectors of chain ends, which prepare_nh never writes)
+    if (hipMemsetAsync(base, 0, total ? total : 256, stream) != hipSuccess)
+      return fail(PCG_E_HIP, "hipMemsetAsync of the arena failed");
     size_t off = 0;
     for (auto &r : req) {
       *r.first = (char *)base + off;
@@ -353,7 +358,7 @@ namespace {
 int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const double *temps,
                const int32_t *seeds, int prec, uint64_t philox_seed, int device,
                std::unique_ptr<DeviceBatch> &out, hipStream_t use_stream = nullptr,
-               size_t *dry_run_bytes = nullptr) {
+               size_t *dry_run_bytes = nullptr, int64_t ran2_skip = 0) {
   if (nsys <= 0 || !models) return fail(PCG_E_ARG, "no systems");
   if (prec != PCG_MIXED && prec != PCG_FP64) return fail(PCG_E_ARG, "unknown precision");
   auto B = std::make_unique<DeviceBatch>();
@@ -369,7 +374,8 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   fill_dev_params(p, B->hp);
   B->hs.resize(nsys);
   for (int s = 0; s < nsys; ++s) {
-    int rc = build_system(p, models[s], temps ? temps[s] : -1.0, seeds ? seeds[s] : 448, B->hs[s]);
+    int rc = build_system(p, models[s], temps ? temps[s] : -1.0, seeds ? seeds[s] : 448, B->hs[s],
+                          ran2_skip);
     if (rc) return rc;
   }
   // offsets
@@ -507,7 +513,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     *dry_run_bytes = A.total;
     return PCG_OK;
   }
-  int rc = A.alloc();
+  int rc = A.alloc(B->stream);
   if (rc) return rc;
   B->bytes = A.total;
   D.sys = dsys;
@@ -771,12 +777,14 @@ int pcg_create(const pcg_params *p, const pcg_model *m, const pcg_options *o, pc
   double T = opt.temperature > 0 ? opt.temperature : -1.0;
   int32_t seed = opt.seed;
   auto c = std::make_unique<pcg_ctx>();
-  int rc = make_batch(*p, 1, m, &T, &seed, opt.precision, opt.philox_seed, opt.device, c->B);
+  int rc = make_batch(*p, 1, m, &T, &seed, opt.precision, opt.philox_seed, opt.device, c->B,
+                      nullptr, nullptr, opt.ran2_skip);
   if (rc) return rc;
   c->B->noise = opt.noise;
   if (opt.noise == PCG_NOISE_RAN2) {
     // the noise stream continues the one the initial velocities came from
     Ran2 g(seed);
+    for (int64_t k = 0; k < opt.ran2_skip; ++k) g.draw();
     if (p->lang_on)
       for (int k = 0; k < 3 * m->n; ++k) g.draw();
     c->B->ran2.push_back(g);

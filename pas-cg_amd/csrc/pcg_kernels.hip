@@ -13,6 +13,7 @@ namespace pcg {
 // st.V zeroed; NL verify decision (nl::verify, cg/src/cg/nl/verify.cpp:3-34)
 // from the displacement maximum the previous Langevin step left behind.
 __global__ void k_begin(DevBatch b) {
+#pragma clang fp contract(off)
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= b.nsys) return;
   SysState &st = b.st[s];

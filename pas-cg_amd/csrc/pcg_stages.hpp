@@ -162,6 +162,7 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
 __device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd, const SysState &st,
                                             int i, double Fx, double Fy, double Fz,
                                             const double nz[3], double out_r[3]) {
+#pragma clang fp contract(off)
   const DevParams &P = *b.P;
   const double f02 = (double)(3.0f / 16.0f), f12 = (double)(251.0f / 360.0f),
                f32 = (double)(11.0f / 18.0f), f42 = (double)(1.0f / 6.0f),
@@ -201,6 +202,7 @@ __device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd
 
 __device__ __forceinline__ double disp_from_orig(const DevBatch &b, const SysDesc &sd, int i,
                                                  const double r[3]) {
+#pragma clang fp contract(off)
   v3<double> d{__dsub_rn(b.orig_r[0][i], r[0]), __dsub_rn(b.orig_r[1][i], r[1]),
                __dsub_rn(b.orig_r[2][i], r[2])};
   return xnorm3(xwrapd(d, sd));

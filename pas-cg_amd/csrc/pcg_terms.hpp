@@ -110,6 +110,7 @@ __device__ double pauli_item(const DevBatch &b, const SysDesc &s, const G &g, A 
 template <typename R, typename G, typename A>
 __device__ double nat_cont_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
                                 A &acc, int k, double t) {
+#pragma clang fp contract(off)
   const pcg_params &p = b.P->p;
   int c = b.ncur[k];
   int i1 = b.nc_i[0][c], i2 = b.nc_i[1][c];
@@ -445,6 +446,7 @@ __device__ void prepare_nh_item(const DevBatch &b, const G &g, int i) {
 }
 
 __device__ __forceinline__ double xdot(double ax, double ay, double az, v3<double> u) {
+#pragma clang fp contract(off)
   return __dadd_rn(__dadd_rn(__dmul_rn(ax, u.x), __dmul_rn(ay, u.y)), __dmul_rn(az, u.z));
 }
 
@@ -454,6 +456,7 @@ __device__ __forceinline__ double xdot(double ax, double ay, double az, v3<doubl
 template <typename R, typename G, typename A>
 __device__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
                                A &acc, int k) {
+#pragma clang fp contract(off)
   const DevParams &P = *b.P;
   const pcg_params &p = P.p;
   int slot = s.qf0 + k;
@@ -524,6 +527,7 @@ __device__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st
 template <typename R, typename G, typename A>
 __device__ double qa_cont_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
                                A &acc, int k) {
+#pragma clang fp contract(off)
   const DevParams &P = *b.P;
   const pcg_params &p = P.p;
   int slot = s.qc0 + k;

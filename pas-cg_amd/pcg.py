@@ -75,7 +75,7 @@ class PcgModel(C.Structure):
 
 class PcgOptions(C.Structure):
     _fields_ = [("precision", I), ("noise", I), ("seed", I), ("philox_seed", C.c_uint64),
-                ("temperature", D), ("device", I)]
+                ("temperature", D), ("device", I), ("ran2_skip", I64)]
 
 
 class PcgBatchOptions(C.Structure):
@@ -227,12 +227,12 @@ class Context:
 
     def __init__(self, params: dict, model: dict, precision=MIXED, noise=PHILOX,
                  seed: int = 448, philox_seed: int = 1, temperature: float = -1.0,
-                 device: int = 0):
+                 device: int = 0, ran2_skip: int = 0):
         L = lib()
         self.n = int(model["n"])
         self._m = _ModelRef(model)
         self._p = fill_struct(PcgParams(), params)
-        o = PcgOptions(precision, noise, seed, philox_seed, temperature, device)
+        o = PcgOptions(precision, noise, seed, philox_seed, temperature, device, ran2_skip)
         h = C.c_void_p()
         check(L.pcg_create(C.byref(self._p), C.byref(self._m.s), C.byref(o), C.byref(h)))
         self.h = h

@@ -139,6 +139,8 @@ def lib():
             getattr(L, f).restype = I
             getattr(L, f).argtypes = [P]
         L.ora_sync.argtypes = [P, C.POINTER(C.c_int8)]
+        L.ora_saw_draws.restype = C.c_int64
+        L.ora_saw_draws.argtypes = [P]
         L.ora_static_list.restype = I
         L.ora_static_list.argtypes = [P, I, I, C.POINTER(I), C.POINTER(D), C.POINTER(D),
                                       C.POINTER(I)]
@@ -337,6 +339,9 @@ class Oracle:
 
     def num_qa_contacts(self) -> int:
         return self.L.ora_num_qa_contacts(self.h)
+
+    def saw_draws(self) -> int:
+        return self.L.ora_saw_draws(self.h)
 
     def nl_rebuilds(self) -> int:
         return self.L.ora_nl_rebuilds(self.h)

@@ -56,7 +56,7 @@ CONFIGS = {
     "protein_sink_pid": lambda: (
         _sink(full_params(qa_on=0, nat_cont_on=0, dh_variant=1,
                           dh_perm=4.0 * P.EPS_0 / P.ANGSTROM)), protein(seed=6)),
-    "multichain": lambda: (full_params(qa_on=0, nat_cont_on=0), multichain()),
+    "multichain": lambda: (_sink(full_params(qa_on=0, nat_cont_on=0)), multichain()),
     "multichain_qa": lambda: (full_params(qa_on=1, pid_on=0), multichain(seed=3)),
 }
 
@@ -154,7 +154,7 @@ def test_fp64_ran2_trajectory_matches_golden(name):
             m["orig_pos"] = o.pos0.copy()
     steps = 100
     ref = np.asarray(fx["trace"]["V_fort_7" if name == "start2" else "V_cxx"][:steps])
-    c = G.Context(p, m, precision=G.FP64, noise=G.RAN2, seed=seed)
+    c = G.Context(p, m, precision=G.FP64, noise=G.RAN2, seed=seed, ran2_skip=o.saw_draws())
     tr_o = o.step(steps)
     tr_g = np.zeros(steps)
     for k in range(steps):
