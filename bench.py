@@ -1,0 +1,331 @@
+#!/usr/bin/env python3
+"""Throughput of pas-cg's per-step loop on one MI355X.
+
+A step is one time step of every system of the batch: NL verify (and
+rebuild when the displacement check fails), per-term list updates, all force
+terms, the QA contact bookkeeping and the Langevin Gear-5 predictor-corrector
+(cg/src/cg/simul/thread_main.cpp:373-536). The timed workload is the ragged
+batch: 256 distinct synthetic proteins with N ~ U[30, 600], 10 % of them
+pulled, full force field, forces in FP32 and integration in FP64, advanced by
+the persistent workgroup-per-system kernel. Beside it, on the same workload:
+
+* the plain single-protein path (one protein at a time, hipGraph-captured
+  stage kernels) on a systematic sample of the proteins;
+* the max relative force error per term of the mixed-precision kernels
+  against the FP64 GPU path on the same positions;
+* a served Poisson stream (continuous batching, HBM-budget admission) with
+  p50/p99 job latency;
+* the FP64 CPU restatement of the loop on a bounded sample (1 core).
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import pcg_import  # noqa: E402
+
+pcg_import.load()
+from pas_cg_amd import model as M  # noqa: E402
+from pas_cg_amd import workloads as W  # noqa: E402
+
+METRIC = "residue-steps/s batched vs plain @force err≤1e-5 vs FP64; p50/p99 job latency"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_persistent_ragged256_traffic.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(spl: int):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per step, copy-kernel calibrated)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    for ln in open(TRAFFIC_FILE):
+        if ":" in ln and not ln.startswith("#"):
+            k, v = ln.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_step"]) * spl
+    except (KeyError, ValueError):
+        return None
+
+
+def batched(p, models, steps, warmup, spl, block):
+    from pas_cg_amd import pcg as G
+    n_res = sum(int(m["n"]) for m in models)
+    b = G.Batch(p, models, precision=G.MIXED, philox_seed=448, block=block,
+                steps_per_launch=spl)
+    info = b.info()
+    spl = info["steps_per_launch"]
+    warmup = max(spl, (warmup + spl - 1) // spl * spl)
+    steps = max(spl, (steps + spl - 1) // spl * spl)
+    log(f"batched: {len(models)} systems, {n_res} residues, block {info['block']}, "
+        f"{spl} steps/launch, {info['bytes'] / 2**20:.1f} MiB; warmup {warmup}")
+    b.run(warmup)
+    reb0 = [b.sizes(s)["rebuilds"] for s in range(len(models))]
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    ms, launches = b.last_ms()
+    sizes = [b.sizes(s) for s in range(len(models))]
+    step_bytes = sum(W.step_bytes(p, sz, (sz["rebuilds"] - r0) / steps)
+                     for sz, r0 in zip(sizes, reb0))
+    Vs = [b.energy(s)[0] for s in range(len(models))]
+    if not all(math.isfinite(v) for v in Vs):
+        raise RuntimeError("non-finite energy in the batched run")
+    log(f"batched: {steps} steps in {wall:.3f} s wall, {ms:.1f} ms in {launches} launches")
+    return {"b": b, "steps": steps, "warmup": warmup, "wall": wall, "ms": ms,
+            "launches": launches, "spl": spl, "block": info["block"], "n_res": n_res,
+            "step_bytes": step_bytes, "sizes": sizes,
+            "nl_pairs": sum(sz["nl"] for sz in sizes)}
+
+
+def force_error(p, models, bat, picks):
+    """Mixed-precision per-term force and energy error against the FP64 GPU
+    path, on the batch's current positions."""
+    from pas_cg_amd import pcg as G
+    worst_f, worst_v = {}, {}
+    for s in picks:
+        r = bat["b"].get(s, 0)
+        ctx = {}
+        for prec in (G.FP64, G.MIXED):
+            c = G.Context(p, models[s], precision=prec, noise=G.PHILOX, seed=s)
+            c.set(0, r)
+            c.fix_nl()
+            ctx[prec] = c
+        for t in G.TERMS:
+            if t.startswith("qa"):
+                continue
+            F64, V64 = ctx[G.FP64].eval_term(t)
+            F32, V32 = ctx[G.MIXED].eval_term(t)
+            scale = np.abs(F64).max()
+            if scale == 0:
+                continue
+            worst_f[t] = max(worst_f.get(t, 0.0), float(np.abs(F32 - F64).max() / scale))
+            if V64 != 0:
+                worst_v[t] = max(worst_v.get(t, 0.0), abs(V32 - V64) / abs(V64))
+        for c in ctx.values():
+            c.close()
+    return worst_f, worst_v
+
+
+def plain(p, models, picks, steps=200):
+    from pas_cg_amd import pcg as G
+    rs, tt, per = 0, 0.0, []
+    for s in picks:
+        c = G.Context(p, models[s], precision=G.MIXED, noise=G.PHILOX, seed=s)
+        c.use_graph(True)
+        c.step(20)
+        t0 = time.perf_counter()
+        c.step(steps)
+        dt = time.perf_counter() - t0
+        n = int(models[s]["n"])
+        rs += n * steps
+        tt += dt
+        per.append((n, dt / steps))
+        c.close()
+    log(f"plain: {len(picks)} proteins, {rs / tt:.3e} residue-steps/s")
+    return rs / tt, per
+
+
+def fifo_latency(jobs, per):
+    """One-job-at-a-time plain GPU path as a FIFO queue, with each job's
+    service time from the plain per-step times measured above (linear in N)."""
+    ns = np.array([n for n, _ in per], float)
+    ts = np.array([t for _, t in per], float)
+    a, b0 = np.polyfit(ns, ts, 1)
+    free = 0.0
+    lat = []
+    for j in jobs:
+        start = max(free, j["arrival"])
+        free = start + j["steps"] * max(a * j["n"] + b0, ts.min())
+        lat.append(free - j["arrival"])
+    return np.percentile(lat, 50), np.percentile(lat, 99)
+
+
+SERVE_STEPS = (2000, 20000)
+SERVE_POOL = 800
+
+
+def serve_pool():
+    """Jobs of the served stream at unit arrival rate, and their models, made
+    before the GPU is touched (the generator forks worker processes)."""
+    from concurrent.futures import ProcessPoolExecutor
+    jobs = W.c4_jobs(SERVE_POOL, 1.0, *SERVE_STEPS)
+    with ProcessPoolExecutor(min(8, os.cpu_count() or 1)) as ex:
+        models = list(ex.map(W.c4_model, jobs, chunksize=8))
+    return jobs, models
+
+
+def served(p, rate_rs, seconds, pool, per_plain):
+    """Poisson stream at about half the batched capacity for `seconds` of
+    arrivals; continuous batching with HBM-budget admission."""
+    from pas_cg_amd import pcg as G
+    steps_lo, steps_hi = SERVE_STEPS
+    mean_n = 150.0 * math.exp(0.6 ** 2 / 2)
+    mean_steps = (steps_hi - steps_lo) / math.log(steps_hi / steps_lo)
+    rate = 0.5 * rate_rs / (mean_n * mean_steps)
+    count = max(8, min(SERVE_POOL, int(rate * seconds)))
+    jobs = [dict(j, arrival=j["arrival"] / rate) for j in pool[0][:count]]
+    models = pool[1][:count]
+    srv = G.Server(p, precision=G.MIXED, hbm_budget_bytes=64 << 30, max_resident=1024,
+                   chunk_steps=200, philox_seed=448)
+    t0 = time.perf_counter()
+    ids = {}
+    nxt = 0
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < count and jobs[nxt]["arrival"] <= now:
+            j = jobs[nxt]
+            ids[srv.submit(models[nxt], j["steps"], temperature=j["T"], seed=j["id"])] = j
+            nxt += 1
+        active = srv.round()
+        if nxt == count and active == 0:
+            break
+        if active == 0:
+            time.sleep(max(0.0, jobs[nxt]["arrival"] - (time.perf_counter() - t0)))
+    span = time.perf_counter() - t0
+    lat, work, failed = [], 0, 0
+    for jid, j in ids.items():
+        st, done = srv.status(jid)
+        if st != "done":
+            failed += 1
+            continue
+        sub, _start, end = srv.times(jid)
+        lat.append(end - sub)
+        work += j["n"] * j["steps"]
+    srv.close()
+    p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
+    f50, f99 = fifo_latency(jobs, per_plain)
+    log(f"served: {count} jobs at {rate:.1f}/s, {work / span:.3e} residue-steps/s, "
+        f"p50 {p50:.3f} s p99 {p99:.3f} s")
+    return {"jobs": count, "arrival_rate_per_s": round(rate, 2),
+            "steps_range": [steps_lo, steps_hi], "failed": failed,
+            "residue_steps_per_s": work / span, "p50_latency_s": round(float(p50), 4),
+            "p99_latency_s": round(float(p99), 4),
+            "plain_fifo_p50_latency_s": round(float(f50), 4),
+            "plain_fifo_p99_latency_s": round(float(f99), 4)}
+
+
+def cpu_baseline(p, models, picks, budget_s=15.0):
+    """The FP64 CPU restatement of the loop (oracle/), one core, on a bounded
+    sample of the same proteins."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from oracle_ffi import Oracle
+    ors = [Oracle(p, models[s], seed=s) for s in picks]
+    for o in ors:
+        o.step(2)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        for o in ors:
+            o.step(20)
+        steps += 20
+        dt = time.perf_counter() - t0
+        if dt >= budget_s:
+            break
+    rs = sum(int(models[s]["n"]) for s in picks) * steps
+    ns = [int(models[s]["n"]) for s in picks]
+    log(f"cpu: {rs / dt:.3e} residue-steps/s over {dt:.1f} s")
+    return {"value": rs / dt, "unit": "residue-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{len(picks)} proteins of the ragged workload (N={ns}), "
+                      f"{steps} steps each after 2 warm-up steps, FP64, one thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps-per-launch", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--pad", type=float, default=2.0, help="NL pad (skin), A, with --no-tune")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (pad and block from the flags)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} (world size {world}) is not "
+              "supported: the workload's systems are independent and would shard by job, "
+              "with no exchange during stepping", file=sys.stderr)
+        sys.exit(2)
+
+    p, models = W.c3()
+    pool = None if (args.no_serve or args.no_plain) else serve_pool()
+    tuning = None
+    if args.no_tune:
+        pad, block = args.pad, args.block
+    else:
+        from pas_cg_amd import tune
+        tuning = tune.tune(p, models, log=log)
+        pad, block = tuning["pad_A"], tuning["block"]
+    args.pad = pad
+    p.update(nl_pad=pad * W.A)
+    bat = batched(p, models, args.steps, args.warmup, args.steps_per_launch, block)
+    value = bat["n_res"] * bat["steps"] / bat["wall"]
+    avg_launch_s = bat["ms"] * 1e-3 / bat["launches"]
+    bytes_launch = bat["step_bytes"] * bat["spl"]
+    achieved = bytes_launch / avg_launch_s / 1e9
+    order = np.argsort([int(m["n"]) for m in models])
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "residue-steps/s",
+           "n_gpus": 1, "steps": bat["steps"], "warmup": bat["warmup"],
+           "ms_per_step": round(bat["wall"] * 1e3 / bat["steps"], 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 forces, f64 integration", "data": "synthetic",
+           "config": {"workload": "ragged-256: 256 proteins, N~U[30,600], 10% pulled, "
+                                  "full force field, 1 persistent workgroup per protein",
+                      "residues": bat["n_res"], "block": bat["block"],
+                      "steps_per_launch": bat["spl"], "nl_pad_A": args.pad,
+                      "nl_pairs": bat["nl_pairs"],
+                      "autotune": tuning and tuning["table"]},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": read_traffic(bat["spl"]),
+                        "kernel": "k_persistent",
+                        "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                        "algorithmic_bytes_per_launch": round(bytes_launch)}}
+    if not args.no_check:
+        picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+        ef, ev = force_error(p, models, bat, picks)
+        out["force_err_vs_fp64"] = {"max": max(ef.values()),
+                                    "per_term": {k: float(f"{v:.3g}") for k, v in ef.items()},
+                                    "energy_per_term": {k: float(f"{v:.3g}")
+                                                        for k, v in ev.items()}}
+    per_plain = None
+    if not args.no_plain:
+        prate, per_plain = plain(p, models, [int(i) for i in order[8::16]])
+        out["plain"] = {"value": round(prate, 1), "unit": "residue-steps/s",
+                        "sample": "16 proteins (every 16th by length), one at a time, "
+                                  "hipGraph-captured step, 200 steps each"}
+        out["speedup_vs_plain"] = round(value / prate, 2)
+    bat["b"].close()
+    if not args.no_serve and per_plain is not None:
+        out["served"] = served(W.full_params(args.pad), value, args.serve_seconds, pool,
+                               per_plain)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, models, [int(i) for i in order[16::32]])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

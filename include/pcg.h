@@ -225,6 +225,15 @@ int pcg_batch_get(pcg_batch *b, int32_t sys, int32_t which, double *out);
 int pcg_batch_energy(pcg_batch *b, int32_t sys, double *V, double *term_V);
 int pcg_batch_info(pcg_batch *b, int32_t *block, int32_t *steps_per_launch,
                    int64_t *device_bytes);
+/* current list sizes of one system: [0, PCG_NUM_TERMS) per term (its
+ * total_size()), then the NL size, the residue count and the number of NL
+ * rebuilds so far */
+int pcg_batch_sizes(pcg_batch *b, int32_t sys, int32_t *sizes /*[PCG_NUM_TERMS + 3]*/);
+/* Autotuner: times `trial_steps` steps of the persistent kernel at each
+ * workgroup size (128, 256, 512, 1024 threads) on the live state, restoring
+ * the state after every trial, and keeps the fastest. Returns the choice and
+ * its time per step (ms). */
+int pcg_batch_autotune(pcg_batch *b, int32_t trial_steps, int32_t *block, double *ms_per_step);
 /* HIP events around the last pcg_batch_run's launches, milliseconds */
 int pcg_batch_last_ms(pcg_batch *b, double *ms, int32_t *launches);
 

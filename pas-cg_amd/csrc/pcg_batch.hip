@@ -227,6 +227,7 @@ __device__ void wg_fix_nl(const DevBatch &b, const SysDesc &sd, SysState &st, co
     st.rebuilds += 1;
   }
   __syncthreads();
+  if (b.P->p.qa_on) build_qa_near(b, sd, st);
 }
 
 template <typename R, typename RF>
@@ -253,10 +254,13 @@ __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
   AccL<RF> acc{fx, fy, fz, r0};
   __syncthreads();
 
+  __shared__ int want_v;
+  bool evaluated = false;
   for (int it = 0; it < nsteps; ++it) {
-    bool want_v = (it == nsteps - 1);
     if (threadIdx.x == 0) {
       stop = st.step >= sd.steps_target;
+      // energies on the last step this launch makes for the system
+      want_v = (it == nsteps - 1) || (st.step + 1 >= sd.steps_target);
       // nl::verify (cg/src/cg/nl/verify.cpp:3-34)
       if (!stop && P.nl_required) {
         if (st.verify_first) {
@@ -272,13 +276,15 @@ __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
       flag_rebuild = st.nl_invalid;
       for (int k = 0; k < NT; ++k) vacc[k] = 0.0;
     }
+    __syncthreads();
+    // a finished system keeps the forces of its last evaluation
+    if (stop) break;
+    evaluated = true;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
       fx[k] = 0;
       fy[k] = 0;
       fz[k] = 0;
     }
-    __syncthreads();
-    if (stop) break;
     if (P.p.qa_on)
       for (int k = threadIdx.x; k < n; k += blockDim.x) prepare_nh_item(b, g, r0 + k);
     if (flag_rebuild) wg_fix_nl(b, sd, st, g);
@@ -336,6 +342,7 @@ __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
     __syncthreads();
   }
   // forces of the last evaluation back to HBM
+  if (!evaluated) return;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     ((RF *)b.F[0])[r0 + k] = fx[k];
     ((RF *)b.F[1])[r0 + k] = fy[k];

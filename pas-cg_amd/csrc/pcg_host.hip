@@ -502,6 +502,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.cand_perm, nl);
   A.add(&D.qf_d, qf);
   A.add(&D.qf_vac, qf);
+  A.add(&D.qh, qf);
   A.add(&D.qc_d, qc);
   A.add(&D.qc_t, qc);
   A.add(&D.qc_sat, qc);
@@ -874,13 +875,7 @@ int pcg_eval_forces(pcg_ctx *c) {
   return check_overflow(B);
 }
 
-int pcg_term_size(pcg_ctx *c, int32_t term, int32_t *size) {
-  if (!c || !size || term < 0 || term >= NT) return fail(PCG_E_ARG, "bad arguments");
-  DeviceBatch &B = *c->B;
-  SysState st;
-  int rc = read_state(B, 0, st);
-  if (rc) return rc;
-  const SysDesc &s = B.sys[0];
+static int host_term_size(const SysDesc &s, const SysState &st, int term) {
   int v = 0;
   switch (term) {
   case PCG_CHIR: v = s.nchir; break;
@@ -889,7 +884,7 @@ int pcg_term_size(pcg_ctx *c, int32_t term, int32_t *size) {
   case PCG_NAT_CONT: v = st.nc_n; break;
   case PCG_PAULI: v = st.pauli_n; break;
   case PCG_CONST_DH: case PCG_REL_DH: v = st.dh_n; break;
-  case PCG_QA_CAND: v = st.qf_n; break;
+  case PCG_QA_CAND: v = st.qh_n; break;
   case PCG_QA_CONT: v = std::min(st.qc_n, st.qf_n); break;
   case PCG_PID: v = st.pid_n; break;
   case PCG_NAT_ANG: v = s.nnang; break;
@@ -898,7 +893,16 @@ int pcg_term_size(pcg_ctx *c, int32_t term, int32_t *size) {
   case PCG_CND: case PCG_SND: v = s.nndih; break;
   case PCG_VEL_AFM: v = s.ntip; break;
   }
-  *size = v;
+  return v;
+}
+
+int pcg_term_size(pcg_ctx *c, int32_t term, int32_t *size) {
+  if (!c || !size || term < 0 || term >= NT) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  SysState st;
+  int rc = read_state(B, 0, st);
+  if (rc) return rc;
+  *size = host_term_size(B.sys[0], st, term);
   return PCG_OK;
 }
 
@@ -1197,6 +1201,9 @@ int pcg_batch_create(const pcg_params *p, int32_t nsys, const pcg_model *models,
   b->lds = lds_for(B);
   // systems whose positions and forces do not fit in LDS take the stage kernels
   b->persistent = b->lds <= 150 * 1024;
+  if (opt.block != 0 && (opt.block < 64 || opt.block > 1024 || opt.block % 64 != 0))
+    return fail(PCG_E_ARG, "block must be a multiple of 64 in [64, 1024]");
+  if (opt.steps_per_launch < 0) return fail(PCG_E_ARG, "steps_per_launch must be >= 0");
   b->block = opt.block;
   if (b->block <= 0) {
     int want = 64;
@@ -1266,6 +1273,69 @@ int pcg_batch_info(pcg_batch *b, int32_t *block, int32_t *spl, int64_t *bytes) {
   if (block) *block = b->persistent ? b->block : 0;
   if (spl) *spl = b->spl;
   if (bytes) *bytes = (int64_t)b->B->bytes;
+  return PCG_OK;
+}
+
+int pcg_batch_autotune(pcg_batch *b, int32_t trial_steps, int32_t *block, double *ms_per_step) {
+  if (!b || trial_steps <= 0) return fail(PCG_E_ARG, "bad arguments");
+  HIPCHK(hipSetDevice(b->B->device));
+  DeviceBatch &B = *b->B;
+  if (!b->persistent) {  // stage kernels: fixed launch shapes, nothing to tune
+    if (block) *block = 0;
+    if (ms_per_step) *ms_per_step = 0.0;
+    return PCG_OK;
+  }
+  // trials run on the live state, which is restored after each one, so the
+  // trajectory does not depend on whether (or how) the batch was tuned
+  size_t total = B.arena.total ? B.arena.total : 256;
+  void *snap = nullptr;
+  if (hipMalloc(&snap, total) != hipSuccess)
+    return fail(PCG_E_NOMEM, "no device memory for the autotuner's state snapshot");
+  int rc = PCG_OK;
+  int spl0 = b->spl, block0 = b->block;
+  b->spl = std::min(b->spl, (int)trial_steps);
+  const int cands[] = {128, 256, 512, 1024};
+  int best = block0;
+  double best_ms = 1e300;
+  if (hipMemcpyAsync(snap, B.arena.base, total, hipMemcpyDeviceToDevice, B.stream) != hipSuccess)
+    rc = fail(PCG_E_HIP, "snapshot copy failed");
+  for (int pass = 0; pass < 2 && rc == PCG_OK; ++pass) {
+    for (int c : cands) {
+      if (pass == 0 && c != cands[0]) continue;  // pass 0 warms up the code objects
+      b->block = c;
+      if ((rc = run_persistent(b, trial_steps)) != PCG_OK) break;
+      double ms = b->last_ms;
+      if (hipMemcpyAsync(B.arena.base, snap, total, hipMemcpyDeviceToDevice, B.stream) != hipSuccess ||
+          hipStreamSynchronize(B.stream) != hipSuccess) {
+        rc = fail(PCG_E_HIP, "snapshot restore failed");
+        break;
+      }
+      if (pass == 1 && ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  }
+  (void)hipFree(snap);
+  b->spl = spl0;
+  b->block = rc == PCG_OK ? best : block0;
+  if (rc) return rc;
+  b->tuned = true;
+  if (block) *block = best;
+  if (ms_per_step) *ms_per_step = best_ms / trial_steps;
+  return PCG_OK;
+}
+
+int pcg_batch_sizes(pcg_batch *b, int32_t sys, int32_t *sizes) {
+  if (!b || !sizes || sys < 0 || sys >= (int)b->B->sys.size()) return fail(PCG_E_ARG, "bad arguments");
+  SysState st;
+  int rc = read_state(*b->B, sys, st);
+  if (rc) return rc;
+  const SysDesc &d = b->B->sys[sys];
+  for (int t = 0; t < NT; ++t) sizes[t] = host_term_size(d, st, t);
+  sizes[NT] = st.nl_n;
+  sizes[NT + 1] = d.n;
+  sizes[NT + 2] = st.rebuilds;
   return PCG_OK;
 }
 

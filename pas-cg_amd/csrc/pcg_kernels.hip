@@ -276,6 +276,14 @@ __global__ void k_part_write(DevBatch b) {
   }
 }
 
+// QA free pairs within the maximum required distance (one block per system).
+__global__ void k_qa_near(DevBatch b) {
+  int s = blockIdx.x;
+  SysState &st = b.st[s];
+  if (!st.active || !st.nl_invalid || !b.P->p.qa_on) return;
+  build_qa_near(b, b.sys[s], st);
+}
+
 // The rebuild is complete: record it and clear nl_invalid.
 __global__ void k_nl_done(DevBatch b) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -386,6 +394,7 @@ void launch_fix_nl(const DevBatch &b, int max_nlcap, hipStream_t s) {
   hipLaunchKernelGGL(k_part_count, pg, dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_part_scan, grid1(b.nsys, 64), dim3(64), 0, s, b);
   hipLaunchKernelGGL(k_part_write, pg, dim3(BLK), 0, s, b);
+  hipLaunchKernelGGL(k_qa_near, dim3(b.nsys), dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_nl_done, grid1(b.nsys, 64), dim3(64), 0, s, b);
 }
 void launch_terms(const DevBatch &b, int prec, int max_items, int only_term, int from, int to,

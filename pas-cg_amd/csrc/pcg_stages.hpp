@@ -46,19 +46,59 @@ __device__ inline int classify(const DevBatch &b, const SysDesc &s, const SysSta
 // free-pair index), sequential acceptance against the sync budgets, and
 // vect::set::compress, which moves items but not vacancy flags
 // (cg/include/cg/vect/ind/set.h:62-71).
+// IEEE-754 totalOrder as a signed integer key: equal to the numeric order
+// on the non-negative distances ranked below, and still a strict total order
+// when a diverged system produces NaN, so the ranks always form a permutation
+__device__ __forceinline__ long long total_key(double d) {
+  long long k = __double_as_longlong(d);
+  return k ^ ((k >> 63) & 0x7fffffffffffffffLL);
+}
+
+// The QA free pairs that can become candidates, in slot order (an ordered
+// compaction of the free list by !(orig_dist > max_req_dist), the test that
+// qa_cand_item makes first). Run by one workgroup per system after the free
+// list is rebuilt.
+__device__ inline void build_qa_near(const DevBatch &b, const SysDesc &sd, SysState &st) {
+  __shared__ int wc[16];
+  __shared__ int run;
+  if (threadIdx.x == 0) run = 0;
+  __syncthreads();
+  const double req = b.P->qa_max_req;
+  const int n = st.qf_n;
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int base = 0; base < n; base += blockDim.x) {
+    int k = base + threadIdx.x;
+    bool near = k < n && !(b.qf_d[sd.qf0 + k] > req);
+    unsigned long long m = __ballot(near);
+    if (lane == 0) wc[w] = __popcll(m);
+    __syncthreads();
+    int off = run, tot = 0;
+    for (int q = 0; q < nw; ++q) {
+      if (q < w) off += wc[q];
+      tot += wc[q];
+    }
+    if (near) b.qh[sd.qf0 + off + __popcll(m & ((1ull << lane) - 1ull))] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) run += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) st.qh_n = run;
+  __syncthreads();
+}
+
 __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, SysState &st) {
   int rem_n = st.rem_n < sd.qccap ? st.rem_n : sd.qccap;
   int cand_n = st.cand_n < sd.nlcap ? st.cand_n : sd.nlcap;
   // rank of every candidate by (dist, free-pair index) -> permutation
   for (int c = threadIdx.x; c < cand_n; c += blockDim.x) {
     int sc = sd.nl0 + c;
-    double dc = b.cand_dist[sc];
+    long long kc = total_key(b.cand_dist[sc]);
     int ic = b.cand_idx[sc];
     int rank = 0;
     for (int o = 0; o < cand_n; ++o) {
       int so = sd.nl0 + o;
-      double dd = b.cand_dist[so];
-      rank += (dd < dc || (dd == dc && b.cand_idx[so] < ic)) ? 1 : 0;
+      long long kd = total_key(b.cand_dist[so]);
+      rank += (kd < kc || (kd == kc && b.cand_idx[so] < ic)) ? 1 : 0;
     }
     b.cand_perm[sd.nl0 + rank] = c;
   }
@@ -87,6 +127,8 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
         b.qf_i[1][f] = i2;
         b.qf_d[f] = b.qc_d[q];
         b.qf_vac[f] = 0;
+        // appended at the end, so the near list stays in slot order
+        if (!(b.qf_d[f] > b.P->qa_max_req)) b.qh[sd.qf0 + st.qh_n++] = st.qf_n;
         st.qf_n += 1;
       } else {
         st.overflow = 1;

@@ -535,6 +535,7 @@ __device__ double qa_cont_item(const DevBatch &b, const SysDesc &s, SysState &st
   int i1 = b.qc_i[0][slot], i2 = b.qc_i[1][slot];
   int type = b.qc_t[slot];
   double sat = b.qc_sat[slot];
+  const double sat_used = sat;
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
   double n2 = __dadd_rn(__dadd_rn(__dmul_rn(d.x, d.x), __dmul_rn(d.y, d.y)), __dmul_rn(d.z, d.z));
   double rn = __ddiv_rn(1.0, __dsqrt_rn(n2));
@@ -560,7 +561,8 @@ __device__ double qa_cont_item(const DevBatch &b, const SysDesc &s, SysState &st
     sat = fmax(__dadd_rn(sat, step), 0.0);
   }
   b.qc_sat[slot] = sat;
-  return (double)sat * (double)Vij;
+  // the energy carries the saturation the force used (qa/process_contacts.cpp:40)
+  return (double)sat_used * (double)Vij;
 }
 
 } // namespace pcg

@@ -119,37 +119,38 @@ def _compact_saw(n: int, rng: np.random.Generator, bond: float, excl: float,
     of mass (a Rosenbluth-style biased growth): among `trials` random bond
     directions giving a C-alpha bond angle between 85 and 150 degrees (so that
     i, i+2 stay beyond 5 A), the one closest to the running centroid that keeps
-    every |i-j| >= 3 pair farther than `excl` is taken. Restarts from scratch
-    if the chain traps itself."""
-    for _attempt in range(200):
-        pos = np.zeros((n, 3))
-        ok = True
-        for i in range(1, n):
-            cen = pos[:i].mean(axis=0)
-            dirs = rng.normal(size=(trials, 3))
-            dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
-            if i >= 2:
-                prev = pos[i - 1] - pos[i - 2]
-                prev /= np.linalg.norm(prev)
-                c = dirs @ prev  # = -cos(bond angle)
-                keep = (c > -0.0872) & (c < 0.866)
-                dirs = dirs[keep]
-            if len(dirs) == 0:
-                ok = False
-                break
-            cand = pos[i - 1] + bond * dirs
-            if i >= 3:
-                d = np.linalg.norm(cand[:, None, :] - pos[None, :i - 2, :], axis=2)
-                good = (d > excl).all(axis=1)
-                cand = cand[good]
-            if len(cand) == 0:
-                ok = False
-                break
-            k = int(np.argmin(np.linalg.norm(cand - cen, axis=1)))
-            pos[i] = cand[k]
-        if ok:
-            return pos
-    raise RuntimeError("could not grow a self-avoiding chain")
+    every |i-j| >= 3 pair farther than `excl` is taken. When the chain traps
+    itself it backs up a few residues (more after repeated traps) and regrows."""
+    pos = np.zeros((n, 3))
+    csum = np.zeros(3)
+    i, fails = 1, 0
+    while i < n:
+        cen = csum / i
+        dirs = rng.normal(size=(trials, 3))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        if i >= 2:
+            prev = pos[i - 1] - pos[i - 2]
+            prev /= np.linalg.norm(prev)
+            c = dirs @ prev  # = -cos(bond angle)
+            dirs = dirs[(c > -0.0872) & (c < 0.866)]
+        cand = pos[i - 1] + bond * dirs
+        if i >= 3 and len(cand):
+            d = np.linalg.norm(cand[:, None, :] - pos[None, :i - 2, :], axis=2)
+            cand = cand[(d > excl).all(axis=1)]
+        if len(cand) == 0:
+            fails += 1
+            if fails > 50 * n:
+                raise RuntimeError("could not grow a self-avoiding chain")
+            back = min(i - 1, 2 + int(rng.integers(0, 4 + fails // 8)))
+            for k in range(i - back, i):
+                csum -= pos[k]
+            i -= back
+            continue
+        k = int(np.argmin(np.linalg.norm(cand - cen, axis=1)))
+        pos[i] = cand[k]
+        csum += pos[i]
+        i += 1
+    return pos
 
 
 def synthetic_protein(n: int, seed: int, native_cutoff_A: float = 7.5,

@@ -92,6 +92,54 @@ class PcgError(RuntimeError):
     pass
 
 
+_V = C.c_void_p
+# argument types of every entry point of include/pcg.h (all return int
+# status codes except pcg_last_error, which returns the message)
+SIGNATURES = {
+    "pcg_last_error": [],
+    "pcg_device_count": [C.POINTER(C.c_int)],
+    "pcg_create": [C.POINTER(PcgParams), C.POINTER(PcgModel), C.POINTER(PcgOptions),
+                   C.POINTER(_V)],
+    "pcg_destroy": [_V], "pcg_step": [_V, I],
+    "pcg_nl_verify": [_V, C.POINTER(I)], "pcg_fix_nl": [_V], "pcg_prepare_nh": [_V],
+    "pcg_eval_forces": [_V],
+    "pcg_eval_term": [_V, I, I, I, C.POINTER(D), C.POINTER(D)],
+    "pcg_term_size": [_V, I, C.POINTER(I)],
+    "pcg_qa_finish": [_V], "pcg_lang_step": [_V],
+    "pcg_get": [_V, I, C.POINTER(D)], "pcg_set": [_V, I, C.POINTER(D)],
+    "pcg_get_energy": [_V, C.POINTER(D), C.POINTER(D)],
+    "pcg_get_time": [_V, C.POINTER(D), C.POINTER(I64)],
+    "pcg_set_time": [_V, D], "pcg_set_noise_scale": [_V, D],
+    "pcg_get_list": [_V, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I),
+                     C.POINTER(D), C.POINTER(U8)],
+    "pcg_nat_cont_state": [_V, C.POINTER(U8), C.POINTER(D), C.POINTER(I)],
+    "pcg_qa_state": [_V, C.POINTER(C.c_int8), C.POINTER(I)],
+    "pcg_nl_rebuilds": [_V, C.POINTER(I)],
+    "pcg_use_graph": [_V, I],
+    "pcg_stream": [_V, C.POINTER(_V)],
+    "pcg_batch_create": [C.POINTER(PcgParams), I, C.POINTER(PcgModel), C.POINTER(D),
+                         C.POINTER(I), C.POINTER(PcgBatchOptions), C.POINTER(_V)],
+    "pcg_batch_destroy": [_V], "pcg_batch_run": [_V, I], "pcg_batch_sync": [_V],
+    "pcg_batch_get": [_V, I, I, C.POINTER(D)],
+    "pcg_batch_energy": [_V, I, C.POINTER(D), C.POINTER(D)],
+    "pcg_batch_info": [_V, C.POINTER(I), C.POINTER(I), C.POINTER(I64)],
+    "pcg_batch_last_ms": [_V, C.POINTER(D), C.POINTER(I)],
+    "pcg_server_create": [C.POINTER(PcgParams), C.POINTER(PcgServerOptions),
+                          C.POINTER(_V)],
+    "pcg_server_destroy": [_V],
+    "pcg_server_submit": [_V, C.POINTER(PcgModel), D, I, I64, C.POINTER(I64)],
+    "pcg_server_round": [_V, C.POINTER(I)],
+    "pcg_server_status": [_V, I64, C.POINTER(I), C.POINTER(I64)],
+    "pcg_server_result": [_V, I64, C.POINTER(D), C.POINTER(D)],
+    "pcg_server_cancel": [_V, I64],
+    "pcg_server_stats": [_V, C.POINTER(I64), C.POINTER(I), C.POINTER(I)],
+    "pcg_server_job_times": [_V, I64, C.POINTER(D), C.POINTER(D), C.POINTER(D)],
+    "pcg_batch_set_noise_scale": [_V, D],
+    "pcg_batch_sizes": [_V, I, C.POINTER(I)],
+    "pcg_batch_autotune": [_V, I, C.POINTER(I), C.POINTER(D)],
+}
+
+
 _lib = None
 
 
@@ -101,48 +149,11 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise PcgError(f"HIP library not built: {LIB_PATH} (run pas-cg_amd/build.py)")
         L = C.CDLL(LIB_PATH)
-        V = C.c_void_p
         L.pcg_last_error.restype = C.c_char_p
-        sigs = {
-            "pcg_device_count": [C.POINTER(C.c_int)],
-            "pcg_create": [C.POINTER(PcgParams), C.POINTER(PcgModel), C.POINTER(PcgOptions),
-                           C.POINTER(V)],
-            "pcg_destroy": [V], "pcg_step": [V, I],
-            "pcg_nl_verify": [V, C.POINTER(I)], "pcg_fix_nl": [V], "pcg_prepare_nh": [V],
-            "pcg_eval_forces": [V],
-            "pcg_eval_term": [V, I, I, I, C.POINTER(D), C.POINTER(D)],
-            "pcg_term_size": [V, I, C.POINTER(I)],
-            "pcg_qa_finish": [V], "pcg_lang_step": [V],
-            "pcg_get": [V, I, C.POINTER(D)], "pcg_set": [V, I, C.POINTER(D)],
-            "pcg_get_energy": [V, C.POINTER(D), C.POINTER(D)],
-            "pcg_get_time": [V, C.POINTER(D), C.POINTER(I64)],
-            "pcg_set_time": [V, D], "pcg_set_noise_scale": [V, D],
-            "pcg_get_list": [V, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I),
-                             C.POINTER(D), C.POINTER(U8)],
-            "pcg_nat_cont_state": [V, C.POINTER(U8), C.POINTER(D), C.POINTER(I)],
-            "pcg_qa_state": [V, C.POINTER(C.c_int8), C.POINTER(I)],
-            "pcg_nl_rebuilds": [V, C.POINTER(I)],
-            "pcg_use_graph": [V, I],
-            "pcg_batch_create": [C.POINTER(PcgParams), I, C.POINTER(PcgModel), C.POINTER(D),
-                                 C.POINTER(I), C.POINTER(PcgBatchOptions), C.POINTER(V)],
-            "pcg_batch_destroy": [V], "pcg_batch_run": [V, I], "pcg_batch_sync": [V],
-            "pcg_batch_get": [V, I, I, C.POINTER(D)],
-            "pcg_batch_energy": [V, I, C.POINTER(D), C.POINTER(D)],
-            "pcg_batch_info": [V, C.POINTER(I), C.POINTER(I), C.POINTER(I64)],
-            "pcg_batch_last_ms": [V, C.POINTER(D), C.POINTER(I)],
-            "pcg_server_create": [C.POINTER(PcgParams), C.POINTER(PcgServerOptions),
-                                  C.POINTER(V)],
-            "pcg_server_destroy": [V],
-            "pcg_server_submit": [V, C.POINTER(PcgModel), D, I, I64, C.POINTER(I64)],
-            "pcg_server_round": [V, C.POINTER(I)],
-            "pcg_server_status": [V, I64, C.POINTER(I), C.POINTER(I64)],
-            "pcg_server_result": [V, I64, C.POINTER(D), C.POINTER(D)],
-            "pcg_server_cancel": [V, I64],
-            "pcg_server_stats": [V, C.POINTER(I64), C.POINTER(I), C.POINTER(I)],
-            "pcg_server_job_times": [V, I64, C.POINTER(D), C.POINTER(D), C.POINTER(D)],
-            "pcg_batch_set_noise_scale": [V, D],
-        }
-        for name, args in sigs.items():
+        L.pcg_last_error.argtypes = []
+        for name, args in SIGNATURES.items():
+            if name == "pcg_last_error":
+                continue
             f = getattr(L, name)
             f.argtypes = args
             f.restype = C.c_int
@@ -341,6 +352,12 @@ class Context:
         check(self.L.pcg_qa_state(self.h, _p(sync, C.c_int8), C.byref(num)))
         return sync, num.value
 
+    def stream(self) -> int:
+        """The HIP stream every launch of this context is enqueued on."""
+        st = C.c_void_p()
+        check(self.L.pcg_stream(self.h, C.byref(st)))
+        return st.value or 0
+
     def nl_rebuilds(self) -> int:
         v = C.c_int32()
         check(self.L.pcg_nl_rebuilds(self.h, C.byref(v)))
@@ -381,6 +398,16 @@ class Batch:
     def run(self, nsteps: int):
         check(self.L.pcg_batch_run(self.h, nsteps))
 
+    def sync(self):
+        check(self.L.pcg_batch_sync(self.h))
+
+    def autotune(self, trial_steps: int = 100):
+        """Fastest workgroup size for this batch (state left unchanged)."""
+        blk = C.c_int32()
+        ms = C.c_double()
+        check(self.L.pcg_batch_autotune(self.h, trial_steps, C.byref(blk), C.byref(ms)))
+        return blk.value, ms.value
+
     def get(self, sys: int, which: int) -> np.ndarray:
         out = np.zeros((self.ns[sys], 3))
         check(self.L.pcg_batch_get(self.h, sys, which, _p(out, D)))
@@ -407,6 +434,15 @@ class Batch:
 
     def set_noise_scale(self, s: float):
         check(self.L.pcg_batch_set_noise_scale(self.h, s))
+
+    def sizes(self, sys: int) -> dict:
+        out = np.zeros(len(TERMS) + 3, np.int32)
+        check(self.L.pcg_batch_sizes(self.h, sys, _p(out, I)))
+        d = dict(zip(TERMS, (int(x) for x in out[:len(TERMS)])))
+        d["nl"] = int(out[len(TERMS)])
+        d["n"] = int(out[len(TERMS) + 1])
+        d["rebuilds"] = int(out[len(TERMS) + 2])
+        return d
 
 
 JOB_STATUS = {0: "queued", 1: "running", 2: "done", 3: "cancelled", 4: "failed"}

@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every entry point the public header
+declares (no compute: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from pas_cg_amd import pcg as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "pcg.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w ]+\*?\s*\b(pcg_\w+)\s*\(", src, re.M)))
+
+
+def test_header_declares_the_interface():
+    names = declared()
+    for must in ("pcg_create", "pcg_step", "pcg_eval_term", "pcg_batch_run",
+                 "pcg_server_submit", "pcg_last_error"):
+        assert must in names
+    assert len(names) >= 40
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(G.LIB_PATH):
+        pytest.skip("libpcg.so not built")
+    lib = ctypes.CDLL(G.LIB_PATH)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    if not os.path.exists(G.LIB_PATH):
+        pytest.skip("libpcg.so not built")
+    L = G.lib()
+    bound = set(G.SIGNATURES)
+    assert set(declared()) <= bound, sorted(set(declared()) - bound)
+    assert L is G.lib()
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(G, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(G, "_lib", None)
+    with pytest.raises(G.PcgError):
+        G.lib()

@@ -1,0 +1,223 @@
+"""GPU parity of the batched (persistent workgroup-per-system) and served
+(continuous batching) forms against the FP64 oracle.
+
+* noise switched off, every system of a ragged batch follows its own oracle
+  trajectory (FP64 to 1e-8 of the box, through NL rebuilds and contact
+  changes); mixed precision forces of the batch match the oracle's to 1e-5;
+* with Philox noise the batch samples the same ensemble as the oracle with
+  the reference's ran2 stream (kinetic temperature, radius of gyration,
+  native-contact fraction) within statistical error;
+* the server gives a job the same result alone and in a mix, retires a job
+  that diverges (FAILED) while the others finish, cancels, and refuses a job
+  larger than its HBM budget.
+"""
+import numpy as np
+import pytest
+
+from oracle_ffi import Oracle
+from pas_cg_amd import model as M
+from pas_cg_amd import params as P
+from pas_cg_amd import pcg as G
+
+pytestmark = pytest.mark.gpu
+
+
+def prm_full(**kw):
+    p = P.default_params()
+    p.update(chir_on=1, afm_on=1)
+    p.update(kw)
+    return p
+
+
+def ragged_models(sizes, seed0=100, pulled=(1,)):
+    out = []
+    for k, n in enumerate(sizes):
+        m = M.synthetic_protein(n, M.job_seed(seed0 + k))
+        rng = np.random.default_rng(seed0 + k)
+        m["pos"] = m["pos"] + rng.normal(scale=0.3 * P.ANGSTROM, size=m["pos"].shape)
+        m["atype"][::6] = P.AA_INDEX["LYS"]
+        m["atype"][2::9] = P.AA_INDEX["GLU"]
+        if k in pulled:
+            m = M.add_pulling(m)
+        out.append(m)
+    return out
+
+
+@pytest.mark.parametrize("qa", [0, 1], ids=["nat_cont", "qa"])
+def test_batch_noise_off_follows_oracle(qa):
+    prm = prm_full(qa_on=qa, nat_cont_on=1 - qa)
+    models = ragged_models([33, 120, 57, 200, 41])
+    steps = 150
+    # seed 1 everywhere: the initial velocities are drawn from ran2(seed)
+    b = G.Batch(prm, models, seeds=[1] * len(models), precision=G.FP64, steps_per_launch=40)
+    b.set_noise_scale(0.0)
+    b.run(steps)
+    for s, m in enumerate(models):
+        o = Oracle(prm, m, seed=1)
+        o.set_noise_scale(0.0)
+        tr = o.step(steps)
+        r = b.get(s, 0)
+        err = np.abs(r - o.r()).max()
+        assert err < 1e-8, (s, m["n"], err)
+        V, _ = b.energy(s)
+        assert abs(V - tr[-1]) <= 1e-9 * abs(tr[-1]), (s, V, tr[-1])
+
+
+def test_batch_mixed_forces_match_oracle():
+    prm = prm_full(qa_on=0)
+    models = ragged_models([64, 150, 47], seed0=7)
+    b = G.Batch(prm, models, seeds=[1] * len(models), precision=G.MIXED)
+    b.set_noise_scale(0.0)
+    b.run(1)
+    for s, m in enumerate(models):
+        o = Oracle(prm, m, seed=1)
+        o.fix_nl()
+        o.eval_forces()
+        F1 = o.F()
+        F2 = b.get(s, 6)
+        err = np.abs(F2 - F1).max() / np.abs(F1).max()
+        assert err <= 1e-5, (s, err)
+        V1 = o.V
+        V2, _ = b.energy(s)
+        assert abs(V2 - V1) <= 1e-6 * abs(V1), (s, V1, V2)
+
+
+def test_batch_list_sizes_match_oracle():
+    prm = prm_full(qa_on=0, pid_on=0)
+    models = ragged_models([80, 35], seed0=3)
+    b = G.Batch(prm, models, seeds=[1] * len(models), precision=G.FP64)
+    b.set_noise_scale(0.0)
+    b.run(1)
+    for s, m in enumerate(models):
+        o = Oracle(prm, m, seed=1)
+        o.fix_nl()
+        sz = b.sizes(s)
+        assert sz["n"] == m["n"]
+        assert sz["nl"] == len(o.list(0)[0])
+        assert sz["pauli"] == len(o.list(2)[0])
+        assert sz["const_dh"] == sz["rel_dh"] == len(o.list(3)[0])
+        assert sz["tether"] == len(o.static_list(0)[0])
+        assert sz["lrep"] == len(o.static_list(1)[0])
+
+
+def _observables(r, m, nat_d):
+    ii, jj = m["cont"][:, 0], m["cont"][:, 1]
+    d = np.linalg.norm(r[ii] - r[jj], axis=1)
+    q = float(np.mean(d < 1.2 * nat_d))
+    rg = float(np.sqrt(((r - r.mean(0)) ** 2).sum(1).mean()))
+    return rg, q
+
+
+def test_philox_ensemble_matches_oracle_ran2():
+    """Statistical parity: the GPU's Philox noise and the reference's ran2
+    stream give the same kinetic temperature, radius of gyration and
+    native-contact fraction (mean over copies and time, after 500 steps)."""
+    prm = prm_full(qa_on=0, afm_on=0, temperature=P.f32(0.35))
+    m = M.synthetic_protein(48, M.job_seed(5))
+    nat_d = m["cont_len"]
+    copies, chunks, chunk = 24, 12, 250
+    b = G.Batch(prm, [m] * copies, seeds=list(range(copies)), precision=G.MIXED,
+                philox_seed=2024, steps_per_launch=chunk)
+    stats_g = []
+    b.run(500)
+    for _ in range(chunks):
+        b.run(chunk)
+        for s in range(copies):
+            v = b.get(s, 7)
+            rg, q = _observables(b.get(s, 0), m, nat_d)
+            stats_g.append(((v * v).sum(1).mean() / 3.0, rg, q))
+    stats_o = []
+    for s in range(8):
+        o = Oracle(prm, m, seed=1000 + s)
+        o.step(500)
+        for _ in range(chunks):
+            o.step(chunk)
+            v = o.y(1) / P.f32(5e-3)
+            rg, q = _observables(o.r(), m, nat_d)
+            stats_o.append(((v * v).sum(1).mean() / 3.0, rg, q))
+    g = np.asarray(stats_g)
+    o = np.asarray(stats_o)
+    for k, name in enumerate(["T_kin", "Rg", "Q"]):
+        mg, mo = g[:, k].mean(), o[:, k].mean()
+        se = np.sqrt(g[:, k].var() / len(g) + o[:, k].var() / len(o))
+        assert abs(mg - mo) < 5 * se + 0.02 * abs(mo), (name, mg, mo, se)
+    assert abs(g[:, 0].mean() - 0.35) < 0.035
+
+
+def _run_all(srv, cap=200):
+    for _ in range(cap):
+        if srv.round() == 0:
+            return
+    raise AssertionError("server did not drain")
+
+
+def test_server_job_alone_equals_job_in_mix():
+    prm = prm_full(qa_on=0)
+    models = ragged_models([90, 40, 160, 70], seed0=40)
+    one = G.Server(prm, precision=G.FP64, chunk_steps=64)
+    j1 = one.submit(models[0], 300, seed=11)
+    _run_all(one)
+    mix = G.Server(prm, precision=G.FP64, chunk_steps=64, max_resident=2)
+    ids = [mix.submit(m, 120 + 90 * k, seed=20 + k) for k, m in enumerate(models[1:])]
+    j2 = mix.submit(models[0], 300, seed=11)
+    _run_all(mix)
+    assert one.status(j1) == ("done", 300)
+    assert mix.status(j2) == ("done", 300)
+    for k, j in enumerate(ids):
+        assert mix.status(j) == ("done", 120 + 90 * k)
+    r1, V1 = one.result(j1)
+    r2, V2 = mix.result(j2)
+    assert np.abs(r1 - r2).max() < 1e-8
+    assert abs(V1 - V2) <= 1e-9 * abs(V1)
+    t_sub, t_start, t_end = mix.times(j2)
+    assert t_sub <= t_start <= t_end
+
+
+def test_server_isolates_failed_job_and_cancels():
+    prm = prm_full(qa_on=0)
+    good = ragged_models([60, 90], seed0=60)
+    bad = M.synthetic_protein(50, M.job_seed(61))
+    bad["pos"] = bad["pos"].copy()
+    bad["pos"][10] = bad["pos"][11]  # coincident bonded residues: NaN force
+    srv = G.Server(prm, precision=G.MIXED, chunk_steps=50)
+    g1 = srv.submit(good[0], 200, seed=1)
+    jb = srv.submit(bad, 200, seed=2)
+    g2 = srv.submit(good[1], 200, seed=3)
+    jc = srv.submit(good[0], 200, seed=4)
+    srv.cancel(jc)
+    _run_all(srv)
+    assert srv.status(jb)[0] == "failed"
+    assert srv.status(jc)[0] == "cancelled"
+    for j in (g1, g2):
+        st, done = srv.status(j)
+        assert (st, done) == ("done", 200)
+        r, V = srv.result(j)
+        assert np.isfinite(r).all() and np.isfinite(V)
+    assert srv.stats()["resident"] == 0
+
+
+def test_server_refuses_job_over_budget():
+    prm = prm_full(qa_on=0)
+    srv = G.Server(prm, precision=G.MIXED, hbm_budget_bytes=1 << 16)
+    with pytest.raises(G.PcgError):
+        srv.submit(M.synthetic_protein(300, M.job_seed(1)), 10)
+
+
+def test_autotune_keeps_the_trajectory():
+    """The autotuner's trials run on the live state and restore it: a tuned
+    batch follows an untuned one (noise off, FP64; the workgroup size only
+    changes the force summation order)."""
+    prm = prm_full(qa_on=1, nat_cont_on=0)
+    models = ragged_models([70, 150, 45], seed0=80)
+    a = G.Batch(prm, models, seeds=[1] * 3, precision=G.FP64, steps_per_launch=50)
+    b = G.Batch(prm, models, seeds=[1] * 3, precision=G.FP64, steps_per_launch=50)
+    for x in (a, b):
+        x.set_noise_scale(0.0)
+        x.run(50)
+    block, ms = b.autotune(60)
+    assert block in (128, 256, 512, 1024) and ms > 0
+    assert b.info()["block"] == block
+    a.run(100)
+    b.run(100)
+    for s in range(3):
+        assert np.abs(a.get(s, 0) - b.get(s, 0)).max() < 1e-9

@@ -187,6 +187,8 @@ def test_qa_trajectory_fp64_matches_oracle():
         gi = c.list(6)
         assert np.array_equal(ci[0], gi[0]) and np.array_equal(ci[1], gi[1])
         assert np.allclose(ci[2], gi[2], rtol=0, atol=1e-12)
+        # energy of the last step, QA contact term included
+        assert abs(c.energy()[0] - o.V) <= 1e-9 * abs(o.V), (c.energy()[0], o.V)
         formed = max(formed, no)
     assert formed > 0
 
