@@ -261,6 +261,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--gen-workers", type=int, default=8,
+                    help="processes generating the synthetic proteins (1: in-process)")
+    ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
+                    help="run only the streaming-copy calibration kernel on BYTES")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -269,7 +273,14 @@ def main():
               "with no exchange during stepping", file=sys.stderr)
         sys.exit(2)
 
-    p, models = W.c3()
+    if args.copy_only:
+        from pas_cg_amd import pcg as G
+        gbs, ms = G.stream_copy(args.copy_only, 20)
+        print(json.dumps({"kernel": "k_stream_copy", "bytes_read": args.copy_only,
+                          "bytes_written": args.copy_only, "launches": 21,
+                          "GBs": round(gbs, 1), "ms_per_copy": round(ms, 4)}), flush=True)
+        return
+    p, models = W.c3(workers=args.gen_workers)
     pool = None if (args.no_serve or args.no_plain) else serve_pool()
     tuning = None
     if args.no_tune:
@@ -285,6 +296,8 @@ def main():
     avg_launch_s = bat["ms"] * 1e-3 / bat["launches"]
     bytes_launch = bat["step_bytes"] * bat["spl"]
     achieved = bytes_launch / avg_launch_s / 1e9
+    from pas_cg_amd import pcg as G
+    copy_gbs, _ = G.stream_copy(1 << 30, 20)
     order = np.argsort([int(m["n"]) for m in models])
     out = {"metric": METRIC, "value": round(value, 1), "unit": "residue-steps/s",
            "n_gpus": 1, "steps": bat["steps"], "warmup": bat["warmup"],
@@ -302,6 +315,7 @@ def main():
                         "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": read_traffic(bat["spl"]),
                         "kernel": "k_persistent",
+                        "stream_copy_GBs": round(copy_gbs, 1),
                         "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                         "algorithmic_bytes_per_launch": round(bytes_launch)}}
     if not args.no_check:

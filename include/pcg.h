@@ -174,7 +174,10 @@ int pcg_eval_forces(pcg_ctx *c);                     /* thread::eval_forces */
  * term's forces alone, V its energy. to < 0 means the whole list. */
 int pcg_eval_term(pcg_ctx *c, int32_t term, int32_t from, int32_t to,
                   double *F, double *V);
-int pcg_term_size(pcg_ctx *c, int32_t term, int32_t *size); /* total_size() */
+/* total_size(); for PCG_QA_CAND the free pairs whose NL distance is within
+ * the QA maximum required distance, the only ones loop_over_candidates
+ * evaluates past its first test (qa/loop_over_candidates.cpp:20) */
+int pcg_term_size(pcg_ctx *c, int32_t term, int32_t *size);
 int pcg_qa_finish(pcg_ctx *c);                       /* qa::finish_processing */
 int pcg_lang_step(pcg_ctx *c);                       /* lang::step */
 
@@ -206,9 +209,11 @@ typedef struct pcg_batch pcg_batch;
 typedef struct pcg_batch_options {
   int32_t precision;
   uint64_t philox_seed; /* per-system streams: (seed, system index) */
-  int32_t block;        /* threads per workgroup (0: autotune) */
-  int32_t steps_per_launch; /* steps fused in one persistent launch (0: auto) */
-  int32_t use_graph;    /* replay the launch sequence as a hipGraph */
+  int32_t block;        /* threads per workgroup, multiple of 64 in [64, 1024]
+                           (0: from the largest system; pcg_batch_autotune
+                           measures) */
+  int32_t steps_per_launch; /* steps fused in one persistent launch (0: 100) */
+  int32_t use_graph;    /* replay each run's launch sequence as a hipGraph */
   int32_t device;
 } pcg_batch_options;
 
@@ -272,6 +277,13 @@ int pcg_server_job_times(pcg_server *s, int64_t job, double *submit, double *sta
                          double *end);
 int pcg_server_stats(pcg_server *s, int64_t *resident_bytes, int32_t *resident,
                      int32_t *queued);
+
+/* ------------------------------------------------------------ measurement
+ * No counterpart in the reference. A streaming copy of `bytes` (read once,
+ * written once) timed over `iters` launches: the achievable HBM bandwidth
+ * (GB/s, read + write) and the calibration kernel for traffic counters. */
+int pcg_stream_copy(int64_t bytes, int32_t iters, int32_t device, double *gbs,
+                    double *ms_per_copy);
 
 #ifdef __cplusplus
 }

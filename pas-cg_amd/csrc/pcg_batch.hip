@@ -230,6 +230,20 @@ __device__ void wg_fix_nl(const DevBatch &b, const SysDesc &sd, SysState &st, co
   if (b.P->p.qa_on) build_qa_near(b, sd, st);
 }
 
+// The items of one term: item k is flat index sizes[T] + k, taken by thread
+// (sizes[T] + k) mod blockDim, as in one loop over all terms.
+template <int T, typename R, typename RF>
+__device__ __forceinline__ void term_loop(const DevBatch &b, const SysDesc &sd, SysState &st,
+                                          const GeoL &g, AccL<RF> &acc, const int *sizes,
+                                          double *vacc, bool want_v) {
+  const int lo = sizes[T], n = sizes[T + 1] - lo, B = blockDim.x;
+  if (n <= 0) return;
+  int k = ((int)threadIdx.x - lo % B + B) % B;
+  double v = 0.0;
+  for (; k < n; k += B) v += eval_item<R>(b, sd, st, g, acc, T, k);
+  if (want_v && v != 0.0) atomicAdd(&vacc[T], v);
+}
+
 template <typename R, typename RF>
 __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
 #pragma clang fp contract(off)
@@ -297,14 +311,26 @@ __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
       sizes[NT] = acc_n;
     }
     __syncthreads();
-    // every term's items in one flat index space, term order preserved
-    int total = sizes[NT];
-    int t = 0;
-    for (int gidx = threadIdx.x; gidx < total; gidx += blockDim.x) {
-      while (gidx >= sizes[t + 1]) ++t;
-      double v = eval_item<R>(b, sd, st, g, acc, t, gidx - sizes[t]);
-      if (want_v && v != 0.0) atomicAdd(&vacc[t], v);
-    }
+    // every term's items over one flat index space (term order preserved,
+    // a thread moves on to the next term without waiting); one loop per term
+    // so each keeps only its own constants in registers
+    const bool wv = want_v;
+    term_loop<PCG_CHIR, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_TETHER, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_LREP, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_NAT_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_PAULI, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_CONST_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_REL_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_QA_CAND, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_QA_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_PID, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_NAT_ANG, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_HEUR_ANG, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_HEUR_DIH, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_CND, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_SND, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_VEL_AFM, R>(b, sd, st, g, acc, sizes, vacc, wv);
     __syncthreads();
     if (P.p.qa_on) qa_finish_block(b, sd, st);
     // lang::step (cg/src/cg/langevin/step.cpp:23-77)
@@ -350,35 +376,51 @@ __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
   }
 }
 
-template <typename R, typename RF>
-__global__ __launch_bounds__(1024) void k_persistent(DevBatch b, int nsteps) {
+// The launch bound sets the register budget: 1024 threads leave 128 VGPRs
+// per lane (the FP64 dihedral and PID items then spill), 512 leave 256.
+template <typename R, typename RF, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_persistent(DevBatch b, int nsteps) {
   persistent_body<R, RF>(b, blockIdx.x, nsteps);
 }
 
 // served form: one resident job (a one-system batch) per workgroup
-template <typename R, typename RF>
-__global__ __launch_bounds__(1024) void k_persistent_jobs(const DevBatch *jobs, int nsteps) {
+template <typename R, typename RF, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_persistent_jobs(const DevBatch *jobs, int nsteps) {
   persistent_body<R, RF>(jobs[blockIdx.x], 0, nsteps);
+}
+
+template <int MAXT>
+void launch_jobs_t(const DevBatch *jobs, int njobs, int prec, int block, int nsteps,
+                   size_t lds_bytes, hipStream_t s) {
+  if (prec == PCG_FP64)
+    hipLaunchKernelGGL((k_persistent_jobs<double, double, MAXT>), dim3(njobs), dim3(block),
+                       lds_bytes, s, jobs, nsteps);
+  else
+    hipLaunchKernelGGL((k_persistent_jobs<float, float, MAXT>), dim3(njobs), dim3(block),
+                       lds_bytes, s, jobs, nsteps);
+}
+
+template <int MAXT>
+void launch_batch_t(const DevBatch &b, int prec, int block, int nsteps, size_t lds_bytes,
+                    hipStream_t s) {
+  if (prec == PCG_FP64)
+    hipLaunchKernelGGL((k_persistent<double, double, MAXT>), dim3(b.nsys), dim3(block), lds_bytes,
+                       s, b, nsteps);
+  else
+    hipLaunchKernelGGL((k_persistent<float, float, MAXT>), dim3(b.nsys), dim3(block), lds_bytes,
+                       s, b, nsteps);
 }
 
 void launch_persistent_jobs(const DevBatch *jobs, int njobs, int prec, int block, int nsteps,
                             size_t lds_bytes, hipStream_t s) {
-  if (prec == PCG_FP64)
-    hipLaunchKernelGGL((k_persistent_jobs<double, double>), dim3(njobs), dim3(block), lds_bytes,
-                       s, jobs, nsteps);
-  else
-    hipLaunchKernelGGL((k_persistent_jobs<float, float>), dim3(njobs), dim3(block), lds_bytes, s,
-                       jobs, nsteps);
+  if (block > 512) launch_jobs_t<1024>(jobs, njobs, prec, block, nsteps, lds_bytes, s);
+  else launch_jobs_t<512>(jobs, njobs, prec, block, nsteps, lds_bytes, s);
 }
 
 void launch_persistent(const DevBatch &b, int prec, int block, int nsteps, size_t lds_bytes,
                        hipStream_t s) {
-  if (prec == PCG_FP64)
-    hipLaunchKernelGGL((k_persistent<double, double>), dim3(b.nsys), dim3(block), lds_bytes, s, b,
-                       nsteps);
-  else
-    hipLaunchKernelGGL((k_persistent<float, float>), dim3(b.nsys), dim3(block), lds_bytes, s, b,
-                       nsteps);
+  if (block > 512) launch_batch_t<1024>(b, prec, block, nsteps, lds_bytes, s);
+  else launch_batch_t<512>(b, prec, block, nsteps, lds_bytes, s);
 }
 
 } // namespace pcg

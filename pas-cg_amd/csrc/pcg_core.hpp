@@ -37,7 +37,7 @@ struct SysState {
   int nl_n, nc_n, pauli_n, dh_n, pid_n, qf_n, qc_n, cand_n, rem_n, num_qa;
   int nl_invalid, verify_first, num_changed, overflow;
   int rebuilds, active;
-  int qh_n;                   // QA free pairs that can become candidates (see qh)
+  int qh_n;                   // QA free pairs that can become candidates (qh_slot)
   int64_t step;
   double t, true_t, orig_pad;
   double orig_cell[3];
@@ -92,12 +92,15 @@ struct DevBatch {
   int *dh_i[2]; double *dh_q;
   int *pid_i[2]; double *pid_d; int16_t *pid_t;
   int *qf_i[2]; double *qf_d; uint8_t *qf_vac;
-  // free-pair slots whose NL distance is within the QA maximum required
-  // distance, ascending: loop_over_candidates skips every other free pair
-  // before reading a position (orig_dist - total_disp > max_req_dist with
-  // total_disp == 0 after each verify, loop_over_candidates.cpp:20), so the
-  // candidate search visits only these
-  int *qh;
+  // the free pairs whose NL distance is within the QA maximum required
+  // distance, in slot order, with their residues and a vacancy flag copied
+  // next to the slot (dense, as the candidate search reads them every step):
+  // loop_over_candidates skips every other free pair before reading a
+  // position (orig_dist - total_disp > max_req_dist with total_disp == 0
+  // after each verify, loop_over_candidates.cpp:20), so the search visits
+  // only these
+  int *qh_slot; int *qh_i[2]; uint8_t *qh_vac;
+  int *cand_h;                // near-list index of each candidate
   int *qc_i[2]; double *qc_d; int16_t *qc_t; double *qc_sat; int8_t *qc_sd; uint8_t *qc_vac;
   int *cand_i[2]; double *cand_od, *cand_dist; int *cand_idx; int16_t *cand_t; int8_t *cand_sd;
   int *cand_perm;

@@ -502,7 +502,11 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.cand_perm, nl);
   A.add(&D.qf_d, qf);
   A.add(&D.qf_vac, qf);
-  A.add(&D.qh, qf);
+  A.add(&D.qh_slot, qf);
+  A.add(&D.qh_i[0], qf);
+  A.add(&D.qh_i[1], qf);
+  A.add(&D.qh_vac, qf);
+  A.add(&D.cand_h, nl);
   A.add(&D.qc_d, qc);
   A.add(&D.qc_t, qc);
   A.add(&D.qc_sat, qc);
@@ -1124,7 +1128,12 @@ struct pcg_batch {
   double last_ms = 0.0;
   int last_launches = 0;
   bool tuned = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_nsteps = 0, g_block = 0, g_spl = 0;
   ~pcg_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
   }
@@ -1138,27 +1147,35 @@ size_t lds_for(const DeviceBatch &B) {
 }
 
 // steps owed by every system = current + nsteps
-int set_targets(DeviceBatch &B, int64_t nsteps) {
-  std::vector<SysState> st(B.sys.size());
-  HIPCHK(hipMemcpyAsync(st.data(), B.d.st, st.size() * sizeof(SysState), hipMemcpyDeviceToHost,
-                        B.stream));
-  HIPCHK(hipStreamSynchronize(B.stream));
-  for (size_t s = 0; s < B.sys.size(); ++s) B.sys[s].steps_target = st[s].step + nsteps;
-  HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), B.sys.size() * sizeof(SysDesc),
-                        hipMemcpyHostToDevice, B.stream));
-  return PCG_OK;
-}
-
+// The batch's systems never stop on their own (steps_target is INT64_MAX), so
+// a run is a fixed sequence of launches: captured once per (nsteps, block,
+// steps per launch) as a hipGraph when the batch asks for it.
 int run_persistent(pcg_batch *b, int nsteps) {
   DeviceBatch &B = *b->B;
-  int rc = set_targets(B, nsteps);
-  if (rc) return rc;
-  HIPCHK(hipEventRecord(b->ev0, B.stream));
-  int launches = 0;
-  for (int done = 0; done < nsteps; done += b->spl) {
-    int k = std::min(b->spl, nsteps - done);
-    launch_persistent(B.d, B.prec, b->block, k, b->lds, B.stream);
-    ++launches;
+  int launches = (nsteps + b->spl - 1) / b->spl;
+  auto enqueue = [&]() {
+    for (int done = 0; done < nsteps; done += b->spl)
+      launch_persistent(B.d, B.prec, b->block, std::min(b->spl, nsteps - done), b->lds, B.stream);
+  };
+  if (b->use_graph && nsteps > 0) {
+    if (!b->gexec || b->g_nsteps != nsteps || b->g_block != b->block || b->g_spl != b->spl) {
+      if (b->gexec) HIPCHK(hipGraphExecDestroy(b->gexec));
+      if (b->graph) HIPCHK(hipGraphDestroy(b->graph));
+      b->gexec = nullptr;
+      b->graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+      enqueue();
+      HIPCHK(hipStreamEndCapture(B.stream, &b->graph));
+      HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      b->g_nsteps = nsteps;
+      b->g_block = b->block;
+      b->g_spl = b->spl;
+    }
+    HIPCHK(hipEventRecord(b->ev0, B.stream));
+    HIPCHK(hipGraphLaunch(b->gexec, B.stream));
+  } else {
+    HIPCHK(hipEventRecord(b->ev0, B.stream));
+    enqueue();
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(b->ev1, B.stream));
@@ -1212,6 +1229,11 @@ int pcg_batch_create(const pcg_params *p, int32_t nsys, const pcg_model *models,
   }
   b->spl = opt.steps_per_launch > 0 ? opt.steps_per_launch : 100;
   b->use_graph = opt.use_graph != 0;
+  // batch systems run until told otherwise: each launch makes its steps
+  for (auto &d : B.sys) d.steps_target = INT64_MAX;
+  HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), B.sys.size() * sizeof(SysDesc),
+                        hipMemcpyHostToDevice, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
   HIPCHK(hipEventCreate(&b->ev0));
   HIPCHK(hipEventCreate(&b->ev1));
   *out = b.release();

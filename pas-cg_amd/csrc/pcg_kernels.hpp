@@ -134,7 +134,7 @@ __device__ __forceinline__ double eval_item(const DevBatch &b, const SysDesc &s,
   case PCG_PAULI: return pauli_item<R>(b, s, g, acc, s.nl0 + k);
   case PCG_CONST_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 0);
   case PCG_REL_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 1);
-  case PCG_QA_CAND: return qa_cand_item<R>(b, s, st, g, acc, b.qh[s.qf0 + k]);
+  case PCG_QA_CAND: return qa_cand_item<R>(b, s, st, g, acc, k);
   case PCG_QA_CONT: return qa_cont_item<R>(b, s, st, g, acc, k);
   case PCG_PID: return pid_item<double>(b, s, g, acc, s.nl0 + k);
   case PCG_NAT_ANG: return nat_ang_item<double>(b, g, acc, s.nang0 + k);

@@ -77,7 +77,13 @@ __device__ inline void build_qa_near(const DevBatch &b, const SysDesc &sd, SysSt
       if (q < w) off += wc[q];
       tot += wc[q];
     }
-    if (near) b.qh[sd.qf0 + off + __popcll(m & ((1ull << lane) - 1ull))] = k;
+    if (near) {
+      int h = sd.qf0 + off + __popcll(m & ((1ull << lane) - 1ull));
+      b.qh_slot[h] = k;
+      b.qh_i[0][h] = b.qf_i[0][sd.qf0 + k];
+      b.qh_i[1][h] = b.qf_i[1][sd.qf0 + k];
+      b.qh_vac[h] = b.qf_vac[sd.qf0 + k];
+    }
     __syncthreads();
     if (threadIdx.x == 0) run += tot;
     __syncthreads();
@@ -128,7 +134,13 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
         b.qf_d[f] = b.qc_d[q];
         b.qf_vac[f] = 0;
         // appended at the end, so the near list stays in slot order
-        if (!(b.qf_d[f] > b.P->qa_max_req)) b.qh[sd.qf0 + st.qh_n++] = st.qf_n;
+        if (!(b.qf_d[f] > b.P->qa_max_req)) {
+          int h = sd.qf0 + st.qh_n++;
+          b.qh_slot[h] = st.qf_n;
+          b.qh_i[0][h] = i1;
+          b.qh_i[1][h] = i2;
+          b.qh_vac[h] = 0;
+        }
         st.qf_n += 1;
       } else {
         st.overflow = 1;
@@ -169,6 +181,7 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
         b.sync[4 * i2 + c] = n2[c];
       }
       b.qf_vac[sd.qf0 + b.cand_idx[sc]] = 1;
+      b.qh_vac[sd.qf0 + b.cand_h[sc]] = 1;
     }
     // compress
     if (2 * st.num_qa < st.qc_n) {

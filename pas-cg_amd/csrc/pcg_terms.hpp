@@ -453,17 +453,16 @@ __device__ __forceinline__ double xdot(double ax, double ay, double az, v3<doubl
 // qa::loop_over_candidates::iter (cg/src/cg/qa/loop_over_candidates.cpp:12-99):
 // criteria in exact FP64, repulsion force in R; a candidate is appended to
 // the system's candidate buffer (its order is restored by the finish sort).
+// Item hk of the near list (every entry passed orig_dist <= max_req_dist).
 template <typename R, typename G, typename A>
 __device__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
-                               A &acc, int k) {
+                               A &acc, int hk) {
 #pragma clang fp contract(off)
   const DevParams &P = *b.P;
   const pcg_params &p = P.p;
-  int slot = s.qf0 + k;
-  if (b.qf_vac[slot]) return 0.0;
-  double od = b.qf_d[slot];
-  if (od > P.qa_max_req) return 0.0;  // orig_dist - total_disp, total_disp == 0
-  int i1 = b.qf_i[0][slot], i2 = b.qf_i[1][slot];
+  int h = s.qf0 + hk;
+  if (b.qh_vac[h]) return 0.0;
+  int i1 = b.qh_i[0][h], i2 = b.qh_i[1][h];
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
   double rn = __ddiv_rn(1.0, xnorm3(d));
   v3<double> u{__dmul_rn(d.x, rn), __dmul_rn(d.y, rn), __dmul_rn(d.z, rn)};
@@ -510,11 +509,13 @@ __device__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st
     return V;
   }
   int cs = s.nl0 + c;
+  int k = b.qh_slot[h];
   b.cand_i[0][cs] = i1;
   b.cand_i[1][cs] = i2;
-  b.cand_od[cs] = od;
+  b.cand_od[cs] = b.qf_d[s.qf0 + k];
   b.cand_dist[cs] = __ddiv_rn(1.0, rn);
   b.cand_idx[cs] = k;
+  b.cand_h[cs] = hk;
   b.cand_t[cs] = (int16_t)type;
   for (int a = 0; a < 4; ++a) {
     b.cand_sd[8 * cs + a] = s1[a];

@@ -137,6 +137,7 @@ SIGNATURES = {
     "pcg_batch_set_noise_scale": [_V, D],
     "pcg_batch_sizes": [_V, I, C.POINTER(I)],
     "pcg_batch_autotune": [_V, I, C.POINTER(I), C.POINTER(D)],
+    "pcg_stream_copy": [I64, I, I, C.POINTER(D), C.POINTER(D)],
 }
 
 
@@ -510,3 +511,13 @@ class Server:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         check(self.L.pcg_server_job_times(self.h, job, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+
+def stream_copy(nbytes: int = 1 << 30, iters: int = 20, device: int = 0):
+    """Achieved bandwidth (GB/s, read + write) and time per copy (ms) of a
+    streaming copy of `nbytes`."""
+    L = lib()
+    g = C.c_double()
+    ms = C.c_double()
+    check(L.pcg_stream_copy(nbytes, iters, device, C.byref(g), C.byref(ms)))
+    return g.value, ms.value

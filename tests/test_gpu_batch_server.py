@@ -221,3 +221,22 @@ def test_autotune_keeps_the_trajectory():
     b.run(100)
     for s in range(3):
         assert np.abs(a.get(s, 0) - b.get(s, 0)).max() < 1e-9
+
+
+def test_batch_graph_replays_match_direct_launches():
+    """The hipGraph of a run's launch sequence, replayed several times with
+    host reads in between, follows the directly launched batch."""
+    prm = prm_full(qa_on=1, nat_cont_on=1)
+    models = ragged_models([64, 130, 38], seed0=90)
+    a = G.Batch(prm, models, seeds=[1] * 3, precision=G.FP64, steps_per_launch=20)
+    g = G.Batch(prm, models, seeds=[1] * 3, precision=G.FP64, steps_per_launch=20,
+                use_graph=True)
+    for x in (a, g):
+        x.set_noise_scale(0.0)
+    for rep in range(4):
+        a.run(60)
+        g.run(60)
+        for s in range(3):
+            err = np.abs(a.get(s, 0) - g.get(s, 0)).max()
+            assert err < 1e-9, (rep, s, err)
+            assert abs(a.energy(s)[0] - g.energy(s)[0]) <= 1e-9 * abs(a.energy(s)[0])
