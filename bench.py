@@ -162,13 +162,16 @@ SERVE_STEPS = (2000, 20000)
 SERVE_POOL = 800
 
 
-def serve_pool():
+def serve_pool(workers: int):
     """Jobs of the served stream at unit arrival rate, and their models, made
     before the GPU is touched (the generator forks worker processes)."""
-    from concurrent.futures import ProcessPoolExecutor
     jobs = W.c4_jobs(SERVE_POOL, 1.0, *SERVE_STEPS)
-    with ProcessPoolExecutor(min(8, os.cpu_count() or 1)) as ex:
-        models = list(ex.map(W.c4_model, jobs, chunksize=8))
+    if workers > 1:
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(workers) as ex:
+            models = list(ex.map(W.c4_model, jobs, chunksize=8))
+    else:
+        models = [W.c4_model(j) for j in jobs]
     return jobs, models
 
 
@@ -211,15 +214,15 @@ def served(p, rate_rs, seconds, pool, per_plain):
         work += j["n"] * j["steps"]
     srv.close()
     p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
-    f50, f99 = fifo_latency(jobs, per_plain)
+    f50, f99 = fifo_latency(jobs, per_plain) if per_plain else (float("nan"), float("nan"))
     log(f"served: {count} jobs at {rate:.1f}/s, {work / span:.3e} residue-steps/s, "
         f"p50 {p50:.3f} s p99 {p99:.3f} s")
     return {"jobs": count, "arrival_rate_per_s": round(rate, 2),
             "steps_range": [steps_lo, steps_hi], "failed": failed,
             "residue_steps_per_s": work / span, "p50_latency_s": round(float(p50), 4),
             "p99_latency_s": round(float(p99), 4),
-            "plain_fifo_p50_latency_s": round(float(f50), 4),
-            "plain_fifo_p99_latency_s": round(float(f99), 4)}
+            "plain_fifo_p50_latency_s": None if math.isnan(f50) else round(float(f50), 4),
+            "plain_fifo_p99_latency_s": None if math.isnan(f99) else round(float(f99), 4)}
 
 
 def cpu_baseline(p, models, picks, budget_s=15.0):
@@ -281,7 +284,7 @@ def main():
                           "GBs": round(gbs, 1), "ms_per_copy": round(ms, 4)}), flush=True)
         return
     p, models = W.c3(workers=args.gen_workers)
-    pool = None if (args.no_serve or args.no_plain) else serve_pool()
+    pool = None if args.no_serve else serve_pool(min(args.gen_workers, os.cpu_count() or 1))
     tuning = None
     if args.no_tune:
         pad, block = args.pad, args.block
@@ -333,7 +336,7 @@ def main():
                                   "hipGraph-captured step, 200 steps each"}
         out["speedup_vs_plain"] = round(value / prate, 2)
     bat["b"].close()
-    if not args.no_serve and per_plain is not None:
+    if not args.no_serve:
         out["served"] = served(W.full_params(args.pad), value, args.serve_seconds, pool,
                                per_plain)
     if not args.no_cpu:
