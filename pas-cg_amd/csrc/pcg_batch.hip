@@ -230,6 +230,140 @@ __device__ void wg_fix_nl(const DevBatch &b, const SysDesc &sd, SysState &st, co
   if (b.P->p.qa_on) build_qa_near(b, sd, st);
 }
 
+// Shares of one chain-local item: slot s receives the force on residue
+// base + s (s = 0..3), in registers.
+template <typename RF> struct AccSlots {
+  int base;
+  RF c[4][3];
+  template <typename T> __device__ __forceinline__ void add(int i, v3<T> f) {
+    int s = i - base;
+    // explicit selects keep c in registers
+    RF fx = (RF)f.x, fy = (RF)f.y, fz = (RF)f.z;
+    if (s == 0) { c[0][0] += fx; c[0][1] += fy; c[0][2] += fz; }
+    else if (s == 1) { c[1][0] += fx; c[1][1] += fy; c[1][2] += fz; }
+    else if (s == 2) { c[2][0] += fx; c[2][1] += fy; c[2][2] += fz; }
+    else { c[3][0] += fx; c[3][1] += fy; c[3][2] += fz; }
+  }
+};
+
+// Per-residue bonded parameters of a chain-local system in LDS: the item
+// starting at residue k of each list exists if its flag bit is set.
+enum { BF_TETHER = 1, BF_LREP = 2, BF_NAT_ANG = 4, BF_HEUR_ANG = 8, BF_NAT_DIH = 16,
+       BF_HEUR_DIH = 32, BF_CHIR = 64 };
+struct BondLds {
+  double *d;   // [5][n]: tether length, native angle, native dihedral, chiral f0, f1
+  uint8_t *u;  // [3][n]: flags, heuristic angle type, heuristic dihedral type
+  int n;
+  __device__ double &teth(int k) const { return d[k]; }
+  __device__ double &ang(int k) const { return d[n + k]; }
+  __device__ double &dih(int k) const { return d[2 * n + k]; }
+  __device__ double &c0(int k) const { return d[3 * n + k]; }
+  __device__ double &c1(int k) const { return d[4 * n + k]; }
+  __device__ uint8_t &flag(int k) const { return u[k]; }
+  __device__ uint8_t &at(int k) const { return u[n + k]; }
+  __device__ uint8_t &dt(int k) const { return u[2 * n + k]; }
+};
+
+// Once per launch: the static bonded items of every base residue, from HBM
+// into LDS (the per-step loop then reads no list of these terms from HBM).
+__device__ __forceinline__ void stage_bonded(const DevBatch &b, const SysDesc &sd, const BondLds &bl) {
+  for (int k = threadIdx.x; k < sd.n; k += blockDim.x) {
+    int r = sd.res0 + k, it;
+    uint8_t f = 0;
+    if ((it = b.bond_item[BOND_TETHER][r]) >= 0) f |= BF_TETHER, bl.teth(k) = b.teth_nat[it];
+    if ((it = b.bond_item[BOND_LREP][r]) >= 0) f |= BF_LREP;
+    if ((it = b.bond_item[BOND_NAT_ANG][r]) >= 0) f |= BF_NAT_ANG, bl.ang(k) = b.nang_th[it];
+    if ((it = b.bond_item[BOND_HEUR_ANG][r]) >= 0) f |= BF_HEUR_ANG, bl.at(k) = b.hang_t[it];
+    if ((it = b.bond_item[BOND_NAT_DIH][r]) >= 0) f |= BF_NAT_DIH, bl.dih(k) = b.ndih_ph[it];
+    if ((it = b.bond_item[BOND_HEUR_DIH][r]) >= 0) f |= BF_HEUR_DIH, bl.dt(k) = b.hdih_t[it];
+    if ((it = b.bond_item[BOND_CHIR][r]) >= 0) {
+      f |= BF_CHIR;
+      bl.c0(k) = b.chir_f0[it];
+      bl.c1(k) = b.chir_f1[it];
+    }
+    bl.flag(k) = f;
+  }
+}
+
+template <typename RF> __device__ __forceinline__ RF shfl_up_rf(RF v, int d) {
+  return __shfl_up(v, (unsigned)d, 64);
+}
+
+// The static bonded terms of a system whose items are all chain-local. Each
+// thread takes one base residue per pass (consecutive residues on consecutive
+// lanes) and evaluates the items that start there; a residue's force is its
+// own share plus the shares lanes 1..3 below it computed for it, gathered
+// with cross-lane shifts, and one LDS add per component. Shares that would
+// cross into the next wave go to LDS directly.
+template <typename RF, bool STAGED>
+__device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc &sd, SysState &st,
+                                               const GeoL &g, const BondLds &bl, RF *fx, RF *fy,
+                                               RF *fz, double *vacc, bool want_v) {
+  const pcg_params &p = b.P->p;
+  const int n = sd.n, r0 = sd.res0, lane = threadIdx.x & 63;
+  double vt = 0, vl = 0, vna = 0, vha = 0, vnd = 0, vhd = 0, vch = 0;
+  for (int base = 0; base < n; base += blockDim.x) {
+    int k = base + (int)threadIdx.x;
+    AccSlots<RF> a;
+    a.base = r0 + k;
+    for (int s = 0; s < 4; ++s) a.c[s][0] = a.c[s][1] = a.c[s][2] = (RF)0;
+    if (STAGED && k < n) {
+      int r = r0 + k;
+      uint8_t f = bl.flag(k);
+      if (p.tether_on && (f & BF_TETHER)) vt += tether_core<double>(p, g, a, r, r + 1, bl.teth(k));
+      if (p.lrep_on && (f & BF_LREP)) vl += lrep_core<double>(p, g, a, r, r + 2);
+      if (p.nat_ang_on && (f & BF_NAT_ANG)) vna += nat_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.ang(k));
+      if (p.heur_ang_on && (f & BF_HEUR_ANG)) vha += heur_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.at(k));
+      if (p.nat_dih_on && (f & BF_NAT_DIH))
+        vnd += nat_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dih(k), p.nat_dih_variant);
+      if (p.heur_dih_on && (f & BF_HEUR_DIH))
+        vhd += heur_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dt(k));
+      if (p.chir_on && (f & BF_CHIR))
+        vch += chir_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.c0(k), bl.c1(k));
+    } else if (!STAGED && k < n) {
+      int r = r0 + k, it;
+      if (p.tether_on && (it = b.bond_item[BOND_TETHER][r]) >= 0) vt += tether_item<double>(b, g, a, it);
+      if (p.lrep_on && (it = b.bond_item[BOND_LREP][r]) >= 0) vl += lrep_item<double>(b, g, a, it);
+      if (p.nat_ang_on && (it = b.bond_item[BOND_NAT_ANG][r]) >= 0) vna += nat_ang_item<double>(b, g, a, it);
+      if (p.heur_ang_on && (it = b.bond_item[BOND_HEUR_ANG][r]) >= 0) vha += heur_ang_item<double>(b, g, a, it);
+      if (p.nat_dih_on && (it = b.bond_item[BOND_NAT_DIH][r]) >= 0)
+        vnd += nat_dih_item<double>(b, g, a, it, p.nat_dih_variant);
+      if (p.heur_dih_on && (it = b.bond_item[BOND_HEUR_DIH][r]) >= 0) vhd += heur_dih_item<double>(b, g, a, it);
+      if (p.chir_on && (it = b.bond_item[BOND_CHIR][r]) >= 0) vch += chir_item<double>(b, g, a, it);
+    }
+    RF t[3];
+    for (int c = 0; c < 3; ++c) {
+      RF s1 = shfl_up_rf(a.c[1][c], 1), s2 = shfl_up_rf(a.c[2][c], 2), s3 = shfl_up_rf(a.c[3][c], 3);
+      t[c] = a.c[0][c];
+      if (lane >= 1) t[c] += s1;
+      if (lane >= 2) t[c] += s2;
+      if (lane >= 3) t[c] += s3;
+    }
+    if (k < n) {
+      atomicAdd(fx + k, t[0]);
+      atomicAdd(fy + k, t[1]);
+      atomicAdd(fz + k, t[2]);
+      // shares for residues handled by the next wave (they exist: an item
+      // only reaches residues of its own system)
+      for (int s = 1; s < 4; ++s)
+        if (lane + s >= 64 && (a.c[s][0] != (RF)0 || a.c[s][1] != (RF)0 || a.c[s][2] != (RF)0)) {
+          atomicAdd(fx + k + s, a.c[s][0]);
+          atomicAdd(fy + k + s, a.c[s][1]);
+          atomicAdd(fz + k + s, a.c[s][2]);
+        }
+    }
+  }
+  if (want_v) {
+    if (vt != 0.0) atomicAdd(&vacc[PCG_TETHER], vt);
+    if (vl != 0.0) atomicAdd(&vacc[PCG_LREP], vl);
+    if (vna != 0.0) atomicAdd(&vacc[PCG_NAT_ANG], vna);
+    if (vha != 0.0) atomicAdd(&vacc[PCG_HEUR_ANG], vha);
+    if (vnd != 0.0) atomicAdd(&vacc[p.nat_dih_variant ? PCG_SND : PCG_CND], vnd);
+    if (vhd != 0.0) atomicAdd(&vacc[PCG_HEUR_DIH], vhd);
+    if (vch != 0.0) atomicAdd(&vacc[PCG_CHIR], vch);
+  }
+}
+
 // The items of one term: item k is flat index sizes[T] + k, taken by thread
 // (sizes[T] + k) mod blockDim, as in one loop over all terms.
 template <int T, typename R, typename RF>
@@ -245,7 +379,7 @@ __device__ __forceinline__ void term_loop(const DevBatch &b, const SysDesc &sd, 
 }
 
 template <typename R, typename RF>
-__device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
+__device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int nsteps) {
 #pragma clang fp contract(off)
   const SysDesc &sd = b.sys[s];
   SysState &st = b.st[s];
@@ -253,8 +387,11 @@ __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
   const DevParams &P = *b.P;
   const int n = sd.n, r0 = sd.res0;
   extern __shared__ __align__(16) unsigned char lds[];
+  const bool staged = sd.bond_regular && b.bond_lds;
+  const LdsLayout lay((size_t)n, sizeof(RF), staged);
   double *rx = (double *)lds, *ry = rx + n, *rz = ry + n;
-  RF *fx = (RF *)(rz + n), *fy = fx + n, *fz = fy + n;
+  RF *fx = (RF *)(lds + lay.f), *fy = fx + n, *fz = fy + n;
+  BondLds bl{(double *)(lds + lay.bd), (uint8_t *)(lds + lay.bu), n};
   __shared__ double vacc[NT];
   __shared__ int sizes[NT + 1];
   __shared__ int flag_rebuild, stop;
@@ -266,6 +403,7 @@ __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
   }
   GeoL g{rx, ry, rz, r0};
   AccL<RF> acc{fx, fy, fz, r0};
+  if (staged) stage_bonded(b, sd, bl);
   __syncthreads();
 
   __shared__ int want_v;
@@ -315,9 +453,20 @@ __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
     // a thread moves on to the next term without waiting); one loop per term
     // so each keeps only its own constants in registers
     const bool wv = want_v;
-    term_loop<PCG_CHIR, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    term_loop<PCG_TETHER, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    term_loop<PCG_LREP, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    if (staged) {
+      bonded_by_base<RF, true>(b, sd, st, g, bl, fx, fy, fz, vacc, wv);
+    } else if (sd.bond_regular) {
+      bonded_by_base<RF, false>(b, sd, st, g, bl, fx, fy, fz, vacc, wv);
+    } else {
+      term_loop<PCG_CHIR, R>(b, sd, st, g, acc, sizes, vacc, wv);
+      term_loop<PCG_TETHER, R>(b, sd, st, g, acc, sizes, vacc, wv);
+      term_loop<PCG_LREP, R>(b, sd, st, g, acc, sizes, vacc, wv);
+      term_loop<PCG_NAT_ANG, R>(b, sd, st, g, acc, sizes, vacc, wv);
+      term_loop<PCG_HEUR_ANG, R>(b, sd, st, g, acc, sizes, vacc, wv);
+      term_loop<PCG_HEUR_DIH, R>(b, sd, st, g, acc, sizes, vacc, wv);
+      term_loop<PCG_CND, R>(b, sd, st, g, acc, sizes, vacc, wv);
+      term_loop<PCG_SND, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    }
     term_loop<PCG_NAT_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_PAULI, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_CONST_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
@@ -325,11 +474,6 @@ __device__ void persistent_body(const DevBatch &b, int s, int nsteps) {
     term_loop<PCG_QA_CAND, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_QA_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_PID, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    term_loop<PCG_NAT_ANG, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    term_loop<PCG_HEUR_ANG, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    term_loop<PCG_HEUR_DIH, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    term_loop<PCG_CND, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    term_loop<PCG_SND, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_VEL_AFM, R>(b, sd, st, g, acc, sizes, vacc, wv);
     __syncthreads();
     if (P.p.qa_on) qa_finish_block(b, sd, st);

@@ -15,6 +15,12 @@
 namespace pcg {
 
 constexpr int NT = PCG_NUM_TERMS;
+// Static bonded lists whose items are chain-local: item of base residue r
+// spans r, r+1 (tether), r, r+2 (local repulsion), r..r+2 (angles) or
+// r..r+3 (dihedrals, chiral quads). The batched kernel evaluates them per
+// base residue and sums the shares of neighbouring residues across lanes.
+enum { BOND_TETHER, BOND_LREP, BOND_NAT_ANG, BOND_HEUR_ANG, BOND_NAT_DIH, BOND_HEUR_DIH,
+       BOND_CHIR, NBOND };
 constexpr double C216_INV = 0.89089871814033927; // cg/include/cg/utils/math.h:6
 
 struct SysDesc {
@@ -25,11 +31,26 @@ struct SysDesc {
   int nl0, nlcap;   // NL and the NL-sized lists (pauli, dh, pid, QA candidates)
   int qf0, qfcap;   // QA free pairs
   int qc0, qccap;   // QA contacts and removed indices
-  int pad0;
+  int bond_regular;  // every static bonded item is chain-local (see bond_item)
   double cell[3], cell_inv[3];
   double temperature, const2; // const2 = sqrt(2 kB T gamma dt) dt (step.cpp:7-17)
   uint64_t philox_key;
   int64_t steps_target;       // batched/served: steps this system still owes
+};
+
+// LDS layout of the batched kernel for a system of n residues: positions
+// (3 n FP64), forces (3 n of fsz bytes) and, when staged, the per-residue
+// bonded parameters (5 n FP64: tether length, native angle, native dihedral,
+// the two chiral fields; 3 n bytes: flags, heuristic angle and dihedral
+// types).
+struct LdsLayout {
+  size_t f, bd, bu, total;
+  __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged) {
+    f = 24 * n;
+    bd = (f + 3 * n * fsz + 15) / 16 * 16;
+    bu = bd + 40 * n;
+    total = staged ? bu + 3 * n + 64 : bd + 64;
+  }
 };
 
 // Mutable per-system scalars.
@@ -88,6 +109,10 @@ struct DevBatch {
   // dynamic lists
   int *nl_i[2]; double *nl_d; uint8_t *nl_cls;  // class bits of the partition
   int *ncur;                                      // current native contacts (index into all)
+  // bond_item[L][r]: global index of the item of list L whose first residue
+  // is r, or -1 (systems with bond_regular set)
+  const int *bond_item[NBOND];
+  int bond_lds;               // batched kernel: stage per-residue bonded data in LDS
   int *pauli_i[2];
   int *dh_i[2]; double *dh_q;
   int *pid_i[2]; double *pid_d; int16_t *pid_t;
@@ -100,6 +125,10 @@ struct DevBatch {
   // after each verify, loop_over_candidates.cpp:20), so the search visits
   // only these
   int *qh_slot; int *qh_i[2]; uint8_t *qh_vac;
+  // per near pair: the largest formation distance any contact type of its
+  // residue pair can have (exact FP64), and the squared FP32 radius beyond
+  // which the pair can have no effect at all (with a 1e-5 margin)
+  double *qh_req; float *qh_thr2;
   int *cand_h;                // near-list index of each candidate
   int *qc_i[2]; double *qc_d; int16_t *qc_t; double *qc_sat; int8_t *qc_sd; uint8_t *qc_vac;
   int *cand_i[2]; double *cand_od, *cand_dist; int *cand_idx; int16_t *cand_t; int8_t *cand_sd;

@@ -475,6 +475,34 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.nc_active, std::max(nnc, 1LL));
   A.add(&D.nc_change_t, std::max(nnc, 1LL));
   A.add(&D.ncur, std::max(nnc, 1LL));
+  // chain-local bonded items by base residue (see BOND_* in pcg_core.hpp)
+  std::vector<int> bond_item((size_t)NBOND * std::max(ntot, 1LL), -1);
+  for (int s = 0; s < nsys; ++s) {
+    HostSys &h = B->hs[s];
+    SysDesc &d = B->sys[s];
+    bool ok = true;
+    auto put = [&](int L, int base, const std::vector<int> *idx, const int *step, int slots) {
+      size_t m = idx[0].size();
+      for (size_t k = 0; k < m && ok; ++k) {
+        int r = idx[0][k];
+        for (int sl = 1; sl < slots; ++sl) ok = ok && idx[sl][k] == r + step[sl];
+        size_t at = (size_t)L * ntot + d.res0 + r;
+        ok = ok && r >= 0 && r + step[slots - 1] < h.n && bond_item[at] < 0;
+        if (ok) bond_item[at] = base + (int)k;
+      }
+    };
+    const int pair1[2] = {0, 1}, pair2[2] = {0, 2}, tri[3] = {0, 1, 2}, quad[4] = {0, 1, 2, 3};
+    put(BOND_TETHER, d.teth0, h.teth, pair1, 2);
+    put(BOND_LREP, d.lrep0, h.lrep, pair2, 2);
+    put(BOND_NAT_ANG, d.nang0, h.nang, tri, 3);
+    put(BOND_HEUR_ANG, d.hang0, h.hang, tri, 3);
+    put(BOND_NAT_DIH, d.ndih0, h.ndih, quad, 4);
+    put(BOND_HEUR_DIH, d.hdih0, h.hdih, quad, 4);
+    put(BOND_CHIR, d.chir0, h.chir, quad, 4);
+    d.bond_regular = ok ? 1 : 0;
+  }
+  int *d_bond_item;
+  A.add(&d_bond_item, bond_item.size());
   A.add(&tip_res, std::max(ntip, 1LL));
   for (int c = 0; c < 3; ++c) {
     A.add(&tip_orig[c], std::max(ntip, 1LL));
@@ -506,6 +534,8 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.qh_i[0], qf);
   A.add(&D.qh_i[1], qf);
   A.add(&D.qh_vac, qf);
+  A.add(&D.qh_req, qf);
+  A.add(&D.qh_thr2, qf);
   A.add(&D.cand_h, nl);
   A.add(&D.qc_d, qc);
   A.add(&D.qc_t, qc);
@@ -533,6 +563,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   D.ndih_ph = ndih_ph, D.nc_nat = nc_nat, D.hang_t = hang_t, D.hdih_t = hdih_t, D.nc_type = nc_type;
   D.tip_res = tip_res;
   for (int c = 0; c < 3; ++c) D.tip_orig[c] = tip_orig[c], D.tip_vel[c] = tip_vel[c];
+  for (int L = 0; L < NBOND; ++L) D.bond_item[L] = d_bond_item + (size_t)L * std::max(ntot, 1LL);
 
   // host staging, then one copy per array
   std::vector<SysState> st(nsys);
@@ -585,6 +616,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   if ((rc = up(sysid, vsys.data(), ntot * 4))) return rc;
   if ((rc = up(local, vloc.data(), ntot * 4))) return rc;
   if ((rc = up(D.sync, sync.data(), 4 * ntot))) return rc;
+  if ((rc = up(d_bond_item, bond_item.data(), bond_item.size() * 4))) return rc;
   // static lists, concatenated with residue indices shifted to the batch
   auto cat_i = [&](int *dst_, auto getv, long long total) -> int {
     std::vector<int> v;
@@ -1141,9 +1173,9 @@ struct pcg_batch {
 
 namespace {
 
-size_t lds_for(const DeviceBatch &B) {
+size_t lds_for(const DeviceBatch &B, bool staged) {
   size_t fsz = B.prec == PCG_FP64 ? 8 : 4;
-  return (size_t)B.max_n * (3 * 8 + 3 * fsz) + 64;
+  return LdsLayout((size_t)B.max_n, fsz, staged).total;
 }
 
 // steps owed by every system = current + nsteps
@@ -1215,8 +1247,11 @@ int pcg_batch_create(const pcg_params *p, int32_t nsys, const pcg_model *models,
                       b->B);
   if (rc) return rc;
   DeviceBatch &B = *b->B;
-  b->lds = lds_for(B);
-  // systems whose positions and forces do not fit in LDS take the stage kernels
+  // per-residue bonded data joins positions and forces in LDS when it fits;
+  // systems whose positions and forces do not fit take the stage kernels
+  bool staged = lds_for(B, true) <= 150 * 1024;
+  b->lds = lds_for(B, staged);
+  B.d.bond_lds = staged ? 1 : 0;
   b->persistent = b->lds <= 150 * 1024;
   if (opt.block != 0 && (opt.block < 64 || opt.block > 1024 || opt.block % 64 != 0))
     return fail(PCG_E_ARG, "block must be a multiple of 64 in [64, 1024]");
@@ -1545,6 +1580,10 @@ int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature, int
   if (rc) return rc;
   if ((int64_t)bytes > s->o.hbm_budget_bytes)
     return fail(PCG_E_NOMEM, "job needs " + std::to_string(bytes) + " bytes, over the budget");
+  size_t fsz = s->o.precision == PCG_FP64 ? 8 : 4;
+  if (LdsLayout((size_t)m->n, fsz, false).total > 150 * 1024)
+    return fail(PCG_E_ARG, "job of " + std::to_string(m->n) +
+                               " residues does not fit the persistent kernel's LDS");
   j->bytes = bytes;
   if (job_id) *job_id = j->id;
   s->queue.push_back(j->id);
@@ -1586,12 +1625,18 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
     // per-job step budget for this round
     std::vector<DevBatch> descs(nres);
     int max_n = 0;
+    size_t fsz = s->o.precision == PCG_FP64 ? 8 : 4;
+    size_t lds = 0;
     for (int k = 0; k < nres; ++k) {
       Job &j = *s->jobs[s->resident[k]];
       DeviceBatch &B = *j.B;
       B.sys[0].steps_target = std::min<int64_t>(j.done + s->o.chunk_steps, j.nsteps);
       HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), sizeof(SysDesc), hipMemcpyHostToDevice,
                             s->stream));
+      // each job stages its bonded data in LDS when that fits on its own
+      size_t with = LdsLayout((size_t)B.max_n, fsz, true).total;
+      B.d.bond_lds = with <= 150 * 1024 ? 1 : 0;
+      lds = std::max(lds, LdsLayout((size_t)B.max_n, fsz, B.d.bond_lds != 0).total);
       descs[k] = B.d;
       max_n = std::max(max_n, B.max_n);
     }
@@ -1602,11 +1647,9 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
     }
     HIPCHK(hipMemcpyAsync(s->d_jobs, descs.data(), nres * sizeof(DevBatch), hipMemcpyHostToDevice,
                           s->stream));
-    size_t fsz = s->o.precision == PCG_FP64 ? 8 : 4;
-    size_t lds = (size_t)max_n * (24 + 3 * fsz) + 64;
     if (lds > 150 * 1024) return fail(PCG_E_ARG, "job too large for the persistent kernel");
     int block = 64;
-    while (block < 512 && block < 2 * max_n) block *= 2;
+    while (block < 1024 && block < 2 * max_n) block *= 2;
     block = std::max(block, 128);
     launch_persistent_jobs(s->d_jobs, nres, s->o.precision, block, s->o.chunk_steps, lds,
                            s->stream);

@@ -54,6 +54,25 @@ __device__ __forceinline__ long long total_key(double d) {
   return k ^ ((k >> 63) & 0x7fffffffffffffffLL);
 }
 
+// Filters of one near pair, from its residue types: req is the largest
+// r_high * (1 + formation_tol) over the contact types the pair can form
+// (bb, bs, sb and its side-side type), compared exactly as the per-type test
+// of loop_over_candidates (multiplication by the inverse distance is
+// monotonic, so rn * req < 1 implies rn * req_type < 1 for every type).
+// Beyond min(max_req, max(rep_cutoff, req)) the pair has no repulsion and no
+// candidate: thr2 is that radius squared in FP32, widened by 1e-5.
+__device__ inline void qa_near_filters(const DevBatch &b, int i1, int i2, double &req, float &thr2) {
+  const pcg_params &p = b.P->p;
+  double one_tol = __dadd_rn(1.0, p.qa_formation_tol);
+  int ss = 4 + b.atype[i1] + 21 * b.atype[i2];
+  req = __dmul_rn(p.qa_lj_r_high[1], one_tol);
+  req = fmax(req, __dmul_rn(p.qa_lj_r_high[2], one_tol));
+  req = fmax(req, __dmul_rn(p.qa_lj_r_high[3], one_tol));
+  req = fmax(req, __dmul_rn(p.qa_lj_r_high[ss], one_tol));
+  double thr = fmin(b.P->qa_max_req, fmax(p.rep_cutoff, req)) * (1.0 + 1e-5);
+  thr2 = (float)(thr * thr);
+}
+
 // The QA free pairs that can become candidates, in slot order (an ordered
 // compaction of the free list by !(orig_dist > max_req_dist), the test that
 // qa_cand_item makes first). Run by one workgroup per system after the free
@@ -83,6 +102,7 @@ __device__ inline void build_qa_near(const DevBatch &b, const SysDesc &sd, SysSt
       b.qh_i[0][h] = b.qf_i[0][sd.qf0 + k];
       b.qh_i[1][h] = b.qf_i[1][sd.qf0 + k];
       b.qh_vac[h] = b.qf_vac[sd.qf0 + k];
+      qa_near_filters(b, b.qh_i[0][h], b.qh_i[1][h], b.qh_req[h], b.qh_thr2[h]);
     }
     __syncthreads();
     if (threadIdx.x == 0) run += tot;
@@ -140,6 +160,7 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
           b.qh_i[0][h] = i1;
           b.qh_i[1][h] = i2;
           b.qh_vac[h] = 0;
+          qa_near_filters(b, i1, i2, b.qh_req[h], b.qh_thr2[h]);
         }
         st.qf_n += 1;
       } else {

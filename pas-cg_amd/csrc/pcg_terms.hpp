@@ -59,24 +59,27 @@ template <typename R> __device__ __forceinline__ void sink_ljR(R depth, R r_low,
 // ---------------------------------------------------------------- terms ----
 
 // tether::eval_forces::iter (cg/src/cg/tether/eval_forces.cpp:18-33)
+// (each *_core takes the item's residues and parameters; *_item reads them
+// from the term's list)
 template <typename R, typename G, typename A>
-__device__ double tether_item(const DevBatch &b, const G &g, A &acc, int k) {
-  const pcg_params &p = b.P->p;
-  int i1 = b.teth_i[0][k], i2 = b.teth_i[1][k];
+__device__ __forceinline__ double tether_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+                                              double nat) {
   v3<R> r12 = cast3<R>(g.r(i2) - g.r(i1));
   R rn = sq(nsq(r12)), V, dV;
-  harmonicR<R>((R)p.tether_H1, (R)p.tether_H2, (R)b.teth_nat[k], rn, V, dV);
+  harmonicR<R>((R)p.tether_H1, (R)p.tether_H2, (R)nat, rn, V, dV);
   v3<R> f = (dV / rn) * r12;
   acc.add(i1, f);
   acc.add(i2, -f);
   return (double)V;
 }
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double tether_item(const DevBatch &b, const G &g, A &acc, int k) {
+  return tether_core<R>(b.P->p, g, acc, b.teth_i[0][k], b.teth_i[1][k], b.teth_nat[k]);
+}
 
 // local_rep::eval_forces::iter (cg/src/cg/local_rep/eval_forces.cpp:11-28)
 template <typename R, typename G, typename A>
-__device__ double lrep_item(const DevBatch &b, const G &g, A &acc, int k) {
-  const pcg_params &p = b.P->p;
-  int i1 = b.lrep_i[0][k], i2 = b.lrep_i[1][k];
+__device__ __forceinline__ double lrep_core(const pcg_params &p, const G &g, A &acc, int i1, int i2) {
   v3<R> r12 = cast3<R>(g.r(i2) - g.r(i1));
   R rn = rsq(nsq(r12)), cut = (R)p.rep_cutoff;
   if ((R)1 > rn * cut) return 0.0;
@@ -86,6 +89,11 @@ __device__ double lrep_item(const DevBatch &b, const G &g, A &acc, int k) {
   acc.add(i1, f);
   acc.add(i2, -f);
   return (double)V;
+}
+
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double lrep_item(const DevBatch &b, const G &g, A &acc, int k) {
+  return lrep_core<R>(b.P->p, g, acc, b.lrep_i[0][k], b.lrep_i[1][k]);
 }
 
 // pauli::eval_forces::iter (cg/src/cg/pauli/eval_forces.cpp:12-30)
@@ -105,14 +113,14 @@ __device__ double pauli_item(const DevBatch &b, const SysDesc &s, const G &g, A 
 }
 
 // nat_cont::eval_forces::iter (cg/src/cg/nat_cont/eval_forces.cpp:11-50).
-// k indexes the current list, which holds indices of the native contacts;
-// active/change_t live once, in the list of all contacts.
+// c indexes the list of all native contacts (where active/change_t live);
+// the state transition is made only when update_state is set, so that a
+// contact evaluated once per residue changes state once.
 template <typename R, typename G, typename A>
-__device__ double nat_cont_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
-                                A &acc, int k, double t) {
+__device__ double nat_cont_core(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
+                                A &acc, int c, double t, bool update_state) {
 #pragma clang fp contract(off)
   const pcg_params &p = b.P->p;
-  int c = b.ncur[k];
   int i1 = b.nc_i[0][c], i2 = b.nc_i[1][c];
   double nat = b.nc_nat[c];
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
@@ -134,6 +142,7 @@ __device__ double nat_cont_item(const DevBatch &b, const SysDesc &s, SysState &s
   v3<R> f = (dV / rn) * r12;
   acc.add(i1, f);
   acc.add(i2, -f);
+  if (!update_state) return (double)V;
   // state transition, exact FP64
   bool now = rnx <= __dmul_rn(__dmul_rn(nat, C216_INV), p.counting_factor);
   if ((uint8_t)now != b.nc_active[c]) {
@@ -144,6 +153,13 @@ __device__ double nat_cont_item(const DevBatch &b, const SysDesc &s, SysState &s
     }
   }
   return (double)V;
+}
+
+// k indexes the current list, which holds indices of the native contacts
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double nat_cont_item(const DevBatch &b, const SysDesc &s, SysState &st,
+                                                const G &g, A &acc, int k, double t) {
+  return nat_cont_core<R>(b, s, st, g, acc, b.ncur[k], t, true);
 }
 
 // const_dh / rel_dh ::iter (cg/src/cg/dh/const.cpp:18-40, rel.cpp:17-40)
@@ -173,13 +189,12 @@ __device__ double dh_item(const DevBatch &b, const SysDesc &s, const G &g, A &ac
 // chir::eval_forces::iter (cg/src/cg/chir/eval_forces.cpp:10-31); f0/f1 are
 // the stored nat_factor/nat_chir fields of the quad.
 template <typename R, typename G, typename A>
-__device__ double chir_item(const DevBatch &b, const G &g, A &acc, int k) {
-  const pcg_params &p = b.P->p;
-  int i1 = b.chir_i[0][k], i2 = b.chir_i[1][k], i3 = b.chir_i[2][k], i4 = b.chir_i[3][k];
+__device__ __forceinline__ double chir_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+                                            int i3, int i4, double f0d, double f1d) {
   v3<double> r1 = g.r(i1), r2 = g.r(i2), r3 = g.r(i3), r4 = g.r(i4);
   v3<R> r12 = cast3<R>(r2 - r1), r23 = cast3<R>(r3 - r2), r34 = cast3<R>(r4 - r3);
   v3<R> x12_23 = cross(r12, r23), x12_34 = cross(r12, r34), x23_34 = cross(r23, r34);
-  R f0 = (R)b.chir_f0[k], f1 = (R)b.chir_f1[k], e = (R)p.e_chi;
+  R f0 = (R)f0d, f1 = (R)f1d, e = (R)p.e_chi;
   R cd = dot(r12, x23_34) * f0 - f1;
   R f = e * cd * f0;
   acc.add(i1, f * x12_23);
@@ -187,6 +202,11 @@ __device__ double chir_item(const DevBatch &b, const G &g, A &acc, int k) {
   acc.add(i3, f * (x12_23 + x12_34));
   acc.add(i4, -(f * x12_23));
   return (double)((R)0.5 * e * cd * cd);
+}
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double chir_item(const DevBatch &b, const G &g, A &acc, int k) {
+  return chir_core<R>(b.P->p, g, acc, b.chir_i[0][k], b.chir_i[1][k], b.chir_i[2][k],
+                      b.chir_i[3][k], b.chir_f0[k], b.chir_f1[k]);
 }
 
 // angle geometry of nat_ang / heur_ang (angles/*/eval_forces.cpp:10-38)
@@ -205,23 +225,27 @@ __device__ __forceinline__ R angle_geom(const G &g, int i1, int i2, int i3, v3<R
 }
 
 template <typename R, typename G, typename A>
-__device__ double nat_ang_item(const DevBatch &b, const G &g, A &acc, int k) {
-  int i1 = b.nang_i[0][k], i2 = b.nang_i[1][k], i3 = b.nang_i[2][k];
+__device__ __forceinline__ double nat_ang_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+                                               int i3, double th0) {
   v3<R> d1, d2, d3;
   R th = angle_geom<R>(g, i1, i2, i3, d1, d2, d3);
-  R CBA = (R)b.P->p.CBA, dth = th - (R)b.nang_th[k], dV = (R)2 * CBA * dth;
+  R CBA = (R)p.CBA, dth = th - (R)th0, dV = (R)2 * CBA * dth;
   acc.add(i1, -(dV * d1));
   acc.add(i2, -(dV * d2));
   acc.add(i3, -(dV * d3));
   return (double)(CBA * dth * dth);
 }
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double nat_ang_item(const DevBatch &b, const G &g, A &acc, int k) {
+  return nat_ang_core<R>(b.P->p, g, acc, b.nang_i[0][k], b.nang_i[1][k], b.nang_i[2][k], b.nang_th[k]);
+}
 
 template <typename R, typename G, typename A>
-__device__ double heur_ang_item(const DevBatch &b, const G &g, A &acc, int k) {
-  int i1 = b.hang_i[0][k], i2 = b.hang_i[1][k], i3 = b.hang_i[2][k], ty = b.hang_t[k];
+__device__ __forceinline__ double heur_ang_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+                                                int i3, int ty) {
   v3<R> d1, d2, d3;
   R th = angle_geom<R>(g, i1, i2, i3, d1, d2, d3);
-  const double(*c)[9] = b.P->p.heur_ang_poly;
+  const double(*c)[9] = p.heur_ang_poly;
   // the coefficients (~1e5) cancel to O(1): evaluated as the reference writes
   // it, in the precision R (callers use FP64)
   R t2 = th * th, t3 = t2 * th, t4 = t3 * th, t5 = t4 * th, t6 = t3 * t3;
@@ -234,12 +258,15 @@ __device__ double heur_ang_item(const DevBatch &b, const G &g, A &acc, int k) {
   acc.add(i3, -(dV * d3));
   return (double)V;
 }
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double heur_ang_item(const DevBatch &b, const G &g, A &acc, int k) {
+  return heur_ang_core<R>(b.P->p, g, acc, b.hang_i[0][k], b.hang_i[1][k], b.hang_i[2][k], b.hang_t[k]);
+}
 
 // cnd / snd ::iter (angles/nat_dih/complex.cpp:10-47, simple.cpp:10-42)
 template <typename R, typename G, typename A>
-__device__ double nat_dih_item(const DevBatch &b, const G &g, A &acc, int k, int variant) {
-  const pcg_params &p = b.P->p;
-  int i1 = b.ndih_i[0][k], i2 = b.ndih_i[1][k], i3 = b.ndih_i[2][k], i4 = b.ndih_i[3][k];
+__device__ __forceinline__ double nat_dih_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+                                               int i3, int i4, double phi0, int variant) {
   v3<double> q1 = g.r(i1), q2 = g.r(i2), q3 = g.r(i3), q4 = g.r(i4);
   v3<R> r12 = cast3<R>(q2 - q1), r23 = cast3<R>(q3 - q2), r34 = cast3<R>(q4 - q3);
   v3<R> x1 = cross(r12, r23), x2 = cross(r23, r34);
@@ -247,7 +274,7 @@ __device__ double nat_dih_item(const DevBatch &b, const G &g, A &acc, int k, int
   v3<R> x1u = x1 * x1rn, x2u = x2 * x2rn;
   R phi = acosc(dot(x1u, x2u));
   if (dot(x1, r34) < (R)0) phi = -phi;
-  R diff = phi - (R)b.ndih_ph[k], V, dVdphi;
+  R diff = phi - (R)phi0, V, dVdphi;
   if (variant == 0) {
     R CDA = (R)p.CDA, CDB = (R)p.CDB;
     V = CDA * ((R)1 - cosR(diff)) + CDB * ((R)1 - cosR((R)3 * diff));
@@ -266,13 +293,17 @@ __device__ double nat_dih_item(const DevBatch &b, const G &g, A &acc, int k, int
   acc.add(i4, -(dVdphi * dr4));
   return (double)V;
 }
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double nat_dih_item(const DevBatch &b, const G &g, A &acc, int k, int variant) {
+  return nat_dih_core<R>(b.P->p, g, acc, b.ndih_i[0][k], b.ndih_i[1][k], b.ndih_i[2][k],
+                         b.ndih_i[3][k], b.ndih_ph[k], variant);
+}
 
 // heur_dih::eval_forces::iter (angles/heur_dih/eval_forces.cpp:10-55)
 template <typename R, typename G, typename A>
-__device__ double heur_dih_item(const DevBatch &b, const G &g, A &acc, int k) {
-  int i1 = b.hdih_i[0][k], i2 = b.hdih_i[1][k], i3 = b.hdih_i[2][k], i4 = b.hdih_i[3][k];
-  int ty = b.hdih_t[k];
-  const double(*c)[9] = b.P->p.heur_dih_coeffs;
+__device__ __forceinline__ double heur_dih_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+                                                int i3, int i4, int ty) {
+  const double(*c)[9] = p.heur_dih_coeffs;
   v3<double> q1 = g.r(i1), q2 = g.r(i2), q3 = g.r(i3), q4 = g.r(i4);
   v3<R> r12 = cast3<R>(q2 - q1), r23 = cast3<R>(q3 - q2), r34 = cast3<R>(q4 - q3);
   v3<R> x1 = cross(r23, r12), x2 = cross(r34, r23);
@@ -296,6 +327,11 @@ __device__ double heur_dih_item(const DevBatch &b, const G &g, A &acc, int k) {
   acc.add(i3, -(dV * dr3));
   acc.add(i4, -(dV * dr4));
   return (double)V;
+}
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double heur_dih_item(const DevBatch &b, const G &g, A &acc, int k) {
+  return heur_dih_core<R>(b.P->p, g, acc, b.hdih_i[0][k], b.hdih_i[1][k], b.hdih_i[2][k],
+                          b.hdih_i[3][k], b.hdih_t[k]);
 }
 
 // ---------------------------------------------------------------- PID ----
@@ -464,6 +500,10 @@ __device__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st
   if (b.qh_vac[h]) return 0.0;
   int i1 = b.qh_i[0][h], i2 = b.qh_i[1][h];
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
+  {  // no repulsion and no candidate beyond the pair's radius
+    float dx = (float)d.x, dy = (float)d.y, dz = (float)d.z;
+    if (dx * dx + dy * dy + dz * dz > b.qh_thr2[h]) return 0.0;
+  }
   double rn = __ddiv_rn(1.0, xnorm3(d));
   v3<double> u{__dmul_rn(d.x, rn), __dmul_rn(d.y, rn), __dmul_rn(d.z, rn)};
   if (1.0 > __dmul_rn(rn, P.qa_max_req)) return 0.0;
@@ -477,6 +517,7 @@ __device__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st
     acc.add(i2, -f);
     V = (double)Vr;
   }
+  if (1.0 > __dmul_rn(rn, b.qh_req[h])) return V;  // beyond every type's formation distance
   double h1u = fabs(xdot(b.nh[3][i1], b.nh[4][i1], b.nh[5][i1], u));
   double h2u = fabs(xdot(b.nh[3][i2], b.nh[4][i2], b.nh[5][i2], u));
   double hh = fabs(__dadd_rn(__dadd_rn(__dmul_rn(b.nh[3][i1], b.nh[3][i2]),

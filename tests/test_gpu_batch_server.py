@@ -240,3 +240,27 @@ def test_batch_graph_replays_match_direct_launches():
             err = np.abs(a.get(s, 0) - g.get(s, 0)).max()
             assert err < 1e-9, (rep, s, err)
             assert abs(a.energy(s)[0] - g.energy(s)[0]) <= 1e-9 * abs(a.energy(s)[0])
+
+
+def test_batch_bonded_paths_follow_oracle():
+    """The three ways the batched kernel evaluates the static bonded terms
+    give the oracle's trajectory (noise off, FP64): chain-local lists staged
+    in LDS (small protein), chain-local lists read from HBM (a protein too
+    large to stage in FP64), and per-term loops for a system with a bonded
+    item that is not chain-local (an extra tether)."""
+    prm = prm_full(qa_on=0)
+    small = ragged_models([70], seed0=120, pulled=())[0]
+    large = ragged_models([1700], seed0=121, pulled=())[0]
+    odd = ragged_models([60], seed0=122, pulled=())[0]
+    odd["teth"] = np.concatenate([odd["teth"], np.array([[5, 40]], np.int32)])
+    for m, steps in ((small, 60), (large, 20), (odd, 60)):
+        b = G.Batch(prm, [m], seeds=[1], precision=G.FP64, steps_per_launch=steps)
+        b.set_noise_scale(0.0)
+        b.run(steps)
+        o = Oracle(prm, m, seed=1)
+        o.set_noise_scale(0.0)
+        tr = o.step(steps)
+        err = np.abs(b.get(0, 0) - o.r()).max()
+        assert err < 1e-8, (m["n"], err)
+        V, _ = b.energy(0)
+        assert abs(V - tr[-1]) <= 1e-9 * abs(tr[-1]), (m["n"], V, tr[-1])
