@@ -181,7 +181,7 @@ __device__ void wg_fix_nl(const DevBatch &b, const SysDesc &sd, SysState &st, co
     if (valid && cls) {
       int i1 = b.nl_i[0][slot], i2 = b.nl_i[1][slot];
       double d = b.nl_d[slot];
-      if (cls & 1) b.ncur[sd.nc0 + run[0] + ranks[0]] = find_contact(b, sd, i1, i2);
+      if (cls & 1) set_current_contact(b, sd.nc0 + run[0] + ranks[0], find_contact(b, sd, i1, i2));
       if (cls & 2) {
         int pos = run[1] + ranks[1];
         if (pos < sd.qfcap) {
@@ -297,8 +297,8 @@ template <typename RF> __device__ __forceinline__ RF shfl_up_rf(RF v, int d) {
 // cross into the next wave go to LDS directly.
 template <typename RF, bool STAGED>
 __device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc &sd, SysState &st,
-                                               const GeoL &g, const BondLds &bl, RF *fx, RF *fy,
-                                               RF *fz, double *vacc, bool want_v) {
+                                               const GeoL &g, const BondLds &bl, double *fx,
+                                               double *fy, double *fz, double *vacc, bool want_v) {
   const pcg_params &p = b.P->p;
   const int n = sd.n, r0 = sd.res0, lane = threadIdx.x & 63;
   double vt = 0, vl = 0, vna = 0, vha = 0, vnd = 0, vhd = 0, vch = 0;
@@ -340,16 +340,16 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc 
       if (lane >= 3) t[c] += s3;
     }
     if (k < n) {
-      atomicAdd(fx + k, t[0]);
-      atomicAdd(fy + k, t[1]);
-      atomicAdd(fz + k, t[2]);
+      atomicAdd(fx + k, (double)t[0]);
+      atomicAdd(fy + k, (double)t[1]);
+      atomicAdd(fz + k, (double)t[2]);
       // shares for residues handled by the next wave (they exist: an item
       // only reaches residues of its own system)
       for (int s = 1; s < 4; ++s)
         if (lane + s >= 64 && (a.c[s][0] != (RF)0 || a.c[s][1] != (RF)0 || a.c[s][2] != (RF)0)) {
-          atomicAdd(fx + k + s, a.c[s][0]);
-          atomicAdd(fy + k + s, a.c[s][1]);
-          atomicAdd(fz + k + s, a.c[s][2]);
+          atomicAdd(fx + k + s, (double)a.c[s][0]);
+          atomicAdd(fy + k + s, (double)a.c[s][1]);
+          atomicAdd(fz + k + s, (double)a.c[s][2]);
         }
     }
   }
@@ -364,11 +364,40 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc 
   }
 }
 
+// Native contacts and QA near pairs, two items per thread per pass with both
+// items' list fields loaded before either is evaluated (their chains of
+// list load -> LDS positions -> exact decisions overlap).
+template <int T, typename R>
+__device__ __forceinline__ void term_loop2(const DevBatch &b, const SysDesc &sd, SysState &st,
+                                           const GeoL &g, AccL<double> &acc, const int *sizes,
+                                           double *vacc, bool want_v) {
+  const int lo = sizes[T], n = sizes[T + 1] - lo, B = blockDim.x;
+  if (n <= 0) return;
+  int k = ((int)threadIdx.x - lo % B + B) % B;
+  double v = 0.0;
+  for (; k < n; k += 2 * B) {
+    const int k2 = k + B;
+    const bool two = k2 < n;
+    if constexpr (T == PCG_NAT_CONT) {
+      NcFetch q1 = nat_cont_fetch(b, sd.nc0 + k), q2{};
+      if (two) q2 = nat_cont_fetch(b, sd.nc0 + k2);
+      v += nat_cont_core<R>(b, sd, st, g, acc, q1, st.t, true);
+      if (two) v += nat_cont_core<R>(b, sd, st, g, acc, q2, st.t, true);
+    } else {
+      QhFetch q1 = qa_cand_fetch(b, sd, k), q2{};
+      if (two) q2 = qa_cand_fetch(b, sd, k2);
+      v += qa_cand_core<R>(b, sd, st, g, acc, q1);
+      if (two) v += qa_cand_core<R>(b, sd, st, g, acc, q2);
+    }
+  }
+  if (want_v && v != 0.0) atomicAdd(&vacc[T], v);
+}
+
 // The items of one term: item k is flat index sizes[T] + k, taken by thread
 // (sizes[T] + k) mod blockDim, as in one loop over all terms.
-template <int T, typename R, typename RF>
+template <int T, typename R>
 __device__ __forceinline__ void term_loop(const DevBatch &b, const SysDesc &sd, SysState &st,
-                                          const GeoL &g, AccL<RF> &acc, const int *sizes,
+                                          const GeoL &g, AccL<double> &acc, const int *sizes,
                                           double *vacc, bool want_v) {
   const int lo = sizes[T], n = sizes[T + 1] - lo, B = blockDim.x;
   if (n <= 0) return;
@@ -388,9 +417,11 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   const int n = sd.n, r0 = sd.res0;
   extern __shared__ __align__(16) unsigned char lds[];
   const bool staged = sd.bond_regular && b.bond_lds;
-  const LdsLayout lay((size_t)n, sizeof(RF), staged);
+  // forces accumulate in FP64 in LDS: on gfx950 an LDS FP64 atomic add
+  // (ds_add_f64) runs at integer-atomic speed, an FP32 one far slower
+  const LdsLayout lay((size_t)n, sizeof(double), staged);
   double *rx = (double *)lds, *ry = rx + n, *rz = ry + n;
-  RF *fx = (RF *)(lds + lay.f), *fy = fx + n, *fz = fy + n;
+  double *fx = (double *)(lds + lay.f), *fy = fx + n, *fz = fy + n;
   BondLds bl{(double *)(lds + lay.bd), (uint8_t *)(lds + lay.bu), n};
   __shared__ double vacc[NT];
   __shared__ int sizes[NT + 1];
@@ -402,7 +433,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     rz[k] = b.y[0][2][r0 + k];
   }
   GeoL g{rx, ry, rz, r0};
-  AccL<RF> acc{fx, fy, fz, r0};
+  AccL<double> acc{fx, fy, fz, r0};
   if (staged) stage_bonded(b, sd, bl);
   __syncthreads();
 
@@ -467,11 +498,11 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
       term_loop<PCG_CND, R>(b, sd, st, g, acc, sizes, vacc, wv);
       term_loop<PCG_SND, R>(b, sd, st, g, acc, sizes, vacc, wv);
     }
-    term_loop<PCG_NAT_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop2<PCG_NAT_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_PAULI, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_CONST_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_REL_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    term_loop<PCG_QA_CAND, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop2<PCG_QA_CAND, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_QA_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_PID, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_VEL_AFM, R>(b, sd, st, g, acc, sizes, vacc, wv);
@@ -514,9 +545,9 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   // forces of the last evaluation back to HBM
   if (!evaluated) return;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    ((RF *)b.F[0])[r0 + k] = fx[k];
-    ((RF *)b.F[1])[r0 + k] = fy[k];
-    ((RF *)b.F[2])[r0 + k] = fz[k];
+    ((RF *)b.F[0])[r0 + k] = (RF)fx[k];
+    ((RF *)b.F[1])[r0 + k] = (RF)fy[k];
+    ((RF *)b.F[2])[r0 + k] = (RF)fz[k];
   }
 }
 

@@ -109,6 +109,7 @@ struct DevBatch {
   // dynamic lists
   int *nl_i[2]; double *nl_d; uint8_t *nl_cls;  // class bits of the partition
   int *ncur;                                      // current native contacts (index into all)
+  int *ncur_i[2]; double *ncur_nat; uint8_t *ncur_t;  // their residues, length, type
   // bond_item[L][r]: global index of the item of list L whose first residue
   // is r, or -1 (systems with bond_regular set)
   const int *bond_item[NBOND];

@@ -475,6 +475,10 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.nc_active, std::max(nnc, 1LL));
   A.add(&D.nc_change_t, std::max(nnc, 1LL));
   A.add(&D.ncur, std::max(nnc, 1LL));
+  A.add(&D.ncur_i[0], std::max(nnc, 1LL));
+  A.add(&D.ncur_i[1], std::max(nnc, 1LL));
+  A.add(&D.ncur_nat, std::max(nnc, 1LL));
+  A.add(&D.ncur_t, std::max(nnc, 1LL));
   // chain-local bonded items by base residue (see BOND_* in pcg_core.hpp)
   std::vector<int> bond_item((size_t)NBOND * std::max(ntot, 1LL), -1);
   for (int s = 0; s < nsys; ++s) {
@@ -1174,8 +1178,7 @@ struct pcg_batch {
 namespace {
 
 size_t lds_for(const DeviceBatch &B, bool staged) {
-  size_t fsz = B.prec == PCG_FP64 ? 8 : 4;
-  return LdsLayout((size_t)B.max_n, fsz, staged).total;
+  return LdsLayout((size_t)B.max_n, sizeof(double), staged).total;  // FP64 LDS forces
 }
 
 // steps owed by every system = current + nsteps
@@ -1580,8 +1583,7 @@ int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature, int
   if (rc) return rc;
   if ((int64_t)bytes > s->o.hbm_budget_bytes)
     return fail(PCG_E_NOMEM, "job needs " + std::to_string(bytes) + " bytes, over the budget");
-  size_t fsz = s->o.precision == PCG_FP64 ? 8 : 4;
-  if (LdsLayout((size_t)m->n, fsz, false).total > 150 * 1024)
+  if (LdsLayout((size_t)m->n, sizeof(double), false).total > 150 * 1024)
     return fail(PCG_E_ARG, "job of " + std::to_string(m->n) +
                                " residues does not fit the persistent kernel's LDS");
   j->bytes = bytes;
@@ -1625,7 +1627,7 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
     // per-job step budget for this round
     std::vector<DevBatch> descs(nres);
     int max_n = 0;
-    size_t fsz = s->o.precision == PCG_FP64 ? 8 : 4;
+    const size_t fsz = sizeof(double);  // the persistent kernel's LDS forces are FP64
     size_t lds = 0;
     for (int k = 0; k < nres; ++k) {
       Job &j = *s->jobs[s->resident[k]];

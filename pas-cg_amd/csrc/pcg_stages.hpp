@@ -20,6 +20,16 @@ __device__ inline int find_contact(const DevBatch &b, const SysDesc &s, int i1, 
   return -1;
 }
 
+// Slot q of the current native-contact list takes contact c of the list of
+// all contacts, with a copy of its fields.
+__device__ __forceinline__ void set_current_contact(const DevBatch &b, int q, int c) {
+  b.ncur[q] = c;
+  b.ncur_i[0][q] = b.nc_i[0][c];
+  b.ncur_i[1][q] = b.nc_i[1][c];
+  b.ncur_nat[q] = b.nc_nat[c];
+  b.ncur_t[q] = b.nc_type[c];
+}
+
 __device__ inline int classify(const DevBatch &b, const SysDesc &s, const SysState &st,
                                         int slot) {
   const DevParams &P = *b.P;

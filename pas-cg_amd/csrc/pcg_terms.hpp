@@ -116,19 +116,25 @@ __device__ double pauli_item(const DevBatch &b, const SysDesc &s, const G &g, A 
 // c indexes the list of all native contacts (where active/change_t live);
 // the state transition is made only when update_state is set, so that a
 // contact evaluated once per residue changes state once.
+struct NcFetch {
+  int c, i1, i2;
+  double nat;
+  int type;
+};
 template <typename R, typename G, typename A>
-__device__ double nat_cont_core(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
-                                A &acc, int c, double t, bool update_state) {
+__device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc &s, SysState &st,
+                                                const G &g, A &acc, const NcFetch &q, double t,
+                                                bool update_state) {
 #pragma clang fp contract(off)
   const pcg_params &p = b.P->p;
-  int i1 = b.nc_i[0][c], i2 = b.nc_i[1][c];
-  double nat = b.nc_nat[c];
+  const int c = q.c, i1 = q.i1, i2 = q.i2;
+  const double nat = q.nat;
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
   double rnx = xnorm3(d);
   if (rnx >= p.nl_cutoff) return 0.0;
   v3<R> r12 = cast3<R>(d);
   R rn = (R)rnx, V, dV;
-  if (p.nc_ss_kind != 0 && b.nc_type[c] == 5) {
+  if (p.nc_ss_kind != 0 && q.type == 5) {
     // disulfide_force::operator() receives the distance as its inverse-distance
     // argument (nat_cont/eval_forces.cpp:27, disulfide.h:12-20)
     if (p.nc_ss_kind == 1)
@@ -155,11 +161,16 @@ __device__ double nat_cont_core(const DevBatch &b, const SysDesc &s, SysState &s
   return (double)V;
 }
 
-// k indexes the current list, which holds indices of the native contacts
+// Item k of the current list: it carries the contact's index into the list
+// of all contacts and a copy of its residues, length and type (independent
+// loads, issued together).
+__device__ __forceinline__ NcFetch nat_cont_fetch(const DevBatch &b, int k) {
+  return NcFetch{b.ncur[k], b.ncur_i[0][k], b.ncur_i[1][k], b.ncur_nat[k], b.ncur_t[k]};
+}
 template <typename R, typename G, typename A>
 __device__ __forceinline__ double nat_cont_item(const DevBatch &b, const SysDesc &s, SysState &st,
                                                 const G &g, A &acc, int k, double t) {
-  return nat_cont_core<R>(b, s, st, g, acc, b.ncur[k], t, true);
+  return nat_cont_core<R>(b, s, st, g, acc, nat_cont_fetch(b, k), t, true);
 }
 
 // const_dh / rel_dh ::iter (cg/src/cg/dh/const.cpp:18-40, rel.cpp:17-40)
@@ -489,20 +500,29 @@ __device__ __forceinline__ double xdot(double ax, double ay, double az, v3<doubl
 // qa::loop_over_candidates::iter (cg/src/cg/qa/loop_over_candidates.cpp:12-99):
 // criteria in exact FP64, repulsion force in R; a candidate is appended to
 // the system's candidate buffer (its order is restored by the finish sort).
-// Item hk of the near list (every entry passed orig_dist <= max_req_dist).
+// Item hk of the near list (every entry passed orig_dist <= max_req_dist):
+// its fields are loaded together, ahead of the geometry.
+struct QhFetch {
+  int hk, i1, i2;
+  float thr2;
+  int vac;
+};
+__device__ __forceinline__ QhFetch qa_cand_fetch(const DevBatch &b, const SysDesc &s, int hk) {
+  int h = s.qf0 + hk;
+  return QhFetch{hk, b.qh_i[0][h], b.qh_i[1][h], b.qh_thr2[h], b.qh_vac[h]};
+}
 template <typename R, typename G, typename A>
-__device__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
-                               A &acc, int hk) {
+__device__ __forceinline__ double qa_cand_core(const DevBatch &b, const SysDesc &s, SysState &st,
+                                               const G &g, A &acc, const QhFetch &q) {
 #pragma clang fp contract(off)
   const DevParams &P = *b.P;
   const pcg_params &p = P.p;
-  int h = s.qf0 + hk;
-  if (b.qh_vac[h]) return 0.0;
-  int i1 = b.qh_i[0][h], i2 = b.qh_i[1][h];
+  if (q.vac) return 0.0;
+  const int hk = q.hk, h = s.qf0 + hk, i1 = q.i1, i2 = q.i2;
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
   {  // no repulsion and no candidate beyond the pair's radius
     float dx = (float)d.x, dy = (float)d.y, dz = (float)d.z;
-    if (dx * dx + dy * dy + dz * dz > b.qh_thr2[h]) return 0.0;
+    if (dx * dx + dy * dy + dz * dz > q.thr2) return 0.0;
   }
   double rn = __ddiv_rn(1.0, xnorm3(d));
   v3<double> u{__dmul_rn(d.x, rn), __dmul_rn(d.y, rn), __dmul_rn(d.z, rn)};
@@ -563,6 +583,11 @@ __device__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st
     b.cand_sd[8 * cs + 4 + a] = s2[a];
   }
   return V;
+}
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st,
+                                               const G &g, A &acc, int hk) {
+  return qa_cand_core<R>(b, s, st, g, acc, qa_cand_fetch(b, s, hk));
 }
 
 // qa::process_contacts::iter (cg/src/cg/qa/process_contacts.cpp:12-72)
