@@ -393,6 +393,42 @@ __device__ __forceinline__ void term_loop2(const DevBatch &b, const SysDesc &sd,
   if (want_v && v != 0.0) atomicAdd(&vacc[T], v);
 }
 
+// QA near pairs in two phases per pass of 4 items per thread: every pair
+// takes the cheap first test, the few that pass are queued in LDS, then the
+// queue is evaluated exactly, densely (a wave no longer waits on the exact
+// path of the one lane in it that needs it, item after item).
+template <typename R>
+__device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, const SysDesc &sd, SysState &st,
+                                                  const GeoL &g, AccL<double> &acc, const int *sizes,
+                                                  int *queue, double *vacc, bool want_v) {
+  const int n = sizes[PCG_QA_CAND + 1] - sizes[PCG_QA_CAND], B = blockDim.x;
+  const int lane = threadIdx.x & 63;
+  __shared__ int qcount;
+  double v = 0.0;
+  for (int base = 0; base < n; base += 4 * B) {
+    if (threadIdx.x == 0) qcount = 0;
+    __syncthreads();
+    QhFetch q[4];
+    for (int j = 0; j < 4; ++j) {
+      int hk = base + j * B + (int)threadIdx.x;
+      q[j] = hk < n ? qa_cand_fetch(b, sd, hk) : QhFetch{hk, 0, 0, 0.f, 1};
+    }
+    for (int j = 0; j < 4; ++j) {
+      bool keep = q[j].hk < n && qa_cand_may_act(sd, g, q[j]);
+      unsigned long long m = __ballot(keep);
+      int off = 0;
+      if (lane == 0 && m) off = atomicAdd(&qcount, __popcll(m));
+      off = __shfl(off, 0, 64);
+      if (keep) queue[off + __popcll(m & ((1ull << lane) - 1ull))] = q[j].hk;
+    }
+    __syncthreads();
+    const int m = qcount;
+    for (int e = threadIdx.x; e < m; e += B) v += qa_cand_exact<R>(b, sd, st, g, acc, qa_cand_fetch(b, sd, queue[e]));
+    __syncthreads();
+  }
+  if (want_v && v != 0.0) atomicAdd(&vacc[PCG_QA_CAND], v);
+}
+
 // The items of one term: item k is flat index sizes[T] + k, taken by thread
 // (sizes[T] + k) mod blockDim, as in one loop over all terms.
 template <int T, typename R>
@@ -423,6 +459,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   double *rx = (double *)lds, *ry = rx + n, *rz = ry + n;
   double *fx = (double *)(lds + lay.f), *fy = fx + n, *fz = fy + n;
   BondLds bl{(double *)(lds + lay.bd), (uint8_t *)(lds + lay.bu), n};
+  int *qa_queue = (int *)(lds + lay.q);
   __shared__ double vacc[NT];
   __shared__ int sizes[NT + 1];
   __shared__ int flag_rebuild, stop;
@@ -502,7 +539,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     term_loop<PCG_PAULI, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_CONST_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_REL_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    term_loop2<PCG_QA_CAND, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    if (P.p.qa_on) qa_cand_two_phase<R>(b, sd, st, g, acc, sizes, qa_queue, vacc, wv);
     term_loop<PCG_QA_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_PID, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_VEL_AFM, R>(b, sd, st, g, acc, sizes, vacc, wv);

@@ -44,12 +44,14 @@ struct SysDesc {
 // the two chiral fields; 3 n bytes: flags, heuristic angle and dihedral
 // types).
 struct LdsLayout {
-  size_t f, bd, bu, total;
+  static constexpr size_t QA_QUEUE = 4 * 1024;  // QA pairs past the first test, per pass
+  size_t f, bd, bu, q, total;
   __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged) {
     f = 24 * n;
     bd = (f + 3 * n * fsz + 15) / 16 * 16;
     bu = bd + 40 * n;
-    total = staged ? bu + 3 * n + 64 : bd + 64;
+    q = ((staged ? bu + 3 * n : bd) + 15) / 16 * 16;
+    total = q + 4 * QA_QUEUE + 64;
   }
 };
 

@@ -511,19 +511,25 @@ __device__ __forceinline__ QhFetch qa_cand_fetch(const DevBatch &b, const SysDes
   int h = s.qf0 + hk;
   return QhFetch{hk, b.qh_i[0][h], b.qh_i[1][h], b.qh_thr2[h], b.qh_vac[h]};
 }
+// The cheap first test: a vacant pair, or one beyond its radius, has no
+// effect at all.
+template <typename G>
+__device__ __forceinline__ bool qa_cand_may_act(const SysDesc &s, const G &g, const QhFetch &q) {
+  if (q.vac) return false;
+  v3<double> d = xwrapd(xsub(g.r(q.i2), g.r(q.i1)), s);
+  float dx = (float)d.x, dy = (float)d.y, dz = (float)d.z;
+  return !(dx * dx + dy * dy + dz * dz > q.thr2);
+}
+
+// Everything after the first test, in exact FP64 as the reference.
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double qa_cand_core(const DevBatch &b, const SysDesc &s, SysState &st,
-                                               const G &g, A &acc, const QhFetch &q) {
+__device__ __forceinline__ double qa_cand_exact(const DevBatch &b, const SysDesc &s, SysState &st,
+                                                const G &g, A &acc, const QhFetch &q) {
 #pragma clang fp contract(off)
   const DevParams &P = *b.P;
   const pcg_params &p = P.p;
-  if (q.vac) return 0.0;
   const int hk = q.hk, h = s.qf0 + hk, i1 = q.i1, i2 = q.i2;
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
-  {  // no repulsion and no candidate beyond the pair's radius
-    float dx = (float)d.x, dy = (float)d.y, dz = (float)d.z;
-    if (dx * dx + dy * dy + dz * dz > q.thr2) return 0.0;
-  }
   double rn = __ddiv_rn(1.0, xnorm3(d));
   v3<double> u{__dmul_rn(d.x, rn), __dmul_rn(d.y, rn), __dmul_rn(d.z, rn)};
   if (1.0 > __dmul_rn(rn, P.qa_max_req)) return 0.0;
@@ -583,6 +589,12 @@ __device__ __forceinline__ double qa_cand_core(const DevBatch &b, const SysDesc 
     b.cand_sd[8 * cs + 4 + a] = s2[a];
   }
   return V;
+}
+template <typename R, typename G, typename A>
+__device__ __forceinline__ double qa_cand_core(const DevBatch &b, const SysDesc &s, SysState &st,
+                                               const G &g, A &acc, const QhFetch &q) {
+  if (!qa_cand_may_act(s, g, q)) return 0.0;
+  return qa_cand_exact<R>(b, s, st, g, acc, q);
 }
 template <typename R, typename G, typename A>
 __device__ __forceinline__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st,
