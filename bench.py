@@ -124,6 +124,26 @@ def force_error(p, models, bat, picks):
     return worst_f, worst_v
 
 
+def batched_force_error(p, models, bat, picks):
+    """Total force of the batched kernel in mixed precision against the same
+    kernel in FP64, on the timed batch's current positions (one step with
+    the noise off evaluates the forces there)."""
+    from pas_cg_amd import pcg as G
+    sub = [models[s] for s in picks]
+    pos = [bat["b"].get(s, 0) for s in picks]
+    F = {}
+    for prec in (G.FP64, G.MIXED):
+        b = G.Batch(p, sub, precision=prec, philox_seed=448, block=bat["block"],
+                    steps_per_launch=1)
+        b.set_noise_scale(0.0)
+        for k, r in enumerate(pos):
+            b.set(k, 0, r)
+        b.run(1)
+        F[prec] = [b.get(k, 6) for k in range(len(sub))]
+        b.close()
+    return max(float(np.abs(a - c).max() / np.abs(c).max()) for a, c in zip(F[G.MIXED], F[G.FP64]))
+
+
 def plain(p, models, picks, steps=200):
     from pas_cg_amd import pcg as G
     rs, tt, per = 0, 0.0, []
@@ -254,7 +274,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--steps-per-launch", type=int, default=0)
+    ap.add_argument("--steps-per-launch", type=int, default=500)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--pad", type=float, default=2.0, help="NL pad (skin), A, with --no-tune")
     ap.add_argument("--no-tune", action="store_true",
@@ -290,7 +310,7 @@ def main():
         pad, block = args.pad, args.block
     else:
         from pas_cg_amd import tune
-        tuning = tune.tune(p, models, log=log)
+        tuning = tune.tune(p, models, trial_steps=args.steps_per_launch, log=log)
         pad, block = tuning["pad_A"], tuning["block"]
     args.pad = pad
     p.update(nl_pad=pad * W.A)
@@ -325,6 +345,7 @@ def main():
         picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
         ef, ev = force_error(p, models, bat, picks)
         out["force_err_vs_fp64"] = {"max": max(ef.values()),
+                                    "batched_total": batched_force_error(p, models, bat, picks),
                                     "per_term": {k: float(f"{v:.3g}") for k, v in ef.items()},
                                     "energy_per_term": {k: float(f"{v:.3g}")
                                                         for k, v in ev.items()}}

@@ -227,6 +227,9 @@ int pcg_batch_run(pcg_batch *b, int32_t nsteps);
 int pcg_batch_sync(pcg_batch *b);
 int pcg_batch_set_noise_scale(pcg_batch *b, double scale); /* 0: no noise */
 int pcg_batch_get(pcg_batch *b, int32_t sys, int32_t which, double *out);
+/* which: 0..5 Gear y0..y5 of one system; setting positions (0) makes its
+ * next step rebuild the neighbour list */
+int pcg_batch_set(pcg_batch *b, int32_t sys, int32_t which, const double *in);
 int pcg_batch_energy(pcg_batch *b, int32_t sys, double *V, double *term_V);
 int pcg_batch_info(pcg_batch *b, int32_t *block, int32_t *steps_per_launch,
                    int64_t *device_bytes);

@@ -311,7 +311,8 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc 
       int r = r0 + k;
       uint8_t f = bl.flag(k);
       if (p.tether_on && (f & BF_TETHER)) vt += tether_core<double>(p, g, a, r, r + 1, bl.teth(k));
-      if (p.lrep_on && (f & BF_LREP)) vl += lrep_core<double>(p, g, a, r, r + 2);
+      // local repulsion and chirality have no ill-conditioned step: in R
+      if (p.lrep_on && (f & BF_LREP)) vl += lrep_core<RF>(p, g, a, r, r + 2);
       if (p.nat_ang_on && (f & BF_NAT_ANG)) vna += nat_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.ang(k));
       if (p.heur_ang_on && (f & BF_HEUR_ANG)) vha += heur_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.at(k));
       if (p.nat_dih_on && (f & BF_NAT_DIH))
@@ -319,7 +320,7 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc 
       if (p.heur_dih_on && (f & BF_HEUR_DIH))
         vhd += heur_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dt(k));
       if (p.chir_on && (f & BF_CHIR))
-        vch += chir_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.c0(k), bl.c1(k));
+        vch += chir_core<RF>(p, g, a, r, r + 1, r + 2, r + 3, bl.c0(k), bl.c1(k));
     } else if (!STAGED && k < n) {
       int r = r0 + k, it;
       if (p.tether_on && (it = b.bond_item[BOND_TETHER][r]) >= 0) vt += tether_item<double>(b, g, a, it);

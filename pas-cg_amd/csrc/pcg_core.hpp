@@ -167,6 +167,8 @@ __device__ __forceinline__ float sinR(float x) { return sinf(x); }
 __device__ __forceinline__ double sinR(double x) { return sin(x); }
 __device__ __forceinline__ float cosR(float x) { return cosf(x); }
 __device__ __forceinline__ double cosR(double x) { return cos(x); }
+__device__ __forceinline__ void sincosR(float x, float &s, float &c) { sincosf(x, &s, &c); }
+__device__ __forceinline__ void sincosR(double x, double &s, double &c) { sincos(x, &s, &c); }
 __device__ __forceinline__ float expR(float x) { return expf(x); }
 __device__ __forceinline__ double expR(double x) { return exp(x); }
 

@@ -1317,6 +1317,28 @@ int pcg_batch_get(pcg_batch *b, int32_t sys, int32_t which, double *out) {
   return get_cols(B, sys, B.d.vel, out);
 }
 
+int pcg_batch_set(pcg_batch *b, int32_t sys, int32_t which, const double *in) {
+  if (!b || !in || sys < 0 || sys >= (int)b->B->sys.size() || which < 0 || which > 5)
+    return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *b->B;
+  const SysDesc &d = B.sys[sys];
+  std::vector<double> tmp(3 * (size_t)d.n);
+  for (int i = 0; i < d.n; ++i)
+    for (int c2 = 0; c2 < 3; ++c2) tmp[c2 * d.n + i] = in[3 * i + c2];
+  for (int c2 = 0; c2 < 3; ++c2)
+    HIPCHK(hipMemcpyAsync(B.d.y[which][c2] + d.res0, tmp.data() + c2 * d.n, d.n * 8,
+                          hipMemcpyHostToDevice, B.stream));
+  if (which == 0) {  // moved residues: the next step rebuilds the neighbour list
+    SysState st;
+    int rc = read_state(B, sys, st);
+    if (rc) return rc;
+    st.verify_first = 1;
+    if ((rc = write_state(B, sys, st))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
 int pcg_batch_energy(pcg_batch *b, int32_t sys, double *V, double *term_V) {
   if (!b || sys < 0 || sys >= (int)b->B->sys.size()) return fail(PCG_E_ARG, "bad arguments");
   SysState st;

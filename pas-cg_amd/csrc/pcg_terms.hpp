@@ -287,9 +287,12 @@ __device__ __forceinline__ double nat_dih_core(const pcg_params &p, const G &g, 
   if (dot(x1, r34) < (R)0) phi = -phi;
   R diff = phi - (R)phi0, V, dVdphi;
   if (variant == 0) {
-    R CDA = (R)p.CDA, CDB = (R)p.CDB;
-    V = CDA * ((R)1 - cosR(diff)) + CDB * ((R)1 - cosR((R)3 * diff));
-    dVdphi = CDA * sinR(diff) + CDB * (R)3 * sinR((R)3 * diff);
+    // one sincos; the triple angle by cos 3x = c (4c^2 - 3), sin 3x = s (3 - 4s^2)
+    R CDA = (R)p.CDA, CDB = (R)p.CDB, s1, c1;
+    sincosR(diff, s1, c1);
+    R c3 = c1 * ((R)4 * c1 * c1 - (R)3), s3 = s1 * ((R)3 - (R)4 * s1 * s1);
+    V = CDA * ((R)1 - c1) + CDB * ((R)1 - c3);
+    dVdphi = CDA * s1 + CDB * (R)3 * s3;
   } else {
     V = (R)0.5 * (R)p.CDH * diff * diff;
     dVdphi = (R)p.CDH * diff;

@@ -121,6 +121,7 @@ SIGNATURES = {
                          C.POINTER(I), C.POINTER(PcgBatchOptions), C.POINTER(_V)],
     "pcg_batch_destroy": [_V], "pcg_batch_run": [_V, I], "pcg_batch_sync": [_V],
     "pcg_batch_get": [_V, I, I, C.POINTER(D)],
+    "pcg_batch_set": [_V, I, I, C.POINTER(D)],
     "pcg_batch_energy": [_V, I, C.POINTER(D), C.POINTER(D)],
     "pcg_batch_info": [_V, C.POINTER(I), C.POINTER(I), C.POINTER(I64)],
     "pcg_batch_last_ms": [_V, C.POINTER(D), C.POINTER(I)],
@@ -413,6 +414,10 @@ class Batch:
         out = np.zeros((self.ns[sys], 3))
         check(self.L.pcg_batch_get(self.h, sys, which, _p(out, D)))
         return out
+
+    def set(self, sys: int, which: int, a):
+        a = np.ascontiguousarray(np.asarray(a, np.float64))
+        check(self.L.pcg_batch_set(self.h, sys, which, _p(a, D)))
 
     def energy(self, sys: int):
         V = C.c_double()

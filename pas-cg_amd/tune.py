@@ -14,7 +14,7 @@ from . import pcg as G
 from . import params as P
 
 
-def tune(params: dict, models: list, pads_A=(2.0, 3.0, 4.0, 5.0, 6.0), trial_steps: int = 200,
+def tune(params: dict, models: list, pads_A=(4.0, 6.0, 8.0, 10.0, 12.0), trial_steps: int = 500,
          precision: int = G.MIXED, philox_seed: int = 1, log=None) -> dict:
     """Best (pad, block) by measured time per step; returns the choice and
     the whole table."""

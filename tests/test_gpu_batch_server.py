@@ -264,3 +264,31 @@ def test_batch_bonded_paths_follow_oracle():
         assert err < 1e-8, (m["n"], err)
         V, _ = b.energy(0)
         assert abs(V - tr[-1]) <= 1e-9 * abs(tr[-1]), (m["n"], V, tr[-1])
+
+
+def test_batch_set_positions_then_forces_match_oracle():
+    """pcg_batch_set moves a system (and makes it rebuild its lists): the
+    next step's forces match the oracle's at the new positions."""
+    prm = prm_full(qa_on=0)
+    models = ragged_models([90, 44], seed0=140)
+    b = G.Batch(prm, models, seeds=[1, 1], precision=G.MIXED, steps_per_launch=1)
+    b.set_noise_scale(0.0)
+    b.run(3)
+    rng = np.random.default_rng(3)
+    for s, m in enumerate(models):
+        r = m["pos"] + rng.normal(scale=0.4 * P.ANGSTROM, size=m["pos"].shape)
+        b.set(s, 0, r)
+    b.run(1)
+    # the oracle evaluated at the same positions (and time, for the AFM tip)
+    rng = np.random.default_rng(3)
+    for s, m in enumerate(models):
+        r = m["pos"] + rng.normal(scale=0.4 * P.ANGSTROM, size=m["pos"].shape)
+        o = Oracle(prm, m, seed=1)
+        o.set_r(r)
+        o.set_t(3 * P.f32(5e-3))
+        o.fix_nl()
+        o.eval_forces()
+        F1 = o.F()
+        F2 = b.get(s, 6)
+        err = np.abs(F2 - F1).max() / np.abs(F1).max()
+        assert err <= 1e-5, (s, err)
