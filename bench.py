@@ -163,6 +163,38 @@ def plain(p, models, picks, steps=200):
     return rs / tt, per
 
 
+def other_configs(c2, c1):
+    """The other configurations, briefly: 1024 replicas of the 56-residue
+    chain in one batch, and the 6400-residue periodic box through the plain
+    (stage-kernel) path."""
+    from pas_cg_amd import pcg as G
+    out = {}
+    p, models = c2
+    b = G.Batch(p, models, precision=G.MIXED, philox_seed=448, seeds=list(range(len(models))),
+                steps_per_launch=500)
+    block, _ = b.autotune(500)
+    b.run(500)
+    t0 = time.perf_counter()
+    b.run(2000)
+    dt = time.perf_counter() - t0
+    n = sum(int(m["n"]) for m in models)
+    out["replicas_1024x56"] = {"residue_steps_per_s": round(n * 2000 / dt, 1), "block": block,
+                               "steps": 2000}
+    b.close()
+    p, m = c1
+    c = G.Context(p, m, precision=G.MIXED, noise=G.PHILOX, seed=1)
+    c.use_graph(True)
+    c.step(50)
+    t0 = time.perf_counter()
+    c.step(300)
+    dt = time.perf_counter() - t0
+    out["box_64x100_1mM_plain"] = {"residue_steps_per_s": round(m["n"] * 300 / dt, 1),
+                                   "steps": 300}
+    c.close()
+    log(f"other: {out}")
+    return out
+
+
 def fifo_latency(jobs, per):
     """One-job-at-a-time plain GPU path as a FIFO queue, with each job's
     service time from the plain per-step times measured above (linear in N)."""
@@ -284,6 +316,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the 1024-replica and periodic-box configurations")
     ap.add_argument("--gen-workers", type=int, default=8,
                     help="processes generating the synthetic proteins (1: in-process)")
     ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
@@ -304,6 +338,7 @@ def main():
                           "GBs": round(gbs, 1), "ms_per_copy": round(ms, 4)}), flush=True)
         return
     p, models = W.c3(workers=args.gen_workers)
+    extra = None if args.no_other else (W.c2(), W.c1())
     pool = None if args.no_serve else serve_pool(min(args.gen_workers, os.cpu_count() or 1))
     tuning = None
     if args.no_tune:
@@ -360,6 +395,8 @@ def main():
     if not args.no_serve:
         out["served"] = served(W.full_params(args.pad), value, args.serve_seconds, pool,
                                per_plain)
+    if extra is not None:
+        out["other_configs"] = other_configs(*extra)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, models, [int(i) for i in order[16::32]])
     print(json.dumps(out), flush=True)
