@@ -448,8 +448,12 @@ template <typename R, typename RF>
 __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int nsteps) {
 #pragma clang fp contract(off)
   const SysDesc &sd = b.sys[s];
-  SysState &st = b.st[s];
-  if (!st.active) return;
+  if (!b.st[s].active) return;
+  // the system's scalars live in LDS for the launch (only this workgroup
+  // touches them) and go back to HBM at its end
+  __shared__ SysState st;
+  if (threadIdx.x == 0) st = b.st[s];
+  __syncthreads();
   const DevParams &P = *b.P;
   const int n = sd.n, r0 = sd.res0;
   extern __shared__ __align__(16) unsigned char lds[];
@@ -580,7 +584,9 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     }
     __syncthreads();
   }
-  // forces of the last evaluation back to HBM
+  // scalars and the forces of the last evaluation back to HBM
+  __syncthreads();
+  if (threadIdx.x == 0) b.st[s] = st;
   if (!evaluated) return;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     ((RF *)b.F[0])[r0 + k] = (RF)fx[k];
