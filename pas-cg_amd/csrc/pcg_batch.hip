@@ -460,11 +460,13 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   const bool staged = sd.bond_regular && b.bond_lds;
   // forces accumulate in FP64 in LDS: on gfx950 an LDS FP64 atomic add
   // (ds_add_f64) runs at integer-atomic speed, an FP32 one far slower
-  const LdsLayout lay((size_t)n, sizeof(double), staged);
+  const bool gear_l = b.gear_lds != 0;
+  const LdsLayout lay((size_t)n, sizeof(double), staged, gear_l);
   double *rx = (double *)lds, *ry = rx + n, *rz = ry + n;
   double *fx = (double *)(lds + lay.f), *fy = fx + n, *fz = fy + n;
   BondLds bl{(double *)(lds + lay.bd), (uint8_t *)(lds + lay.bu), n};
   int *qa_queue = (int *)(lds + lay.q);
+  double *ys = (double *)(lds + lay.g);  // [15][n]: y1..y5 x, y, z (gear_l)
   __shared__ double vacc[NT];
   __shared__ int sizes[NT + 1];
   __shared__ int flag_rebuild, stop;
@@ -477,6 +479,9 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   GeoL g{rx, ry, rz, r0};
   AccL<double> acc{fx, fy, fz, r0};
   if (staged) stage_bonded(b, sd, bl);
+  if (gear_l)
+    for (int k = threadIdx.x; k < n; k += blockDim.x)
+      for (int j = 0; j < 15; ++j) ys[j * n + k] = b.y[1 + j / 3][j % 3][r0 + k];
   __syncthreads();
 
   __shared__ int want_v;
@@ -558,7 +563,14 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
         double nz[3];
         philox_normals(sd.philox_key, st.step, k, nz);
         double r[3];
-        gear_update(b, sd, st, i, (double)fx[k], (double)fy[k], (double)fz[k], nz, r);
+        if (gear_l) {
+          double vel[3];
+          gear_core(b, sd, st, i, fx[k], fy[k], fz[k], nz, r, vel, [&](int kk, int c) -> double & {
+            return kk == 0 ? (c == 0 ? rx[k] : c == 1 ? ry[k] : rz[k]) : ys[((kk - 1) * 3 + c) * n + k];
+          });
+        } else {
+          gear_update(b, sd, st, i, fx[k], fy[k], fz[k], nz, r);
+        }
         rx[k] = r[0];
         ry[k] = r[1];
         rz[k] = r[2];
@@ -587,6 +599,14 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   // scalars and the forces of the last evaluation back to HBM
   __syncthreads();
   if (threadIdx.x == 0) b.st[s] = st;
+  if (gear_l)
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      b.y[0][0][r0 + k] = rx[k];
+      b.y[0][1][r0 + k] = ry[k];
+      b.y[0][2][r0 + k] = rz[k];
+      for (int j = 0; j < 15; ++j) b.y[1 + j / 3][j % 3][r0 + k] = ys[j * n + k];
+      for (int c = 0; c < 3; ++c) b.vel[c][r0 + k] = ys[c * n + k] * P.dt_inv;
+    }
   if (!evaluated) return;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     ((RF *)b.F[0])[r0 + k] = (RF)fx[k];

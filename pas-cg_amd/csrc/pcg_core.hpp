@@ -43,14 +43,16 @@ struct SysDesc {
 // bonded parameters (5 n FP64: tether length, native angle, native dihedral,
 // the two chiral fields; 3 n bytes: flags, heuristic angle and dihedral
 // types).
+// With gear set, the Gear derivatives y1..y5 (15 n FP64) stay in LDS too.
 struct LdsLayout {
   static constexpr size_t QA_QUEUE = 4 * 1024;  // QA pairs past the first test, per pass
-  size_t f, bd, bu, q, total;
-  __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged) {
+  size_t f, bd, bu, g, q, total;
+  __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged, bool gear) {
     f = 24 * n;
     bd = (f + 3 * n * fsz + 15) / 16 * 16;
     bu = bd + 40 * n;
-    q = ((staged ? bu + 3 * n : bd) + 15) / 16 * 16;
+    g = ((staged ? bu + 3 * n : bd) + 15) / 16 * 16;
+    q = gear ? g + 120 * n : g;
     total = q + 4 * QA_QUEUE + 64;
   }
 };
@@ -112,10 +114,12 @@ struct DevBatch {
   int *nl_i[2]; double *nl_d; uint8_t *nl_cls;  // class bits of the partition
   int *ncur;                                      // current native contacts (index into all)
   int *ncur_i[2]; double *ncur_nat; uint8_t *ncur_t;  // their residues, length, type
+  float *ncur_th2;  // squared transition distance (nat * 2^(-1/6) * counting factor), FP32
   // bond_item[L][r]: global index of the item of list L whose first residue
   // is r, or -1 (systems with bond_regular set)
   const int *bond_item[NBOND];
   int bond_lds;               // batched kernel: stage per-residue bonded data in LDS
+  int gear_lds;               // batched kernel: keep y1..y5 in LDS for the launch
   int *pauli_i[2];
   int *dh_i[2]; double *dh_q;
   int *pid_i[2]; double *pid_d; int16_t *pid_t;

@@ -479,6 +479,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.ncur_i[1], std::max(nnc, 1LL));
   A.add(&D.ncur_nat, std::max(nnc, 1LL));
   A.add(&D.ncur_t, std::max(nnc, 1LL));
+  A.add(&D.ncur_th2, std::max(nnc, 1LL));
   // chain-local bonded items by base residue (see BOND_* in pcg_core.hpp)
   std::vector<int> bond_item((size_t)NBOND * std::max(ntot, 1LL), -1);
   for (int s = 0; s < nsys; ++s) {
@@ -1178,7 +1179,7 @@ struct pcg_batch {
 namespace {
 
 size_t lds_for(const DeviceBatch &B, bool staged) {
-  return LdsLayout((size_t)B.max_n, sizeof(double), staged).total;  // FP64 LDS forces
+  return LdsLayout((size_t)B.max_n, sizeof(double), staged, false).total;  // FP64 LDS forces
 }
 
 // steps owed by every system = current + nsteps
@@ -1253,8 +1254,10 @@ int pcg_batch_create(const pcg_params *p, int32_t nsys, const pcg_model *models,
   // per-residue bonded data joins positions and forces in LDS when it fits;
   // systems whose positions and forces do not fit take the stage kernels
   bool staged = lds_for(B, true) <= 150 * 1024;
-  b->lds = lds_for(B, staged);
+  bool gear = staged && LdsLayout((size_t)B.max_n, sizeof(double), true, true).total <= 150 * 1024;
+  b->lds = LdsLayout((size_t)B.max_n, sizeof(double), staged, gear).total;
   B.d.bond_lds = staged ? 1 : 0;
+  B.d.gear_lds = gear ? 1 : 0;
   b->persistent = b->lds <= 150 * 1024;
   if (opt.block != 0 && (opt.block < 64 || opt.block > 1024 || opt.block % 64 != 0))
     return fail(PCG_E_ARG, "block must be a multiple of 64 in [64, 1024]");
@@ -1605,7 +1608,7 @@ int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature, int
   if (rc) return rc;
   if ((int64_t)bytes > s->o.hbm_budget_bytes)
     return fail(PCG_E_NOMEM, "job needs " + std::to_string(bytes) + " bytes, over the budget");
-  if (LdsLayout((size_t)m->n, sizeof(double), false).total > 150 * 1024)
+  if (LdsLayout((size_t)m->n, sizeof(double), false, false).total > 150 * 1024)
     return fail(PCG_E_ARG, "job of " + std::to_string(m->n) +
                                " residues does not fit the persistent kernel's LDS");
   j->bytes = bytes;
@@ -1658,9 +1661,10 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
       HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), sizeof(SysDesc), hipMemcpyHostToDevice,
                             s->stream));
       // each job stages its bonded data in LDS when that fits on its own
-      size_t with = LdsLayout((size_t)B.max_n, fsz, true).total;
-      B.d.bond_lds = with <= 150 * 1024 ? 1 : 0;
-      lds = std::max(lds, LdsLayout((size_t)B.max_n, fsz, B.d.bond_lds != 0).total);
+      const size_t nj = (size_t)B.max_n;
+      B.d.bond_lds = LdsLayout(nj, fsz, true, false).total <= 150 * 1024 ? 1 : 0;
+      B.d.gear_lds = B.d.bond_lds && LdsLayout(nj, fsz, true, true).total <= 150 * 1024 ? 1 : 0;
+      lds = std::max(lds, LdsLayout(nj, fsz, B.d.bond_lds != 0, B.d.gear_lds != 0).total);
       descs[k] = B.d;
       max_n = std::max(max_n, B.max_n);
     }

@@ -28,6 +28,8 @@ __device__ __forceinline__ void set_current_contact(const DevBatch &b, int q, in
   b.ncur_i[1][q] = b.nc_i[1][c];
   b.ncur_nat[q] = b.nc_nat[c];
   b.ncur_t[q] = b.nc_type[c];
+  double th = __dmul_rn(__dmul_rn(b.nc_nat[c], C216_INV), b.P->p.counting_factor);
+  b.ncur_th2[q] = (float)(th * th);
 }
 
 __device__ inline int classify(const DevBatch &b, const SysDesc &s, const SysState &st,
@@ -245,9 +247,12 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
 // predictor-corrector with the reference's float-computed coefficients
 // (cg/include/cg/langevin/step.h:10-12). The displacement from the last NL
 // rebuild is reduced here for the next step's verify.
-__device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd, const SysState &st,
-                                            int i, double Fx, double Fy, double Fz,
-                                            const double nz[3], double out_r[3]) {
+// ya(k, c) is a reference to y_k, component c, of the residue (in HBM or LDS)
+template <typename YA>
+__device__ __forceinline__ void gear_core(const DevBatch &b, const SysDesc &sd, const SysState &st,
+                                          int i, double Fx, double Fy, double Fz,
+                                          const double nz[3], double out_r[3], double vel[3],
+                                          YA ya) {
 #pragma clang fp contract(off)
   const DevParams &P = *b.P;
   const double f02 = (double)(3.0f / 16.0f), f12 = (double)(251.0f / 360.0f),
@@ -258,8 +263,8 @@ __device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd
   double gam2 = P.gamma2 * minv, con2 = sd.const2 * sqrt(minv);
   double F[3] = {Fx, Fy, Fz};
   for (int c = 0; c < 3; ++c) {
-    double y0 = b.y[0][c][i], y1 = b.y[1][c][i], y2 = b.y[2][c][i], y3 = b.y[3][c][i],
-           y4 = b.y[4][c][i], y5 = b.y[5][c][i];
+    double y0 = ya(0, c), y1 = ya(1, c), y2 = ya(2, c), y3 = ya(3, c), y4 = ya(4, c),
+           y5 = ya(5, c);
     if (st.t == 0) y2 = F[c] * P.deltsq;
     double f = F[c] - gam2 * y1;
     y1 = y1 + con2 * nz[c];
@@ -275,15 +280,24 @@ __device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd
     y2 += 3.0 * y3 + 6.0 * y4 + 10.0 * y5;
     y3 += 4.0 * y4 + 10.0 * y5;
     y4 += 5.0 * y5;
-    b.y[0][c][i] = y0;
-    b.y[1][c][i] = y1;
-    b.y[2][c][i] = y2;
-    b.y[3][c][i] = y3;
-    b.y[4][c][i] = y4;
-    b.y[5][c][i] = y5;
-    b.vel[c][i] = y1 * P.dt_inv;
+    ya(0, c) = y0;
+    ya(1, c) = y1;
+    ya(2, c) = y2;
+    ya(3, c) = y3;
+    ya(4, c) = y4;
+    ya(5, c) = y5;
+    vel[c] = y1 * P.dt_inv;
     out_r[c] = y0;
   }
+}
+
+__device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd, const SysState &st,
+                                            int i, double Fx, double Fy, double Fz,
+                                            const double nz[3], double out_r[3]) {
+  double vel[3];
+  gear_core(b, sd, st, i, Fx, Fy, Fz, nz, out_r, vel,
+            [&](int k, int c) -> double & { return b.y[k][c][i]; });
+  for (int c = 0; c < 3; ++c) b.vel[c][i] = vel[c];
 }
 
 __device__ __forceinline__ double disp_from_orig(const DevBatch &b, const SysDesc &sd, int i,

@@ -120,6 +120,7 @@ struct NcFetch {
   int c, i1, i2;
   double nat;
   int type;
+  float th2;
 };
 template <typename R, typename G, typename A>
 __device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc &s, SysState &st,
@@ -130,10 +131,26 @@ __device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc
   const int c = q.c, i1 = q.i1, i2 = q.i2;
   const double nat = q.nat;
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
-  double rnx = xnorm3(d);
-  if (rnx >= p.nl_cutoff) return 0.0;
   v3<R> r12 = cast3<R>(d);
-  R rn = (R)rnx, V, dV;
+  R rn, V, dV;
+  double rnx = 0.0;
+  int now = -1;  // transition decision, -1 while undecided
+  if constexpr (sizeof(R) == 4) {
+    // both decisions (cutoff, transition) from the FP32 square when it is
+    // clear of the thresholds by 2e-5; the exact FP64 norm otherwise
+    const float d2 = nsq(r12), cut2 = (float)(p.nl_cutoff * p.nl_cutoff);
+    const float lo = 1.0f - 2e-5f, hi = 1.0f + 2e-5f;
+    if (d2 > cut2 * hi) return 0.0;
+    if (d2 < cut2 * lo && (d2 < q.th2 * lo || d2 > q.th2 * hi)) {
+      now = d2 < q.th2 * lo ? 1 : 0;
+      rn = sq(d2);
+    }
+  }
+  if (now < 0) {
+    rnx = xnorm3(d);
+    if (rnx >= p.nl_cutoff) return 0.0;
+    rn = (R)rnx;
+  }
   if (p.nc_ss_kind != 0 && q.type == 5) {
     // disulfide_force::operator() receives the distance as its inverse-distance
     // argument (nat_cont/eval_forces.cpp:27, disulfide.h:12-20)
@@ -149,8 +166,8 @@ __device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc
   acc.add(i1, f);
   acc.add(i2, -f);
   if (!update_state) return (double)V;
-  // state transition, exact FP64
-  bool now = rnx <= __dmul_rn(__dmul_rn(nat, C216_INV), p.counting_factor);
+  // state transition, exact FP64 unless decided above
+  if (now < 0) now = rnx <= __dmul_rn(__dmul_rn(nat, C216_INV), p.counting_factor) ? 1 : 0;
   if ((uint8_t)now != b.nc_active[c]) {
     b.nc_active[c] = (uint8_t)now;
     if (b.nc_change_t[c] < 0) {
@@ -165,7 +182,8 @@ __device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc
 // of all contacts and a copy of its residues, length and type (independent
 // loads, issued together).
 __device__ __forceinline__ NcFetch nat_cont_fetch(const DevBatch &b, int k) {
-  return NcFetch{b.ncur[k], b.ncur_i[0][k], b.ncur_i[1][k], b.ncur_nat[k], b.ncur_t[k]};
+  return NcFetch{b.ncur[k], b.ncur_i[0][k], b.ncur_i[1][k], b.ncur_nat[k], b.ncur_t[k],
+                 b.ncur_th2[k]};
 }
 template <typename R, typename G, typename A>
 __device__ __forceinline__ double nat_cont_item(const DevBatch &b, const SysDesc &s, SysState &st,
