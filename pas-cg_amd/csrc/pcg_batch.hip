@@ -445,7 +445,8 @@ __device__ __forceinline__ void term_loop(const DevBatch &b, const SysDesc &sd, 
 }
 
 template <typename R, typename RF>
-__device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int nsteps) {
+__device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int nsteps,
+                                                JobSlot *slot = nullptr) {
 #pragma clang fp contract(off)
   const SysDesc &sd = b.sys[s];
   if (!b.st[s].active) return;
@@ -456,6 +457,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   __syncthreads();
   const DevParams &P = *b.P;
   const int n = sd.n, r0 = sd.res0;
+  const int64_t target = slot ? slot->target : sd.steps_target;
   extern __shared__ __align__(16) unsigned char lds[];
   const bool staged = sd.bond_regular && b.bond_lds;
   // forces accumulate in FP64 in LDS: on gfx950 an LDS FP64 atomic add
@@ -488,9 +490,9 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   bool evaluated = false;
   for (int it = 0; it < nsteps; ++it) {
     if (threadIdx.x == 0) {
-      stop = st.step >= sd.steps_target;
+      stop = st.step >= target;
       // energies on the last step this launch makes for the system
-      want_v = (it == nsteps - 1) || (st.step + 1 >= sd.steps_target);
+      want_v = (it == nsteps - 1) || (st.step + 1 >= target);
       // nl::verify (cg/src/cg/nl/verify.cpp:3-34)
       if (!stop && P.nl_required) {
         if (st.verify_first) {
@@ -598,7 +600,14 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   }
   // scalars and the forces of the last evaluation back to HBM
   __syncthreads();
-  if (threadIdx.x == 0) b.st[s] = st;
+  if (threadIdx.x == 0) {
+    b.st[s] = st;
+    if (slot) {
+      slot->step = st.step;
+      slot->Vtot = st.Vtot;
+      slot->overflow = st.overflow;
+    }
+  }
   if (gear_l)
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
       b.y[0][0][r0 + k] = rx[k];
@@ -624,19 +633,20 @@ __global__ __launch_bounds__(MAXT) void k_persistent(DevBatch b, int nsteps) {
 
 // served form: one resident job (a one-system batch) per workgroup
 template <typename R, typename RF, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_persistent_jobs(const DevBatch *jobs, int nsteps) {
-  persistent_body<R, RF>(jobs[blockIdx.x], 0, nsteps);
+__global__ __launch_bounds__(MAXT) void k_persistent_jobs(const DevBatch *jobs, JobSlot *slots,
+                                                          int nsteps) {
+  persistent_body<R, RF>(jobs[blockIdx.x], 0, nsteps, slots + blockIdx.x);
 }
 
 template <int MAXT>
-void launch_jobs_t(const DevBatch *jobs, int njobs, int prec, int block, int nsteps,
+void launch_jobs_t(const DevBatch *jobs, JobSlot *slots, int njobs, int prec, int block, int nsteps,
                    size_t lds_bytes, hipStream_t s) {
   if (prec == PCG_FP64)
     hipLaunchKernelGGL((k_persistent_jobs<double, double, MAXT>), dim3(njobs), dim3(block),
-                       lds_bytes, s, jobs, nsteps);
+                       lds_bytes, s, jobs, slots, nsteps);
   else
     hipLaunchKernelGGL((k_persistent_jobs<float, float, MAXT>), dim3(njobs), dim3(block),
-                       lds_bytes, s, jobs, nsteps);
+                       lds_bytes, s, jobs, slots, nsteps);
 }
 
 template <int MAXT>
@@ -650,10 +660,10 @@ void launch_batch_t(const DevBatch &b, int prec, int block, int nsteps, size_t l
                        s, b, nsteps);
 }
 
-void launch_persistent_jobs(const DevBatch *jobs, int njobs, int prec, int block, int nsteps,
-                            size_t lds_bytes, hipStream_t s) {
-  if (block > 512) launch_jobs_t<1024>(jobs, njobs, prec, block, nsteps, lds_bytes, s);
-  else launch_jobs_t<512>(jobs, njobs, prec, block, nsteps, lds_bytes, s);
+void launch_persistent_jobs(const DevBatch *jobs, JobSlot *slots, int njobs, int prec, int block,
+                            int nsteps, size_t lds_bytes, hipStream_t s) {
+  if (block > 512) launch_jobs_t<1024>(jobs, slots, njobs, prec, block, nsteps, lds_bytes, s);
+  else launch_jobs_t<512>(jobs, slots, njobs, prec, block, nsteps, lds_bytes, s);
 }
 
 void launch_persistent(const DevBatch &b, int prec, int block, int nsteps, size_t lds_bytes,

@@ -57,6 +57,14 @@ struct LdsLayout {
   }
 };
 
+// Served form: one slot per resident job of a round, uploaded and read back
+// in one copy each (the step target in, the progress and health out).
+struct JobSlot {
+  int64_t target, step;
+  double Vtot;
+  int overflow, pad;
+};
+
 // Mutable per-system scalars.
 struct SysState {
   int nl_n, nc_n, pauli_n, dh_n, pid_n, qf_n, qc_n, cand_n, rem_n, num_qa;

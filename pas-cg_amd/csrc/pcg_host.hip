@@ -309,9 +309,15 @@ struct Arena {
     req.push_back({(void **)ptr, count * sizeof(T)});
     total += (count * sizeof(T) + 255) / 256 * 256;
   }
-  int alloc(hipStream_t stream) {
-    if (hipMalloc(&base, total ? total : 256) != hipSuccess)
+  bool borrowed = false;  // base came from the caller's pool, which keeps it
+  int alloc(hipStream_t stream, void *pool_block = nullptr, size_t pool_bytes = 0) {
+    if (pool_block && pool_bytes >= (total ? total : 256)) {
+      base = pool_block;
+      borrowed = true;
+    } else if (hipMalloc(&base, total ? total : 256) != hipSuccess) {
+      base = nullptr;
       return fail(PCG_E_NOMEM, "hipMalloc of " + std::to_string(total) + " bytes failed");
+    }
     // every array starts at zero, as the reference's vectors do (e.g. the QA
     // n/h vectors of chain ends, which prepare_nh never writes)
     if (hipMemsetAsync(base, 0, total ? total : 256, stream) != hipSuccess)
@@ -347,7 +353,7 @@ struct DeviceBatch {
   bool own_stream = true;
 
   ~DeviceBatch() {
-    if (arena.base) (void)hipFree(arena.base);
+    if (arena.base && !arena.borrowed) (void)hipFree(arena.base);
     if (noise_pinned) (void)hipHostFree(noise_pinned);
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
@@ -358,7 +364,8 @@ namespace {
 int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const double *temps,
                const int32_t *seeds, int prec, uint64_t philox_seed, int device,
                std::unique_ptr<DeviceBatch> &out, hipStream_t use_stream = nullptr,
-               size_t *dry_run_bytes = nullptr, int64_t ran2_skip = 0) {
+               size_t *dry_run_bytes = nullptr, int64_t ran2_skip = 0,
+               void *pool_block = nullptr, size_t pool_bytes = 0) {
   if (nsys <= 0 || !models) return fail(PCG_E_ARG, "no systems");
   if (prec != PCG_MIXED && prec != PCG_FP64) return fail(PCG_E_ARG, "unknown precision");
   auto B = std::make_unique<DeviceBatch>();
@@ -553,7 +560,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     *dry_run_bytes = A.total;
     return PCG_OK;
   }
-  int rc = A.alloc(B->stream);
+  int rc = A.alloc(B->stream, pool_block, pool_bytes);
   if (rc) return rc;
   B->bytes = A.total;
   D.sys = dsys;
@@ -1518,7 +1525,8 @@ struct Job {
   int32_t seed = 448;
   int64_t nsteps = 0, done = 0;
   int status = PCG_JOB_QUEUED;
-  size_t bytes = 0;
+  size_t bytes = 0;        // arena bytes (dry run)
+  size_t block_bytes = 0;  // the pooled device block it runs in (>= bytes)
   std::unique_ptr<DeviceBatch> B;
   std::vector<double> result;
   double V = 0;
@@ -1537,15 +1545,30 @@ struct pcg_server {
   int64_t next_id = 1;
   size_t resident_bytes = 0;
   DevBatch *d_jobs = nullptr;
+  JobSlot *d_slots = nullptr;
   int d_jobs_cap = 0;
+  // device blocks of retired jobs, by size, reused for the next admissions
+  // (no hipMalloc/hipFree per job); sizes are multiples of 256 KiB
+  std::multimap<size_t, void *> pool;
+  size_t pooled_bytes = 0;
   ~pcg_server() {
+    for (auto &kv : jobs)
+      if (kv.second->B && kv.second->B->arena.base)
+        pool.emplace(kv.second->block_bytes, kv.second->B->arena.base);
     jobs.clear();
+    for (auto &kv : pool) (void)hipFree(kv.second);
     if (d_jobs) (void)hipFree(d_jobs);
+    if (d_slots) (void)hipFree(d_slots);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
 namespace {
+
+size_t block_size(size_t arena_bytes) {
+  const size_t g = (size_t)256 << 10;
+  return (arena_bytes + g - 1) / g * g;
+}
 
 int retire(pcg_server *s, Job &j, int status) {
   if (j.B) {
@@ -1558,7 +1581,10 @@ int retire(pcg_server *s, Job &j, int status) {
       j.V = st.Vtot;
       j.done = st.step;
     }
-    s->resident_bytes -= j.bytes;
+    s->resident_bytes -= j.block_bytes;
+    // the block goes back to the pool (the batch only borrowed it)
+    s->pool.emplace(j.block_bytes, j.B->arena.base);
+    s->pooled_bytes += j.block_bytes;
     j.B.reset();
   }
   j.status = status;
@@ -1606,8 +1632,9 @@ int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature, int
   int rc = make_batch(s->p, 1, &v, &j->T, &j->seed, s->o.precision, 0, s->o.device, probe,
                       s->stream, &bytes);
   if (rc) return rc;
-  if ((int64_t)bytes > s->o.hbm_budget_bytes)
-    return fail(PCG_E_NOMEM, "job needs " + std::to_string(bytes) + " bytes, over the budget");
+  if ((int64_t)block_size(bytes) > s->o.hbm_budget_bytes)
+    return fail(PCG_E_NOMEM, "job needs " + std::to_string(block_size(bytes)) +
+                                 " bytes, over the budget");
   if (LdsLayout((size_t)m->n, sizeof(double), false, false).total > 150 * 1024)
     return fail(PCG_E_ARG, "job of " + std::to_string(m->n) +
                                " residues does not fit the persistent kernel's LDS");
@@ -1628,22 +1655,52 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
       it = s->queue.erase(it);
       continue;
     }
-    if ((int64_t)(s->resident_bytes + j.bytes) > s->o.hbm_budget_bytes) {
+    const size_t need = block_size(j.bytes);
+    if ((int64_t)(s->resident_bytes + need) > s->o.hbm_budget_bytes) {
       ++it;
       continue;
+    }
+    // a pooled block that fits, else a new one; idle pooled blocks are given
+    // back when resident plus pooled memory would pass the budget
+    void *blk = nullptr;
+    size_t blk_bytes = need;
+    auto pit = s->pool.lower_bound(need);
+    if (pit != s->pool.end() && pit->first <= need + need / 4) {
+      blk = pit->second;
+      blk_bytes = pit->first;
+      s->pooled_bytes -= pit->first;
+      s->pool.erase(pit);
+    } else {
+      while (!s->pool.empty() &&
+             (int64_t)(s->resident_bytes + s->pooled_bytes + need) > s->o.hbm_budget_bytes) {
+        auto last = std::prev(s->pool.end());
+        (void)hipFree(last->second);
+        s->pooled_bytes -= last->first;
+        s->pool.erase(last);
+      }
+      if (hipMalloc(&blk, need) != hipSuccess) {
+        blk = nullptr;
+        ++it;
+        continue;
+      }
     }
     pcg_model v = j.model.view();
     uint64_t key = mix64(s->o.philox_seed ^ mix64((uint64_t)(uint32_t)j.seed));
     int rc = make_batch(s->p, 1, &v, &j.T, &j.seed, s->o.precision, key, s->o.device, j.B,
-                        s->stream);
-    if (rc == PCG_E_NOMEM) {
-      ++it;
-      continue;
+                        s->stream, nullptr, 0, blk, blk_bytes);
+    if (rc) {
+      s->pool.emplace(blk_bytes, blk);
+      s->pooled_bytes += blk_bytes;
+      if (rc == PCG_E_NOMEM) {
+        ++it;
+        continue;
+      }
     }
     if (rc) return rc;
     j.status = PCG_JOB_RUNNING;
     j.t_start = now_s();
-    s->resident_bytes += j.bytes;
+    j.block_bytes = blk_bytes;
+    s->resident_bytes += blk_bytes;
     s->resident.push_back(j.id);
     it = s->queue.erase(it);
   }
@@ -1651,15 +1708,14 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
   if (nres > 0) {
     // per-job step budget for this round
     std::vector<DevBatch> descs(nres);
+    std::vector<JobSlot> slots(nres);
     int max_n = 0;
     const size_t fsz = sizeof(double);  // the persistent kernel's LDS forces are FP64
     size_t lds = 0;
     for (int k = 0; k < nres; ++k) {
       Job &j = *s->jobs[s->resident[k]];
       DeviceBatch &B = *j.B;
-      B.sys[0].steps_target = std::min<int64_t>(j.done + s->o.chunk_steps, j.nsteps);
-      HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), sizeof(SysDesc), hipMemcpyHostToDevice,
-                            s->stream));
+      slots[k] = JobSlot{std::min<int64_t>(j.done + s->o.chunk_steps, j.nsteps), 0, 0.0, 0, 0};
       // each job stages its bonded data in LDS when that fits on its own
       const size_t nj = (size_t)B.max_n;
       B.d.bond_lds = LdsLayout(nj, fsz, true, false).total <= 150 * 1024 ? 1 : 0;
@@ -1670,28 +1726,35 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
     }
     if (nres > s->d_jobs_cap) {
       if (s->d_jobs) HIPCHK(hipFree(s->d_jobs));
+      if (s->d_slots) HIPCHK(hipFree(s->d_slots));
+      s->d_jobs = nullptr;
+      s->d_slots = nullptr;
       s->d_jobs_cap = std::max(nres, 2 * s->d_jobs_cap);
       HIPCHK(hipMalloc(&s->d_jobs, s->d_jobs_cap * sizeof(DevBatch)));
+      HIPCHK(hipMalloc(&s->d_slots, s->d_jobs_cap * sizeof(JobSlot)));
     }
     HIPCHK(hipMemcpyAsync(s->d_jobs, descs.data(), nres * sizeof(DevBatch), hipMemcpyHostToDevice,
+                          s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_slots, slots.data(), nres * sizeof(JobSlot), hipMemcpyHostToDevice,
                           s->stream));
     if (lds > 150 * 1024) return fail(PCG_E_ARG, "job too large for the persistent kernel");
     int block = 64;
     while (block < 1024 && block < 2 * max_n) block *= 2;
     block = std::max(block, 128);
-    launch_persistent_jobs(s->d_jobs, nres, s->o.precision, block, s->o.chunk_steps, lds,
-                           s->stream);
+    launch_persistent_jobs(s->d_jobs, s->d_slots, nres, s->o.precision, block, s->o.chunk_steps,
+                           lds, s->stream);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(slots.data(), s->d_slots, nres * sizeof(JobSlot), hipMemcpyDeviceToHost,
+                          s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    // retire finished / failed jobs
+    // retire finished / failed jobs (their progress from the round's slots)
     std::vector<int64_t> keep;
-    for (int64_t id : s->resident) {
+    for (int k = 0; k < nres; ++k) {
+      int64_t id = s->resident[k];
       Job &j = *s->jobs[id];
-      SysState st;
-      int rc = read_state(*j.B, 0, st);
-      if (rc) return rc;
-      j.done = st.step;
-      bool bad = st.overflow || !std::isfinite(st.Vtot);
+      int rc;
+      j.done = slots[k].step;
+      bool bad = slots[k].overflow || !std::isfinite(slots[k].Vtot);
       if (bad) {
         if ((rc = retire(s, j, PCG_JOB_FAILED))) return rc;
       } else if (j.done >= j.nsteps) {

@@ -16,6 +16,6 @@ void launch_tick(const DevBatch &b, int advance, hipStream_t s);
 // persistent form: every system of the batch advances nsteps in one launch
 void launch_persistent(const DevBatch &b, int prec, int block, int nsteps, size_t lds_bytes,
                        hipStream_t s);
-void launch_persistent_jobs(const DevBatch *jobs, int njobs, int prec, int block, int nsteps,
-                            size_t lds_bytes, hipStream_t s);
+void launch_persistent_jobs(const DevBatch *jobs, JobSlot *slots, int njobs, int prec, int block,
+                            int nsteps, size_t lds_bytes, hipStream_t s);
 } // namespace pcg
