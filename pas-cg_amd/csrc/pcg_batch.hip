@@ -251,14 +251,17 @@ template <typename RF> struct AccSlots {
 enum { BF_TETHER = 1, BF_LREP = 2, BF_NAT_ANG = 4, BF_HEUR_ANG = 8, BF_NAT_DIH = 16,
        BF_HEUR_DIH = 32, BF_CHIR = 64 };
 struct BondLds {
-  double *d;   // [5][n]: tether length, native angle, native dihedral, chiral f0, f1
+  // [6][n]: tether length, native angle, native dihedral (phi0 of the simple
+  // variant, cos phi0 of the complex one), sin phi0, chiral f0, f1
+  double *d;
   uint8_t *u;  // [3][n]: flags, heuristic angle type, heuristic dihedral type
   int n;
   __device__ double &teth(int k) const { return d[k]; }
   __device__ double &ang(int k) const { return d[n + k]; }
   __device__ double &dih(int k) const { return d[2 * n + k]; }
-  __device__ double &c0(int k) const { return d[3 * n + k]; }
-  __device__ double &c1(int k) const { return d[4 * n + k]; }
+  __device__ double &dihs(int k) const { return d[3 * n + k]; }
+  __device__ double &c0(int k) const { return d[4 * n + k]; }
+  __device__ double &c1(int k) const { return d[5 * n + k]; }
   __device__ uint8_t &flag(int k) const { return u[k]; }
   __device__ uint8_t &at(int k) const { return u[n + k]; }
   __device__ uint8_t &dt(int k) const { return u[2 * n + k]; }
@@ -274,7 +277,18 @@ __device__ __forceinline__ void stage_bonded(const DevBatch &b, const SysDesc &s
     if ((it = b.bond_item[BOND_LREP][r]) >= 0) f |= BF_LREP;
     if ((it = b.bond_item[BOND_NAT_ANG][r]) >= 0) f |= BF_NAT_ANG, bl.ang(k) = b.nang_th[it];
     if ((it = b.bond_item[BOND_HEUR_ANG][r]) >= 0) f |= BF_HEUR_ANG, bl.at(k) = b.hang_t[it];
-    if ((it = b.bond_item[BOND_NAT_DIH][r]) >= 0) f |= BF_NAT_DIH, bl.dih(k) = b.ndih_ph[it];
+    if ((it = b.bond_item[BOND_NAT_DIH][r]) >= 0) {
+      f |= BF_NAT_DIH;
+      const double phi0 = b.ndih_ph[it];
+      if (b.P->p.nat_dih_variant == 0) {
+        double s0, c0;
+        sincos(phi0, &s0, &c0);
+        bl.dih(k) = c0;
+        bl.dihs(k) = s0;
+      } else {
+        bl.dih(k) = phi0;
+      }
+    }
     if ((it = b.bond_item[BOND_HEUR_DIH][r]) >= 0) f |= BF_HEUR_DIH, bl.dt(k) = b.hdih_t[it];
     if ((it = b.bond_item[BOND_CHIR][r]) >= 0) {
       f |= BF_CHIR;
@@ -316,7 +330,8 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc 
       if (p.nat_ang_on && (f & BF_NAT_ANG)) vna += nat_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.ang(k));
       if (p.heur_ang_on && (f & BF_HEUR_ANG)) vha += heur_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.at(k));
       if (p.nat_dih_on && (f & BF_NAT_DIH))
-        vnd += nat_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dih(k), p.nat_dih_variant);
+        vnd += nat_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dih(k), bl.dih(k), bl.dihs(k),
+                                    p.nat_dih_variant);
       if (p.heur_dih_on && (f & BF_HEUR_DIH))
         vhd += heur_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dt(k));
       if (p.chir_on && (f & BF_CHIR))
@@ -467,6 +482,13 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   double *rx = (double *)lds, *ry = rx + n, *rz = ry + n;
   double *fx = (double *)(lds + lay.f), *fy = fx + n, *fz = fy + n;
   BondLds bl{(double *)(lds + lay.bd), (uint8_t *)(lds + lay.bu), n};
+  uint8_t *atl = (uint8_t *)(lds + lay.at);  // residue types
+  // 1 / mass and its square root per residue type, for the Langevin step
+  __shared__ double minv_t[21], sqminv_t[21];
+  if (threadIdx.x < 21) {
+    minv_t[threadIdx.x] = 1.0 / P.p.mass[threadIdx.x];
+    sqminv_t[threadIdx.x] = sqrt(minv_t[threadIdx.x]);
+  }
   int *qa_queue = (int *)(lds + lay.q);
   double *ys = (double *)(lds + lay.g);  // [15][n]: y1..y5 x, y, z (gear_l)
   __shared__ double vacc[NT];
@@ -477,6 +499,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     rx[k] = b.y[0][0][r0 + k];
     ry[k] = b.y[0][1][r0 + k];
     rz[k] = b.y[0][2][r0 + k];
+    atl[k] = b.atype[r0 + k];
   }
   GeoL g{rx, ry, rz, r0};
   AccL<double> acc{fx, fy, fz, r0};
@@ -562,21 +585,28 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     if (P.p.lang_on) {
       for (int k = threadIdx.x; k < n; k += blockDim.x) {
         int i = r0 + k;
+        // the rebuild snapshot is loaded first: its latency hides behind
+        // the noise and the corrector
+        const double o[3] = {b.orig_r[0][i], b.orig_r[1][i], b.orig_r[2][i]};
         double nz[3];
         philox_normals(sd.philox_key, st.step, k, nz);
-        double r[3];
+        double r[3], vel[3];
+        const int aa = atl[k];
         if (gear_l) {
-          double vel[3];
-          gear_core(b, sd, st, i, fx[k], fy[k], fz[k], nz, r, vel, [&](int kk, int c) -> double & {
-            return kk == 0 ? (c == 0 ? rx[k] : c == 1 ? ry[k] : rz[k]) : ys[((kk - 1) * 3 + c) * n + k];
-          });
+          gear_core(b, sd, st, minv_t[aa], sqminv_t[aa], fx[k], fy[k], fz[k], nz, r, vel,
+                    [&](int kk, int c) -> double & {
+                      return kk == 0 ? (c == 0 ? rx[k] : c == 1 ? ry[k] : rz[k])
+                                     : ys[((kk - 1) * 3 + c) * n + k];
+                    });
         } else {
-          gear_update(b, sd, st, i, fx[k], fy[k], fz[k], nz, r);
+          gear_core(b, sd, st, minv_t[aa], sqminv_t[aa], fx[k], fy[k], fz[k], nz, r, vel,
+                    [&](int kk, int c) -> double & { return b.y[kk][c][i]; });
+          for (int c = 0; c < 3; ++c) b.vel[c][i] = vel[c];
         }
         rx[k] = r[0];
         ry[k] = r[1];
         rz[k] = r[2];
-        maxd = fmax(maxd, disp_from_orig(b, sd, i, r));
+        maxd = fmax(maxd, disp_from(sd, o, r));
       }
     }
     maxd = block_max(maxd);

@@ -46,11 +46,12 @@ struct SysDesc {
 // With gear set, the Gear derivatives y1..y5 (15 n FP64) stay in LDS too.
 struct LdsLayout {
   static constexpr size_t QA_QUEUE = 4 * 1024;  // QA pairs past the first test, per pass
-  size_t f, bd, bu, g, q, total;
+  size_t f, at, bd, bu, g, q, total;
   __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged, bool gear) {
     f = 24 * n;
-    bd = (f + 3 * n * fsz + 15) / 16 * 16;
-    bu = bd + 40 * n;
+    at = f + 3 * n * fsz;
+    bd = (at + n + 15) / 16 * 16;
+    bu = bd + 48 * n;
     g = ((staged ? bu + 3 * n : bd) + 15) / 16 * 16;
     q = gear ? g + 120 * n : g;
     total = q + 4 * QA_QUEUE + 64;

@@ -56,21 +56,23 @@ __device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t 
     k1 += 0xBB67AE85u;
   }
 }
-// three standard normals for (system key, step, residue): Box-Muller
+// Three standard normals for (system key, step, residue): Box-Muller in
+// FP32 on the hardware's transcendental units. v_log_f32 is log2, and
+// v_sin_f32 / v_cos_f32 take the angle in turns (sin(2 pi u)), so no range
+// reduction is needed. u1, u3 lie in [2^-32, 1] (the radius is at most 6.66),
+// u2, u4 in [0, 1]. A normal carries about 1e-7 relative error, far below
+// what the noise's statistics can resolve.
 __device__ __forceinline__ void philox_normals(uint64_t key, int64_t step, int res, double out[3]) {
   uint32_t c[4] = {(uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)res, 0x5043u};
   philox4x32(c, (uint32_t)key, (uint32_t)(key >> 32));
-  const double inv = 2.3283064365386963e-10; // 2^-32
-  double u1 = ((double)c[0] + 1.0) * inv, u2 = (double)c[1] * inv;
-  double u3 = ((double)c[2] + 1.0) * inv, u4 = (double)c[3] * inv;
-  double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
-  double s1, c1, s2, c2;
-  sincos(2.0 * M_PI * u2, &s1, &c1);
-  sincos(2.0 * M_PI * u4, &s2, &c2);
-  out[0] = r1 * c1;
-  out[1] = r1 * s1;
-  out[2] = r2 * c2;
-  (void)s2;
+  const float inv = 0x1p-32f, m2ln2 = -1.3862943611198906f;  // -2 ln 2
+  float u1 = fmaf((float)c[0], inv, inv), u2 = (float)c[1] * inv;
+  float u3 = fmaf((float)c[2], inv, inv), u4 = (float)c[3] * inv;
+  float r1 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1));
+  float r2 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u3));
+  out[0] = (double)(r1 * __builtin_amdgcn_cosf(u2));
+  out[1] = (double)(r1 * __builtin_amdgcn_sinf(u2));
+  out[2] = (double)(r2 * __builtin_amdgcn_cosf(u4));
 }
 
 // ------------------------------------------------------------ per term ----

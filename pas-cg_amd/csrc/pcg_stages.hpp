@@ -248,19 +248,18 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
 // (cg/include/cg/langevin/step.h:10-12). The displacement from the last NL
 // rebuild is reduced here for the next step's verify.
 // ya(k, c) is a reference to y_k, component c, of the residue (in HBM or LDS)
+// minv = 1 / mass and sqminv = sqrt(minv) of the residue's type.
 template <typename YA>
 __device__ __forceinline__ void gear_core(const DevBatch &b, const SysDesc &sd, const SysState &st,
-                                          int i, double Fx, double Fy, double Fz,
-                                          const double nz[3], double out_r[3], double vel[3],
-                                          YA ya) {
+                                          double minv, double sqminv, double Fx, double Fy,
+                                          double Fz, const double nz[3], double out_r[3],
+                                          double vel[3], YA ya) {
 #pragma clang fp contract(off)
   const DevParams &P = *b.P;
   const double f02 = (double)(3.0f / 16.0f), f12 = (double)(251.0f / 360.0f),
                f32 = (double)(11.0f / 18.0f), f42 = (double)(1.0f / 6.0f),
                f52 = (double)(1.0f / 60.0f);
-  int aa = b.atype[i];
-  double minv = 1.0 / P.p.mass[aa];
-  double gam2 = P.gamma2 * minv, con2 = sd.const2 * sqrt(minv);
+  double gam2 = P.gamma2 * minv, con2 = sd.const2 * sqminv;
   double F[3] = {Fx, Fy, Fz};
   for (int c = 0; c < 3; ++c) {
     double y0 = ya(0, c), y1 = ya(1, c), y2 = ya(2, c), y3 = ya(3, c), y4 = ya(4, c),
@@ -295,17 +294,21 @@ __device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd
                                             int i, double Fx, double Fy, double Fz,
                                             const double nz[3], double out_r[3]) {
   double vel[3];
-  gear_core(b, sd, st, i, Fx, Fy, Fz, nz, out_r, vel,
+  const double minv = 1.0 / b.P->p.mass[b.atype[i]];
+  gear_core(b, sd, st, minv, sqrt(minv), Fx, Fy, Fz, nz, out_r, vel,
             [&](int k, int c) -> double & { return b.y[k][c][i]; });
   for (int c = 0; c < 3; ++c) b.vel[c][i] = vel[c];
 }
 
+__device__ __forceinline__ double disp_from(const SysDesc &sd, const double o[3], const double r[3]) {
+#pragma clang fp contract(off)
+  v3<double> d{__dsub_rn(o[0], r[0]), __dsub_rn(o[1], r[1]), __dsub_rn(o[2], r[2])};
+  return xnorm3(xwrapd(d, sd));
+}
 __device__ __forceinline__ double disp_from_orig(const DevBatch &b, const SysDesc &sd, int i,
                                                  const double r[3]) {
-#pragma clang fp contract(off)
-  v3<double> d{__dsub_rn(b.orig_r[0][i], r[0]), __dsub_rn(b.orig_r[1][i], r[1]),
-               __dsub_rn(b.orig_r[2][i], r[2])};
-  return xnorm3(xwrapd(d, sd));
+  const double o[3] = {b.orig_r[0][i], b.orig_r[1][i], b.orig_r[2][i]};
+  return disp_from(sd, o, r);
 }
 
 

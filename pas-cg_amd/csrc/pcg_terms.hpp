@@ -243,10 +243,11 @@ template <typename R, typename G>
 __device__ __forceinline__ R angle_geom(const G &g, int i1, int i2, int i3, v3<R> &d1, v3<R> &d2, v3<R> &d3) {
   v3<R> r12 = cast3<R>(g.r(i2) - g.r(i1)), r23 = cast3<R>(g.r(i3) - g.r(i2));
   v3<R> x = cross(r12, r23);
-  R r12_rn = rsq(nsq(r12)), r23_rn = rsq(nsq(r23));
+  R r12_rn = rsq(nsq(r12)), r23_rn = rsq(nsq(r23)), x_rn = rsq(nsq(x));
   v3<R> c1 = cross(r12, x), c3 = cross(r23, x);
-  d1 = (rsq(nsq(c1)) * r12_rn) * c1;
-  d3 = (rsq(nsq(c3)) * r23_rn) * c3;
+  // |r12 x x| = |r12| |x| and |r23 x x| = |r23| |x|, since x is normal to both
+  d1 = (x_rn * r12_rn * r12_rn) * c1;
+  d3 = (x_rn * r23_rn * r23_rn) * c3;
   d2 = -d1 - d3;
   R cos_t = clampR(-dot(r12, r23) * r12_rn * r23_rn, (R)-1, (R)1);
   if (cos_t == (R)0) d1 = d2 = d3 = v3<R>{0, 0, 0};
@@ -293,29 +294,37 @@ __device__ __forceinline__ double heur_ang_item(const DevBatch &b, const G &g, A
 }
 
 // cnd / snd ::iter (angles/nat_dih/complex.cpp:10-47, simple.cpp:10-42)
+//
+// The complex variant needs only cos and sin of phi - phi0. They come from
+// cos phi = x1u.x2u and sin phi = |r23| (x1.r34) / (|x1| |x2|), since
+// x1 x x2 = r23 (x1.r34); the sign of x1.r34 is the sign the reference gives
+// phi after its acos. cos phi0 and sin phi0 are the caller's (precomputed per
+// item), so the term takes no acos and no sincos.
 template <typename R, typename G, typename A>
 __device__ __forceinline__ double nat_dih_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
-                                               int i3, int i4, double phi0, int variant) {
+                                               int i3, int i4, double phi0, double cphi0,
+                                               double sphi0, int variant) {
   v3<double> q1 = g.r(i1), q2 = g.r(i2), q3 = g.r(i3), q4 = g.r(i4);
   v3<R> r12 = cast3<R>(q2 - q1), r23 = cast3<R>(q3 - q2), r34 = cast3<R>(q4 - q3);
   v3<R> x1 = cross(r12, r23), x2 = cross(r23, r34);
   R x1rn = rsq(nsq(x1)), x2rn = rsq(nsq(x2));
   v3<R> x1u = x1 * x1rn, x2u = x2 * x2rn;
-  R phi = acosc(dot(x1u, x2u));
-  if (dot(x1, r34) < (R)0) phi = -phi;
-  R diff = phi - (R)phi0, V, dVdphi;
+  R r23n = sq(nsq(r23)), V, dVdphi;
   if (variant == 0) {
-    // one sincos; the triple angle by cos 3x = c (4c^2 - 3), sin 3x = s (3 - 4s^2)
-    R CDA = (R)p.CDA, CDB = (R)p.CDB, s1, c1;
-    sincosR(diff, s1, c1);
+    // cos 3x = c (4c^2 - 3), sin 3x = s (3 - 4s^2)
+    R cphi = clampR(dot(x1u, x2u), (R)-1, (R)1), sphi = r23n * dot(x1, r34) * x1rn * x2rn;
+    R c1 = cphi * (R)cphi0 + sphi * (R)sphi0, s1 = sphi * (R)cphi0 - cphi * (R)sphi0;
+    R CDA = (R)p.CDA, CDB = (R)p.CDB;
     R c3 = c1 * ((R)4 * c1 * c1 - (R)3), s3 = s1 * ((R)3 - (R)4 * s1 * s1);
     V = CDA * ((R)1 - c1) + CDB * ((R)1 - c3);
     dVdphi = CDA * s1 + CDB * (R)3 * s3;
   } else {
+    R phi = acosc(dot(x1u, x2u));
+    if (dot(x1, r34) < (R)0) phi = -phi;
+    R diff = phi - (R)phi0;
     V = (R)0.5 * (R)p.CDH * diff * diff;
     dVdphi = (R)p.CDH * diff;
   }
-  R r23n = sq(nsq(r23));
   v3<R> dr1 = -(x1u * (r23n * x1rn)), dr4 = x2u * (r23n * x2rn);
   v3<R> df = (-dr1 * dot(r12, r23) + dr4 * dot(r23, r34)) * ((R)1 / (r23n * r23n));
   v3<R> dr2 = -dr1 + df, dr3 = -dr4 - df;
@@ -327,8 +336,11 @@ __device__ __forceinline__ double nat_dih_core(const pcg_params &p, const G &g, 
 }
 template <typename R, typename G, typename A>
 __device__ __forceinline__ double nat_dih_item(const DevBatch &b, const G &g, A &acc, int k, int variant) {
+  const double phi0 = b.ndih_ph[k];
+  double s0 = 0.0, c0 = 1.0;
+  if (variant == 0) sincos(phi0, &s0, &c0);
   return nat_dih_core<R>(b.P->p, g, acc, b.ndih_i[0][k], b.ndih_i[1][k], b.ndih_i[2][k],
-                         b.ndih_i[3][k], b.ndih_ph[k], variant);
+                         b.ndih_i[3][k], phi0, c0, s0, variant);
 }
 
 // heur_dih::eval_forces::iter (angles/heur_dih/eval_forces.cpp:10-55)
@@ -341,10 +353,11 @@ __device__ __forceinline__ double heur_dih_core(const pcg_params &p, const G &g,
   v3<R> x1 = cross(r23, r12), x2 = cross(r34, r23);
   R x1rn = rsq(nsq(x1)), x2rn = rsq(nsq(x2));
   v3<R> x1u = x1 * x1rn, x2u = x2 * x2rn;
+  // sin phi without acos: x1 x x2 = r23 ((r12 x r23).r34), and the sign the
+  // reference gives phi (that of -r12.x2u) is the sign of (r12 x r23).r34
   R cphi = dot(x1u, x2u);
-  R phi = acosc(cphi);
-  if (-dot(r12, x2u) < (R)0) phi = -phi;
-  R sphi = sinR(phi);
+  R r23l = sq(nsq(r23));
+  R sphi = -(r23l * dot(x1, r34) * x1rn * x2rn);
   R s2 = sphi * sphi, c2 = cphi * cphi, sc = sphi * cphi;
   R V = (R)c[0][ty] + (R)c[1][ty] * sphi + (R)c[2][ty] * cphi + (R)c[3][ty] * s2 +
         (R)c[4][ty] * c2 + (R)c[5][ty] * sc;
