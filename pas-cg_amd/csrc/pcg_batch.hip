@@ -12,12 +12,32 @@
 
 namespace pcg {
 
+// Geometry and per-residue QA data of the workgroup's system, from LDS.
 struct GeoL {
   const double *x, *y, *z;
   int res0;
+  const uint8_t *at;       // residue type | has chain predecessor << 6 | successor << 7
+  const uint32_t *sync;    // the four sync budgets of each residue, one byte each
+  const int8_t *ptype_t;   // polarity class per residue type
+  const double *req_t;     // r_high[type] * (1 + formation_tol) per QA type
   __device__ __forceinline__ v3<double> r(int i) const {
     int k = i - res0;
     return v3<double>{x[k], y[k], z[k]};
+  }
+  __device__ __forceinline__ int atype(const DevBatch &, int i) const { return at[i - res0] & 63; }
+  __device__ __forceinline__ int ptype(const DevBatch &, int a) const { return ptype_t[a]; }
+  __device__ __forceinline__ uint32_t sync4(const DevBatch &, int i) const { return sync[i - res0]; }
+  __device__ __forceinline__ double type_req(const DevBatch &, int t) const { return req_t[t]; }
+  // n and h recomputed from the positions, with prepare_nh's operations (the
+  // chain neighbours of residue i are i - 1 and i + 1); zero at chain ends,
+  // where prepare_nh writes nothing
+  __device__ __forceinline__ void nh(const DevBatch &, int i, double out[6]) const {
+    const uint8_t f = at[i - res0];
+    if ((f & 0xC0) != 0xC0) {
+      for (int c = 0; c < 6; ++c) out[c] = 0.0;
+      return;
+    }
+    nh_from(r(i), r(i - 1), r(i + 1), out);
   }
 };
 
@@ -400,8 +420,8 @@ __device__ __forceinline__ void term_loop2(const DevBatch &b, const SysDesc &sd,
       v += nat_cont_core<R>(b, sd, st, g, acc, q1, st.t, true);
       if (two) v += nat_cont_core<R>(b, sd, st, g, acc, q2, st.t, true);
     } else {
-      QhFetch q1 = qa_cand_fetch(b, sd, k), q2{};
-      if (two) q2 = qa_cand_fetch(b, sd, k2);
+      QhFetch q1 = qa_cand_fetch_full(b, sd, k), q2{};
+      if (two) q2 = qa_cand_fetch_full(b, sd, k2);
       v += qa_cand_core<R>(b, sd, st, g, acc, q1);
       if (two) v += qa_cand_core<R>(b, sd, st, g, acc, q2);
     }
@@ -427,7 +447,7 @@ __device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, const SysDe
     QhFetch q[4];
     for (int j = 0; j < 4; ++j) {
       int hk = base + j * B + (int)threadIdx.x;
-      q[j] = hk < n ? qa_cand_fetch(b, sd, hk) : QhFetch{hk, 0, 0, 0.f, 1};
+      q[j] = hk < n ? qa_cand_fetch(b, sd, hk) : QhFetch{hk, 0, 0, 0.f, 1, 0.0, 0};
     }
     for (int j = 0; j < 4; ++j) {
       bool keep = q[j].hk < n && qa_cand_may_act(sd, g, q[j]);
@@ -439,7 +459,7 @@ __device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, const SysDe
     }
     __syncthreads();
     const int m = qcount;
-    for (int e = threadIdx.x; e < m; e += B) v += qa_cand_exact<R>(b, sd, st, g, acc, qa_cand_fetch(b, sd, queue[e]));
+    for (int e = threadIdx.x; e < m; e += B) v += qa_cand_exact<R>(b, sd, st, g, acc, qa_cand_fetch_full(b, sd, queue[e]));
     __syncthreads();
   }
   if (want_v && v != 0.0) atomicAdd(&vacc[PCG_QA_CAND], v);
@@ -482,13 +502,19 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   double *rx = (double *)lds, *ry = rx + n, *rz = ry + n;
   double *fx = (double *)(lds + lay.f), *fy = fx + n, *fz = fy + n;
   BondLds bl{(double *)(lds + lay.bd), (uint8_t *)(lds + lay.bu), n};
-  uint8_t *atl = (uint8_t *)(lds + lay.at);  // residue types
-  // 1 / mass and its square root per residue type, for the Langevin step
-  __shared__ double minv_t[21], sqminv_t[21];
+  uint8_t *atl = (uint8_t *)(lds + lay.at);      // residue types and chain flags
+  uint32_t *syl = (uint32_t *)(lds + lay.sy);     // QA sync budgets
+  // 1 / mass and its square root per residue type, for the Langevin step;
+  // the QA polarity classes and per-type formation distances
+  __shared__ double minv_t[21], sqminv_t[21], req_t[445];
+  __shared__ int8_t ptype_t[21];
   if (threadIdx.x < 21) {
     minv_t[threadIdx.x] = 1.0 / P.p.mass[threadIdx.x];
     sqminv_t[threadIdx.x] = sqrt(minv_t[threadIdx.x]);
+    ptype_t[threadIdx.x] = (int8_t)P.p.ptype[threadIdx.x];
   }
+  for (int t = threadIdx.x; t < 445; t += blockDim.x)
+    req_t[t] = __dmul_rn(P.p.qa_lj_r_high[t], __dadd_rn(1.0, P.p.qa_formation_tol));
   int *qa_queue = (int *)(lds + lay.q);
   double *ys = (double *)(lds + lay.g);  // [15][n]: y1..y5 x, y, z (gear_l)
   __shared__ double vacc[NT];
@@ -499,9 +525,12 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     rx[k] = b.y[0][0][r0 + k];
     ry[k] = b.y[0][1][r0 + k];
     rz[k] = b.y[0][2][r0 + k];
-    atl[k] = b.atype[r0 + k];
+    // chain neighbours are always the adjacent residues (i - 1, i + 1)
+    atl[k] = (uint8_t)(b.atype[r0 + k] | (b.prev[r0 + k] >= 0 ? 0x40 : 0) |
+                       (b.next[r0 + k] >= 0 ? 0x80 : 0));
+    syl[k] = *(const uint32_t *)(b.sync + 4 * (r0 + k));
   }
-  GeoL g{rx, ry, rz, r0};
+  GeoL g{rx, ry, rz, r0, atl, syl, ptype_t, req_t};
   AccL<double> acc{fx, fy, fz, r0};
   if (staged) stage_bonded(b, sd, bl);
   if (gear_l)
@@ -540,8 +569,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
       fy[k] = 0;
       fz[k] = 0;
     }
-    if (P.p.qa_on)
-      for (int k = threadIdx.x; k < n; k += blockDim.x) prepare_nh_item(b, g, r0 + k);
+    // (n and h of qa::prepare_nh are recomputed where a QA candidate needs them)
     if (flag_rebuild) wg_fix_nl(b, sd, st, g);
     if (threadIdx.x == 0) {
       int acc_n = 0;
@@ -579,7 +607,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     term_loop<PCG_PID, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_VEL_AFM, R>(b, sd, st, g, acc, sizes, vacc, wv);
     __syncthreads();
-    if (P.p.qa_on) qa_finish_block(b, sd, st);
+    if (P.p.qa_on) qa_finish_block(b, sd, st, (int8_t *)syl, r0);
     // lang::step (cg/src/cg/langevin/step.cpp:23-77)
     double maxd = 0.0;
     if (P.p.lang_on) {
@@ -591,7 +619,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
         double nz[3];
         philox_normals(sd.philox_key, st.step, k, nz);
         double r[3], vel[3];
-        const int aa = atl[k];
+        const int aa = atl[k] & 63;  // the type, without the chain flags
         if (gear_l) {
           gear_core(b, sd, st, minv_t[aa], sqminv_t[aa], fx[k], fy[k], fz[k], nz, r, vel,
                     [&](int kk, int c) -> double & {
@@ -646,6 +674,8 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
       for (int j = 0; j < 15; ++j) b.y[1 + j / 3][j % 3][r0 + k] = ys[j * n + k];
       for (int c = 0; c < 3; ++c) b.vel[c][r0 + k] = ys[c * n + k] * P.dt_inv;
     }
+  if (P.p.qa_on)
+    for (int k = threadIdx.x; k < n; k += blockDim.x) *(uint32_t *)(b.sync + 4 * (r0 + k)) = syl[k];
   if (!evaluated) return;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     ((RF *)b.F[0])[r0 + k] = (RF)fx[k];

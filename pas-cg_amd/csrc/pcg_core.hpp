@@ -44,13 +44,18 @@ struct SysDesc {
 // the two chiral fields; 3 n bytes: flags, heuristic angle and dihedral
 // types).
 // With gear set, the Gear derivatives y1..y5 (15 n FP64) stay in LDS too.
+// Dynamic LDS of one persistent workgroup, at most: 160 KiB per CU on
+// gfx950, less the kernel's static LDS (about 5 KiB).
+constexpr size_t LDS_DYN_MAX = 154 * 1024;
+
 struct LdsLayout {
   static constexpr size_t QA_QUEUE = 4 * 1024;  // QA pairs past the first test, per pass
-  size_t f, at, bd, bu, g, q, total;
+  size_t f, at, sy, bd, bu, g, q, total;
   __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged, bool gear) {
     f = 24 * n;
     at = f + 3 * n * fsz;
-    bd = (at + n + 15) / 16 * 16;
+    sy = (at + n + 3) / 4 * 4;
+    bd = (sy + 4 * n + 15) / 16 * 16;
     bu = bd + 48 * n;
     g = ((staged ? bu + 3 * n : bd) + 15) / 16 * 16;
     q = gear ? g + 120 * n : g;

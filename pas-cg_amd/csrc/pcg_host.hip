@@ -1260,12 +1260,12 @@ int pcg_batch_create(const pcg_params *p, int32_t nsys, const pcg_model *models,
   DeviceBatch &B = *b->B;
   // per-residue bonded data joins positions and forces in LDS when it fits;
   // systems whose positions and forces do not fit take the stage kernels
-  bool staged = lds_for(B, true) <= 150 * 1024;
-  bool gear = staged && LdsLayout((size_t)B.max_n, sizeof(double), true, true).total <= 150 * 1024;
+  bool staged = lds_for(B, true) <= LDS_DYN_MAX;
+  bool gear = staged && LdsLayout((size_t)B.max_n, sizeof(double), true, true).total <= LDS_DYN_MAX;
   b->lds = LdsLayout((size_t)B.max_n, sizeof(double), staged, gear).total;
   B.d.bond_lds = staged ? 1 : 0;
   B.d.gear_lds = gear ? 1 : 0;
-  b->persistent = b->lds <= 150 * 1024;
+  b->persistent = b->lds <= LDS_DYN_MAX;
   if (opt.block != 0 && (opt.block < 64 || opt.block > 1024 || opt.block % 64 != 0))
     return fail(PCG_E_ARG, "block must be a multiple of 64 in [64, 1024]");
   if (opt.steps_per_launch < 0) return fail(PCG_E_ARG, "steps_per_launch must be >= 0");
@@ -1635,7 +1635,7 @@ int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature, int
   if ((int64_t)block_size(bytes) > s->o.hbm_budget_bytes)
     return fail(PCG_E_NOMEM, "job needs " + std::to_string(block_size(bytes)) +
                                  " bytes, over the budget");
-  if (LdsLayout((size_t)m->n, sizeof(double), false, false).total > 150 * 1024)
+  if (LdsLayout((size_t)m->n, sizeof(double), false, false).total > LDS_DYN_MAX)
     return fail(PCG_E_ARG, "job of " + std::to_string(m->n) +
                                " residues does not fit the persistent kernel's LDS");
   j->bytes = bytes;
@@ -1718,8 +1718,8 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
       slots[k] = JobSlot{std::min<int64_t>(j.done + s->o.chunk_steps, j.nsteps), 0, 0.0, 0, 0};
       // each job stages its bonded data in LDS when that fits on its own
       const size_t nj = (size_t)B.max_n;
-      B.d.bond_lds = LdsLayout(nj, fsz, true, false).total <= 150 * 1024 ? 1 : 0;
-      B.d.gear_lds = B.d.bond_lds && LdsLayout(nj, fsz, true, true).total <= 150 * 1024 ? 1 : 0;
+      B.d.bond_lds = LdsLayout(nj, fsz, true, false).total <= LDS_DYN_MAX ? 1 : 0;
+      B.d.gear_lds = B.d.bond_lds && LdsLayout(nj, fsz, true, true).total <= LDS_DYN_MAX ? 1 : 0;
       lds = std::max(lds, LdsLayout(nj, fsz, B.d.bond_lds != 0, B.d.gear_lds != 0).total);
       descs[k] = B.d;
       max_n = std::max(max_n, B.max_n);
@@ -1737,7 +1737,7 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
                           s->stream));
     HIPCHK(hipMemcpyAsync(s->d_slots, slots.data(), nres * sizeof(JobSlot), hipMemcpyHostToDevice,
                           s->stream));
-    if (lds > 150 * 1024) return fail(PCG_E_ARG, "job too large for the persistent kernel");
+    if (lds > LDS_DYN_MAX) return fail(PCG_E_ARG, "job too large for the persistent kernel");
     int block = 64;
     while (block < 1024 && block < 2 * max_n) block *= 2;
     block = std::max(block, 128);

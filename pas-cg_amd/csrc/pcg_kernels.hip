@@ -327,7 +327,7 @@ __global__ void k_qa_finish(DevBatch b) {
   int s = blockIdx.x;
   SysState &st = b.st[s];
   if (!st.active || !b.P->p.qa_on) return;
-  qa_finish_block(b, b.sys[s], st);
+  qa_finish_block(b, b.sys[s], st, b.sync, 0);
 }
 
 template <typename RF> __global__ void k_lang(DevBatch b, int noise_mode) {

@@ -9,9 +9,23 @@ namespace pcg {
 
 constexpr int BLK = 256;
 
+// Geometry and per-residue QA data from HBM.
 struct GeoG {
   const double *x, *y, *z;
   __device__ __forceinline__ v3<double> r(int i) const { return v3<double>{x[i], y[i], z[i]}; }
+  __device__ __forceinline__ int atype(const DevBatch &b, int i) const { return b.atype[i]; }
+  __device__ __forceinline__ int ptype(const DevBatch &b, int a) const { return b.P->p.ptype[a]; }
+  // the four sync budgets of residue i, byte c = budget c
+  __device__ __forceinline__ uint32_t sync4(const DevBatch &b, int i) const {
+    return *(const uint32_t *)(b.sync + 4 * i);
+  }
+  __device__ __forceinline__ double type_req(const DevBatch &b, int t) const {
+    return __dmul_rn(b.P->p.qa_lj_r_high[t], __dadd_rn(1.0, b.P->p.qa_formation_tol));
+  }
+  // n and h as prepare_nh left them
+  __device__ __forceinline__ void nh(const DevBatch &b, int i, double out[6]) const {
+    for (int c = 0; c < 6; ++c) out[c] = b.nh[c][i];
+  }
 };
 
 template <typename RF> struct AccG {

@@ -124,7 +124,9 @@ __device__ inline void build_qa_near(const DevBatch &b, const SysDesc &sd, SysSt
   __syncthreads();
 }
 
-__device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, SysState &st) {
+// sy[4 * (i - sy_off) + c] is sync budget c of residue i (HBM or LDS).
+__device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, SysState &st,
+                                       int8_t *sy, int sy_off) {
   int rem_n = st.rem_n < sd.qccap ? st.rem_n : sd.qccap;
   int cand_n = st.cand_n < sd.nlcap ? st.cand_n : sd.nlcap;
   // rank of every candidate by (dist, free-pair index) -> permutation
@@ -156,8 +158,8 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
       int q = sd.qc0 + rem[a];
       int i1 = b.qc_i[0][q], i2 = b.qc_i[1][q];
       for (int c = 0; c < 4; ++c) {
-        b.sync[4 * i1 + c] = (int8_t)(b.sync[4 * i1 + c] + b.qc_sd[8 * q + c]);
-        b.sync[4 * i2 + c] = (int8_t)(b.sync[4 * i2 + c] + b.qc_sd[8 * q + 4 + c]);
+        sy[4 * (i1 - sy_off) + c] = (int8_t)(sy[4 * (i1 - sy_off) + c] + b.qc_sd[8 * q + c]);
+        sy[4 * (i2 - sy_off) + c] = (int8_t)(sy[4 * (i2 - sy_off) + c] + b.qc_sd[8 * q + 4 + c]);
       }
       if (st.qf_n < sd.qfcap) {
         int f = sd.qf0 + st.qf_n;
@@ -188,8 +190,8 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
       bool ok = true;
       int8_t n1[4], n2[4];
       for (int c = 0; c < 4; ++c) {
-        n1[c] = (int8_t)(b.sync[4 * i1 + c] - b.cand_sd[8 * sc + c]);
-        n2[c] = (int8_t)(b.sync[4 * i2 + c] - b.cand_sd[8 * sc + 4 + c]);
+        n1[c] = (int8_t)(sy[4 * (i1 - sy_off) + c] - b.cand_sd[8 * sc + c]);
+        n2[c] = (int8_t)(sy[4 * (i2 - sy_off) + c] - b.cand_sd[8 * sc + 4 + c]);
       }
       for (int c = 0; c < 4; ++c) ok = ok && n1[c] >= 0;
       if (!ok) continue;
@@ -210,8 +212,8 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
       st.qc_n += 1;
       st.num_qa += 1;
       for (int c = 0; c < 4; ++c) {
-        b.sync[4 * i1 + c] = n1[c];
-        b.sync[4 * i2 + c] = n2[c];
+        sy[4 * (i1 - sy_off) + c] = n1[c];
+        sy[4 * (i2 - sy_off) + c] = n2[c];
       }
       b.qf_vac[sd.qf0 + b.cand_idx[sc]] = 1;
       b.qh_vac[sd.qf0 + b.cand_h[sc]] = 1;

@@ -505,25 +505,31 @@ __device__ double vel_afm_item(const DevBatch &b, const G &g, A &acc, int k, dou
 }
 
 // ------------------------------------------------------------------ QA ----
-// qa::prepare_nh::iter (cg/src/cg/qa/prepare_nh.cpp:10-19), exact FP64.
-template <typename G>
-__device__ void prepare_nh_item(const DevBatch &b, const G &g, int i) {
+// qa::prepare_nh::iter (cg/src/cg/qa/prepare_nh.cpp:10-19), exact FP64:
+// n and h of residue i from its own and its chain neighbours' positions.
+__device__ __forceinline__ void nh_from(v3<double> ri, v3<double> rp, v3<double> rn, double nh[6]) {
 #pragma clang fp contract(off)
-  int ip = b.prev[i], in = b.next[i];
-  if (ip < 0 || in < 0) return;
-  v3<double> v1 = xsub(g.r(i), g.r(ip)), v2 = xsub(g.r(in), g.r(i));
+  v3<double> v1 = xsub(ri, rp), v2 = xsub(rn, ri);
   v3<double> nn = xsub(v2, v1);
   double ni = __ddiv_rn(1.0, xnorm3(nn));
   v3<double> h{__dsub_rn(__dmul_rn(v2.y, v1.z), __dmul_rn(v2.z, v1.y)),
                __dsub_rn(__dmul_rn(v2.z, v1.x), __dmul_rn(v2.x, v1.z)),
                __dsub_rn(__dmul_rn(v2.x, v1.y), __dmul_rn(v2.y, v1.x))};
   double hi = __ddiv_rn(1.0, xnorm3(h));
-  b.nh[0][i] = __dmul_rn(ni, nn.x);
-  b.nh[1][i] = __dmul_rn(ni, nn.y);
-  b.nh[2][i] = __dmul_rn(ni, nn.z);
-  b.nh[3][i] = __dmul_rn(hi, h.x);
-  b.nh[4][i] = __dmul_rn(hi, h.y);
-  b.nh[5][i] = __dmul_rn(hi, h.z);
+  nh[0] = __dmul_rn(ni, nn.x);
+  nh[1] = __dmul_rn(ni, nn.y);
+  nh[2] = __dmul_rn(ni, nn.z);
+  nh[3] = __dmul_rn(hi, h.x);
+  nh[4] = __dmul_rn(hi, h.y);
+  nh[5] = __dmul_rn(hi, h.z);
+}
+template <typename G>
+__device__ void prepare_nh_item(const DevBatch &b, const G &g, int i) {
+  int ip = b.prev[i], in = b.next[i];
+  if (ip < 0 || in < 0) return;
+  double nh[6];
+  nh_from(g.r(i), g.r(ip), g.r(in), nh);
+  for (int c = 0; c < 6; ++c) b.nh[c][i] = nh[c];
 }
 
 __device__ __forceinline__ double xdot(double ax, double ay, double az, v3<double> u) {
@@ -540,10 +546,17 @@ struct QhFetch {
   int hk, i1, i2;
   float thr2;
   int vac;
+  double req;  // the fields below only from qa_cand_fetch_full
+  int slot;
 };
 __device__ __forceinline__ QhFetch qa_cand_fetch(const DevBatch &b, const SysDesc &s, int hk) {
   int h = s.qf0 + hk;
-  return QhFetch{hk, b.qh_i[0][h], b.qh_i[1][h], b.qh_thr2[h], b.qh_vac[h]};
+  return QhFetch{hk, b.qh_i[0][h], b.qh_i[1][h], b.qh_thr2[h], b.qh_vac[h], 0.0, 0};
+}
+// with what the exact path needs, in the same round trip
+__device__ __forceinline__ QhFetch qa_cand_fetch_full(const DevBatch &b, const SysDesc &s, int hk) {
+  int h = s.qf0 + hk;
+  return QhFetch{hk, b.qh_i[0][h], b.qh_i[1][h], b.qh_thr2[h], b.qh_vac[h], b.qh_req[h], b.qh_slot[h]};
 }
 // The cheap first test: a vacant pair, or one beyond its radius, has no
 // effect at all.
@@ -555,14 +568,19 @@ __device__ __forceinline__ bool qa_cand_may_act(const SysDesc &s, const G &g, co
   return !(dx * dx + dy * dy + dz * dz > q.thr2);
 }
 
-// Everything after the first test, in exact FP64 as the reference.
+// Everything after the first test, in exact FP64 as the reference. q comes
+// from qa_cand_fetch_full. The residue data (types, sync budgets, n and h)
+// comes through the geometry accessor: from HBM, or from LDS and positions.
 template <typename R, typename G, typename A>
 __device__ __forceinline__ double qa_cand_exact(const DevBatch &b, const SysDesc &s, SysState &st,
                                                 const G &g, A &acc, const QhFetch &q) {
 #pragma clang fp contract(off)
   const DevParams &P = *b.P;
   const pcg_params &p = P.p;
-  const int hk = q.hk, h = s.qf0 + hk, i1 = q.i1, i2 = q.i2;
+  const int hk = q.hk, i1 = q.i1, i2 = q.i2;
+  // independent of the geometry: issued first
+  const uint32_t sy1 = g.sync4(b, i1), sy2 = g.sync4(b, i2);
+  const int a1 = g.atype(b, i1), a2 = g.atype(b, i2);
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
   double rn = __ddiv_rn(1.0, xnorm3(d));
   v3<double> u{__dmul_rn(d.x, rn), __dmul_rn(d.y, rn), __dmul_rn(d.z, rn)};
@@ -577,40 +595,42 @@ __device__ __forceinline__ double qa_cand_exact(const DevBatch &b, const SysDesc
     acc.add(i2, -f);
     V = (double)Vr;
   }
-  if (1.0 > __dmul_rn(rn, b.qh_req[h])) return V;  // beyond every type's formation distance
-  double h1u = fabs(xdot(b.nh[3][i1], b.nh[4][i1], b.nh[5][i1], u));
-  double h2u = fabs(xdot(b.nh[3][i2], b.nh[4][i2], b.nh[5][i2], u));
-  double hh = fabs(__dadd_rn(__dadd_rn(__dmul_rn(b.nh[3][i1], b.nh[3][i2]),
-                                       __dmul_rn(b.nh[4][i1], b.nh[4][i2])),
-                             __dmul_rn(b.nh[5][i1], b.nh[5][i2])));
-  double n1u = xdot(b.nh[0][i1], b.nh[1][i1], b.nh[2][i1], u);
-  double n2u = xdot(b.nh[0][i2], b.nh[1][i2], b.nh[2][i2], u);
+  if (1.0 > __dmul_rn(rn, q.req)) return V;  // beyond every type's formation distance
+  double nh1[6], nh2[6];
+  g.nh(b, i1, nh1);
+  g.nh(b, i2, nh2);
+  double h1u = fabs(xdot(nh1[3], nh1[4], nh1[5], u));
+  double h2u = fabs(xdot(nh2[3], nh2[4], nh2[5], u));
+  double hh = fabs(__dadd_rn(__dadd_rn(__dmul_rn(nh1[3], nh2[3]), __dmul_rn(nh1[4], nh2[4])),
+                             __dmul_rn(nh1[5], nh2[5])));
+  double n1u = xdot(nh1[0], nh1[1], nh1[2], u);
+  double n2u = xdot(nh2[0], nh2[1], nh2[2], u);
   bool back1 = h1u > p.qa_min_cos_hr, back2 = h2u > p.qa_min_cos_hr;
   bool bbx = hh > p.qa_min_cos_hh;
   bool side1 = n1u < p.qa_max_cos_nr, side2 = -n2u < p.qa_max_cos_nr;
-  int a1 = b.atype[i1], a2 = b.atype[i2];
   int type = 0;
   if (back1 && back2 && bbx) type = 1;
   else if (back1 && side2) type = 2;
   else if (side1 && back2) type = 3;
   else if (side1 && side2) type = 4 + a1 + a2 * 21;
   if (type == 0) return V;
-  double req = __dmul_rn(p.qa_lj_r_high[type], __dadd_rn(1.0, p.qa_formation_tol));
+  // r_high[type] * (1 + formation_tol)
+  const double req = g.type_req(b, type);
   if (1.0 > __dmul_rn(rn, req)) return V;
-  int pt1 = p.ptype[a1], pt2 = p.ptype[a2];
+  int pt1 = g.ptype(b, a1), pt2 = g.ptype(b, a2);
   int8_t s1[4] = {(int8_t)back1, (int8_t)side1, (int8_t)(side1 && pt2 == 1), (int8_t)(side1 && pt2 == 2)};
   int8_t s2[4] = {(int8_t)back2, (int8_t)side2, (int8_t)(side2 && pt1 == 1), (int8_t)(side2 && pt1 == 2)};
   for (int a = 0; a < 4; ++a)
-    if ((int8_t)(b.sync[4 * i1 + a] - s1[a]) < 0) return V;
+    if ((int8_t)((int8_t)(sy1 >> (8 * a)) - s1[a]) < 0) return V;
   for (int a = 0; a < 4; ++a)
-    if ((int8_t)(b.sync[4 * i2 + a] - s2[a]) < 0) return V;
+    if ((int8_t)((int8_t)(sy2 >> (8 * a)) - s2[a]) < 0) return V;
   int c = atomicAdd(&st.cand_n, 1);
   if (c >= s.nlcap) {
     st.overflow = 1;
     return V;
   }
   int cs = s.nl0 + c;
-  int k = b.qh_slot[h];
+  int k = q.slot;
   b.cand_i[0][cs] = i1;
   b.cand_i[1][cs] = i2;
   b.cand_od[cs] = b.qf_d[s.qf0 + k];
@@ -633,7 +653,7 @@ __device__ __forceinline__ double qa_cand_core(const DevBatch &b, const SysDesc 
 template <typename R, typename G, typename A>
 __device__ __forceinline__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st,
                                                const G &g, A &acc, int hk) {
-  return qa_cand_core<R>(b, s, st, g, acc, qa_cand_fetch(b, s, hk));
+  return qa_cand_core<R>(b, s, st, g, acc, qa_cand_fetch_full(b, s, hk));
 }
 
 // qa::process_contacts::iter (cg/src/cg/qa/process_contacts.cpp:12-72)
