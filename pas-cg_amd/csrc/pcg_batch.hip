@@ -400,66 +400,87 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc 
   }
 }
 
-// Native contacts and QA near pairs, two items per thread per pass with both
-// items' list fields loaded before either is evaluated (their chains of
-// list load -> LDS positions -> exact decisions overlap).
-template <int T, typename R>
-__device__ __forceinline__ void term_loop2(const DevBatch &b, const SysDesc &sd, SysState &st,
-                                           const GeoL &g, AccL<double> &acc, const int *sizes,
-                                           double *vacc, bool want_v) {
-  const int lo = sizes[T], n = sizes[T + 1] - lo, B = blockDim.x;
+// Native contacts, NK items per thread per pass, with every item's list
+// fields loaded before any is evaluated (their chains of list load -> LDS
+// positions -> decisions overlap).
+template <int NK, typename R>
+__device__ __forceinline__ void nat_cont_loop(const DevBatch &b, const SysDesc &sd, SysState &st,
+                                              const GeoL &g, AccL<double> &acc, const int *sizes,
+                                              double *vacc, bool want_v) {
+  const int lo = sizes[PCG_NAT_CONT], n = sizes[PCG_NAT_CONT + 1] - lo, B = blockDim.x;
   if (n <= 0) return;
   int k = ((int)threadIdx.x - lo % B + B) % B;
   double v = 0.0;
-  for (; k < n; k += 2 * B) {
-    const int k2 = k + B;
-    const bool two = k2 < n;
-    if constexpr (T == PCG_NAT_CONT) {
-      NcFetch q1 = nat_cont_fetch(b, sd.nc0 + k), q2{};
-      if (two) q2 = nat_cont_fetch(b, sd.nc0 + k2);
-      v += nat_cont_core<R>(b, sd, st, g, acc, q1, st.t, true);
-      if (two) v += nat_cont_core<R>(b, sd, st, g, acc, q2, st.t, true);
-    } else {
-      QhFetch q1 = qa_cand_fetch_full(b, sd, k), q2{};
-      if (two) q2 = qa_cand_fetch_full(b, sd, k2);
-      v += qa_cand_core<R>(b, sd, st, g, acc, q1);
-      if (two) v += qa_cand_core<R>(b, sd, st, g, acc, q2);
-    }
+  for (; k < n; k += NK * B) {
+    NcFetch q[NK];
+    for (int j = 0; j < NK; ++j)
+      if (k + j * B < n) q[j] = nat_cont_fetch(b, sd.nc0 + k + j * B);
+    for (int j = 0; j < NK; ++j)
+      if (k + j * B < n) v += nat_cont_core<R>(b, sd, st, g, acc, q[j], st.t, true);
   }
-  if (want_v && v != 0.0) atomicAdd(&vacc[T], v);
+  if (want_v && v != 0.0) atomicAdd(&vacc[PCG_NAT_CONT], v);
 }
 
-// QA near pairs in two phases per pass of 4 items per thread: every pair
-// takes the cheap first test, the few that pass are queued in LDS, then the
-// queue is evaluated exactly, densely (a wave no longer waits on the exact
-// path of the one lane in it that needs it, item after item).
+// QA near pairs in two phases per pass of QK items per thread: every pair
+// takes the cheap first test, and the few that pass are queued in LDS with
+// their residues; the queue is then evaluated exactly and densely. A wave no
+// longer waits, item after item, on the exact path of the one lane in it
+// that needs it, and the exact phase reads nothing from HBM (a pair's
+// formation distance bound is recomputed from the LDS tables, its free-pair
+// slot loaded only if it becomes a candidate). A pair that finds the queue
+// full is evaluated at once by its own lane.
 template <typename R>
 __device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, const SysDesc &sd, SysState &st,
                                                   const GeoL &g, AccL<double> &acc, const int *sizes,
-                                                  int *queue, double *vacc, bool want_v) {
+                                                  uint2 *queue, double *vacc, bool want_v) {
+  constexpr int QK = 8;
+  constexpr int QCAP = (int)(LdsLayout::QA_QUEUE * sizeof(int) / sizeof(uint2));
   const int n = sizes[PCG_QA_CAND + 1] - sizes[PCG_QA_CAND], B = blockDim.x;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, r0 = sd.res0;
   __shared__ int qcount;
   double v = 0.0;
-  for (int base = 0; base < n; base += 4 * B) {
+  // the exact path's q: every field but the free-pair slot
+  auto full = [&](int hk, int i1, int i2) {
+    const int a1 = g.atype(b, i1), a2 = g.atype(b, i2);
+    double req = fmax(fmax(fmax(g.req_t[1], g.req_t[2]), g.req_t[3]), g.req_t[4 + a1 + 21 * a2]);
+    return QhFetch{hk, i1, i2, 0.f, 0, req, -1};
+  };
+  for (int base = 0; base < n; base += QK * B) {
     if (threadIdx.x == 0) qcount = 0;
     __syncthreads();
-    QhFetch q[4];
-    for (int j = 0; j < 4; ++j) {
+    QhFetch q[QK];
+    for (int j = 0; j < QK; ++j) {
       int hk = base + j * B + (int)threadIdx.x;
       q[j] = hk < n ? qa_cand_fetch(b, sd, hk) : QhFetch{hk, 0, 0, 0.f, 1, 0.0, 0};
     }
-    for (int j = 0; j < 4; ++j) {
+    unsigned ovf = 0;  // items of this thread that found the queue full
+    for (int j = 0; j < QK; ++j) {
       bool keep = q[j].hk < n && qa_cand_may_act(sd, g, q[j]);
       unsigned long long m = __ballot(keep);
       int off = 0;
       if (lane == 0 && m) off = atomicAdd(&qcount, __popcll(m));
       off = __shfl(off, 0, 64);
-      if (keep) queue[off + __popcll(m & ((1ull << lane) - 1ull))] = q[j].hk;
+      if (keep) {
+        int e = off + __popcll(m & ((1ull << lane) - 1ull));
+        if (e < QCAP)
+          queue[e] = make_uint2((unsigned)q[j].hk,
+                                (unsigned)(q[j].i1 - r0) | ((unsigned)(q[j].i2 - r0) << 16));
+        else
+          ovf |= 1u << j;
+      }
+    }
+    for (unsigned o = ovf; o; o &= o - 1) {
+      const int hk = base + (__ffs(o) - 1) * B + (int)threadIdx.x;
+      const QhFetch f = qa_cand_fetch(b, sd, hk);
+      v += qa_cand_exact<R>(b, sd, st, g, acc, full(hk, f.i1, f.i2));
     }
     __syncthreads();
-    const int m = qcount;
-    for (int e = threadIdx.x; e < m; e += B) v += qa_cand_exact<R>(b, sd, st, g, acc, qa_cand_fetch_full(b, sd, queue[e]));
+    const int m = qcount < QCAP ? qcount : QCAP;
+    for (int e = threadIdx.x; e < m; e += B) {
+      const uint2 w = queue[e];
+      v += qa_cand_exact<R>(b, sd, st, g, acc,
+                            full((int)w.x, r0 + (int)(w.y & 0xFFFFu), r0 + (int)(w.y >> 16)));
+    }
     __syncthreads();
   }
   if (want_v && v != 0.0) atomicAdd(&vacc[PCG_QA_CAND], v);
@@ -515,7 +536,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   }
   for (int t = threadIdx.x; t < 445; t += blockDim.x)
     req_t[t] = __dmul_rn(P.p.qa_lj_r_high[t], __dadd_rn(1.0, P.p.qa_formation_tol));
-  int *qa_queue = (int *)(lds + lay.q);
+  uint2 *qa_queue = (uint2 *)(lds + lay.q);
   double *ys = (double *)(lds + lay.g);  // [15][n]: y1..y5 x, y, z (gear_l)
   __shared__ double vacc[NT];
   __shared__ int sizes[NT + 1];
@@ -598,7 +619,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
       term_loop<PCG_CND, R>(b, sd, st, g, acc, sizes, vacc, wv);
       term_loop<PCG_SND, R>(b, sd, st, g, acc, sizes, vacc, wv);
     }
-    term_loop2<PCG_NAT_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    nat_cont_loop<4, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_PAULI, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_CONST_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_REL_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);

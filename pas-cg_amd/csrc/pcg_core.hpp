@@ -49,7 +49,8 @@ struct SysDesc {
 constexpr size_t LDS_DYN_MAX = 154 * 1024;
 
 struct LdsLayout {
-  static constexpr size_t QA_QUEUE = 4 * 1024;  // QA pairs past the first test, per pass
+  // 32-bit words of the queue of QA pairs past the first test (8 bytes a pair)
+  static constexpr size_t QA_QUEUE = 4 * 1024;
   size_t f, at, sy, bd, bu, g, q, total;
   __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged, bool gear) {
     f = 24 * n;

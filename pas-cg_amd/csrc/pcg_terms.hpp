@@ -547,7 +547,7 @@ struct QhFetch {
   float thr2;
   int vac;
   double req;  // the fields below only from qa_cand_fetch_full
-  int slot;
+  int slot;    // -1: load it when the pair becomes a candidate
 };
 __device__ __forceinline__ QhFetch qa_cand_fetch(const DevBatch &b, const SysDesc &s, int hk) {
   int h = s.qf0 + hk;
@@ -630,7 +630,7 @@ __device__ __forceinline__ double qa_cand_exact(const DevBatch &b, const SysDesc
     return V;
   }
   int cs = s.nl0 + c;
-  int k = q.slot;
+  int k = q.slot >= 0 ? q.slot : b.qh_slot[s.qf0 + hk];
   b.cand_i[0][cs] = i1;
   b.cand_i[1][cs] = i2;
   b.cand_od[cs] = b.qf_d[s.qf0 + k];
