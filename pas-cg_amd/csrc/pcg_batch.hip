@@ -347,8 +347,13 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc 
       if (p.tether_on && (f & BF_TETHER)) vt += tether_core<double>(p, g, a, r, r + 1, bl.teth(k));
       // local repulsion and chirality have no ill-conditioned step: in R
       if (p.lrep_on && (f & BF_LREP)) vl += lrep_core<RF>(p, g, a, r, r + 2);
-      if (p.nat_ang_on && (f & BF_NAT_ANG)) vna += nat_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.ang(k));
-      if (p.heur_ang_on && (f & BF_HEUR_ANG)) vha += heur_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.at(k));
+      if constexpr (sizeof(RF) == 4) {
+        if (p.nat_ang_on && (f & BF_NAT_ANG)) vna += nat_ang_mixed(p, g, a, r, r + 1, r + 2, bl.ang(k));
+        if (p.heur_ang_on && (f & BF_HEUR_ANG)) vha += heur_ang_mixed(p, g, a, r, r + 1, r + 2, bl.at(k));
+      } else {
+        if (p.nat_ang_on && (f & BF_NAT_ANG)) vna += nat_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.ang(k));
+        if (p.heur_ang_on && (f & BF_HEUR_ANG)) vha += heur_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.at(k));
+      }
       if (p.nat_dih_on && (f & BF_NAT_DIH))
         vnd += nat_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dih(k), bl.dih(k), bl.dihs(k),
                                     p.nat_dih_variant);

@@ -128,7 +128,8 @@ struct DevBatch {
   // dynamic lists
   int *nl_i[2]; double *nl_d; uint8_t *nl_cls;  // class bits of the partition
   int *ncur;                                      // current native contacts (index into all)
-  int *ncur_i[2]; double *ncur_nat; uint8_t *ncur_t;  // their residues, length, type
+  int *ncur_i[2]; double *ncur_nat;  // their residues, length,
+  uint8_t *ncur_t;                   // type | active << 7
   float *ncur_th2;  // squared transition distance (nat * 2^(-1/6) * counting factor), FP32
   // bond_item[L][r]: global index of the item of list L whose first residue
   // is r, or -1 (systems with bond_regular set)
@@ -174,8 +175,20 @@ template <typename R> __device__ __forceinline__ R nsq(v3<R> a) { return a.x * a
 template <typename R> __device__ __forceinline__ v3<R> cross(v3<R> a, v3<R> b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ float rsq(float x) { return 1.0f / sqrtf(x); }
-__device__ __forceinline__ double rsq(double x) { return 1.0 / sqrt(x); }
+// 1/sqrt(x) for forces (never for a discrete decision): FP32 from the
+// hardware's v_rsq_f32 (about 1 ulp); FP64 from v_rsq_f64 refined by two
+// Newton steps y <- y + y (1 - x y^2) / 2 (an ulp or two from the correctly
+// rounded value, against some 50 instructions for sqrt and a division).
+// x = 0 and x = inf keep the hardware's inf and 0.
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsq(double x) {
+  const double y0 = __builtin_amdgcn_rsq(x);
+  double e = __builtin_fma(-(x * y0), y0, 1.0);
+  const double y1 = __builtin_fma(0.5 * y0, e, y0);
+  e = __builtin_fma(-(x * y1), y1, 1.0);
+  const double y2 = __builtin_fma(0.5 * y1, e, y1);
+  return (x == 0.0 || x == __builtin_inf()) ? y0 : y2;
+}
 __device__ __forceinline__ float sq(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sq(double x) { return sqrt(x); }
 template <typename R> __device__ __forceinline__ R clampR(R v, R lo, R hi) { return v < lo ? lo : (v < hi ? v : hi); }

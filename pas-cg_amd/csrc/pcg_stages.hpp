@@ -27,7 +27,8 @@ __device__ __forceinline__ void set_current_contact(const DevBatch &b, int q, in
   b.ncur_i[0][q] = b.nc_i[0][c];
   b.ncur_i[1][q] = b.nc_i[1][c];
   b.ncur_nat[q] = b.nc_nat[c];
-  b.ncur_t[q] = b.nc_type[c];
+  // the type, and the contact's state in bit 7 (kept equal to nc_active[c])
+  b.ncur_t[q] = (uint8_t)(b.nc_type[c] | (b.nc_active[c] ? 0x80 : 0));
   double th = __dmul_rn(__dmul_rn(b.nc_nat[c], C216_INV), b.P->p.counting_factor);
   b.ncur_th2[q] = (float)(th * th);
 }

@@ -65,9 +65,10 @@ template <typename R, typename G, typename A>
 __device__ __forceinline__ double tether_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
                                               double nat) {
   v3<R> r12 = cast3<R>(g.r(i2) - g.r(i1));
-  R rn = sq(nsq(r12)), V, dV;
+  const R r2 = nsq(r12), rinv = rsq(r2);
+  R rn = r2 * rinv, V, dV;
   harmonicR<R>((R)p.tether_H1, (R)p.tether_H2, (R)nat, rn, V, dV);
-  v3<R> f = (dV / rn) * r12;
+  v3<R> f = (dV * rinv) * r12;
   acc.add(i1, f);
   acc.add(i2, -f);
   return (double)V;
@@ -117,9 +118,9 @@ __device__ double pauli_item(const DevBatch &b, const SysDesc &s, const G &g, A 
 // the state transition is made only when update_state is set, so that a
 // contact evaluated once per residue changes state once.
 struct NcFetch {
-  int c, i1, i2;
+  int k, c, i1, i2;
   double nat;
-  int type;
+  int type;  // contact type | active << 7
   float th2;
 };
 template <typename R, typename G, typename A>
@@ -151,7 +152,7 @@ __device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc
     if (rnx >= p.nl_cutoff) return 0.0;
     rn = (R)rnx;
   }
-  if (p.nc_ss_kind != 0 && q.type == 5) {
+  if (p.nc_ss_kind != 0 && (q.type & 0x7F) == 5) {
     // disulfide_force::operator() receives the distance as its inverse-distance
     // argument (nat_cont/eval_forces.cpp:27, disulfide.h:12-20)
     if (p.nc_ss_kind == 1)
@@ -168,8 +169,9 @@ __device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc
   if (!update_state) return (double)V;
   // state transition, exact FP64 unless decided above
   if (now < 0) now = rnx <= __dmul_rn(__dmul_rn(nat, C216_INV), p.counting_factor) ? 1 : 0;
-  if ((uint8_t)now != b.nc_active[c]) {
+  if (now != (q.type >> 7)) {
     b.nc_active[c] = (uint8_t)now;
+    b.ncur_t[q.k] = (uint8_t)((q.type & 0x7F) | (now << 7));
     if (b.nc_change_t[c] < 0) {
       b.nc_change_t[c] = t;
       atomicAdd(&st.num_changed, 1);
@@ -182,7 +184,7 @@ __device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc
 // of all contacts and a copy of its residues, length and type (independent
 // loads, issued together).
 __device__ __forceinline__ NcFetch nat_cont_fetch(const DevBatch &b, int k) {
-  return NcFetch{b.ncur[k], b.ncur_i[0][k], b.ncur_i[1][k], b.ncur_nat[k], b.ncur_t[k],
+  return NcFetch{k, b.ncur[k], b.ncur_i[0][k], b.ncur_i[1][k], b.ncur_nat[k], b.ncur_t[k],
                  b.ncur_th2[k]};
 }
 template <typename R, typename G, typename A>
@@ -293,6 +295,61 @@ __device__ __forceinline__ double heur_ang_item(const DevBatch &b, const G &g, A
   return heur_ang_core<R>(b.P->p, g, acc, b.hang_i[0][k], b.hang_i[1][k], b.hang_i[2][k], b.hang_t[k]);
 }
 
+// Angles of the batched mixed form: FP32 geometry from the FP64 differences,
+// and theta = atan2(|r12 x r23|, -r12.r23), which is well conditioned at
+// every angle (an acos of a rounded cosine is not, near 0 and pi): theta
+// carries about 1e-7 rad of absolute error. Near-collinear triples
+// (sin theta < 0.05), where the FP32 directions of the derivatives lose
+// accuracy, return false and are evaluated in FP64 by the caller.
+template <typename G>
+__device__ __forceinline__ bool angle_geom_f32(const G &g, int i1, int i2, int i3, v3<float> &d1,
+                                               v3<float> &d2, v3<float> &d3, float &th) {
+  v3<float> r12 = cast3<float>(g.r(i2) - g.r(i1)), r23 = cast3<float>(g.r(i3) - g.r(i2));
+  v3<float> x = cross(r12, r23);
+  const float a2 = nsq(r12), b2 = nsq(r23), x2 = nsq(x);
+  if (!(x2 > 0.0025f * a2 * b2)) return false;
+  const float ar = rsq(a2), br = rsq(b2), xr = rsq(x2);
+  v3<float> c1 = cross(r12, x), c3 = cross(r23, x);
+  d1 = (xr * ar * ar) * c1;
+  d3 = (xr * br * br) * c3;
+  d2 = -d1 - d3;
+  const float dt = dot(r12, r23);
+  if (dt == 0.0f) d1 = d2 = d3 = v3<float>{0.f, 0.f, 0.f};  // the cos == 0 case of angle_geom
+  th = atan2f(x2 * xr, -dt);
+  return true;
+}
+template <typename G, typename A>
+__device__ __forceinline__ double nat_ang_mixed(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+                                                int i3, double th0) {
+  v3<float> d1, d2, d3;
+  float th;
+  if (!angle_geom_f32(g, i1, i2, i3, d1, d2, d3, th)) return nat_ang_core<double>(p, g, acc, i1, i2, i3, th0);
+  const double dth = (double)th - th0;
+  const float dV = (float)(2.0 * p.CBA * dth);
+  acc.add(i1, -(dV * d1));
+  acc.add(i2, -(dV * d2));
+  acc.add(i3, -(dV * d3));
+  return p.CBA * dth * dth;
+}
+// the polynomial cancels heavily: FP64 in theta
+template <typename G, typename A>
+__device__ __forceinline__ double heur_ang_mixed(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+                                                 int i3, int ty) {
+  v3<float> d1, d2, d3;
+  float thf;
+  if (!angle_geom_f32(g, i1, i2, i3, d1, d2, d3, thf)) return heur_ang_core<double>(p, g, acc, i1, i2, i3, ty);
+  const double(*c)[9] = p.heur_ang_poly, th = thf;
+  double t2 = th * th, t3 = t2 * th, t4 = t3 * th, t5 = t4 * th, t6 = t3 * t3;
+  double V = c[0][ty] + c[1][ty] * th + c[2][ty] * t2 + c[3][ty] * t3 + c[4][ty] * t4 +
+             c[5][ty] * t5 + c[6][ty] * t6;
+  float dV = (float)(c[1][ty] + (2.0 * c[2][ty]) * th + (3.0 * c[3][ty]) * t2 + (4.0 * c[4][ty]) * t3 +
+                     (5.0 * c[5][ty]) * t4 + (6.0 * c[6][ty]) * t5);
+  acc.add(i1, -(dV * d1));
+  acc.add(i2, -(dV * d2));
+  acc.add(i3, -(dV * d3));
+  return V;
+}
+
 // cnd / snd ::iter (angles/nat_dih/complex.cpp:10-47, simple.cpp:10-42)
 //
 // The complex variant needs only cos and sin of phi - phi0. They come from
@@ -309,7 +366,8 @@ __device__ __forceinline__ double nat_dih_core(const pcg_params &p, const G &g, 
   v3<R> x1 = cross(r12, r23), x2 = cross(r23, r34);
   R x1rn = rsq(nsq(x1)), x2rn = rsq(nsq(x2));
   v3<R> x1u = x1 * x1rn, x2u = x2 * x2rn;
-  R r23n = sq(nsq(r23)), V, dVdphi;
+  const R r23sq = nsq(r23), r23rn = rsq(r23sq);
+  R r23n = r23sq * r23rn, V, dVdphi;
   if (variant == 0) {
     // cos 3x = c (4c^2 - 3), sin 3x = s (3 - 4s^2)
     R cphi = clampR(dot(x1u, x2u), (R)-1, (R)1), sphi = r23n * dot(x1, r34) * x1rn * x2rn;
@@ -326,7 +384,7 @@ __device__ __forceinline__ double nat_dih_core(const pcg_params &p, const G &g, 
     dVdphi = (R)p.CDH * diff;
   }
   v3<R> dr1 = -(x1u * (r23n * x1rn)), dr4 = x2u * (r23n * x2rn);
-  v3<R> df = (-dr1 * dot(r12, r23) + dr4 * dot(r23, r34)) * ((R)1 / (r23n * r23n));
+  v3<R> df = (-dr1 * dot(r12, r23) + dr4 * dot(r23, r34)) * (r23rn * r23rn);
   v3<R> dr2 = -dr1 + df, dr3 = -dr4 - df;
   acc.add(i1, -(dVdphi * dr1));
   acc.add(i2, -(dVdphi * dr2));
@@ -356,16 +414,16 @@ __device__ __forceinline__ double heur_dih_core(const pcg_params &p, const G &g,
   // sin phi without acos: x1 x x2 = r23 ((r12 x r23).r34), and the sign the
   // reference gives phi (that of -r12.x2u) is the sign of (r12 x r23).r34
   R cphi = dot(x1u, x2u);
-  R r23l = sq(nsq(r23));
-  R sphi = -(r23l * dot(x1, r34) * x1rn * x2rn);
+  const R r23sq = nsq(r23), r23rn = rsq(r23sq);
+  R sphi = -(r23sq * r23rn * dot(x1, r34) * x1rn * x2rn);
   R s2 = sphi * sphi, c2 = cphi * cphi, sc = sphi * cphi;
   R V = (R)c[0][ty] + (R)c[1][ty] * sphi + (R)c[2][ty] * cphi + (R)c[3][ty] * s2 +
         (R)c[4][ty] * c2 + (R)c[5][ty] * sc;
   R dV = (R)c[1][ty] * cphi - (R)c[2][ty] * sphi + (R)2 * ((R)c[3][ty] - (R)c[4][ty]) * sc +
          (R)c[5][ty] * (c2 - s2);
-  R r23n = sq(nsq(r23));
+  R r23n = r23sq * r23rn;
   v3<R> dr1 = x1u * (r23n * x1rn), dr4 = -(x2u * (r23n * x2rn));
-  v3<R> df = (dr1 * (-dot(r12, r23)) - dr4 * (-dot(r34, r23))) * ((R)1 / (r23n * r23n));
+  v3<R> df = (dr1 * (-dot(r12, r23)) - dr4 * (-dot(r34, r23))) * (r23rn * r23rn);
   v3<R> dr2 = -dr1 + df, dr3 = -dr4 - df;
   acc.add(i1, -(dV * dr1));
   acc.add(i2, -(dV * dr2));
