@@ -127,7 +127,7 @@ __device__ __forceinline__ void block_ranks5(int cls, int valid, int counts[5], 
 
 
 // NL rebuild + ordered partition inside the workgroup.
-__device__ void wg_fix_nl(const DevBatch &b, const SysDesc &sd, SysState &st, const GeoL &g) {
+__device__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysState &st, const GeoL &g) {
 #pragma clang fp contract(off)
   const double req = b.P->nl_eff_cutoff + b.P->p.nl_pad;
   int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -189,7 +189,7 @@ __device__ void wg_fix_nl(const DevBatch &b, const SysDesc &sd, SysState &st, co
   if (threadIdx.x == 0)
     for (int c = 0; c < 5; ++c) run[c] = 0;
   __syncthreads();
-  const pcg_params &p = b.P->p;
+  ParamsC &p = b.P->p;
   int nl_n = st.nl_n;
   for (int base = 0; base < nl_n; base += blockDim.x) {
     int k = base + threadIdx.x;
@@ -289,7 +289,7 @@ struct BondLds {
 
 // Once per launch: the static bonded items of every base residue, from HBM
 // into LDS (the per-step loop then reads no list of these terms from HBM).
-__device__ __forceinline__ void stage_bonded(const DevBatch &b, const SysDesc &sd, const BondLds &bl) {
+__device__ __forceinline__ void stage_bonded(const DevBatch &b, SysDescC &sd, const BondLds &bl) {
   for (int k = threadIdx.x; k < sd.n; k += blockDim.x) {
     int r = sd.res0 + k, it;
     uint8_t f = 0;
@@ -330,10 +330,10 @@ template <typename RF> __device__ __forceinline__ RF shfl_up_rf(RF v, int d) {
 // with cross-lane shifts, and one LDS add per component. Shares that would
 // cross into the next wave go to LDS directly.
 template <typename RF, bool STAGED>
-__device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc &sd, SysState &st,
+__device__ __forceinline__ void bonded_by_base(const DevBatch &b, SysDescC &sd, SysState &st,
                                                const GeoL &g, const BondLds &bl, double *fx,
                                                double *fy, double *fz, double *vacc, bool want_v) {
-  const pcg_params &p = b.P->p;
+  ParamsC &p = b.P->p;
   const int n = sd.n, r0 = sd.res0, lane = threadIdx.x & 63;
   double vt = 0, vl = 0, vna = 0, vha = 0, vnd = 0, vhd = 0, vch = 0;
   for (int base = 0; base < n; base += blockDim.x) {
@@ -409,7 +409,7 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, const SysDesc 
 // fields loaded before any is evaluated (their chains of list load -> LDS
 // positions -> decisions overlap).
 template <int NK, typename R>
-__device__ __forceinline__ void nat_cont_loop(const DevBatch &b, const SysDesc &sd, SysState &st,
+__device__ __forceinline__ void nat_cont_loop(const DevBatch &b, SysDescC &sd, SysState &st,
                                               const GeoL &g, AccL<double> &acc, const int *sizes,
                                               double *vacc, bool want_v) {
   const int lo = sizes[PCG_NAT_CONT], n = sizes[PCG_NAT_CONT + 1] - lo, B = blockDim.x;
@@ -420,6 +420,8 @@ __device__ __forceinline__ void nat_cont_loop(const DevBatch &b, const SysDesc &
     NcFetch q[NK];
     for (int j = 0; j < NK; ++j)
       if (k + j * B < n) q[j] = nat_cont_fetch(b, sd.nc0 + k + j * B);
+    for (int j = 0; j < NK; ++j)
+      if (k + j * B < n) keep_fetch(q[j]);
     for (int j = 0; j < NK; ++j)
       if (k + j * B < n) v += nat_cont_core<R>(b, sd, st, g, acc, q[j], st.t, true);
   }
@@ -435,7 +437,7 @@ __device__ __forceinline__ void nat_cont_loop(const DevBatch &b, const SysDesc &
 // slot loaded only if it becomes a candidate). A pair that finds the queue
 // full is evaluated at once by its own lane.
 template <typename R>
-__device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, const SysDesc &sd, SysState &st,
+__device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, SysDescC &sd, SysState &st,
                                                   const GeoL &g, AccL<double> &acc, const int *sizes,
                                                   uint2 *queue, double *vacc, bool want_v) {
   constexpr int QK = 8;
@@ -458,6 +460,7 @@ __device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, const SysDe
       int hk = base + j * B + (int)threadIdx.x;
       q[j] = hk < n ? qa_cand_fetch(b, sd, hk) : QhFetch{hk, 0, 0, 0.f, 1, 0.0, 0};
     }
+    for (int j = 0; j < QK; ++j) keep_fetch(q[j]);
     unsigned ovf = 0;  // items of this thread that found the queue full
     for (int j = 0; j < QK; ++j) {
       bool keep = q[j].hk < n && qa_cand_may_act(sd, g, q[j]);
@@ -494,7 +497,7 @@ __device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, const SysDe
 // The items of one term: item k is flat index sizes[T] + k, taken by thread
 // (sizes[T] + k) mod blockDim, as in one loop over all terms.
 template <int T, typename R>
-__device__ __forceinline__ void term_loop(const DevBatch &b, const SysDesc &sd, SysState &st,
+__device__ __forceinline__ void term_loop(const DevBatch &b, SysDescC &sd, SysState &st,
                                           const GeoL &g, AccL<double> &acc, const int *sizes,
                                           double *vacc, bool want_v) {
   const int lo = sizes[T], n = sizes[T + 1] - lo, B = blockDim.x;
@@ -509,14 +512,14 @@ template <typename R, typename RF>
 __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int nsteps,
                                                 JobSlot *slot = nullptr) {
 #pragma clang fp contract(off)
-  const SysDesc &sd = b.sys[s];
+  SysDescC &sd = b.sys[s];
   if (!b.st[s].active) return;
   // the system's scalars live in LDS for the launch (only this workgroup
   // touches them) and go back to HBM at its end
   __shared__ SysState st;
   if (threadIdx.x == 0) st = b.st[s];
   __syncthreads();
-  const DevParams &P = *b.P;
+  DevParamsC &P = *b.P;
   const int n = sd.n, r0 = sd.res0;
   const int64_t target = slot ? slot->target : sd.steps_target;
   extern __shared__ __align__(16) unsigned char lds[];

@@ -98,12 +98,24 @@ struct DevParams {
   int nl_required;
 };
 
+// Device code reads the parameters through the constant address space: they
+// are never written while a kernel runs, so uniform fields become scalar
+// loads (into SGPRs, hoisted freely) instead of per-lane loads at each use.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PCG_CONST_AS __attribute__((address_space(4)))
+#else
+#define PCG_CONST_AS
+#endif
+typedef const PCG_CONST_AS DevParams DevParamsC;
+typedef const PCG_CONST_AS pcg_params ParamsC;
+typedef const PCG_CONST_AS SysDesc SysDescC;  // written by the host between launches only
+
 // Device arrays of a batch (pointers into one allocation).
 struct DevBatch {
   int nsys, ntot;
-  const SysDesc *sys;
+  SysDescC *sys;
   SysState *st;
-  const DevParams *P;
+  DevParamsC *P;
   // residues
   double *y[6][3];            // Gear y0..y5 (y0 = positions)
   double *vel[3];
@@ -189,8 +201,16 @@ __device__ __forceinline__ double rsq(double x) {
   const double y2 = __builtin_fma(0.5 * y1, e, y1);
   return (x == 0.0 || x == __builtin_inf()) ? y0 : y2;
 }
-__device__ __forceinline__ float sq(float x) { return sqrtf(x); }
+// sqrt and 1/x for forces: FP32 on the hardware's v_sqrt_f32 / v_rcp_f32
+// (about 1 ulp), FP64 correctly rounded
+__device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double sq(double x) { return sqrt(x); }
+__device__ __forceinline__ float rcpR(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcpR(double x) { return 1.0 / x; }
+// Makes a loaded value count as used here, so that the compiler issues its
+// load where it stands (with the loads beside it) instead of sinking it past
+// the branches that first read it. An empty statement: no instruction.
+template <typename T> __device__ __forceinline__ void keep(T &x) { asm volatile("" : "+v"(x)); }
 template <typename R> __device__ __forceinline__ R clampR(R v, R lo, R hi) { return v < lo ? lo : (v < hi ? v : hi); }
 template <typename R> __device__ __forceinline__ R ipow6(R s) { R s2 = s * s; return s2 * s2 * s2; }
 __device__ __forceinline__ float acosc(float x) { return acosf(fminf(fmaxf(x, -1.0f), 1.0f)); }

@@ -563,9 +563,9 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   int rc = A.alloc(B->stream, pool_block, pool_bytes);
   if (rc) return rc;
   B->bytes = A.total;
-  D.sys = dsys;
+  D.sys = (SysDescC *)dsys;
   D.st = dst;
-  D.P = B->dP;
+  D.P = (DevParamsC *)B->dP;
   D.atype = atype, D.chain = chain, D.seq = seq, D.prev = prev, D.next = next;
   D.sysid = sysid, D.local = local;
   for (int k = 0; k < 2; ++k) D.teth_i[k] = teth_i[k], D.lrep_i[k] = lrep_i[k], D.nc_i[k] = nc_i[k];

@@ -17,7 +17,7 @@ __global__ void k_begin(DevBatch b) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= b.nsys) return;
   SysState &st = b.st[s];
-  const SysDesc &d = b.sys[s];
+  SysDescC &d = b.sys[s];
   if (!st.active) return;
   for (int k = 0; k < NT; ++k) st.V[k] = 0.0;
   if (b.P->nl_required) {
@@ -49,7 +49,7 @@ template <typename RF> __global__ void k_zero_forces(DevBatch b) {
 // of a system, same-chain |seq difference| < 3 skipped, kept if the wrapped
 // distance is below max(nl.cutoff, dh.cutoff) + pad. One wavefront per row;
 // rows come out in (i1, i2) order, which is the reference's sorted order.
-__device__ __forceinline__ bool nl_pred(const DevBatch &b, const SysDesc &s, const GeoG &g,
+__device__ __forceinline__ bool nl_pred(const DevBatch &b, SysDescC &s, const GeoG &g,
                                         int i, int j, double req, double &dist) {
   if (b.chain[i] == b.chain[j]) {
     int d = b.seq[i] - b.seq[j];
@@ -68,7 +68,7 @@ template <bool WRITE> __global__ void k_nl_rows(DevBatch b) {
   int s = b.sysid[row];
   SysState &st = b.st[s];
   if (!st.active || !st.nl_invalid) return;
-  const SysDesc &sd = b.sys[s];
+  SysDescC &sd = b.sys[s];
   GeoG g{b.y[0][0], b.y[0][1], b.y[0][2]};
   double req = b.P->nl_eff_cutoff + b.P->p.nl_pad;
   int end = sd.res0 + sd.n;
@@ -98,7 +98,7 @@ __global__ void k_nl_scan(DevBatch b) {
   int s = blockIdx.x;
   SysState &st = b.st[s];
   if (!st.active || !st.nl_invalid) return;
-  const SysDesc &sd = b.sys[s];
+  SysDescC &sd = b.sys[s];
   __shared__ int carry;
   __shared__ int wsum[BLK / 64];
   if (threadIdx.x == 0) carry = 0;
@@ -185,7 +185,7 @@ __global__ void k_part_count(DevBatch b) {
   int s = blockIdx.y;
   const SysState &st = b.st[s];
   if (!st.active || !st.nl_invalid) return;
-  const SysDesc &sd = b.sys[s];
+  SysDescC &sd = b.sys[s];
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int nchunks = (sd.nlcap + BLK - 1) / BLK;
   if ((int)blockIdx.x >= nchunks) return;
@@ -205,7 +205,7 @@ __global__ void k_part_scan(DevBatch b) {
   if (s >= b.nsys) return;
   SysState &st = b.st[s];
   if (!st.active || !st.nl_invalid) return;
-  const SysDesc &sd = b.sys[s];
+  SysDescC &sd = b.sys[s];
   int nchunks = (st.nl_n + BLK - 1) / BLK;
   int tot[5] = {0, 0, 0, 0, 0};
   for (int ch = 0; ch < nchunks; ++ch) {
@@ -228,7 +228,7 @@ __global__ void k_part_write(DevBatch b) {
   int s = blockIdx.y;
   SysState &st = b.st[s];
   if (!st.active || !st.nl_invalid) return;
-  const SysDesc &sd = b.sys[s];
+  SysDescC &sd = b.sys[s];
   int nchunks = (sd.nlcap + BLK - 1) / BLK;
   if ((int)blockIdx.x >= nchunks) return;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -241,7 +241,7 @@ __global__ void k_part_write(DevBatch b) {
   int base = (sd.nl0 / BLK + blockIdx.x) * 5;
   int i1 = b.nl_i[0][slot], i2 = b.nl_i[1][slot];
   double d = b.nl_d[slot];
-  const pcg_params &p = b.P->p;
+  ParamsC &p = b.P->p;
   if (cls & CL_NC) {
     int pos = b.blk_cnt[base + 0] + ranks[0];
     set_current_contact(b, sd.nc0 + pos, find_contact(b, sd, i1, i2));
@@ -302,7 +302,7 @@ template <typename R, typename RF> __global__ void k_terms(DevBatch b, int only_
   int s = blockIdx.y;
   SysState &st = b.st[s];
   if (!st.active || !term_on(*b.P, term)) return;
-  const SysDesc &sd = b.sys[s];
+  SysDescC &sd = b.sys[s];
   int n = term_size(sd, st, term);
   int lo = from > 0 ? from : 0, hi = (to >= 0 && to < n) ? to : n;
   GeoG g{b.y[0][0], b.y[0][1], b.y[0][2]};
@@ -336,7 +336,7 @@ template <typename RF> __global__ void k_lang(DevBatch b, int noise_mode) {
   int s = b.sysid[i];
   SysState &st = b.st[s];
   if (!st.active || !b.P->p.lang_on) return;
-  const SysDesc &sd = b.sys[s];
+  SysDescC &sd = b.sys[s];
   double nz[3];
   if (noise_mode == PCG_NOISE_RAN2) {
     nz[0] = b.noise[0][i];

@@ -90,7 +90,7 @@ __device__ __forceinline__ void philox_normals(uint64_t key, int64_t step, int r
 }
 
 // ------------------------------------------------------------ per term ----
-__device__ __forceinline__ int term_size(const SysDesc &s, const SysState &st, int term) {
+__device__ __forceinline__ int term_size(SysDescC &s, const SysState &st, int term) {
   switch (term) {
   case PCG_CHIR: return s.nchir;
   case PCG_TETHER: return s.nteth;
@@ -112,8 +112,8 @@ __device__ __forceinline__ int term_size(const SysDesc &s, const SysState &st, i
   return 0;
 }
 
-__device__ __forceinline__ bool term_on(const DevParams &P, int term) {
-  const pcg_params &p = P.p;
+__device__ __forceinline__ bool term_on(DevParamsC &P, int term) {
+  ParamsC &p = P.p;
   switch (term) {
   case PCG_CHIR: return p.chir_on;
   case PCG_TETHER: return p.tether_on;
@@ -140,7 +140,7 @@ __device__ __forceinline__ bool term_on(const DevParams &P, int term) {
 // tips in FP64 (acos near 0/pi and the heuristic-angle polynomial lose too
 // much in FP32).
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double eval_item(const DevBatch &b, const SysDesc &s, SysState &st,
+__device__ __forceinline__ double eval_item(const DevBatch &b, SysDescC &s, SysState &st,
                                             const G &g, A &acc, int term, int k) {
   switch (term) {
   case PCG_CHIR: return chir_item<double>(b, g, acc, s.chir0 + k);

@@ -8,7 +8,7 @@ namespace pcg {
 
 enum { CL_NC = 1, CL_QA = 2, CL_PID = 4, CL_DH = 8, CL_PAULI = 16 };
 
-__device__ inline int find_contact(const DevBatch &b, const SysDesc &s, int i1, int i2) {
+__device__ inline int find_contact(const DevBatch &b, SysDescC &s, int i1, int i2) {
   int lo = s.nc0, hi = s.nc0 + s.nnc;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
@@ -33,10 +33,10 @@ __device__ __forceinline__ void set_current_contact(const DevBatch &b, int q, in
   b.ncur_th2[q] = (float)(th * th);
 }
 
-__device__ inline int classify(const DevBatch &b, const SysDesc &s, const SysState &st,
+__device__ inline int classify(const DevBatch &b, SysDescC &s, const SysState &st,
                                         int slot) {
-  const DevParams &P = *b.P;
-  const pcg_params &p = P.p;
+  DevParamsC &P = *b.P;
+  ParamsC &p = P.p;
   int i1 = b.nl_i[0][slot], i2 = b.nl_i[1][slot];
   double d = b.nl_d[slot];
   double pad = st.orig_pad;
@@ -75,7 +75,7 @@ __device__ __forceinline__ long long total_key(double d) {
 // Beyond min(max_req, max(rep_cutoff, req)) the pair has no repulsion and no
 // candidate: thr2 is that radius squared in FP32, widened by 1e-5.
 __device__ inline void qa_near_filters(const DevBatch &b, int i1, int i2, double &req, float &thr2) {
-  const pcg_params &p = b.P->p;
+  ParamsC &p = b.P->p;
   double one_tol = __dadd_rn(1.0, p.qa_formation_tol);
   int ss = 4 + b.atype[i1] + 21 * b.atype[i2];
   req = __dmul_rn(p.qa_lj_r_high[1], one_tol);
@@ -90,7 +90,7 @@ __device__ inline void qa_near_filters(const DevBatch &b, int i1, int i2, double
 // compaction of the free list by !(orig_dist > max_req_dist), the test that
 // qa_cand_item makes first). Run by one workgroup per system after the free
 // list is rebuilt.
-__device__ inline void build_qa_near(const DevBatch &b, const SysDesc &sd, SysState &st) {
+__device__ inline void build_qa_near(const DevBatch &b, SysDescC &sd, SysState &st) {
   __shared__ int wc[16];
   __shared__ int run;
   if (threadIdx.x == 0) run = 0;
@@ -126,7 +126,7 @@ __device__ inline void build_qa_near(const DevBatch &b, const SysDesc &sd, SysSt
 }
 
 // sy[4 * (i - sy_off) + c] is sync budget c of residue i (HBM or LDS).
-__device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, SysState &st,
+__device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState &st,
                                        int8_t *sy, int sy_off) {
   int rem_n = st.rem_n < sd.qccap ? st.rem_n : sd.qccap;
   int cand_n = st.cand_n < sd.nlcap ? st.cand_n : sd.nlcap;
@@ -253,12 +253,12 @@ __device__ inline void qa_finish_block(const DevBatch &b, const SysDesc &sd, Sys
 // ya(k, c) is a reference to y_k, component c, of the residue (in HBM or LDS)
 // minv = 1 / mass and sqminv = sqrt(minv) of the residue's type.
 template <typename YA>
-__device__ __forceinline__ void gear_core(const DevBatch &b, const SysDesc &sd, const SysState &st,
+__device__ __forceinline__ void gear_core(const DevBatch &b, SysDescC &sd, const SysState &st,
                                           double minv, double sqminv, double Fx, double Fy,
                                           double Fz, const double nz[3], double out_r[3],
                                           double vel[3], YA ya) {
 #pragma clang fp contract(off)
-  const DevParams &P = *b.P;
+  DevParamsC &P = *b.P;
   const double f02 = (double)(3.0f / 16.0f), f12 = (double)(251.0f / 360.0f),
                f32 = (double)(11.0f / 18.0f), f42 = (double)(1.0f / 6.0f),
                f52 = (double)(1.0f / 60.0f);
@@ -293,7 +293,7 @@ __device__ __forceinline__ void gear_core(const DevBatch &b, const SysDesc &sd, 
   }
 }
 
-__device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd, const SysState &st,
+__device__ __forceinline__ void gear_update(const DevBatch &b, SysDescC &sd, const SysState &st,
                                             int i, double Fx, double Fy, double Fz,
                                             const double nz[3], double out_r[3]) {
   double vel[3];
@@ -303,12 +303,12 @@ __device__ __forceinline__ void gear_update(const DevBatch &b, const SysDesc &sd
   for (int c = 0; c < 3; ++c) b.vel[c][i] = vel[c];
 }
 
-__device__ __forceinline__ double disp_from(const SysDesc &sd, const double o[3], const double r[3]) {
+__device__ __forceinline__ double disp_from(SysDescC &sd, const double o[3], const double r[3]) {
 #pragma clang fp contract(off)
   v3<double> d{__dsub_rn(o[0], r[0]), __dsub_rn(o[1], r[1]), __dsub_rn(o[2], r[2])};
   return xnorm3(xwrapd(d, sd));
 }
-__device__ __forceinline__ double disp_from_orig(const DevBatch &b, const SysDesc &sd, int i,
+__device__ __forceinline__ double disp_from_orig(const DevBatch &b, SysDescC &sd, int i,
                                                  const double r[3]) {
   const double o[3] = {b.orig_r[0][i], b.orig_r[1][i], b.orig_r[2][i]};
   return disp_from(sd, o, r);

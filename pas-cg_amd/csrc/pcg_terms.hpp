@@ -18,13 +18,21 @@ template <typename R> __device__ __forceinline__ v3<R> cast3(v3<double> d) {
   return v3<R>{(R)d.x, (R)d.y, (R)d.z};
 }
 
-__device__ __forceinline__ v3<double> wrapd(v3<double> d, const SysDesc &s) {
+// Without a periodic cell (all sides 0) both wraps are the identity, bit for
+// bit (v - 0 * round(v * 0) = v); the test is uniform, so a system without
+// one skips the FP64 rounding work.
+__device__ __forceinline__ bool has_cell(SysDescC &s) {
+  return s.cell[0] != 0.0 || s.cell[1] != 0.0 || s.cell[2] != 0.0;
+}
+__device__ __forceinline__ v3<double> wrapd(v3<double> d, SysDescC &s) {
+  if (!has_cell(s)) return d;
   d.x -= s.cell[0] * round(d.x * s.cell_inv[0]);
   d.y -= s.cell[1] * round(d.y * s.cell_inv[1]);
   d.z -= s.cell[2] * round(d.z * s.cell_inv[2]);
   return d;
 }
-__device__ __forceinline__ v3<double> xwrapd(v3<double> d, const SysDesc &s) {
+__device__ __forceinline__ v3<double> xwrapd(v3<double> d, SysDescC &s) {
+  if (!has_cell(s)) return d;
   return v3<double>{xwrap1(d.x, s.cell[0], s.cell_inv[0]), xwrap1(d.y, s.cell[1], s.cell_inv[1]),
                     xwrap1(d.z, s.cell[2], s.cell_inv[2])};
 }
@@ -62,7 +70,7 @@ template <typename R> __device__ __forceinline__ void sink_ljR(R depth, R r_low,
 // (each *_core takes the item's residues and parameters; *_item reads them
 // from the term's list)
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double tether_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+__device__ __forceinline__ double tether_core(ParamsC &p, const G &g, A &acc, int i1, int i2,
                                               double nat) {
   v3<R> r12 = cast3<R>(g.r(i2) - g.r(i1));
   const R r2 = nsq(r12), rinv = rsq(r2);
@@ -80,7 +88,7 @@ __device__ __forceinline__ double tether_item(const DevBatch &b, const G &g, A &
 
 // local_rep::eval_forces::iter (cg/src/cg/local_rep/eval_forces.cpp:11-28)
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double lrep_core(const pcg_params &p, const G &g, A &acc, int i1, int i2) {
+__device__ __forceinline__ double lrep_core(ParamsC &p, const G &g, A &acc, int i1, int i2) {
   v3<R> r12 = cast3<R>(g.r(i2) - g.r(i1));
   R rn = rsq(nsq(r12)), cut = (R)p.rep_cutoff;
   if ((R)1 > rn * cut) return 0.0;
@@ -99,8 +107,8 @@ __device__ __forceinline__ double lrep_item(const DevBatch &b, const G &g, A &ac
 
 // pauli::eval_forces::iter (cg/src/cg/pauli/eval_forces.cpp:12-30)
 template <typename R, typename G, typename A>
-__device__ double pauli_item(const DevBatch &b, const SysDesc &s, const G &g, A &acc, int k) {
-  const pcg_params &p = b.P->p;
+__device__ double pauli_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, int k) {
+  ParamsC &p = b.P->p;
   int i1 = b.pauli_i[0][k], i2 = b.pauli_i[1][k];
   v3<R> r12 = cast3<R>(wrapd(g.r(i2) - g.r(i1), s));
   R rn = rsq(nsq(r12)), cut = (R)p.rep_cutoff;
@@ -124,11 +132,11 @@ struct NcFetch {
   float th2;
 };
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc &s, SysState &st,
+__device__ __forceinline__ double nat_cont_core(const DevBatch &b, SysDescC &s, SysState &st,
                                                 const G &g, A &acc, const NcFetch &q, double t,
                                                 bool update_state) {
 #pragma clang fp contract(off)
-  const pcg_params &p = b.P->p;
+  ParamsC &p = b.P->p;
   const int c = q.c, i1 = q.i1, i2 = q.i2;
   const double nat = q.nat;
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
@@ -156,14 +164,14 @@ __device__ __forceinline__ double nat_cont_core(const DevBatch &b, const SysDesc
     // disulfide_force::operator() receives the distance as its inverse-distance
     // argument (nat_cont/eval_forces.cpp:27, disulfide.h:12-20)
     if (p.nc_ss_kind == 1)
-      harmonicR<R>((R)p.nc_ss_H1, (R)p.nc_ss_H2, (R)p.nc_ss_nat_r, (R)1 / rn, V, dV);
+      harmonicR<R>((R)p.nc_ss_H1, (R)p.nc_ss_H2, (R)p.nc_ss_nat_r, rcpR(rn), V, dV);
     else
       ljR<R>((R)p.nc_ss_depth, (R)p.nc_ss_r_min, rn, V, dV);
   } else {
-    ljR<R>((R)p.nat_cont_depth, (R)nat, (R)1 / rn, V, dV);
+    ljR<R>((R)p.nat_cont_depth, (R)nat, rcpR(rn), V, dV);
   }
   dV = clampR(dV, (R)-1e3, (R)1e3);
-  v3<R> f = (dV / rn) * r12;
+  v3<R> f = (dV * rcpR(rn)) * r12;
   acc.add(i1, f);
   acc.add(i2, -f);
   if (!update_state) return (double)V;
@@ -187,22 +195,33 @@ __device__ __forceinline__ NcFetch nat_cont_fetch(const DevBatch &b, int k) {
   return NcFetch{k, b.ncur[k], b.ncur_i[0][k], b.ncur_i[1][k], b.ncur_nat[k], b.ncur_t[k],
                  b.ncur_th2[k]};
 }
+// every field in registers before the item's first decision (see keep)
+__device__ __forceinline__ void keep_fetch(NcFetch &q) {
+  keep(q.c);
+  keep(q.i1);
+  keep(q.i2);
+  keep(q.nat);
+  keep(q.type);
+  keep(q.th2);
+}
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double nat_cont_item(const DevBatch &b, const SysDesc &s, SysState &st,
+__device__ __forceinline__ double nat_cont_item(const DevBatch &b, SysDescC &s, SysState &st,
                                                 const G &g, A &acc, int k, double t) {
-  return nat_cont_core<R>(b, s, st, g, acc, nat_cont_fetch(b, k), t, true);
+  NcFetch q = nat_cont_fetch(b, k);
+  keep_fetch(q);
+  return nat_cont_core<R>(b, s, st, g, acc, q, t, true);
 }
 
 // const_dh / rel_dh ::iter (cg/src/cg/dh/const.cpp:18-40, rel.cpp:17-40)
 template <typename R, typename G, typename A>
-__device__ double dh_item(const DevBatch &b, const SysDesc &s, const G &g, A &acc, int k, int variant) {
-  const pcg_params &p = b.P->p;
+__device__ double dh_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, int k, int variant) {
+  ParamsC &p = b.P->p;
   int i1 = b.dh_i[0][k], i2 = b.dh_i[1][k];
   v3<double> d = wrapd(g.r(i2) - g.r(i1), s);
   double rnd = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
   if (rnd > p.dh_cutoff) return 0.0;
   v3<R> r12 = cast3<R>(d);
-  R rn = (R)rnd, rinv = (R)1 / rn, sinv = (R)b.P->screen_inv;
+  R rn = (R)rnd, rinv = rcpR(rn), sinv = (R)b.P->screen_inv;
   R Vij, dV;
   if (variant == 0) {
     Vij = (R)b.P->V_factor * (R)b.dh_q[k] * expR(-rn * sinv) * rinv;
@@ -220,7 +239,7 @@ __device__ double dh_item(const DevBatch &b, const SysDesc &s, const G &g, A &ac
 // chir::eval_forces::iter (cg/src/cg/chir/eval_forces.cpp:10-31); f0/f1 are
 // the stored nat_factor/nat_chir fields of the quad.
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double chir_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+__device__ __forceinline__ double chir_core(ParamsC &p, const G &g, A &acc, int i1, int i2,
                                             int i3, int i4, double f0d, double f1d) {
   v3<double> r1 = g.r(i1), r2 = g.r(i2), r3 = g.r(i3), r4 = g.r(i4);
   v3<R> r12 = cast3<R>(r2 - r1), r23 = cast3<R>(r3 - r2), r34 = cast3<R>(r4 - r3);
@@ -257,7 +276,7 @@ __device__ __forceinline__ R angle_geom(const G &g, int i1, int i2, int i3, v3<R
 }
 
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double nat_ang_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+__device__ __forceinline__ double nat_ang_core(ParamsC &p, const G &g, A &acc, int i1, int i2,
                                                int i3, double th0) {
   v3<R> d1, d2, d3;
   R th = angle_geom<R>(g, i1, i2, i3, d1, d2, d3);
@@ -273,7 +292,7 @@ __device__ __forceinline__ double nat_ang_item(const DevBatch &b, const G &g, A 
 }
 
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double heur_ang_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+__device__ __forceinline__ double heur_ang_core(ParamsC &p, const G &g, A &acc, int i1, int i2,
                                                 int i3, int ty) {
   v3<R> d1, d2, d3;
   R th = angle_geom<R>(g, i1, i2, i3, d1, d2, d3);
@@ -319,7 +338,7 @@ __device__ __forceinline__ bool angle_geom_f32(const G &g, int i1, int i2, int i
   return true;
 }
 template <typename G, typename A>
-__device__ __forceinline__ double nat_ang_mixed(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+__device__ __forceinline__ double nat_ang_mixed(ParamsC &p, const G &g, A &acc, int i1, int i2,
                                                 int i3, double th0) {
   v3<float> d1, d2, d3;
   float th;
@@ -333,7 +352,7 @@ __device__ __forceinline__ double nat_ang_mixed(const pcg_params &p, const G &g,
 }
 // the polynomial cancels heavily: FP64 in theta
 template <typename G, typename A>
-__device__ __forceinline__ double heur_ang_mixed(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+__device__ __forceinline__ double heur_ang_mixed(ParamsC &p, const G &g, A &acc, int i1, int i2,
                                                  int i3, int ty) {
   v3<float> d1, d2, d3;
   float thf;
@@ -358,7 +377,7 @@ __device__ __forceinline__ double heur_ang_mixed(const pcg_params &p, const G &g
 // phi after its acos. cos phi0 and sin phi0 are the caller's (precomputed per
 // item), so the term takes no acos and no sincos.
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double nat_dih_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+__device__ __forceinline__ double nat_dih_core(ParamsC &p, const G &g, A &acc, int i1, int i2,
                                                int i3, int i4, double phi0, double cphi0,
                                                double sphi0, int variant) {
   v3<double> q1 = g.r(i1), q2 = g.r(i2), q3 = g.r(i3), q4 = g.r(i4);
@@ -403,7 +422,7 @@ __device__ __forceinline__ double nat_dih_item(const DevBatch &b, const G &g, A 
 
 // heur_dih::eval_forces::iter (angles/heur_dih/eval_forces.cpp:10-55)
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double heur_dih_core(const pcg_params &p, const G &g, A &acc, int i1, int i2,
+__device__ __forceinline__ double heur_dih_core(ParamsC &p, const G &g, A &acc, int i1, int i2,
                                                 int i3, int i4, int ty) {
   const double(*c)[9] = p.heur_dih_coeffs;
   v3<double> q1 = g.r(i1), q2 = g.r(i2), q3 = g.r(i3), q4 = g.r(i4);
@@ -470,7 +489,7 @@ __device__ __forceinline__ bool pid_psi(const G &g, int i, int j, bool vp, bool 
   v3<R> rm = cross(rij, rkj), rn = cross(rkj, rkl);
   R rm_n = sq(nsq(rm)), rn_n = sq(nsq(rn));
   bool ok = !(rm_n < (R)0.1 || rn_n < (R)0.1);
-  R rm_i = (R)1 / rm_n, rn_i = (R)1 / rn_n, rkj_i = rsq(nsq(rkj)), rkj_n = (R)1 / rkj_i;
+  R rm_i = rcpR(rm_n), rn_i = rcpR(rn_n), rkj_i = rsq(nsq(rkj)), rkj_n = rcpR(rkj_i);
   v3<R> fi = rm * (rkj_n * rm_i * rm_i);
   v3<R> fl = -(rn * (rkj_n * rn_i * rn_i));
   v3<R> df = (fi * dot(rij, rkj) - fl * dot(rkl, rkj)) * (rkj_i * rkj_i);
@@ -485,12 +504,12 @@ __device__ __forceinline__ bool pid_psi(const G &g, int i, int j, bool vp, bool 
 }
 
 template <typename R, typename G, typename A>
-__device__ double pid_item(const DevBatch &b, const SysDesc &s, const G &g, A &acc, int k) {
-  const pcg_params &p = b.P->p;
+__device__ double pid_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, int k) {
+  ParamsC &p = b.P->p;
   int i1 = b.pid_i[0][k], i2 = b.pid_i[1][k];
   if (b.atype[i1] == 14 || b.atype[i2] == 14) return 0.0; // PRO
   v3<R> r12 = cast3<R>(wrapd(g.r(i2) - g.r(i1), s));
-  R r12n = sq(nsq(r12)), r12rn = (R)1 / r12n;
+  R r12n = sq(nsq(r12)), r12rn = rcpR(r12n);
   if (!(r12n < (R)p.nl_cutoff)) return 0.0;
   int lo = s.res0, hi = s.res0 + s.n;
   bool v1p = i1 - 1 >= lo, v1n = i1 + 1 < hi, v2p = i2 - 1 >= lo, v2n = i2 + 1 < hi;
@@ -549,7 +568,7 @@ __device__ double pid_item(const DevBatch &b, const SysDesc &s, const G &g, A &a
 // afm::vel::eval_forces::iter (cg/src/cg/afm/vel/eval_forces.cpp:10-31)
 template <typename R, typename G, typename A>
 __device__ double vel_afm_item(const DevBatch &b, const G &g, A &acc, int k, double t) {
-  const pcg_params &p = b.P->p;
+  ParamsC &p = b.P->p;
   int res = b.tip_res[k];
   v3<double> cur{b.tip_orig[0][k] + t * b.tip_vel[0][k], b.tip_orig[1][k] + t * b.tip_vel[1][k],
                  b.tip_orig[2][k] + t * b.tip_vel[2][k]};
@@ -607,19 +626,25 @@ struct QhFetch {
   double req;  // the fields below only from qa_cand_fetch_full
   int slot;    // -1: load it when the pair becomes a candidate
 };
-__device__ __forceinline__ QhFetch qa_cand_fetch(const DevBatch &b, const SysDesc &s, int hk) {
+__device__ __forceinline__ QhFetch qa_cand_fetch(const DevBatch &b, SysDescC &s, int hk) {
   int h = s.qf0 + hk;
   return QhFetch{hk, b.qh_i[0][h], b.qh_i[1][h], b.qh_thr2[h], b.qh_vac[h], 0.0, 0};
 }
+__device__ __forceinline__ void keep_fetch(QhFetch &q) {
+  keep(q.i1);
+  keep(q.i2);
+  keep(q.thr2);
+  keep(q.vac);
+}
 // with what the exact path needs, in the same round trip
-__device__ __forceinline__ QhFetch qa_cand_fetch_full(const DevBatch &b, const SysDesc &s, int hk) {
+__device__ __forceinline__ QhFetch qa_cand_fetch_full(const DevBatch &b, SysDescC &s, int hk) {
   int h = s.qf0 + hk;
   return QhFetch{hk, b.qh_i[0][h], b.qh_i[1][h], b.qh_thr2[h], b.qh_vac[h], b.qh_req[h], b.qh_slot[h]};
 }
 // The cheap first test: a vacant pair, or one beyond its radius, has no
 // effect at all.
 template <typename G>
-__device__ __forceinline__ bool qa_cand_may_act(const SysDesc &s, const G &g, const QhFetch &q) {
+__device__ __forceinline__ bool qa_cand_may_act(SysDescC &s, const G &g, const QhFetch &q) {
   if (q.vac) return false;
   v3<double> d = xwrapd(xsub(g.r(q.i2), g.r(q.i1)), s);
   float dx = (float)d.x, dy = (float)d.y, dz = (float)d.z;
@@ -630,11 +655,11 @@ __device__ __forceinline__ bool qa_cand_may_act(const SysDesc &s, const G &g, co
 // from qa_cand_fetch_full. The residue data (types, sync budgets, n and h)
 // comes through the geometry accessor: from HBM, or from LDS and positions.
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double qa_cand_exact(const DevBatch &b, const SysDesc &s, SysState &st,
+__device__ __forceinline__ double qa_cand_exact(const DevBatch &b, SysDescC &s, SysState &st,
                                                 const G &g, A &acc, const QhFetch &q) {
 #pragma clang fp contract(off)
-  const DevParams &P = *b.P;
-  const pcg_params &p = P.p;
+  DevParamsC &P = *b.P;
+  ParamsC &p = P.p;
   const int hk = q.hk, i1 = q.i1, i2 = q.i2;
   // independent of the geometry: issued first
   const uint32_t sy1 = g.sync4(b, i1), sy2 = g.sync4(b, i2);
@@ -703,24 +728,24 @@ __device__ __forceinline__ double qa_cand_exact(const DevBatch &b, const SysDesc
   return V;
 }
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double qa_cand_core(const DevBatch &b, const SysDesc &s, SysState &st,
+__device__ __forceinline__ double qa_cand_core(const DevBatch &b, SysDescC &s, SysState &st,
                                                const G &g, A &acc, const QhFetch &q) {
   if (!qa_cand_may_act(s, g, q)) return 0.0;
   return qa_cand_exact<R>(b, s, st, g, acc, q);
 }
 template <typename R, typename G, typename A>
-__device__ __forceinline__ double qa_cand_item(const DevBatch &b, const SysDesc &s, SysState &st,
+__device__ __forceinline__ double qa_cand_item(const DevBatch &b, SysDescC &s, SysState &st,
                                                const G &g, A &acc, int hk) {
   return qa_cand_core<R>(b, s, st, g, acc, qa_cand_fetch_full(b, s, hk));
 }
 
 // qa::process_contacts::iter (cg/src/cg/qa/process_contacts.cpp:12-72)
 template <typename R, typename G, typename A>
-__device__ double qa_cont_item(const DevBatch &b, const SysDesc &s, SysState &st, const G &g,
+__device__ double qa_cont_item(const DevBatch &b, SysDescC &s, SysState &st, const G &g,
                                A &acc, int k) {
 #pragma clang fp contract(off)
-  const DevParams &P = *b.P;
-  const pcg_params &p = P.p;
+  DevParamsC &P = *b.P;
+  ParamsC &p = P.p;
   int slot = s.qc0 + k;
   if (b.qc_vac[slot]) return 0.0;
   int i1 = b.qc_i[0][slot], i2 = b.qc_i[1][slot];
