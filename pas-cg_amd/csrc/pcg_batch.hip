@@ -416,6 +416,69 @@ __device__ __forceinline__ void nat_cont_loop(const DevBatch &b, SysDescC &sd, S
   if (n <= 0) return;
   int k = ((int)threadIdx.x - lo % B + B) % B;
   double v = 0.0;
+  ParamsC &p = b.P->p;
+  if constexpr (sizeof(R) == 4) {
+    if (p.nc_ss_kind == 0) {
+      // the FP32 form of nat_cont_core's plain-LJ case, in stages over the NK
+      // items so that the items' loads and arithmetic overlap: the geometry
+      // and the FP32-first decisions of every item, straight-line; the exact
+      // FP64 decisions of the few inside the margins; the forces of every
+      // item, straight-line; then the adds and the state transitions
+      const float cut2 = (float)(p.nl_cutoff * p.nl_cutoff), lo = 1.0f - 2e-5f, hi = 1.0f + 2e-5f;
+      const float depth = (float)p.nat_cont_depth;
+      for (; k < n; k += NK * B) {
+        NcFetch q[NK];
+        for (int j = 0; j < NK; ++j) q[j] = nat_cont_fetch(b, sd.nc0 + min(k + j * B, n - 1));
+        for (int j = 0; j < NK; ++j) keep_fetch(q[j]);
+        v3<double> d[NK];
+        v3<float> r12[NK];
+        float rn[NK];
+        int now[NK];
+        bool act[NK], exact[NK];
+        for (int j = 0; j < NK; ++j) {
+          d[j] = xwrapd(xsub(g.r(q[j].i2), g.r(q[j].i1)), sd);
+          r12[j] = cast3<float>(d[j]);
+          const float d2 = nsq(r12[j]);
+          act[j] = k + j * B < n && !(d2 > cut2 * hi);
+          exact[j] = !(d2 < cut2 * lo && (d2 < q[j].th2 * lo || d2 > q[j].th2 * hi));
+          now[j] = d2 < q[j].th2 * lo ? 1 : 0;
+          rn[j] = sq(d2);
+        }
+        for (int j = 0; j < NK; ++j)
+          if (act[j] && exact[j]) {
+            const double rnx = xnorm3(d[j]);
+            act[j] = rnx < p.nl_cutoff;
+            rn[j] = (float)rnx;
+            now[j] = rnx <= __dmul_rn(__dmul_rn(q[j].nat, C216_INV), p.counting_factor) ? 1 : 0;
+          }
+        float Vj[NK];
+        v3<float> f[NK];
+        for (int j = 0; j < NK; ++j) {
+          float dV;
+          ljR<float>(depth, (float)q[j].nat, rcpR(rn[j]), Vj[j], dV);
+          dV = clampR(dV, -1e3f, 1e3f);
+          f[j] = (dV * rcpR(rn[j])) * r12[j];
+        }
+        for (int j = 0; j < NK; ++j)
+          if (act[j]) {
+            acc.add(q[j].i1, f[j]);
+            acc.add(q[j].i2, -f[j]);
+            v += (double)Vj[j];
+            if (now[j] != (q[j].type >> 7)) {
+              const int c = q[j].c;
+              b.nc_active[c] = (uint8_t)now[j];
+              b.ncur_t[q[j].k] = (uint8_t)((q[j].type & 0x7F) | (now[j] << 7));
+              if (b.nc_change_t[c] < 0) {
+                b.nc_change_t[c] = st.t;
+                atomicAdd(&st.num_changed, 1);
+              }
+            }
+          }
+      }
+      if (want_v && v != 0.0) atomicAdd(&vacc[PCG_NAT_CONT], v);
+      return;
+    }
+  }
   for (; k < n; k += NK * B) {
     NcFetch q[NK];
     for (int j = 0; j < NK; ++j)
