@@ -777,7 +777,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
 }
 
 // The launch bound sets the register budget: 1024 threads leave 128 VGPRs
-// per lane (the FP64 dihedral and PID items then spill), 512 leave 256.
+// per lane (the bonded items then spill), 768 leave 168, 512 leave 256.
 template <typename R, typename RF, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_persistent(DevBatch b, int nsteps) {
   persistent_body<R, RF>(b, blockIdx.x, nsteps);
@@ -814,13 +814,15 @@ void launch_batch_t(const DevBatch &b, int prec, int block, int nsteps, size_t l
 
 void launch_persistent_jobs(const DevBatch *jobs, JobSlot *slots, int njobs, int prec, int block,
                             int nsteps, size_t lds_bytes, hipStream_t s) {
-  if (block > 512) launch_jobs_t<1024>(jobs, slots, njobs, prec, block, nsteps, lds_bytes, s);
+  if (block > 768) launch_jobs_t<1024>(jobs, slots, njobs, prec, block, nsteps, lds_bytes, s);
+  else if (block > 512) launch_jobs_t<768>(jobs, slots, njobs, prec, block, nsteps, lds_bytes, s);
   else launch_jobs_t<512>(jobs, slots, njobs, prec, block, nsteps, lds_bytes, s);
 }
 
 void launch_persistent(const DevBatch &b, int prec, int block, int nsteps, size_t lds_bytes,
                        hipStream_t s) {
-  if (block > 512) launch_batch_t<1024>(b, prec, block, nsteps, lds_bytes, s);
+  if (block > 768) launch_batch_t<1024>(b, prec, block, nsteps, lds_bytes, s);
+  else if (block > 512) launch_batch_t<768>(b, prec, block, nsteps, lds_bytes, s);
   else launch_batch_t<512>(b, prec, block, nsteps, lds_bytes, s);
 }
 

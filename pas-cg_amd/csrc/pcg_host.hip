@@ -1386,7 +1386,7 @@ int pcg_batch_autotune(pcg_batch *b, int32_t trial_steps, int32_t *block, double
   int rc = PCG_OK;
   int spl0 = b->spl, block0 = b->block;
   b->spl = std::min(b->spl, (int)trial_steps);
-  const int cands[] = {128, 256, 512, 1024};
+  const int cands[] = {128, 256, 512, 768, 1024};
   int best = block0;
   double best_ms = 1e300;
   if (hipMemcpyAsync(snap, B.arena.base, total, hipMemcpyDeviceToDevice, B.stream) != hipSuccess)
