@@ -766,14 +766,15 @@ int upload_ran2_noise(DeviceBatch &B) {
 }
 
 int check_overflow(DeviceBatch &B) {
-  for (size_t s = 0; s < B.sys.size(); ++s) {
-    SysState st;
-    int rc = read_state(B, (int)s, st);
-    if (rc) return rc;
-    if (st.overflow)
+  // every system's scalars in one copy
+  std::vector<SysState> st(B.sys.size());
+  HIPCHK(hipMemcpyAsync(st.data(), B.d.st, st.size() * sizeof(SysState), hipMemcpyDeviceToHost,
+                        B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  for (size_t s = 0; s < st.size(); ++s)
+    if (st[s].overflow)
       return fail(PCG_E_OVERFLOW, "a neighbour or QA list of system " + std::to_string(s) +
                                       " outgrew its capacity");
-  }
   return PCG_OK;
 }
 
@@ -1174,7 +1175,7 @@ struct pcg_batch {
   bool tuned = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  int g_nsteps = 0, g_block = 0, g_spl = 0;
+  int g_block = 0, g_spl = 0;
   ~pcg_batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1196,29 +1197,29 @@ size_t lds_for(const DeviceBatch &B, bool staged) {
 int run_persistent(pcg_batch *b, int nsteps) {
   DeviceBatch &B = *b->B;
   int launches = (nsteps + b->spl - 1) / b->spl;
-  auto enqueue = [&]() {
-    for (int done = 0; done < nsteps; done += b->spl)
-      launch_persistent(B.d, B.prec, b->block, std::min(b->spl, nsteps - done), b->lds, B.stream);
-  };
-  if (b->use_graph && nsteps > 0) {
-    if (!b->gexec || b->g_nsteps != nsteps || b->g_block != b->block || b->g_spl != b->spl) {
+  const int full = nsteps / b->spl, rem = nsteps % b->spl;
+  if (b->use_graph && full > 0) {
+    // the graph holds one launch of steps_per_launch steps, replayed; a
+    // shorter last launch goes directly
+    if (!b->gexec || b->g_block != b->block || b->g_spl != b->spl) {
       if (b->gexec) HIPCHK(hipGraphExecDestroy(b->gexec));
       if (b->graph) HIPCHK(hipGraphDestroy(b->graph));
       b->gexec = nullptr;
       b->graph = nullptr;
       HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-      enqueue();
+      launch_persistent(B.d, B.prec, b->block, b->spl, b->lds, B.stream);
       HIPCHK(hipStreamEndCapture(B.stream, &b->graph));
       HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-      b->g_nsteps = nsteps;
       b->g_block = b->block;
       b->g_spl = b->spl;
     }
     HIPCHK(hipEventRecord(b->ev0, B.stream));
-    HIPCHK(hipGraphLaunch(b->gexec, B.stream));
+    for (int k = 0; k < full; ++k) HIPCHK(hipGraphLaunch(b->gexec, B.stream));
+    if (rem) launch_persistent(B.d, B.prec, b->block, rem, b->lds, B.stream);
   } else {
     HIPCHK(hipEventRecord(b->ev0, B.stream));
-    enqueue();
+    for (int done = 0; done < nsteps; done += b->spl)
+      launch_persistent(B.d, B.prec, b->block, std::min(b->spl, nsteps - done), b->lds, B.stream);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(b->ev1, B.stream));

@@ -224,8 +224,10 @@ def test_autotune_keeps_the_trajectory():
 
 
 def test_batch_graph_replays_match_direct_launches():
-    """The hipGraph of a run's launch sequence, replayed several times with
-    host reads in between, follows the directly launched batch."""
+    """The hipGraph of one launch, replayed for each full launch of a run
+    (a shorter last launch goes directly), with host reads in between,
+    follows the directly launched batch; runs of 50 and 30 steps end in a
+    partial launch."""
     prm = prm_full(qa_on=1, nat_cont_on=1)
     models = ragged_models([64, 130, 38], seed0=90)
     a = G.Batch(prm, models, seeds=[1] * 3, precision=G.FP64, steps_per_launch=20)
@@ -233,9 +235,9 @@ def test_batch_graph_replays_match_direct_launches():
                 use_graph=True)
     for x in (a, g):
         x.set_noise_scale(0.0)
-    for rep in range(4):
-        a.run(60)
-        g.run(60)
+    for rep, n in enumerate((60, 50, 60, 30)):
+        a.run(n)
+        g.run(n)
         for s in range(3):
             err = np.abs(a.get(s, 0) - g.get(s, 0)).max()
             assert err < 1e-9, (rep, s, err)
