@@ -42,6 +42,7 @@ from pas_cg_amd import workloads as W  # noqa: E402
 METRIC = "residue-steps/s batched vs plain @force err≤1e-5 vs FP64; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_persistent_ragged256_traffic.txt")
+COUNTERS_FILE = os.path.join(HERE, "profiles", "k_persistent_ragged256_counters.md")
 
 
 def log(*a):
@@ -49,19 +50,37 @@ def log(*a):
 
 
 def read_traffic(spl: int):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE per step, copy-kernel calibrated)."""
+    """HBM bytes per launch of `spl` steps from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, copy-kernel calibrated), with the
+    configuration it was measured at; (None, None) without it."""
     if not os.path.exists(TRAFFIC_FILE):
-        return None
+        return None, None
     vals = {}
     for ln in open(TRAFFIC_FILE):
         if ":" in ln and not ln.startswith("#"):
             k, v = ln.split(":", 1)
             vals[k.strip()] = v.strip()
     try:
-        return float(vals["hbm_bytes_per_step"]) * spl
+        cfg = {"pad_A": float(vals["pad_A"]), "block": int(vals["block"]),
+               "steps_per_launch": int(vals["steps_per_launch"]),
+               "over_algorithmic": float(vals["traffic_over_algorithmic"])}
+        return float(vals["hbm_bytes_per_step"]) * spl, cfg
     except (KeyError, ValueError):
+        return None, None
+
+
+def read_valu_active():
+    """VALU-active fraction of wave cycles of the persistent kernel, from the
+    committed rocprofv3 counter summary (None without it)."""
+    if not os.path.exists(COUNTERS_FILE):
         return None
+    for ln in open(COUNTERS_FILE):
+        if ln.startswith("* VALU active per wave cycle"):
+            try:
+                return float(ln.rsplit(":", 1)[1])
+            except ValueError:
+                return None
+    return None
 
 
 def batched(p, models, steps, warmup, spl, block):
@@ -277,9 +296,33 @@ def served(p, rate_rs, seconds, pool, per_plain):
             "plain_fifo_p99_latency_s": None if math.isnan(f99) else round(float(f99), 4)}
 
 
-def cpu_baseline(p, models, picks, budget_s=15.0):
-    """The FP64 CPU restatement of the loop (oracle/), one core, on a bounded
-    sample of the same proteins."""
+def cpu_pool(workers: int):
+    """Worker processes for the all-cores CPU baseline, forked now, before the
+    GPU is touched (each runs one oracle system later)."""
+    from concurrent.futures import ProcessPoolExecutor
+    ex = ProcessPoolExecutor(workers)
+    list(ex.map(time.sleep, [0.3] * workers))  # start every worker now
+    return ex
+
+
+def _cpu_one(a):
+    """One oracle system stepped for budget_s seconds (a worker of cpu_pool)."""
+    p, m, seed, budget_s = a
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from oracle_ffi import Oracle
+    o = Oracle(p, m, seed=seed)
+    o.step(2)
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        o.step(20)
+        steps += 20
+    return int(m["n"]) * steps, time.perf_counter() - t0
+
+
+def cpu_baseline(p, models, picks, budget_s=15.0, ex=None, picks_all=None, budget_all=10.0):
+    """The FP64 CPU restatement of the loop (oracle/) on bounded samples of the
+    same proteins: one core, and one process per core (every host core up
+    to the pool's size) at once."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
     from oracle_ffi import Oracle
     ors = [Oracle(p, models[s], seed=s) for s in picks]
@@ -296,9 +339,24 @@ def cpu_baseline(p, models, picks, budget_s=15.0):
     rs = sum(int(models[s]["n"]) for s in picks) * steps
     ns = [int(models[s]["n"]) for s in picks]
     log(f"cpu: {rs / dt:.3e} residue-steps/s over {dt:.1f} s")
-    return {"value": rs / dt, "unit": "residue-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{len(picks)} proteins of the ragged workload (N={ns}), "
-                      f"{steps} steps each after 2 warm-up steps, FP64, one thread"}
+    one = {"value": rs / dt, "unit": "residue-steps/s", "cores": 1, "kind": "port",
+           "sample": f"{len(picks)} proteins of the ragged workload (N={ns}), "
+                     f"{steps} steps each after 2 warm-up steps, FP64, one thread"}
+    if ex is None or not picks_all:
+        return one
+    k = ex._max_workers
+    picks_all = picks_all[:k]  # one system per worker, all at once
+    t0 = time.perf_counter()
+    res = list(ex.map(_cpu_one, [(p, models[s], s, budget_all) for s in picks_all]))
+    rs_all = sum(r for r, _ in res)
+    dt_all = time.perf_counter() - t0
+    log(f"cpu: {rs_all / dt_all:.3e} residue-steps/s on {k} cores over {dt_all:.1f} s")
+    return {"value": rs_all / dt_all, "unit": "residue-steps/s", "cores": k, "kind": "port",
+            "sample": f"{len(picks_all)} proteins of the ragged workload (spread over the "
+                      f"lengths, N={[int(models[s]['n']) for s in picks_all]}), one process per "
+                      f"protein on {k} cores at once, {budget_all:.0f} s each after 2 warm-up "
+                      f"steps, FP64; value = their residue-steps / wall time",
+            "one_core": one}
 
 
 def main():
@@ -340,6 +398,7 @@ def main():
     p, models = W.c3(workers=args.gen_workers)
     extra = None if args.no_other else (W.c2(), W.c1())
     pool = None if args.no_serve else serve_pool(min(args.gen_workers, os.cpu_count() or 1))
+    cpu_ex = None if args.no_cpu else cpu_pool(min(16, os.cpu_count() or 1))
     tuning = None
     if args.no_tune:
         pad, block = args.pad, args.block
@@ -357,6 +416,7 @@ def main():
     from pas_cg_amd import pcg as G
     copy_gbs, _ = G.stream_copy(1 << 30, 20)
     order = np.argsort([int(m["n"]) for m in models])
+    traffic, traffic_cfg = read_traffic(bat["spl"])
     out = {"metric": METRIC, "value": round(value, 1), "unit": "residue-steps/s",
            "n_gpus": 1, "steps": bat["steps"], "warmup": bat["warmup"],
            "ms_per_step": round(bat["wall"] * 1e3 / bat["steps"], 5),
@@ -371,7 +431,9 @@ def main():
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": read_traffic(bat["spl"]),
+                        "traffic": traffic,
+                        "traffic_measured_at": traffic_cfg,
+                        "valu_active": read_valu_active(),
                         "kernel": "k_persistent",
                         "stream_copy_GBs": round(copy_gbs, 1),
                         "avg_launch_ms": round(avg_launch_s * 1e3, 4),
@@ -398,7 +460,11 @@ def main():
     if extra is not None:
         out["other_configs"] = other_configs(*extra)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, models, [int(i) for i in order[16::32]])
+        k = cpu_ex._max_workers
+        spread = [int(order[(2 * j + 1) * len(order) // (2 * k)]) for j in range(k)]
+        out["cpu_baseline"] = cpu_baseline(p, models, [int(i) for i in order[16::32]], ex=cpu_ex,
+                                           picks_all=spread)
+        cpu_ex.shutdown()
     print(json.dumps(out), flush=True)
 
 
