@@ -146,7 +146,8 @@ def force_error(p, models, bat, picks):
 def batched_force_error(p, models, bat, picks):
     """Total force of the batched kernel in mixed precision against the same
     kernel in FP64, on the timed batch's current positions (one step with
-    the noise off evaluates the forces there)."""
+    the noise off evaluates the forces there; energies off, so that step
+    takes the FP32 force paths of every other step)."""
     from pas_cg_amd import pcg as G
     sub = [models[s] for s in picks]
     pos = [bat["b"].get(s, 0) for s in picks]
@@ -155,6 +156,7 @@ def batched_force_error(p, models, bat, picks):
         b = G.Batch(p, sub, precision=prec, philox_seed=448, block=bat["block"],
                     steps_per_launch=1)
         b.set_noise_scale(0.0)
+        b.energies(False)
         for k, r in enumerate(pos):
             b.set(k, 0, r)
         b.run(1)

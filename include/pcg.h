@@ -226,6 +226,11 @@ int pcg_batch_destroy(pcg_batch *b);
 int pcg_batch_run(pcg_batch *b, int32_t nsteps);
 int pcg_batch_sync(pcg_batch *b);
 int pcg_batch_set_noise_scale(pcg_batch *b, double scale); /* 0: no noise */
+/* Energies (default on) are evaluated on the last step a launch makes for a
+ * system. Off: no step evaluates them (pcg_batch_energy keeps the last values)
+ * and every step of the mixed precision takes its FP32 force paths. No
+ * counterpart in the reference, which sums V on every step. */
+int pcg_batch_energies(pcg_batch *b, int32_t on);
 int pcg_batch_get(pcg_batch *b, int32_t sys, int32_t which, double *out);
 /* which: 0..5 Gear y0..y5 of one system; setting positions (0) makes its
  * next step rebuild the neighbour list */

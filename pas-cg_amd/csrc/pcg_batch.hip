@@ -347,7 +347,9 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, SysDescC &sd, 
       if (p.tether_on && (f & BF_TETHER)) vt += tether_core<double>(p, g, a, r, r + 1, bl.teth(k));
       // local repulsion and chirality have no ill-conditioned step: in R
       if (p.lrep_on && (f & BF_LREP)) vl += lrep_core<RF>(p, g, a, r, r + 2);
-      if constexpr (sizeof(RF) == 4) {
+      // the mixed form's FP32 angles on every step but the one that reports
+      // energies (want_v, uniform), which takes FP64
+      if (sizeof(RF) == 4 && !want_v) {
         if (p.nat_ang_on && (f & BF_NAT_ANG)) vna += nat_ang_mixed(p, g, a, r, r + 1, r + 2, bl.ang(k));
         if (p.heur_ang_on && (f & BF_HEUR_ANG)) vha += heur_ang_mixed(p, g, a, r, r + 1, r + 2, bl.at(k));
       } else {
@@ -636,7 +638,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     if (threadIdx.x == 0) {
       stop = st.step >= target;
       // energies on the last step this launch makes for the system
-      want_v = (it == nsteps - 1) || (st.step + 1 >= target);
+      want_v = !b.energy_off && ((it == nsteps - 1) || (st.step + 1 >= target));
       // nl::verify (cg/src/cg/nl/verify.cpp:3-34)
       if (!stop && P.nl_required) {
         if (st.verify_first) {

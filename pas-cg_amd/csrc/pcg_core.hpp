@@ -148,6 +148,7 @@ struct DevBatch {
   const int *bond_item[NBOND];
   int bond_lds;               // batched kernel: stage per-residue bonded data in LDS
   int gear_lds;               // batched kernel: keep y1..y5 in LDS for the launch
+  int energy_off;             // batched kernel: no step evaluates energies
   int *pauli_i[2];
   int *dh_i[2]; double *dh_q;
   int *pid_i[2]; double *pid_d; int16_t *pid_t;

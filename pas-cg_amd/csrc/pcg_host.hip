@@ -1308,6 +1308,18 @@ int pcg_batch_sync(pcg_batch *b) {
   return PCG_OK;
 }
 
+int pcg_batch_energies(pcg_batch *b, int32_t on) {
+  if (!b) return fail(PCG_E_ARG, "null batch");
+  b->B->d.energy_off = on ? 0 : 1;
+  if (b->gexec) {  // the captured launch holds the old flag
+    HIPCHK(hipGraphExecDestroy(b->gexec));
+    HIPCHK(hipGraphDestroy(b->graph));
+    b->gexec = nullptr;
+    b->graph = nullptr;
+  }
+  return PCG_OK;
+}
+
 int pcg_batch_set_noise_scale(pcg_batch *b, double scale) {
   if (!b || scale < 0) return fail(PCG_E_ARG, "bad arguments");
   DeviceBatch &B = *b->B;

@@ -136,6 +136,7 @@ SIGNATURES = {
     "pcg_server_stats": [_V, C.POINTER(I64), C.POINTER(I), C.POINTER(I)],
     "pcg_server_job_times": [_V, I64, C.POINTER(D), C.POINTER(D), C.POINTER(D)],
     "pcg_batch_set_noise_scale": [_V, D],
+    "pcg_batch_energies": [_V, I],
     "pcg_batch_sizes": [_V, I, C.POINTER(I)],
     "pcg_batch_autotune": [_V, I, C.POINTER(I), C.POINTER(D)],
     "pcg_stream_copy": [I64, I, I, C.POINTER(D), C.POINTER(D)],
@@ -440,6 +441,11 @@ class Batch:
 
     def set_noise_scale(self, s: float):
         check(self.L.pcg_batch_set_noise_scale(self.h, s))
+
+    def energies(self, on: bool = True):
+        """Whether the last step of a launch evaluates energies (default on);
+        off, every step takes the mixed precision's FP32 force paths."""
+        check(self.L.pcg_batch_energies(self.h, int(on)))
 
     def sizes(self, sys: int) -> dict:
         out = np.zeros(len(TERMS) + 3, np.int32)

@@ -223,6 +223,43 @@ def test_autotune_keeps_the_trajectory():
         assert np.abs(a.get(s, 0) - b.get(s, 0)).max() < 1e-9
 
 
+def test_batch_mixed_forces_match_fp64_with_energies_off():
+    """With energies off no step of the mixed batched kernel takes the FP64
+    energy path, so its FP32 angles, dihedrals, contacts and local terms all
+    act: the total force stays within 1e-5 of the same kernel in FP64
+    (relative to the largest component) on evolved positions. In FP64 the
+    switch leaves the trajectory unchanged."""
+    prm = prm_full(qa_on=1)
+    models = ragged_models([40, 150, 300], seed0=200)
+    b0 = G.Batch(prm, models, precision=G.MIXED, steps_per_launch=200)
+    b0.run(400)
+    pos = [b0.get(s, 0) for s in range(3)]
+    b0.close()
+    F = {}
+    for prec in (G.FP64, G.MIXED):
+        b = G.Batch(prm, models, precision=prec, steps_per_launch=1)
+        b.energies(False)
+        b.set_noise_scale(0.0)
+        for k, r in enumerate(pos):
+            b.set(k, 0, r)
+        b.run(1)
+        F[prec] = [b.get(k, 6) for k in range(3)]
+        b.close()
+    for s, (a, c) in enumerate(zip(F[G.MIXED], F[G.FP64])):
+        err = float(np.abs(a - c).max() / np.abs(c).max())
+        assert err <= 1e-5, (s, err)
+    runs = []
+    for on in (True, False):
+        b = G.Batch(prm, models, seeds=[1] * 3, precision=G.FP64, steps_per_launch=25)
+        b.set_noise_scale(0.0)
+        b.energies(on)
+        b.run(50)
+        runs.append([b.get(s, 0) for s in range(3)])
+        b.close()
+    for s in range(3):
+        assert np.abs(runs[0][s] - runs[1][s]).max() < 1e-9
+
+
 def test_batch_graph_replays_match_direct_launches():
     """The hipGraph of one launch, replayed for each full launch of a run
     (a shorter last launch goes directly), with host reads in between,
