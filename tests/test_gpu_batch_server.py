@@ -305,6 +305,61 @@ def test_batch_bonded_paths_follow_oracle():
         assert abs(V - tr[-1]) <= 1e-9 * abs(tr[-1]), (m["n"], V, tr[-1])
 
 
+def test_batch_tiny_and_empty_systems_follow_oracle():
+    """Edge cases in one batch: systems of 1 to 5 residues (no pairs, no
+    angles, no dihedrals, chain ends everywhere) and a protein without native
+    contacts, beside an ordinary one, follow the oracle (noise off, FP64).
+    One residue has NaN velocities from the start, in the reference as here:
+    removing the mean velocity leaves zero, which is then scaled by
+    sqrt(.../0) (cg/src/cg/simul/state.cpp:316-330)."""
+    prm = prm_full(qa_on=1)
+    models = [M.synthetic_protein(n, M.job_seed(500 + n)) for n in (1, 2, 3, 4, 5)]
+    bare = M.synthetic_protein(40, M.job_seed(600))
+    for key in ("cont", "cont_len", "cont_type"):
+        bare[key] = bare[key][:0]
+    models += [bare] + ragged_models([60], seed0=601, pulled=())
+    steps = 80
+    b = G.Batch(prm, models, seeds=[1] * len(models), precision=G.FP64, steps_per_launch=20)
+    b.set_noise_scale(0.0)
+    b.run(steps)
+    for s, m in enumerate(models):
+        o = Oracle(prm, m, seed=1)
+        o.set_noise_scale(0.0)
+        tr = o.step(steps)
+        if m["n"] == 1:
+            assert np.isnan(b.get(s, 0)).all() and np.isnan(o.r()).all()
+            continue
+        err = np.abs(b.get(s, 0) - o.r()).max()
+        assert err < 1e-8, (s, m["n"], err)
+        V, _ = b.energy(s)
+        assert abs(V - tr[-1]) <= 1e-9 * max(abs(tr[-1]), 1e-12), (s, m["n"], V, tr[-1])
+
+
+def test_batch_beyond_lds_takes_stage_kernels():
+    """A system too large for the persistent workgroup's LDS (2800 residues)
+    runs on the stage kernels, in a batch with a small one, and follows the
+    oracle (noise off, FP64); the served form refuses such a job."""
+    prm = prm_full(qa_on=1)
+    models = ragged_models([2800, 50], seed0=700, pulled=())
+    steps = 12
+    b = G.Batch(prm, models, seeds=[1, 1], precision=G.FP64, steps_per_launch=6)
+    b.set_noise_scale(0.0)
+    b.run(steps)
+    for s, m in enumerate(models):
+        o = Oracle(prm, m, seed=1)
+        o.set_noise_scale(0.0)
+        tr = o.step(steps)
+        err = np.abs(b.get(s, 0) - o.r()).max()
+        assert err < 1e-8, (s, m["n"], err)
+        V, _ = b.energy(s)
+        assert abs(V - tr[-1]) <= 1e-9 * abs(tr[-1]), (s, m["n"], V, tr[-1])
+    b.close()
+    srv = G.Server(prm, precision=G.MIXED)
+    with pytest.raises(G.PcgError):
+        srv.submit(models[0], 10, temperature=0.35, seed=1)
+    srv.close()
+
+
 def test_batch_set_positions_then_forces_match_oracle():
     """pcg_batch_set moves a system (and makes it rebuild its lists): the
     next step's forces match the oracle's at the new positions."""
