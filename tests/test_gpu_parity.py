@@ -239,3 +239,38 @@ def test_c1_box_per_term_errors_mixed():
         assert np.abs(F2 - F1).max() / scale <= MIXED_F_TOL, term
         assert abs(V2 - V1) <= MIXED_V_TOL * abs(V1) + 1e-12, (term, V1, V2)
     assert checked >= 5
+
+
+@pytest.mark.parametrize("cfg", ["multichain", "multichain_qa"])
+def test_batch_periodic_multichain_follows_oracle(cfg):
+    """The persistent kernel on a periodic box of several chains (wrapped
+    pairs; inter-chain Pauli, DH and PID, or QA with its sync budgets)
+    follows the oracle over a noise-free FP64 trajectory."""
+    prm, m = CONFIGS[cfg]()
+    steps = 60
+    b = G.Batch(prm, [m], seeds=[1], precision=G.FP64, steps_per_launch=20)
+    b.set_noise_scale(0.0)
+    b.run(steps)
+    o = Oracle(prm, m, seed=1)
+    o.set_noise_scale(0.0)
+    tr = o.step(steps)
+    err = np.abs(b.get(0, 0) - o.r()).max()
+    assert err < 1e-8, err
+    V, _ = b.energy(0)
+    assert abs(V - tr[-1]) <= 1e-9 * abs(tr[-1]), (V, tr[-1])
+    b.close()
+
+
+def test_plain_context_tiny_systems_follow_oracle():
+    """The drop-in context on systems of 2 to 5 residues (no pairs, chain
+    ends everywhere) follows the oracle with the reference's ran2 noise."""
+    prm = full_params(qa_on=1)
+    for n in (2, 3, 4, 5):
+        m = M.synthetic_protein(n, M.job_seed(800 + n))
+        o = Oracle(prm, m, seed=3)
+        c = G.Context(prm, m, precision=G.FP64, noise=G.RAN2, seed=3)
+        o.step(40)
+        c.step(40)
+        assert np.abs(c.r() - o.r()).max() < 1e-9, n
+        assert abs(c.energy()[0] - o.V) <= 1e-9 * max(abs(o.V), 1e-12), (n, c.energy()[0], o.V)
+        c.close()
