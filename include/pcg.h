@@ -186,6 +186,19 @@ int pcg_lang_step(pcg_ctx *c);                       /* lang::step */
 int pcg_get(pcg_ctx *c, int32_t which, double *out);
 int pcg_set(pcg_ctx *c, int32_t which, const double *in);
 int pcg_get_energy(pcg_ctx *c, double *V, double *term_V /*[PCG_NUM_TERMS]*/);
+/* The scalars make_report::add_cur_scalars writes for the trajectory
+ * (cg/src/cg/output/make_report.cpp:196-247), of the current state, indexed
+ * by pcg_report_field: time, potential, total and kinetic energy, radius of
+ * gyration, end-to-end distance, RMSD to the original positions after the
+ * optimal superposition, asphericity, active native contacts (all and
+ * disulfide) and QA contacts between and within chains. The PID activity
+ * counts (pid/count_active.cpp) are not among them. */
+typedef enum {
+  PCG_REP_TIME = 0, PCG_REP_EPOT, PCG_REP_ETOT, PCG_REP_EKIN, PCG_REP_RG, PCG_REP_L,
+  PCG_REP_RMSD, PCG_REP_W, PCG_REP_ICN, PCG_REP_ICNSS, PCG_REP_INTEHC, PCG_REP_INTRHC,
+  PCG_REPORT_NUM
+} pcg_report_field;
+int pcg_report(pcg_ctx *c, double *out /*[PCG_REPORT_NUM]*/);
 int pcg_get_time(pcg_ctx *c, double *t, int64_t *step);
 int pcg_set_time(pcg_ctx *c, double t);
 int pcg_set_noise_scale(pcg_ctx *c, double scale); /* 0: no noise */
@@ -231,6 +244,8 @@ int pcg_batch_set_noise_scale(pcg_batch *b, double scale); /* 0: no noise */
  * and every step of the mixed precision takes its FP32 force paths. No
  * counterpart in the reference, which sums V on every step. */
 int pcg_batch_energies(pcg_batch *b, int32_t on);
+/* The report scalars of every system, [nsys][PCG_REPORT_NUM] (pcg_report). */
+int pcg_batch_report(pcg_batch *b, double *out);
 int pcg_batch_get(pcg_batch *b, int32_t sys, int32_t which, double *out);
 /* which: 0..5 Gear y0..y5 of one system; setting positions (0) makes its
  * next step rebuild the neighbour list */

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT, "libpcg.so")
-SOURCES = ["pcg_kernels.hip", "pcg_batch.hip", "pcg_host.hip", "pcg_probe.hip"]
+SOURCES = ["pcg_kernels.hip", "pcg_batch.hip", "pcg_host.hip", "pcg_probe.hip", "pcg_report.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result"]

@@ -110,6 +110,16 @@ typedef const PCG_CONST_AS DevParams DevParamsC;
 typedef const PCG_CONST_AS pcg_params ParamsC;
 typedef const PCG_CONST_AS SysDesc SysDescC;  // written by the host between launches only
 
+// What the report of one system needs, summed on the device
+// (cg/src/cg/output/make_report.cpp:160-247).
+struct ReportSums {
+  double n, K, L;
+  double S[6];       // sum of (r - c)(r - c)^T: xx, yy, zz, xy, xz, yz
+  double H[9];       // sum of (q - q0)(r - c)^T, row-major (orig^T cur)
+  double pp, qq;     // sums of |r - c|^2 and |q - q0|^2
+  double icn, icnss, inter, intra;
+};
+
 // Device arrays of a batch (pointers into one allocation).
 struct DevBatch {
   int nsys, ntot;
@@ -121,6 +131,7 @@ struct DevBatch {
   double *vel[3];
   void *F[3];                 // float or double forces (precision)
   double *orig_r[3];          // positions at the last NL rebuild
+  double *nat_r[3];           // the model's original positions (the report's RMSD reference)
   double *nh[6];              // QA n and h vectors (FP64)
   const uint8_t *atype;
   const int *chain, *seq, *prev, *next, *sysid, *local;

@@ -443,6 +443,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     A.add(&D.vel[c], ntot);
     A.add((char **)&D.F[c], ntot * fsz);
     A.add(&D.orig_r[c], ntot);
+    A.add(&D.nat_r[c], ntot);
     A.add(&D.noise[c], ntot);
   }
   for (int c = 0; c < 6; ++c) A.add(&D.nh[c], ntot);
@@ -593,7 +594,8 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   std::vector<int> ci(ntot);
   std::vector<uint8_t> cu(ntot);
   std::vector<int8_t> sync(4 * ntot);
-  std::vector<double> ybuf[6][3], vel[3];
+  std::vector<double> ybuf[6][3], vel[3], natb[3];
+  for (int c = 0; c < 3; ++c) natb[c].assign(ntot, 0.0);
   for (int k = 0; k < 6; ++k)
     for (int c = 0; c < 3; ++c) ybuf[k][c].assign(ntot, 0.0);
   std::vector<int> vchain(ntot), vseq(ntot), vprev(ntot), vnext(ntot), vsys(ntot), vloc(ntot);
@@ -606,6 +608,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
       for (int c = 0; c < 3; ++c) {
         ybuf[0][c][g] = h.pos[3 * i + c];
         ybuf[1][c][g] = h.y1[3 * i + c];
+        natb[c][g] = h.orig[3 * i + c];
       }
       vat[g] = h.atype[i];
       vchain[g] = h.chain[i];
@@ -620,6 +623,8 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   for (int k = 0; k < 6; ++k)
     for (int c = 0; c < 3; ++c)
       if ((rc = up(D.y[k][c], ybuf[k][c].data(), ntot * sizeof(double)))) return rc;
+  for (int c = 0; c < 3; ++c)
+    if ((rc = up(D.nat_r[c], natb[c].data(), ntot * sizeof(double)))) return rc;
   if ((rc = up(atype, vat.data(), ntot))) return rc;
   if ((rc = up(chain, vchain.data(), ntot * 4))) return rc;
   if ((rc = up(seq, vseq.data(), ntot * 4))) return rc;
@@ -762,6 +767,97 @@ int upload_ran2_noise(DeviceBatch &B) {
                           B.stream));
   // the pinned buffer is rewritten next step: wait for the copy
   HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+// Eigenvalues of a symmetric 3x3 matrix, in descending order (cyclic Jacobi
+// rotations until the off-diagonal part vanishes).
+void eig_sym3(double a[3][3], double e[3]) {
+  static const int P[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    const double off = std::fabs(a[0][1]) + std::fabs(a[0][2]) + std::fabs(a[1][2]);
+    const double dia = std::fabs(a[0][0]) + std::fabs(a[1][1]) + std::fabs(a[2][2]);
+    if (off == 0.0 || off <= 1e-300 + 1e-18 * dia) break;
+    for (auto &pq : P) {
+      const int p = pq[0], q = pq[1];
+      if (a[p][q] == 0.0) continue;
+      const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+      const double t = std::fabs(th) > 1e150 ? 0.5 / th
+                                             : (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+      const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+      for (int k = 0; k < 3; ++k) {
+        const double kp = a[k][p], kq = a[k][q];
+        a[k][p] = c * kp - s * kq;
+        a[k][q] = s * kp + c * kq;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double pk = a[p][k], qk = a[q][k];
+        a[p][k] = c * pk - s * qk;
+        a[q][k] = s * pk + c * qk;
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k) e[k] = a[k][k];
+  std::sort(e, e + 3, [](double x, double y) { return x > y; });
+}
+
+// make_report::add_cur_scalars (cg/src/cg/output/make_report.cpp:196-247)
+// from the device sums of one system
+void report_scalars(const ReportSums &o, const SysState &st, double *out) {
+  const double N = o.n;
+  out[PCG_REP_TIME] = st.t;
+  out[PCG_REP_EPOT] = st.Vtot;
+  out[PCG_REP_EKIN] = o.K;
+  out[PCG_REP_ETOT] = st.Vtot + o.K;
+  // gyration_radius: sqrt(sum |r - mean|^2 / (float)N) (make_report.cpp:168-179)
+  out[PCG_REP_RG] = std::sqrt(o.pp / (double)(float)N);
+  out[PCG_REP_L] = o.L;
+  // rmsd (make_report.cpp:132-158): the optimal superposition's residual from
+  // the singular values s1 >= s2 >= s3 of H = orig^T cur and d = sign det H:
+  // sum |cur R - orig|^2 = |cur|^2 + |orig|^2 - 2 (s1 + s2 + d s3)
+  double HtH[3][3], e[3];
+  const double *Hm = o.H;
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) {
+      double x = 0.0;
+      for (int k = 0; k < 3; ++k) x += Hm[3 * k + a] * Hm[3 * k + c];
+      HtH[a][c] = x;
+    }
+  eig_sym3(HtH, e);
+  const double detH = Hm[0] * (Hm[4] * Hm[8] - Hm[5] * Hm[7]) - Hm[1] * (Hm[3] * Hm[8] - Hm[5] * Hm[6]) +
+                      Hm[2] * (Hm[3] * Hm[7] - Hm[4] * Hm[6]);
+  const double s1 = std::sqrt(std::max(e[0], 0.0)), s2 = std::sqrt(std::max(e[1], 0.0)),
+               s3 = std::sqrt(std::max(e[2], 0.0));
+  const double r2 = (o.pp + o.qq - 2.0 * (s1 + s2 + (detH < 0 ? -s3 : s3))) / N;
+  out[PCG_REP_RMSD] = std::sqrt(std::max(r2, 0.0));
+  // asphericity (make_report.cpp:181-194): the singular values of the centred
+  // positions are the square roots of the eigenvalues of sum (r - c)(r - c)^T;
+  // W = 1.5 s1 - 0.5 (s1 + s2 + s3)
+  double S[3][3] = {{o.S[0], o.S[3], o.S[4]}, {o.S[3], o.S[1], o.S[5]}, {o.S[4], o.S[5], o.S[2]}};
+  eig_sym3(S, e);
+  const double g1 = std::sqrt(std::max(e[0], 0.0)), g2 = std::sqrt(std::max(e[1], 0.0)),
+               g3 = std::sqrt(std::max(e[2], 0.0));
+  out[PCG_REP_W] = 1.5 * g1 - 0.5 * (g1 + g2 + g3);
+  out[PCG_REP_ICN] = o.icn;
+  out[PCG_REP_ICNSS] = o.icnss;
+  out[PCG_REP_INTEHC] = o.inter;
+  out[PCG_REP_INTRHC] = o.intra;
+}
+
+int report_into(DeviceBatch &B, double *out) {
+  const size_t ns = B.sys.size();
+  ReportSums *d = nullptr;
+  if (hipMalloc(&d, ns * sizeof(ReportSums)) != hipSuccess)
+    return fail(PCG_E_NOMEM, "no device memory for the report sums");
+  launch_report(B.d, d, B.stream);
+  std::vector<ReportSums> h(ns);
+  std::vector<SysState> st(ns);
+  hipError_t e1 = hipMemcpyAsync(h.data(), d, ns * sizeof(ReportSums), hipMemcpyDeviceToHost, B.stream);
+  hipError_t e2 = hipMemcpyAsync(st.data(), B.d.st, ns * sizeof(SysState), hipMemcpyDeviceToHost, B.stream);
+  hipError_t e3 = hipStreamSynchronize(B.stream);
+  (void)hipFree(d);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(PCG_E_HIP, "report copy failed");
+  for (size_t s = 0; s < ns; ++s) report_scalars(h[s], st[s], out + s * PCG_REPORT_NUM);
   return PCG_OK;
 }
 
@@ -1010,6 +1106,12 @@ int pcg_set(pcg_ctx *c, int32_t which, const double *in) {
                           hipMemcpyHostToDevice, B.stream));
   HIPCHK(hipStreamSynchronize(B.stream));
   return PCG_OK;
+}
+
+int pcg_report(pcg_ctx *c, double *out) {
+  if (!c || !out) return fail(PCG_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->B->device));
+  return report_into(*c->B, out);
 }
 
 int pcg_get_energy(pcg_ctx *c, double *V, double *term_V) {
@@ -1306,6 +1408,12 @@ int pcg_batch_sync(pcg_batch *b) {
   if (!b) return fail(PCG_E_ARG, "null batch");
   HIPCHK(hipStreamSynchronize(b->B->stream));
   return PCG_OK;
+}
+
+int pcg_batch_report(pcg_batch *b, double *out) {
+  if (!b || !out) return fail(PCG_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(b->B->device));
+  return report_into(*b->B, out);
 }
 
 int pcg_batch_energies(pcg_batch *b, int32_t on) {

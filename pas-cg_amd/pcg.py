@@ -19,6 +19,9 @@ TERMS = ["chir", "tether", "lrep", "nat_cont", "pauli", "const_dh", "rel_dh",
          "qa_cand", "qa_cont", "pid", "nat_ang", "heur_ang", "heur_dih", "cnd",
          "snd", "vel_afm"]
 TERM_ID = {t: i for i, t in enumerate(TERMS)}
+# pcg_report_field order: the scalars of make_report::add_cur_scalars
+REPORT_FIELDS = ["TIME", "EPOT", "ETOT", "EKIN", "RG", "L", "RMSD", "W", "ICN", "ICNss",
+                 "INTEHC", "INTRHC"]
 MIXED, FP64 = 0, 1
 PHILOX, RAN2 = 0, 1
 
@@ -108,6 +111,7 @@ SIGNATURES = {
     "pcg_qa_finish": [_V], "pcg_lang_step": [_V],
     "pcg_get": [_V, I, C.POINTER(D)], "pcg_set": [_V, I, C.POINTER(D)],
     "pcg_get_energy": [_V, C.POINTER(D), C.POINTER(D)],
+    "pcg_report": [_V, C.POINTER(D)],
     "pcg_get_time": [_V, C.POINTER(D), C.POINTER(I64)],
     "pcg_set_time": [_V, D], "pcg_set_noise_scale": [_V, D],
     "pcg_get_list": [_V, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I),
@@ -137,6 +141,7 @@ SIGNATURES = {
     "pcg_server_job_times": [_V, I64, C.POINTER(D), C.POINTER(D), C.POINTER(D)],
     "pcg_batch_set_noise_scale": [_V, D],
     "pcg_batch_energies": [_V, I],
+    "pcg_batch_report": [_V, C.POINTER(D)],
     "pcg_batch_sizes": [_V, I, C.POINTER(I)],
     "pcg_batch_autotune": [_V, I, C.POINTER(I), C.POINTER(D)],
     "pcg_stream_copy": [I64, I, I, C.POINTER(D), C.POINTER(D)],
@@ -318,6 +323,12 @@ class Context:
         check(self.L.pcg_get_energy(self.h, C.byref(V), _p(tv, D)))
         return V.value, dict(zip(TERMS, tv))
 
+    def report(self) -> dict:
+        """The trajectory's report scalars (REPORT_FIELDS) of the current state."""
+        out = np.zeros(len(REPORT_FIELDS))
+        check(self.L.pcg_report(self.h, _p(out, D)))
+        return dict(zip(REPORT_FIELDS, out.tolist()))
+
     def time(self):
         t = C.c_double()
         s = C.c_int64()
@@ -425,6 +436,12 @@ class Batch:
         tv = np.zeros(len(TERMS))
         check(self.L.pcg_batch_energy(self.h, sys, C.byref(V), _p(tv, D)))
         return V.value, dict(zip(TERMS, tv))
+
+    def report(self) -> list:
+        """The report scalars (REPORT_FIELDS) of every system, one dict each."""
+        out = np.zeros((len(self.ns), len(REPORT_FIELDS)))
+        check(self.L.pcg_batch_report(self.h, _p(out, D)))
+        return [dict(zip(REPORT_FIELDS, row.tolist())) for row in out]
 
     def info(self):
         b = C.c_int32()
