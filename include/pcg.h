@@ -246,6 +246,16 @@ int pcg_batch_set_noise_scale(pcg_batch *b, double scale); /* 0: no noise */
 int pcg_batch_energies(pcg_batch *b, int32_t on);
 /* The report scalars of every system, [nsys][PCG_REPORT_NUM] (pcg_report). */
 int pcg_batch_report(pcg_batch *b, double *out);
+/* Checkpoint and resume of a whole batch (replaces the state serialisation of
+ * cg/src/cg/simul/state_io.cpp:117-140: positions, time, velocities, Gear
+ * y0..y5, QA contacts, sync budgets; here with the lists, the native-contact
+ * state and the noise position as well, so a resumed run continues the same
+ * trajectory). The buffer is host memory of pcg_batch_checkpoint_size bytes;
+ * restore needs a batch built from the same models, parameters and
+ * precision. */
+int pcg_batch_checkpoint_size(pcg_batch *b, int64_t *bytes);
+int pcg_batch_checkpoint(pcg_batch *b, void *buf, int64_t bytes);
+int pcg_batch_restore(pcg_batch *b, const void *buf, int64_t bytes);
 int pcg_batch_get(pcg_batch *b, int32_t sys, int32_t which, double *out);
 /* which: 0..5 Gear y0..y5 of one system; setting positions (0) makes its
  * next step rebuild the neighbour list */

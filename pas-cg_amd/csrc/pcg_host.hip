@@ -1410,6 +1410,78 @@ int pcg_batch_sync(pcg_batch *b) {
   return PCG_OK;
 }
 
+// Checkpoint of a batch: a header, then the device arena byte for byte. The
+// arena holds every piece of device state (Gear y0..y5, velocities, time and
+// step counters, which are also the Philox position, lists, native-contact
+// and QA state, sync budgets, system descriptors, parameters) and no absolute
+// addresses, so it restores into any batch built from the same models,
+// parameters and precision.
+namespace {
+struct CkptHeader {
+  char magic[8];
+  uint32_t version;
+  int32_t nsys;
+  int64_t ntot;
+  int32_t prec;
+  int32_t pad;
+  uint64_t arena_bytes;
+  uint64_t fingerprint;
+};
+uint64_t ckpt_fingerprint(const DeviceBatch &B) {
+  uint64_t h = mix64((uint64_t)B.sys.size() ^ ((uint64_t)B.arena.total << 20));
+  for (const SysDesc &s : B.sys) h = mix64(h ^ (uint64_t)(uint32_t)s.n ^ ((uint64_t)(uint32_t)s.res0 << 32));
+  return mix64(h ^ (uint64_t)B.prec);
+}
+}  // namespace
+
+int pcg_batch_checkpoint_size(pcg_batch *b, int64_t *bytes) {
+  if (!b || !bytes) return fail(PCG_E_ARG, "null argument");
+  *bytes = (int64_t)(sizeof(CkptHeader) + b->B->arena.total);
+  return PCG_OK;
+}
+
+int pcg_batch_checkpoint(pcg_batch *b, void *buf, int64_t bytes) {
+  if (!b || !buf) return fail(PCG_E_ARG, "null argument");
+  DeviceBatch &B = *b->B;
+  if (bytes < (int64_t)(sizeof(CkptHeader) + B.arena.total))
+    return fail(PCG_E_ARG, "checkpoint buffer too small");
+  HIPCHK(hipSetDevice(B.device));
+  CkptHeader h{};
+  std::memcpy(h.magic, "PCGCKPT1", 8);
+  h.version = 1;
+  h.nsys = (int32_t)B.sys.size();
+  h.ntot = B.d.ntot;
+  h.prec = B.prec;
+  h.arena_bytes = B.arena.total;
+  h.fingerprint = ckpt_fingerprint(B);
+  std::memcpy(buf, &h, sizeof h);
+  HIPCHK(hipMemcpyAsync((char *)buf + sizeof h, B.arena.base, B.arena.total, hipMemcpyDeviceToHost,
+                        B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+int pcg_batch_restore(pcg_batch *b, const void *buf, int64_t bytes) {
+  if (!b || !buf) return fail(PCG_E_ARG, "null argument");
+  DeviceBatch &B = *b->B;
+  CkptHeader h;
+  if (bytes < (int64_t)sizeof h) return fail(PCG_E_ARG, "not a checkpoint");
+  std::memcpy(&h, buf, sizeof h);
+  if (std::memcmp(h.magic, "PCGCKPT1", 8) != 0 || h.version != 1) return fail(PCG_E_ARG, "not a checkpoint");
+  if (h.nsys != (int32_t)B.sys.size() || h.ntot != B.d.ntot || h.prec != B.prec ||
+      h.arena_bytes != B.arena.total || h.fingerprint != ckpt_fingerprint(B))
+    return fail(PCG_E_ARG, "checkpoint of a different batch (models, parameters or precision)");
+  if (bytes < (int64_t)(sizeof h + h.arena_bytes)) return fail(PCG_E_ARG, "truncated checkpoint");
+  HIPCHK(hipSetDevice(B.device));
+  HIPCHK(hipMemcpyAsync(B.arena.base, (const char *)buf + sizeof h, B.arena.total,
+                        hipMemcpyHostToDevice, B.stream));
+  // the host copy of the system descriptors follows the restored ones
+  HIPCHK(hipMemcpyAsync(B.sys.data(), (const void *)B.d.sys, B.sys.size() * sizeof(SysDesc),
+                        hipMemcpyDeviceToHost, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
 int pcg_batch_report(pcg_batch *b, double *out) {
   if (!b || !out) return fail(PCG_E_ARG, "null argument");
   HIPCHK(hipSetDevice(b->B->device));

@@ -142,6 +142,9 @@ SIGNATURES = {
     "pcg_batch_set_noise_scale": [_V, D],
     "pcg_batch_energies": [_V, I],
     "pcg_batch_report": [_V, C.POINTER(D)],
+    "pcg_batch_checkpoint_size": [_V, C.POINTER(I64)],
+    "pcg_batch_checkpoint": [_V, _V, I64],
+    "pcg_batch_restore": [_V, _V, I64],
     "pcg_batch_sizes": [_V, I, C.POINTER(I)],
     "pcg_batch_autotune": [_V, I, C.POINTER(I), C.POINTER(D)],
     "pcg_stream_copy": [I64, I, I, C.POINTER(D), C.POINTER(D)],
@@ -436,6 +439,20 @@ class Batch:
         tv = np.zeros(len(TERMS))
         check(self.L.pcg_batch_energy(self.h, sys, C.byref(V), _p(tv, D)))
         return V.value, dict(zip(TERMS, tv))
+
+    def checkpoint(self) -> bytes:
+        """The batch's whole device state (restore with restore())."""
+        n = I64()
+        check(self.L.pcg_batch_checkpoint_size(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(self.L.pcg_batch_checkpoint(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, ck: bytes):
+        """Resumes from a checkpoint of a batch of the same models,
+        parameters and precision."""
+        buf = C.create_string_buffer(ck, len(ck))
+        check(self.L.pcg_batch_restore(self.h, buf, len(ck)))
 
     def report(self) -> list:
         """The report scalars (REPORT_FIELDS) of every system, one dict each."""

@@ -260,6 +260,38 @@ def test_batch_mixed_forces_match_fp64_with_energies_off():
         assert np.abs(runs[0][s] - runs[1][s]).max() < 1e-9
 
 
+def test_batch_checkpoint_resumes_the_trajectory():
+    """A checkpoint restores a whole batch: resumed in place, or in a new batch
+    of the same models, a run continues the trajectory of the uninterrupted
+    batch (mixed precision, Philox noise on: the noise position is part of
+    the state). A batch of other models refuses it."""
+    prm = prm_full(qa_on=1)
+    models = ragged_models([45, 130, 70], seed0=300)
+    b = G.Batch(prm, models, precision=G.MIXED, philox_seed=9, steps_per_launch=100)
+    b.run(200)
+    ck = b.checkpoint()
+    b.run(300)
+    ref = [b.get(s, 0) for s in range(3)]
+    b.restore(ck)
+    b.run(300)
+    again = [b.get(s, 0) for s in range(3)]
+    b2 = G.Batch(prm, models, precision=G.MIXED, philox_seed=9, steps_per_launch=100)
+    b2.restore(ck)
+    b2.run(300)
+    fresh = [b2.get(s, 0) for s in range(3)]
+    for s in range(3):
+        scale = np.abs(ref[s]).max()
+        # the same trajectory up to the order of the LDS force additions
+        assert np.abs(again[s] - ref[s]).max() <= 1e-7 * scale, s
+        assert np.abs(fresh[s] - ref[s]).max() <= 1e-7 * scale, s
+        assert abs(b2.report()[s]["TIME"] - b.report()[s]["TIME"]) == 0.0
+    b3 = G.Batch(prm, models[:2], precision=G.MIXED, philox_seed=9, steps_per_launch=100)
+    with pytest.raises(G.PcgError):
+        b3.restore(ck)
+    for x in (b, b2, b3):
+        x.close()
+
+
 def test_batch_graph_replays_match_direct_launches():
     """The hipGraph of one launch, replayed for each full launch of a run
     (a shorter last launch goes directly), with host reads in between,
