@@ -201,7 +201,7 @@ __device__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysState &st, const G
     if (valid && cls) {
       int i1 = b.nl_i[0][slot], i2 = b.nl_i[1][slot];
       double d = b.nl_d[slot];
-      if (cls & 1) set_current_contact(b, sd.nc0 + run[0] + ranks[0], find_contact(b, sd, i1, i2));
+      if (cls & 1) set_current_contact(b, sd.nc0 + run[0] + ranks[0], find_contact(b, sd, i1, i2), sd.res0);
       if (cls & 2) {
         int pos = run[1] + ranks[1];
         if (pos < sd.qfcap) {
@@ -428,22 +428,37 @@ __device__ __forceinline__ void nat_cont_loop(const DevBatch &b, SysDescC &sd, S
       // item, straight-line; then the adds and the state transitions
       const float cut2 = (float)(p.nl_cutoff * p.nl_cutoff), lo = 1.0f - 2e-5f, hi = 1.0f + 2e-5f;
       const float depth = (float)p.nat_cont_depth;
+      const int r0 = sd.res0;
       for (; k < n; k += NK * B) {
-        NcFetch q[NK];
-        for (int j = 0; j < NK; ++j) q[j] = nat_cont_fetch(b, sd.nc0 + min(k + j * B, n - 1));
-        for (int j = 0; j < NK; ++j) keep_fetch(q[j]);
+        // three loads an item: the packed residues, type and state; the
+        // native length; the FP32 squared transition distance
+        uint32_t ijt[NK];
+        double nat[NK];
+        float th2[NK];
+        for (int j = 0; j < NK; ++j) {
+          const int kk = sd.nc0 + min(k + j * B, n - 1);
+          ijt[j] = b.ncur_ijt[kk];
+          nat[j] = b.ncur_nat[kk];
+          th2[j] = b.ncur_th2[kk];
+        }
+        for (int j = 0; j < NK; ++j) {
+          keep(ijt[j]);
+          keep(nat[j]);
+          keep(th2[j]);
+        }
         v3<double> d[NK];
         v3<float> r12[NK];
         float rn[NK];
         int now[NK];
         bool act[NK], exact[NK];
         for (int j = 0; j < NK; ++j) {
-          d[j] = xwrapd(xsub(g.r(q[j].i2), g.r(q[j].i1)), sd);
+          const int i1 = r0 + (int)(ijt[j] & 0xFFFu), i2 = r0 + (int)((ijt[j] >> 12) & 0xFFFu);
+          d[j] = xwrapd(xsub(g.r(i2), g.r(i1)), sd);
           r12[j] = cast3<float>(d[j]);
           const float d2 = nsq(r12[j]);
           act[j] = k + j * B < n && !(d2 > cut2 * hi);
-          exact[j] = !(d2 < cut2 * lo && (d2 < q[j].th2 * lo || d2 > q[j].th2 * hi));
-          now[j] = d2 < q[j].th2 * lo ? 1 : 0;
+          exact[j] = !(d2 < cut2 * lo && (d2 < th2[j] * lo || d2 > th2[j] * hi));
+          now[j] = d2 < th2[j] * lo ? 1 : 0;
           rn[j] = sq(d2);
         }
         for (int j = 0; j < NK; ++j)
@@ -451,25 +466,27 @@ __device__ __forceinline__ void nat_cont_loop(const DevBatch &b, SysDescC &sd, S
             const double rnx = xnorm3(d[j]);
             act[j] = rnx < p.nl_cutoff;
             rn[j] = (float)rnx;
-            now[j] = rnx <= __dmul_rn(__dmul_rn(q[j].nat, C216_INV), p.counting_factor) ? 1 : 0;
+            now[j] = rnx <= __dmul_rn(__dmul_rn(nat[j], C216_INV), p.counting_factor) ? 1 : 0;
           }
         float Vj[NK];
         v3<float> f[NK];
         for (int j = 0; j < NK; ++j) {
           float dV;
-          ljR<float>(depth, (float)q[j].nat, rcpR(rn[j]), Vj[j], dV);
+          ljR<float>(depth, (float)nat[j], rcpR(rn[j]), Vj[j], dV);
           dV = clampR(dV, -1e3f, 1e3f);
           f[j] = (dV * rcpR(rn[j])) * r12[j];
         }
         for (int j = 0; j < NK; ++j)
           if (act[j]) {
-            acc.add(q[j].i1, f[j]);
-            acc.add(q[j].i2, -f[j]);
+            acc.add(r0 + (int)(ijt[j] & 0xFFFu), f[j]);
+            acc.add(r0 + (int)((ijt[j] >> 12) & 0xFFFu), -f[j]);
             v += (double)Vj[j];
-            if (now[j] != (q[j].type >> 7)) {
-              const int c = q[j].c;
+            if ((uint32_t)now[j] != (ijt[j] >> 31)) {
+              // a transition (rare): the contact's index into the list of all
+              const int kk = sd.nc0 + k + j * B, c = b.ncur[kk];
               b.nc_active[c] = (uint8_t)now[j];
-              b.ncur_t[q[j].k] = (uint8_t)((q[j].type & 0x7F) | (now[j] << 7));
+              b.ncur_t[kk] = (uint8_t)(((ijt[j] >> 24) & 0x7Fu) | ((uint32_t)now[j] << 7));
+              b.ncur_ijt[kk] = (ijt[j] & 0x7FFFFFFFu) | ((uint32_t)now[j] << 31);
               if (b.nc_change_t[c] < 0) {
                 b.nc_change_t[c] = st.t;
                 atomicAdd(&st.num_changed, 1);
@@ -520,24 +537,32 @@ __device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, SysDescC &s
   for (int base = 0; base < n; base += QK * B) {
     if (threadIdx.x == 0) qcount = 0;
     __syncthreads();
-    QhFetch q[QK];
+    QhLite q[QK];
     for (int j = 0; j < QK; ++j) {
-      int hk = base + j * B + (int)threadIdx.x;
-      q[j] = hk < n ? qa_cand_fetch(b, sd, hk) : QhFetch{hk, 0, 0, 0.f, 1, 0.0, 0};
+      const int hk = base + j * B + (int)threadIdx.x;
+      q[j] = hk < n ? qa_cand_fetch_lite(b, sd, hk) : QhLite{0u, -1.0f};
     }
-    for (int j = 0; j < QK; ++j) keep_fetch(q[j]);
+    for (int j = 0; j < QK; ++j) {
+      keep(q[j].ij);
+      keep(q[j].thr2);
+    }
     unsigned ovf = 0;  // items of this thread that found the queue full
     for (int j = 0; j < QK; ++j) {
-      bool keep = q[j].hk < n && qa_cand_may_act(sd, g, q[j]);
-      unsigned long long m = __ballot(keep);
+      // the first test of qa_cand_may_act; a vacant pair has thr2 < 0
+      bool act = false;
+      if (q[j].thr2 >= 0.0f) {
+        const v3<double> d = xwrapd(xsub(g.r(r0 + (int)(q[j].ij >> 16)), g.r(r0 + (int)(q[j].ij & 0xFFFFu))), sd);
+        const float dx = (float)d.x, dy = (float)d.y, dz = (float)d.z;
+        act = !(dx * dx + dy * dy + dz * dz > q[j].thr2);
+      }
+      unsigned long long m = __ballot(act);
       int off = 0;
       if (lane == 0 && m) off = atomicAdd(&qcount, __popcll(m));
       off = __shfl(off, 0, 64);
-      if (keep) {
+      if (act) {
         int e = off + __popcll(m & ((1ull << lane) - 1ull));
         if (e < QCAP)
-          queue[e] = make_uint2((unsigned)q[j].hk,
-                                (unsigned)(q[j].i1 - r0) | ((unsigned)(q[j].i2 - r0) << 16));
+          queue[e] = make_uint2((unsigned)(base + j * B + (int)threadIdx.x), q[j].ij);
         else
           ovf |= 1u << j;
       }

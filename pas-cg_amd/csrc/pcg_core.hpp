@@ -154,6 +154,9 @@ struct DevBatch {
   int *ncur_i[2]; double *ncur_nat;  // their residues, length,
   uint8_t *ncur_t;                   // type | active << 7
   float *ncur_th2;  // squared transition distance (nat * 2^(-1/6) * counting factor), FP32
+  // the same contact packed for the persistent kernel (systems under 4096
+  // residues): i1 | i2 << 12 within the system, type << 24, active << 31
+  uint32_t *ncur_ijt;
   // bond_item[L][r]: global index of the item of list L whose first residue
   // is r, or -1 (systems with bond_regular set)
   const int *bond_item[NBOND];
@@ -175,7 +178,8 @@ struct DevBatch {
   // per near pair: the largest formation distance any contact type of its
   // residue pair can have (exact FP64), and the squared FP32 radius beyond
   // which the pair can have no effect at all (with a 1e-5 margin)
-  double *qh_req; float *qh_thr2;
+  double *qh_req; float *qh_thr2;  // thr2 < 0 once the pair is vacant
+  uint32_t *qh_ij;  // the pair's residues within its system, i1 | i2 << 16
   int *cand_h;                // near-list index of each candidate
   int *qc_i[2]; double *qc_d; int16_t *qc_t; double *qc_sat; int8_t *qc_sd; uint8_t *qc_vac;
   int *cand_i[2]; double *cand_od, *cand_dist; int *cand_idx; int16_t *cand_t; int8_t *cand_sd;

@@ -488,6 +488,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.ncur_nat, std::max(nnc, 1LL));
   A.add(&D.ncur_t, std::max(nnc, 1LL));
   A.add(&D.ncur_th2, std::max(nnc, 1LL));
+  A.add(&D.ncur_ijt, std::max(nnc, 1LL));
   // chain-local bonded items by base residue (see BOND_* in pcg_core.hpp)
   std::vector<int> bond_item((size_t)NBOND * std::max(ntot, 1LL), -1);
   for (int s = 0; s < nsys; ++s) {
@@ -549,6 +550,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.qh_vac, qf);
   A.add(&D.qh_req, qf);
   A.add(&D.qh_thr2, qf);
+  A.add(&D.qh_ij, qf);
   A.add(&D.cand_h, nl);
   A.add(&D.qc_d, qc);
   A.add(&D.qc_t, qc);

@@ -244,7 +244,7 @@ __global__ void k_part_write(DevBatch b) {
   ParamsC &p = b.P->p;
   if (cls & CL_NC) {
     int pos = b.blk_cnt[base + 0] + ranks[0];
-    set_current_contact(b, sd.nc0 + pos, find_contact(b, sd, i1, i2));
+    set_current_contact(b, sd.nc0 + pos, find_contact(b, sd, i1, i2), sd.res0);
   }
   if (cls & CL_QA) {
     int pos = b.blk_cnt[base + 1] + ranks[1];

@@ -22,7 +22,7 @@ __device__ inline int find_contact(const DevBatch &b, SysDescC &s, int i1, int i
 
 // Slot q of the current native-contact list takes contact c of the list of
 // all contacts, with a copy of its fields.
-__device__ __forceinline__ void set_current_contact(const DevBatch &b, int q, int c) {
+__device__ __forceinline__ void set_current_contact(const DevBatch &b, int q, int c, int res0) {
   b.ncur[q] = c;
   b.ncur_i[0][q] = b.nc_i[0][c];
   b.ncur_i[1][q] = b.nc_i[1][c];
@@ -31,6 +31,8 @@ __device__ __forceinline__ void set_current_contact(const DevBatch &b, int q, in
   b.ncur_t[q] = (uint8_t)(b.nc_type[c] | (b.nc_active[c] ? 0x80 : 0));
   double th = __dmul_rn(__dmul_rn(b.nc_nat[c], C216_INV), b.P->p.counting_factor);
   b.ncur_th2[q] = (float)(th * th);
+  b.ncur_ijt[q] = (uint32_t)((b.nc_i[0][c] - res0) & 0xFFF) |
+                  ((uint32_t)((b.nc_i[1][c] - res0) & 0xFFF) << 12) | ((uint32_t)b.ncur_t[q] << 24);
 }
 
 __device__ inline int classify(const DevBatch &b, SysDescC &s, const SysState &st,
@@ -116,6 +118,8 @@ __device__ inline void build_qa_near(const DevBatch &b, SysDescC &sd, SysState &
       b.qh_i[1][h] = b.qf_i[1][sd.qf0 + k];
       b.qh_vac[h] = b.qf_vac[sd.qf0 + k];
       qa_near_filters(b, b.qh_i[0][h], b.qh_i[1][h], b.qh_req[h], b.qh_thr2[h]);
+      b.qh_ij[h] = (uint32_t)(b.qh_i[0][h] - sd.res0) | ((uint32_t)(b.qh_i[1][h] - sd.res0) << 16);
+      if (b.qh_vac[h]) b.qh_thr2[h] = -1.0f;
     }
     __syncthreads();
     if (threadIdx.x == 0) run += tot;
@@ -176,6 +180,7 @@ __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState
           b.qh_i[1][h] = i2;
           b.qh_vac[h] = 0;
           qa_near_filters(b, i1, i2, b.qh_req[h], b.qh_thr2[h]);
+          b.qh_ij[h] = (uint32_t)(i1 - sd.res0) | ((uint32_t)(i2 - sd.res0) << 16);
         }
         st.qf_n += 1;
       } else {
@@ -218,6 +223,7 @@ __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState
       }
       b.qf_vac[sd.qf0 + b.cand_idx[sc]] = 1;
       b.qh_vac[sd.qf0 + b.cand_h[sc]] = 1;
+      b.qh_thr2[sd.qf0 + b.cand_h[sc]] = -1.0f;
     }
     // compress
     if (2 * st.num_qa < st.qc_n) {

@@ -180,6 +180,7 @@ __device__ __forceinline__ double nat_cont_core(const DevBatch &b, SysDescC &s, 
   if (now != (q.type >> 7)) {
     b.nc_active[c] = (uint8_t)now;
     b.ncur_t[q.k] = (uint8_t)((q.type & 0x7F) | (now << 7));
+    b.ncur_ijt[q.k] = (b.ncur_ijt[q.k] & 0x7FFFFFFFu) | ((uint32_t)now << 31);
     if (b.nc_change_t[c] < 0) {
       b.nc_change_t[c] = t;
       atomicAdd(&st.num_changed, 1);
@@ -635,6 +636,17 @@ __device__ __forceinline__ void keep_fetch(QhFetch &q) {
   keep(q.i2);
   keep(q.thr2);
   keep(q.vac);
+}
+// The first test's view of near pair hk, in two loads: its residues within
+// the system and its FP32 radius, negative once the pair is vacant (the test
+// then fails by itself).
+struct QhLite {
+  uint32_t ij;
+  float thr2;
+};
+__device__ __forceinline__ QhLite qa_cand_fetch_lite(const DevBatch &b, SysDescC &s, int hk) {
+  const int h = s.qf0 + hk;
+  return QhLite{b.qh_ij[h], b.qh_thr2[h]};
 }
 // with what the exact path needs, in the same round trip
 __device__ __forceinline__ QhFetch qa_cand_fetch_full(const DevBatch &b, SysDescC &s, int hk) {
