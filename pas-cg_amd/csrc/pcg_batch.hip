@@ -335,7 +335,12 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, SysDescC &sd, 
                                                double *fy, double *fz, double *vacc, bool want_v) {
   ParamsC &p = b.P->p;
   const int n = sd.n, r0 = sd.res0, lane = threadIdx.x & 63;
-  double vt = 0, vl = 0, vna = 0, vha = 0, vnd = 0, vhd = 0, vch = 0;
+  // energies only on the step that reports them, added per item (no
+  // accumulators live across the items: registers are the scarce resource)
+  auto ev = [&](int term, double V) {
+    if (want_v && V != 0.0) atomicAdd(&vacc[term], V);
+  };
+  const int dih_term = p.nat_dih_variant ? PCG_SND : PCG_CND;
   for (int base = 0; base < n; base += blockDim.x) {
     int k = base + (int)threadIdx.x;
     AccSlots<RF> a;
@@ -344,35 +349,35 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, SysDescC &sd, 
     if (STAGED && k < n) {
       int r = r0 + k;
       uint8_t f = bl.flag(k);
-      if (p.tether_on && (f & BF_TETHER)) vt += tether_core<double>(p, g, a, r, r + 1, bl.teth(k));
+      if (p.tether_on && (f & BF_TETHER)) ev(PCG_TETHER, tether_core<double>(p, g, a, r, r + 1, bl.teth(k)));
       // local repulsion and chirality have no ill-conditioned step: in R
-      if (p.lrep_on && (f & BF_LREP)) vl += lrep_core<RF>(p, g, a, r, r + 2);
+      if (p.lrep_on && (f & BF_LREP)) ev(PCG_LREP, lrep_core<RF>(p, g, a, r, r + 2));
       // the mixed form's FP32 angles on every step but the one that reports
       // energies (want_v, uniform), which takes FP64
       if (sizeof(RF) == 4 && !want_v) {
-        if (p.nat_ang_on && (f & BF_NAT_ANG)) vna += nat_ang_mixed(p, g, a, r, r + 1, r + 2, bl.ang(k));
-        if (p.heur_ang_on && (f & BF_HEUR_ANG)) vha += heur_ang_mixed(p, g, a, r, r + 1, r + 2, bl.at(k));
+        if (p.nat_ang_on && (f & BF_NAT_ANG)) ev(PCG_NAT_ANG, nat_ang_mixed(p, g, a, r, r + 1, r + 2, bl.ang(k)));
+        if (p.heur_ang_on && (f & BF_HEUR_ANG)) ev(PCG_HEUR_ANG, heur_ang_mixed(p, g, a, r, r + 1, r + 2, bl.at(k)));
       } else {
-        if (p.nat_ang_on && (f & BF_NAT_ANG)) vna += nat_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.ang(k));
-        if (p.heur_ang_on && (f & BF_HEUR_ANG)) vha += heur_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.at(k));
+        if (p.nat_ang_on && (f & BF_NAT_ANG)) ev(PCG_NAT_ANG, nat_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.ang(k)));
+        if (p.heur_ang_on && (f & BF_HEUR_ANG)) ev(PCG_HEUR_ANG, heur_ang_core<double>(p, g, a, r, r + 1, r + 2, bl.at(k)));
       }
       if (p.nat_dih_on && (f & BF_NAT_DIH))
-        vnd += nat_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dih(k), bl.dih(k), bl.dihs(k),
-                                    p.nat_dih_variant);
+        ev(dih_term, nat_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dih(k), bl.dih(k), bl.dihs(k),
+                                    p.nat_dih_variant));
       if (p.heur_dih_on && (f & BF_HEUR_DIH))
-        vhd += heur_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dt(k));
+        ev(PCG_HEUR_DIH, heur_dih_core<double>(p, g, a, r, r + 1, r + 2, r + 3, bl.dt(k)));
       if (p.chir_on && (f & BF_CHIR))
-        vch += chir_core<RF>(p, g, a, r, r + 1, r + 2, r + 3, bl.c0(k), bl.c1(k));
+        ev(PCG_CHIR, chir_core<RF>(p, g, a, r, r + 1, r + 2, r + 3, bl.c0(k), bl.c1(k)));
     } else if (!STAGED && k < n) {
       int r = r0 + k, it;
-      if (p.tether_on && (it = b.bond_item[BOND_TETHER][r]) >= 0) vt += tether_item<double>(b, g, a, it);
-      if (p.lrep_on && (it = b.bond_item[BOND_LREP][r]) >= 0) vl += lrep_item<double>(b, g, a, it);
-      if (p.nat_ang_on && (it = b.bond_item[BOND_NAT_ANG][r]) >= 0) vna += nat_ang_item<double>(b, g, a, it);
-      if (p.heur_ang_on && (it = b.bond_item[BOND_HEUR_ANG][r]) >= 0) vha += heur_ang_item<double>(b, g, a, it);
+      if (p.tether_on && (it = b.bond_item[BOND_TETHER][r]) >= 0) ev(PCG_TETHER, tether_item<double>(b, g, a, it));
+      if (p.lrep_on && (it = b.bond_item[BOND_LREP][r]) >= 0) ev(PCG_LREP, lrep_item<double>(b, g, a, it));
+      if (p.nat_ang_on && (it = b.bond_item[BOND_NAT_ANG][r]) >= 0) ev(PCG_NAT_ANG, nat_ang_item<double>(b, g, a, it));
+      if (p.heur_ang_on && (it = b.bond_item[BOND_HEUR_ANG][r]) >= 0) ev(PCG_HEUR_ANG, heur_ang_item<double>(b, g, a, it));
       if (p.nat_dih_on && (it = b.bond_item[BOND_NAT_DIH][r]) >= 0)
-        vnd += nat_dih_item<double>(b, g, a, it, p.nat_dih_variant);
-      if (p.heur_dih_on && (it = b.bond_item[BOND_HEUR_DIH][r]) >= 0) vhd += heur_dih_item<double>(b, g, a, it);
-      if (p.chir_on && (it = b.bond_item[BOND_CHIR][r]) >= 0) vch += chir_item<double>(b, g, a, it);
+        ev(dih_term, nat_dih_item<double>(b, g, a, it, p.nat_dih_variant));
+      if (p.heur_dih_on && (it = b.bond_item[BOND_HEUR_DIH][r]) >= 0) ev(PCG_HEUR_DIH, heur_dih_item<double>(b, g, a, it));
+      if (p.chir_on && (it = b.bond_item[BOND_CHIR][r]) >= 0) ev(PCG_CHIR, chir_item<double>(b, g, a, it));
     }
     RF t[3];
     for (int c = 0; c < 3; ++c) {
@@ -395,15 +400,6 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, SysDescC &sd, 
           atomicAdd(fz + k + s, (double)a.c[s][2]);
         }
     }
-  }
-  if (want_v) {
-    if (vt != 0.0) atomicAdd(&vacc[PCG_TETHER], vt);
-    if (vl != 0.0) atomicAdd(&vacc[PCG_LREP], vl);
-    if (vna != 0.0) atomicAdd(&vacc[PCG_NAT_ANG], vna);
-    if (vha != 0.0) atomicAdd(&vacc[PCG_HEUR_ANG], vha);
-    if (vnd != 0.0) atomicAdd(&vacc[p.nat_dih_variant ? PCG_SND : PCG_CND], vnd);
-    if (vhd != 0.0) atomicAdd(&vacc[PCG_HEUR_DIH], vhd);
-    if (vch != 0.0) atomicAdd(&vacc[PCG_CHIR], vch);
   }
 }
 
@@ -753,12 +749,12 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
         rx[k] = r[0];
         ry[k] = r[1];
         rz[k] = r[2];
-        maxd = fmax(maxd, disp_from(sd, o, r));
+        maxd = fmax(maxd, disp2_from(sd, o, r));  // squared; the root is taken once below
       }
     }
     maxd = block_max(maxd);
     if (threadIdx.x == 0) {
-      st.max_disp_bits = (unsigned long long)__double_as_longlong(maxd);
+      st.max_disp_bits = (unsigned long long)__double_as_longlong(__dsqrt_rn(maxd));
       if (want_v) {
         double v = 0.0;
         for (int k = 0; k < NT; ++k) {

@@ -309,6 +309,15 @@ __device__ __forceinline__ void gear_update(const DevBatch &b, SysDescC &sd, con
   for (int c = 0; c < 3; ++c) b.vel[c][i] = vel[c];
 }
 
+// The square of disp_from, with the same operations before the root: since
+// the correctly rounded square root is monotonic, the root of the largest
+// square is exactly the largest disp_from.
+__device__ __forceinline__ double disp2_from(SysDescC &sd, const double o[3], const double r[3]) {
+#pragma clang fp contract(off)
+  v3<double> d{__dsub_rn(o[0], r[0]), __dsub_rn(o[1], r[1]), __dsub_rn(o[2], r[2])};
+  d = xwrapd(d, sd);
+  return __dadd_rn(__dadd_rn(__dmul_rn(d.x, d.x), __dmul_rn(d.y, d.y)), __dmul_rn(d.z, d.z));
+}
 __device__ __forceinline__ double disp_from(SysDescC &sd, const double o[3], const double r[3]) {
 #pragma clang fp contract(off)
   v3<double> d{__dsub_rn(o[0], r[0]), __dsub_rn(o[1], r[1]), __dsub_rn(o[2], r[2])};

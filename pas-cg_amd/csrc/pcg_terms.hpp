@@ -218,6 +218,10 @@ template <typename R, typename G, typename A>
 __device__ double dh_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, int k, int variant) {
   ParamsC &p = b.P->p;
   int i1 = b.dh_i[0][k], i2 = b.dh_i[1][k];
+  double q1q2 = b.dh_q[k];
+  keep(i1);
+  keep(i2);
+  keep(q1q2);
   v3<double> d = wrapd(g.r(i2) - g.r(i1), s);
   double rnd = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
   if (rnd > p.dh_cutoff) return 0.0;
@@ -225,10 +229,10 @@ __device__ double dh_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, in
   R rn = (R)rnd, rinv = rcpR(rn), sinv = (R)b.P->screen_inv;
   R Vij, dV;
   if (variant == 0) {
-    Vij = (R)b.P->V_factor * (R)b.dh_q[k] * expR(-rn * sinv) * rinv;
+    Vij = (R)b.P->V_factor * (R)q1q2 * expR(-rn * sinv) * rinv;
     dV = -Vij * (sinv + rinv);
   } else {
-    Vij = (R)b.P->V_factor * (R)b.dh_q[k] * expR(-rn * sinv) * rinv * rinv;
+    Vij = (R)b.P->V_factor * (R)q1q2 * expR(-rn * sinv) * rinv * rinv;
     dV = -Vij * (sinv + (R)2 * rinv);
   }
   v3<R> f = (dV * rinv) * r12;
@@ -759,10 +763,15 @@ __device__ double qa_cont_item(const DevBatch &b, SysDescC &s, SysState &st, con
   DevParamsC &P = *b.P;
   ParamsC &p = P.p;
   int slot = s.qc0 + k;
-  if (b.qc_vac[slot]) return 0.0;
-  int i1 = b.qc_i[0][slot], i2 = b.qc_i[1][slot];
-  int type = b.qc_t[slot];
+  // every field loaded before the first branch
+  int vac = b.qc_vac[slot], i1 = b.qc_i[0][slot], i2 = b.qc_i[1][slot], type = b.qc_t[slot];
   double sat = b.qc_sat[slot];
+  keep(vac);
+  keep(i1);
+  keep(i2);
+  keep(type);
+  keep(sat);
+  if (vac) return 0.0;
   const double sat_used = sat;
   v3<double> d = xwrapd(xsub(g.r(i2), g.r(i1)), s);
   double n2 = __dadd_rn(__dadd_rn(__dmul_rn(d.x, d.x), __dmul_rn(d.y, d.y)), __dmul_rn(d.z, d.z));
