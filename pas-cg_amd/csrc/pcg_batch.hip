@@ -127,7 +127,8 @@ __device__ __forceinline__ void block_ranks5(int cls, int valid, int counts[5], 
 
 
 // NL rebuild + ordered partition inside the workgroup.
-__device__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysState &st, const GeoL &g) {
+// (inlined: as a call it would make the kernel save every live register)
+__device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysState &st, const GeoL &g) {
 #pragma clang fp contract(off)
   const double req = b.P->nl_eff_cutoff + b.P->p.nl_pad;
   int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -713,7 +714,9 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
       term_loop<PCG_CND, R>(b, sd, st, g, acc, sizes, vacc, wv);
       term_loop<PCG_SND, R>(b, sd, st, g, acc, sizes, vacc, wv);
     }
-    nat_cont_loop<4, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    // three slots a thread: the largest proteins (some 2600 contacts on 1024
+    // threads) fill them in one pass
+    nat_cont_loop<3, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_PAULI, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_CONST_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_REL_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
