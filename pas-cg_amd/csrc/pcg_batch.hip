@@ -477,7 +477,7 @@ __device__ __forceinline__ void nat_cont_loop(const DevBatch &b, SysDescC &sd, S
           if (act[j]) {
             acc.add(r0 + (int)(ijt[j] & 0xFFFu), f[j]);
             acc.add(r0 + (int)((ijt[j] >> 12) & 0xFFFu), -f[j]);
-            v += (double)Vj[j];
+            if (want_v) v += (double)Vj[j];
             if ((uint32_t)now[j] != (ijt[j] >> 31)) {
               // a transition (rare): the contact's index into the list of all
               const int kk = sd.nc0 + k + j * B, c = b.ncur[kk];
@@ -654,48 +654,58 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
       for (int j = 0; j < 15; ++j) ys[j * n + k] = b.y[1 + j / 3][j % 3][r0 + k];
   __syncthreads();
 
-  __shared__ int want_v;
-  bool evaluated = false;
-  for (int it = 0; it < nsteps; ++it) {
-    if (threadIdx.x == 0) {
-      stop = st.step >= target;
-      // energies on the last step this launch makes for the system
-      want_v = !b.energy_off && ((it == nsteps - 1) || (st.step + 1 >= target));
-      // nl::verify (cg/src/cg/nl/verify.cpp:3-34)
-      if (!stop && P.nl_required) {
-        if (st.verify_first) {
-          st.verify_first = 0;
-          st.nl_invalid = 1;
-        } else {
-          double maxd = __longlong_as_double((long long)st.max_disp_bits);
-          double box = fabs(sd.cell[0] - st.orig_cell[0]) + fabs(sd.cell[1] - st.orig_cell[1]) +
-                       fabs(sd.cell[2] - st.orig_cell[2]);
-          st.nl_invalid = __dadd_rn(__dmul_rn(2.0, maxd), box) > st.orig_pad ? 1 : 0;
-        }
-      }
-      flag_rebuild = st.nl_invalid;
-      for (int k = 0; k < NT; ++k) vacc[k] = 0.0;
+  __shared__ int want_v, last;
+  // term sizes of the flat index space
+  auto set_sizes = [&]() {
+    int acc_n = 0;
+    for (int t = 0; t < NT; ++t) {
+      sizes[t] = acc_n;
+      acc_n += term_on(P, t) ? term_size(sd, st, t) : 0;
     }
-    __syncthreads();
+    sizes[NT] = acc_n;
+  };
+  // The prologue of step it, by thread 0 at the end of the step before (one
+  // barrier a step): the stop test, the energy flag, nl::verify and, without
+  // a rebuild, the term sizes.
+  auto prologue = [&](int it) {
+    stop = st.step >= target;
+    // the last step this launch makes for the system: energies (unless
+    // off) and the write-back of its forces
+    last = (it == nsteps - 1) || (st.step + 1 >= target);
+    want_v = !b.energy_off && last;
+    // nl::verify (cg/src/cg/nl/verify.cpp:3-34)
+    if (!stop && P.nl_required) {
+      if (st.verify_first) {
+        st.verify_first = 0;
+        st.nl_invalid = 1;
+      } else {
+        double maxd = __longlong_as_double((long long)st.max_disp_bits);
+        double box = fabs(sd.cell[0] - st.orig_cell[0]) + fabs(sd.cell[1] - st.orig_cell[1]) +
+                     fabs(sd.cell[2] - st.orig_cell[2]);
+        st.nl_invalid = __dadd_rn(__dmul_rn(2.0, maxd), box) > st.orig_pad ? 1 : 0;
+      }
+    }
+    flag_rebuild = st.nl_invalid;
+    for (int k = 0; k < NT; ++k) vacc[k] = 0.0;
+    if (!flag_rebuild) set_sizes();
+  };
+  // forces start at zero; afterwards each step's Langevin pass clears them
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    fx[k] = 0;
+    fy[k] = 0;
+    fz[k] = 0;
+  }
+  if (threadIdx.x == 0) prologue(0);
+  __syncthreads();
+  for (int it = 0; it < nsteps; ++it) {
     // a finished system keeps the forces of its last evaluation
     if (stop) break;
-    evaluated = true;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-      fx[k] = 0;
-      fy[k] = 0;
-      fz[k] = 0;
-    }
     // (n and h of qa::prepare_nh are recomputed where a QA candidate needs them)
-    if (flag_rebuild) wg_fix_nl(b, sd, st, g);
-    if (threadIdx.x == 0) {
-      int acc_n = 0;
-      for (int t = 0; t < NT; ++t) {
-        sizes[t] = acc_n;
-        acc_n += term_on(P, t) ? term_size(sd, st, t) : 0;
-      }
-      sizes[NT] = acc_n;
+    if (flag_rebuild) {
+      wg_fix_nl(b, sd, st, g);
+      if (threadIdx.x == 0) set_sizes();
+      __syncthreads();
     }
-    __syncthreads();
     // every term's items over one flat index space (term order preserved,
     // a thread moves on to the next term without waiting); one loop per term
     // so each keeps only its own constants in registers
@@ -755,6 +765,18 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
         maxd = fmax(maxd, disp2_from(sd, o, r));  // squared; the root is taken once below
       }
     }
+    // the forces of the last step go to HBM; then every force restarts at 0
+    const bool wl = last;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      if (wl) {
+        ((RF *)b.F[0])[r0 + k] = (RF)fx[k];
+        ((RF *)b.F[1])[r0 + k] = (RF)fy[k];
+        ((RF *)b.F[2])[r0 + k] = (RF)fz[k];
+      }
+      fx[k] = 0;
+      fy[k] = 0;
+      fz[k] = 0;
+    }
     maxd = block_max(maxd);
     if (threadIdx.x == 0) {
       st.max_disp_bits = (unsigned long long)__double_as_longlong(__dsqrt_rn(maxd));
@@ -771,6 +793,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
         st.t = st.true_t;
         st.step += 1;
       }
+      prologue(it + 1);
     }
     __syncthreads();
   }
@@ -794,12 +817,6 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     }
   if (P.p.qa_on)
     for (int k = threadIdx.x; k < n; k += blockDim.x) *(uint32_t *)(b.sync + 4 * (r0 + k)) = syl[k];
-  if (!evaluated) return;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    ((RF *)b.F[0])[r0 + k] = (RF)fx[k];
-    ((RF *)b.F[1])[r0 + k] = (RF)fy[k];
-    ((RF *)b.F[2])[r0 + k] = (RF)fz[k];
-  }
 }
 
 // The launch bound sets the register budget: 1024 threads leave 128 VGPRs
