@@ -191,11 +191,15 @@ int pcg_get_energy(pcg_ctx *c, double *V, double *term_V /*[PCG_NUM_TERMS]*/);
  * by pcg_report_field: time, potential, total and kinetic energy, radius of
  * gyration, end-to-end distance, RMSD to the original positions after the
  * optimal superposition, asphericity, active native contacts (all and
- * disulfide) and QA contacts between and within chains. The PID activity
- * counts (pid/count_active.cpp) are not among them. */
+ * disulfide) and QA contacts between and within chains, then the PID
+ * activity counts of pid::count_active (cg/src/cg/pid/count_active.cpp:4-164,
+ * columns PID-INTRHC, PID-B1-B1, PID-B1-S1, PID-INTEHC, PID-B1-B2, PID-S1-S2
+ * of make_report.cpp:253-261; zero when PID is off). */
 typedef enum {
   PCG_REP_TIME = 0, PCG_REP_EPOT, PCG_REP_ETOT, PCG_REP_EKIN, PCG_REP_RG, PCG_REP_L,
   PCG_REP_RMSD, PCG_REP_W, PCG_REP_ICN, PCG_REP_ICNSS, PCG_REP_INTEHC, PCG_REP_INTRHC,
+  PCG_REP_PID_INTRHC, PCG_REP_PID_B1B1, PCG_REP_PID_B1S1,
+  PCG_REP_PID_INTEHC, PCG_REP_PID_B1B2, PCG_REP_PID_S1S2,
   PCG_REPORT_NUM
 } pcg_report_field;
 int pcg_report(pcg_ctx *c, double *out /*[PCG_REPORT_NUM]*/);

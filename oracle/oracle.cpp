@@ -704,6 +704,71 @@ struct ora_sys {
     }
   }
 
+  /* pid::count_active (cg/src/cg/pid/count_active.cpp:4-164), the report's
+   * PID counts over the current bundles: out = {intra, intra_bb, intra_ss,
+   * inter, inter_bb, inter_ss}. Unlike the force evaluation it skips a
+   * bundle when 1/|rm| or 1/|rn| exceeds 10, keeps pairs with
+   * r12 <= cutoff, takes acos of the unclamped cosine, and counts a lambda
+   * pair whose product exceeds 5e-5 when r12 < r_high * 2^(-1/6) * factor. */
+  void improper_psi(int i, int j, double &psi, bool &skip) const {
+    int ip = i - 1, in = i + 1;
+    v3 zero{0, 0, 0};
+    v3 r_in = in < n ? r[in] : zero, r_ip = ip >= 0 ? r[ip] : zero;
+    v3 rij = r[i] - r_in;
+    v3 rkj = ip >= 0 ? (r_ip - r_in) : zero;
+    v3 rkl = r_ip - r[j];
+    v3 rm = cross(rij, rkj), rn = cross(rkj, rkl);
+    double rm_ninv = norm_inv(rm), rn_ninv = norm_inv(rn);
+    skip = rm_ninv > 10.0 || rn_ninv > 10.0;
+    psi = std::acos(dot(rm, rn) * rm_ninv * rn_ninv);
+    if (dot(rij, rn) < 0.0) psi = -psi;
+  }
+  void pid_counts(int32_t out[6]) const {
+    for (int k = 0; k < 6; ++k) out[k] = 0;
+    const double factor = p.counting_factor, cutoff = p.nl_cutoff;
+    for (auto const &b : pid_bundles) {
+      int i1 = b.i1, i2 = b.i2;
+      if (atype[i1] == 14 || atype[i2] == 14) continue; /* PRO */
+      double r12_n = std::sqrt(norm_sq(pbc.wrap(r[i1], r[i2])));
+      if (r12_n > cutoff) continue;
+      double psi1, psi2;
+      bool skip1, skip2;
+      improper_psi(i1, i2, psi1, skip1);
+      if (skip1) continue;
+      improper_psi(i2, i1, psi2, skip2);
+      if (skip2) continue;
+      bool sep3 = (i2 - i1 == 3);
+      bool opt1 = lam_supp(p.pid_bbp_alpha, p.pid_bbp_psi0, psi1) || sep3;
+      bool opt2 = !sep3 && lam_supp(p.pid_bbp_alpha, p.pid_bbp_psi0, psi2);
+      double l1a = opt1 ? p.pid_bbp_alpha : p.pid_bbm_alpha;
+      double l1p = opt1 ? p.pid_bbp_psi0 : p.pid_bbm_psi0;
+      double l2a = opt2 ? p.pid_bbp_alpha : p.pid_bbm_alpha;
+      double l2p = opt2 ? p.pid_bbp_psi0 : p.pid_bbm_psi0;
+      double bb_r_high = opt2 ? p.pid_bbp_lj[2] : p.pid_bbm_lj[2];
+      bool same = chain[i1] == chain[i2];
+      if (lam_supp(l1a, l1p, psi1) && lam_supp(l2a, l2p, psi2)) {
+        double lam1, der1, lam2, der2;
+        lam_eval(l1a, l1p, psi1, lam1, der1);
+        lam_eval(l2a, l2p, psi2, lam2, der2);
+        if (lam1 * lam2 > 5e-5 && r12_n < bb_r_high * C216_INV * factor) {
+          ++out[same ? 0 : 3];
+          ++out[same ? 1 : 4];
+        }
+      }
+      double sa = p.pid_ss_alpha, sp = p.pid_ss_psi0;
+      if (lam_supp(sa, sp, psi1) && lam_supp(sa, sp, psi2)) {
+        double lam1, der1, lam2, der2;
+        lam_eval(sa, sp, psi1, lam1, der1);
+        lam_eval(sa, sp, psi2, lam2, der2);
+        if (lam1 * lam2 > 5e-5 && p.pid_ss_depth[b.type] > 5e-5 &&
+            r12_n < p.pid_ss_r_high[b.type] * C216_INV * factor) {
+          ++out[same ? 0 : 3];
+          ++out[same ? 2 : 5];
+        }
+      }
+    }
+  }
+
   /* qa::prepare_nh::iter, cg/src/cg/qa/prepare_nh.cpp:10-19 */
   void prepare_nh() {
     for (int i = 0; i < n; ++i) {
@@ -1614,6 +1679,7 @@ void ora_sync(const ora_sys *s, int8_t *out) {
       out[4 * i + k] = s->sync.empty() ? 0 : s->sync[i].v[k];
 }
 int32_t ora_nl_rebuilds(const ora_sys *s) { return s->nl_rebuilds; }
+void ora_pid_counts(const ora_sys *s, int32_t *out) { s->pid_counts(out); }
 int64_t ora_saw_draws(const ora_sys *s) { return s->saw_draws; }
 
 struct ora_rng {

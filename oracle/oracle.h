@@ -206,6 +206,9 @@ int32_t ora_num_changed(const ora_sys *s);
 int32_t ora_num_qa_contacts(const ora_sys *s);
 void ora_sync(const ora_sys *s, int8_t *sync /*[4n]*/);
 int32_t ora_nl_rebuilds(const ora_sys *s);
+/* pid::count_active over the current PID bundles and positions:
+ * {intra, intra_bb, intra_ss, inter, inter_bb, inter_ss} */
+void ora_pid_counts(const ora_sys *s, int32_t *out /*[6]*/);
 int64_t ora_saw_draws(const ora_sys *s); /* ran2 draws the SAW consumed */
 
 /* The ran2 generator by itself (cg/include/cg/utils/random/nr.h). */

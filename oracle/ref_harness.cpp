@@ -28,6 +28,7 @@
 #include <cg/nl/verify.h>
 #include <cg/pauli/eval_forces.h>
 #include <cg/pauli/update_pairs.h>
+#include <cg/pid/count_active.h>
 #include <cg/pid/eval_forces.h>
 #include <cg/pid/update_bundles.h>
 #include <cg/qa/finish_processing.h>
@@ -336,24 +337,8 @@ double ref_heur_dih(int n, const double *r, int k, const int *q,
   return V;
 }
 
-double ref_pid(int n, const double *r, const double *cell, const uint8_t *atype,
-               int k, const int *ij, const double *orig_dist, const int *type,
-               const ora_params *p, double *F) {
-  auto R = to_vec(n, r);
-  vect::vector<vec3r> Fv(n);
-  auto at = to_aa(n, atype);
-  vect::vector<pid::bundle> bundles;
-  for (int a = 0; a < k; ++a)
-    bundles.emplace_back(ij[2 * a], ij[2 * a + 1], orig_dist[a], (int16_t)type[a]);
-  auto box = make_pbc(cell);
-  vect::vector<int> prev(n, -1), next(n, -1);
-  real V = 0, total_disp = 0;
-  pid::eval_forces e;
-  e.V = &V;
-  e.F = Fv;
-  e.total_disp = &total_disp;
-  e.active_thr = p->counting_factor;
-  e.atype = at;
+// the lambdas and sink LJs of pid::eval_forces (thread_setup.cpp:589-686)
+static void pid_setup(pid::eval_forces &e, const ora_params *p) {
   auto ver = p->pid_lambda_variant == 0 ? pid::COSINE : pid::ALGEBRAIC;
   e.bb_plus_lam.version() = ver;
   e.bb_plus_lam.alpha() = p->pid_bbp_alpha;
@@ -369,15 +354,68 @@ double ref_pid(int n, const double *r, const double *cell, const uint8_t *atype,
   e.ss_ljs = vect::vector<sink_lj>(441);
   for (int t = 0; t < 441; ++t)
     e.ss_ljs[t] = sink_lj(p->pid_ss_depth[t], p->pid_ss_r_low[t], p->pid_ss_r_high[t]);
+  e.cutoff = p->nl_cutoff;
+}
+
+double ref_pid(int n, const double *r, const double *cell, const uint8_t *atype,
+               int k, const int *ij, const double *orig_dist, const int *type,
+               const ora_params *p, double *F) {
+  auto R = to_vec(n, r);
+  vect::vector<vec3r> Fv(n);
+  auto at = to_aa(n, atype);
+  vect::vector<pid::bundle> bundles;
+  for (int a = 0; a < k; ++a)
+    bundles.emplace_back(ij[2 * a], ij[2 * a + 1], orig_dist[a], (int16_t)type[a]);
+  auto box = make_pbc(cell);
+  vect::vector<int> prev(n, -1), next(n, -1);
+  real V = 0, total_disp = 0;
+  pid::eval_forces e;
+  pid_setup(e, p);
+  e.V = &V;
+  e.F = Fv;
+  e.total_disp = &total_disp;
+  e.active_thr = p->counting_factor;
+  e.atype = at;
   e.r = R;
   e.simul_box = &box;
   e.bundles = &bundles;
   e.prev = prev;
   e.next = next;
-  e.cutoff = p->nl_cutoff;
   run(e);
   from_vec(Fv, F);
   return V;
+}
+
+// pid::count_active (cg/src/cg/pid/count_active.cpp:4-164) over the given
+// bundles: out = {intra, intra_bb, intra_ss, inter, inter_bb, inter_ss}
+void ref_pid_count(int n, const double *r, const double *cell, const uint8_t *atype,
+                   const int *chain, int k, const int *ij, const double *orig_dist,
+                   const int *type, const ora_params *p, int *out) {
+  auto R = to_vec(n, r);
+  auto at = to_aa(n, atype);
+  vect::vector<pid::bundle> bundles;
+  for (int a = 0; a < k; ++a)
+    bundles.emplace_back(ij[2 * a], ij[2 * a + 1], orig_dist[a], (int16_t)type[a]);
+  auto box = make_pbc(cell);
+  vect::vector<int> ch(n);
+  for (int i = 0; i < n; ++i) ch[i] = chain[i];
+  pid::eval_forces e;
+  pid_setup(e, p);
+  e.atype = at;
+  e.r = R;
+  e.simul_box = &box;
+  e.bundles = &bundles;
+  pid::count_active c;
+  c.eval = &e;
+  c.chain_idx = ch;
+  c.count_factor = p->counting_factor;
+  auto cnt = c();
+  out[0] = cnt.intra;
+  out[1] = cnt.intra_bb;
+  out[2] = cnt.intra_ss;
+  out[3] = cnt.inter;
+  out[4] = cnt.inter_bb;
+  out[5] = cnt.inter_ss;
 }
 
 double ref_vel_afm(int n, const double *r, double t, int k, const int *res,

@@ -118,6 +118,8 @@ struct ReportSums {
   double H[9];       // sum of (q - q0)(r - c)^T, row-major (orig^T cur)
   double pp, qq;     // sums of |r - c|^2 and |q - q0|^2
   double icn, icnss, inter, intra;
+  int pid[6];        // pid::count_active: intra, intra bb, intra ss, inter, inter bb, inter ss
+  int pad_;
 };
 
 // Device arrays of a batch (pointers into one allocation).

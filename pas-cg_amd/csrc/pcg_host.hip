@@ -844,6 +844,7 @@ void report_scalars(const ReportSums &o, const SysState &st, double *out) {
   out[PCG_REP_ICNSS] = o.icnss;
   out[PCG_REP_INTEHC] = o.inter;
   out[PCG_REP_INTRHC] = o.intra;
+  for (int k = 0; k < 6; ++k) out[PCG_REP_PID_INTRHC + k] = o.pid[k];
 }
 
 int report_into(DeviceBatch &B, double *out) {
@@ -851,7 +852,7 @@ int report_into(DeviceBatch &B, double *out) {
   ReportSums *d = nullptr;
   if (hipMalloc(&d, ns * sizeof(ReportSums)) != hipSuccess)
     return fail(PCG_E_NOMEM, "no device memory for the report sums");
-  launch_report(B.d, d, B.stream);
+  launch_report(B.d, B.hp.p.pid_on, d, B.stream);
   std::vector<ReportSums> h(ns);
   std::vector<SysState> st(ns);
   hipError_t e1 = hipMemcpyAsync(h.data(), d, ns * sizeof(ReportSums), hipMemcpyDeviceToHost, B.stream);

@@ -19,5 +19,5 @@ void launch_persistent(const DevBatch &b, int prec, int block, int nsteps, size_
 void launch_persistent_jobs(const DevBatch *jobs, JobSlot *slots, int njobs, int prec, int block,
                             int nsteps, size_t lds_bytes, hipStream_t s);
 // report sums of every system (pcg_report.hip)
-void launch_report(const DevBatch &b, ReportSums *out, hipStream_t s);
+void launch_report(const DevBatch &b, int pid_on, ReportSums *out, hipStream_t s);
 } // namespace pcg

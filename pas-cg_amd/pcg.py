@@ -21,7 +21,8 @@ TERMS = ["chir", "tether", "lrep", "nat_cont", "pauli", "const_dh", "rel_dh",
 TERM_ID = {t: i for i, t in enumerate(TERMS)}
 # pcg_report_field order: the scalars of make_report::add_cur_scalars
 REPORT_FIELDS = ["TIME", "EPOT", "ETOT", "EKIN", "RG", "L", "RMSD", "W", "ICN", "ICNss",
-                 "INTEHC", "INTRHC"]
+                 "INTEHC", "INTRHC", "PID-INTRHC", "PID-B1-B1", "PID-B1-S1", "PID-INTEHC",
+                 "PID-B1-B2", "PID-S1-S2"]
 MIXED, FP64 = 0, 1
 PHILOX, RAN2 = 0, 1
 

@@ -139,6 +139,7 @@ def lib():
             getattr(L, f).restype = I
             getattr(L, f).argtypes = [P]
         L.ora_sync.argtypes = [P, C.POINTER(C.c_int8)]
+        L.ora_pid_counts.argtypes = [P, C.POINTER(I)]
         L.ora_saw_draws.restype = C.c_int64
         L.ora_saw_draws.argtypes = [P]
         L.ora_static_list.restype = I
@@ -349,6 +350,13 @@ class Oracle:
     def sync(self) -> np.ndarray:
         out = np.zeros((self.n, 4), np.int8)
         self.L.ora_sync(self.h, _ptr(out, C.c_int8))
+        return out
+
+    def pid_counts(self) -> np.ndarray:
+        """{intra, intra_bb, intra_ss, inter, inter_bb, inter_ss} of the
+        current PID bundles at the current positions."""
+        out = np.zeros(6, np.int32)
+        self.L.ora_pid_counts(self.h, _ptr(out, I))
         return out
 
 

@@ -50,6 +50,8 @@ def lib():
         L.ref_fix_nl.argtypes = [I, PD, PD, PU8, PI, PI, PP, I, PI, PD, I, PI, PI, PD,
                                  PU8, PI, PI, PI, PI, PI]
         L.ref_fix_nl.restype = None
+        L.ref_pid_count.argtypes = [I, PD, PD, PU8, PI, I, PI, PD, PI, PP, PI]
+        L.ref_pid_count.restype = None
         _lib = L
     return _lib
 

@@ -350,3 +350,43 @@ def test_qa_round_matches_reference():
         checked += k
         # keep the trajectory going from the oracle's state
     assert checked > 0, "no QA contacts formed"
+
+
+def _ref_pid_counts(o, prm, m):
+    i1, i2, od, _ = o.list(4)
+    at = np.ascontiguousarray(m["atype"], np.uint8)
+    ij = np.ascontiguousarray(np.stack([i1, i2], 1).astype(np.int32))
+    ty = np.ascontiguousarray((at[i1].astype(np.int32) * 21 + at[i2]).astype(np.int32))
+    ch = np.ascontiguousarray(m["chain"], np.int32)
+    r = np.ascontiguousarray(o.r())
+    cell = np.ascontiguousarray(m["cell"], np.float64)
+    out = np.zeros(6, np.int32)
+    R.lib().ref_pid_count(o.n, ptr(r, D), ptr(cell, D), ptr(at, U8), ptr(ch, I), len(ij),
+                          ptr(ij, I), ptr(np.ascontiguousarray(od), D), ptr(ty, I),
+                          C.byref(R.params_struct(prm)), ptr(out, I))
+    return out
+
+
+@pytest.mark.parametrize("variant", ["lj-algebraic", "sink-cosine"])
+def test_pid_activity_counts(variant):
+    """pid::count_active of the report: the oracle's counts over its current
+    bundles equal the reference's on the same positions and bundles, with
+    intra- and inter-chain, backbone and side-chain counts all exercised."""
+    prm = full_params(qa_on=0, nat_cont_on=0)
+    if variant == "sink-cosine":
+        prm["pid_lambda_variant"] = 0
+        P.set_pid_forces(prm, "sink lj", bbp_depth=0.3, bbm_depth=0.25,
+                         ss_depth_table=[[0.1 + 0.01 * ((a * 7 + b) % 13) for b in range(21)]
+                                         for a in range(21)])
+    m = multichain_box()
+    o = Oracle(prm, m, seed=11)
+    seen = np.zeros(6, np.int64)
+    for _ in range(2):
+        o.step(3)
+        got, want = o.pid_counts(), _ref_pid_counts(o, prm, m)
+        assert (got == want).all(), (got, want)
+        assert got[0] == got[1] + got[2] and got[3] == got[4] + got[5]
+        seen += got
+    assert seen[1] > 0 and seen[2] > 0
+    if variant == "lj-algebraic":
+        assert seen[4] > 0 and seen[5] > 0

@@ -6,6 +6,14 @@ quaternion solution (an eigenvector of a 4x4 matrix, no SVD), Rg of a cube's
 corners, asphericity of three collinear points, and a mirror image of a
 chiral set keeps a non-zero RMSD (no reflections, d = sign det).
 GPU: pcg_report and pcg_batch_report against the oracle on live states.
+
+PID activity counts (pid/count_active.cpp:4-164): the oracle's count is
+pinned against the reference's own count_active in test_oracle_vs_ref.py.
+The device counts every pair of a system that the list rules admit instead
+of walking the PID list; the CPU test here pins the fact that makes the two
+equal (the count over a list rebuilt some steps ago equals the count over a
+list rebuilt at the current positions), and the GPU tests compare the
+device's counts with the oracle's on the same positions.
 """
 import os
 import sys
@@ -129,4 +137,94 @@ def test_batch_report_matches_oracle():
         for k, w in want.items():
             assert abs(reps[s][k] - w) <= 1e-9 * max(abs(w), 1.0), (s, k, reps[s][k], w)
         assert 0 <= reps[s]["ICN"] <= len(m["cont"])
+    b.close()
+
+
+def _pid_params(**kw):
+    """PID on with the sink-LJ forces and a side-chain depth table, which stay
+    stable on compact synthetic chains; QA off (its free pairs would take the
+    PID pairs)."""
+    prm = P.default_params()
+    prm.update(pid_on=1, qa_on=0, pid_lambda_variant=0)
+    P.set_pid_forces(prm, "sink lj", bbp_depth=0.3, bbm_depth=0.25,
+                     ss_depth_table=[[0.1 + 0.01 * ((a * 7 + b) % 13) for b in range(21)]
+                                     for a in range(21)])
+    prm.update(kw)
+    return prm
+
+
+def _three_chains(seed=5):
+    one = M.synthetic_protein(40, M.job_seed(seed))
+    return M.replicate(one, 3, 45.0 * P.ANGSTROM, seed)
+
+
+@pytest.mark.parametrize("nat_cont_on", [1, 0])
+def test_pid_counts_do_not_depend_on_list_age(nat_cont_on):
+    from oracle_ffi import Oracle
+    o = Oracle(_pid_params(nat_cont_on=nat_cont_on), _three_chains(), seed=3)
+    seen = np.zeros(6, np.int64)
+    for _ in range(4):
+        o.step(80)
+        aged = o.pid_counts()
+        o.fix_nl()                     # rebuild at the current positions
+        fresh = o.pid_counts()
+        assert (aged == fresh).all(), (aged, fresh)
+        seen += fresh
+    assert seen[0] > 0 and seen[3] > 0 and seen[2] > 0
+
+
+def _oracle_pid_counts(prm, m, r):
+    from oracle_ffi import Oracle
+    o = Oracle(prm, m, seed=1)
+    o.set_r(r)
+    o.fix_nl()
+    return o.pid_counts()
+
+
+PID_KEYS = ["PID-INTRHC", "PID-B1-B1", "PID-B1-S1", "PID-INTEHC", "PID-B1-B2", "PID-S1-S2"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nat_cont_on", [1, 0])
+def test_context_pid_counts_match_oracle(nat_cont_on):
+    """The six PID counts of pcg_report on a periodic three-chain box, at
+    several points of an FP64 trajectory, equal the oracle's count_active on
+    the same positions."""
+    from pas_cg_amd import pcg as G
+    m = _three_chains()
+    prm = _pid_params(nat_cont_on=nat_cont_on)
+    c = G.Context(prm, m, precision=G.FP64, noise=G.PHILOX, seed=7)
+    seen = np.zeros(6)
+    for _ in range(3):
+        c.step(150)
+        got = np.array([c.report()[k] for k in PID_KEYS])
+        want = _oracle_pid_counts(prm, m, c.r())
+        assert (got == want).all(), (got, want)
+        seen += got
+    assert seen[0] > 0 and seen[3] > 0
+    c.close()
+
+
+@pytest.mark.gpu
+def test_batch_pid_counts_match_oracle():
+    """pcg_batch_report's PID counts of every system of a ragged mixed batch
+    (one periodic multichain system among single chains) equal the oracle's
+    on the same positions; PID off gives zeros."""
+    from pas_cg_amd import pcg as G
+    prm = _pid_params()
+    models = [M.synthetic_protein(n, M.job_seed(60 + n)) for n in (33, 150)] + [_three_chains(8)]
+    b = G.Batch(prm, models, precision=G.MIXED, steps_per_launch=100)
+    b.run(300)
+    reps = b.report()
+    for s, m in enumerate(models):
+        got = np.array([reps[s][k] for k in PID_KEYS])
+        want = _oracle_pid_counts(prm, m, b.get(s, 0))
+        assert (got == want).all(), (s, got, want)
+    assert sum(reps[s]["PID-INTRHC"] for s in range(3)) > 0
+    assert reps[2]["PID-INTEHC"] > 0
+    b.close()
+    off = dict(prm, pid_on=0)
+    b = G.Batch(off, models[:1], precision=G.MIXED, steps_per_launch=50)
+    b.run(50)
+    assert all(b.report()[0][k] == 0 for k in PID_KEYS)
     b.close()
