@@ -315,6 +315,30 @@ int pcg_server_job_times(pcg_server *s, int64_t job, double *submit, double *sta
 int pcg_server_stats(pcg_server *s, int64_t *resident_bytes, int32_t *resident,
                      int32_t *queued);
 
+/* ------------------------------------------------------ input preparation
+ * model::morph_into_saw (cg/src/cg/input/model.cpp:108-234): every chain is
+ * redrawn as a self-avoiding walk of fixed bond length from the ran2
+ * generator of `seed`, up to max_tries times per chain until no residues
+ * i, j >= i + 3 come closer than `intersection`. Chains are the runs of equal
+ * chain_idx (contiguous, in index order, as state::compile_model lays them
+ * out). The start point of each chain is drawn in the cube of side
+ * box_size centred at the origin (box_size 0 and density > 0: side
+ * cbrt(n / density); both 0: the origin); pbc wraps the intersection test in
+ * that cube. pos receives 3n coordinates; draws the number of ran2 draws
+ * used, which pcg_options.ran2_skip takes so that the initial velocities
+ * and the noise continue the same stream, as in the reference. Host code;
+ * no GPU is touched. */
+typedef struct pcg_saw_options {
+  double bond;          /* bond length (> 0) */
+  double intersection;  /* minimum distance of residues three or more apart */
+  double box_size;
+  double density;
+  int32_t pbc;
+  int32_t max_tries;    /* <= 0: 9000, the reference's */
+} pcg_saw_options;
+int pcg_morph_saw(const pcg_saw_options *o, int32_t n, const int32_t *chain_idx, int32_t seed,
+                  double *pos /*[3n]*/, int64_t *draws);
+
 /* ------------------------------------------------------------ measurement
  * No counterpart in the reference. A streaming copy of `bytes` (read once,
  * written once) timed over `iters` launches: the achievable HBM bandwidth

@@ -2019,4 +2019,100 @@ int pcg_server_job_times(pcg_server *s, int64_t job, double *submit, double *sta
   return PCG_OK;
 }
 
+
+// ------------------------------------------------------ input preparation ----
+// model::morph_into_saw (cg/src/cg/input/model.cpp:108-234). Each attempt
+// draws the chain's bend and torsion angles, chains the local frames
+// T_0 .. T_{n-2} (rotation i applied to the bond of direction (theta0, phi0)
+// after rotations i-1 .. 0), places the chain at a point drawn in the start
+// box and accepts it when no pair i, j >= i + 3 is closer than the
+// intersection distance.
+int pcg_morph_saw(const pcg_saw_options *o, int32_t n, const int32_t *chain_idx, int32_t seed,
+                  double *pos, int64_t *draws) {
+  if (!o || (n > 0 && (!chain_idx || !pos))) return fail(PCG_E_ARG, "null argument");
+  if (n < 0) return fail(PCG_E_ARG, "negative residue count");
+  if (!(o->bond > 0.0)) return fail(PCG_E_ARG, "SAW bond length must be positive");
+  for (int i = 1; i < n; ++i)
+    if (chain_idx[i] < chain_idx[i - 1])
+      return fail(PCG_E_ARG, "chains must be contiguous and in chain-index order");
+  double side = 0.0;
+  if (o->box_size > 0.0) side = o->box_size;
+  else if (o->density > 0.0) side = std::cbrt((double)n / o->density);
+  if (o->pbc && !(side > 0.0))
+    return fail(PCG_E_ARG, "SAW with periodic boundaries needs a non-zero start box");
+  const int tries = o->max_tries > 0 ? o->max_tries : 9000;
+  const double lo = -side / 2.0, hi = side / 2.0, span = hi - lo;
+  const double cell = o->pbc ? span : 0.0, cell_inv = o->pbc ? 1.0 / span : 0.0;
+  const double min_d2 = std::pow(o->intersection, 2.0);
+  const double pi = std::acos(-1.0);
+  Ran2 g(seed);
+  int64_t used = 0;
+  auto u = [&]() {
+    ++used;
+    return g.draw();
+  };
+  auto wrap = [&](double d) { return d - cell * std::round(d * cell_inv); };
+  std::vector<double> phi, theta, T, R;
+  for (int a = 0; a < n;) {
+    int b = a;
+    while (b < n && chain_idx[b] == chain_idx[a]) ++b;
+    const int m = b - a;
+    phi.assign(m + 1, 0.0);
+    theta.assign(m + 1, 0.0);
+    T.assign(9 * (size_t)m, 0.0);
+    R.assign(3 * (size_t)m, 0.0);
+    for (int attempt = 0; attempt < tries; ++attempt) {
+      phi[0] = pi / 2.0;
+      theta[0] = 0.0;
+      phi[1] = 0.0;
+      theta[1] = u() * pi / 3.0;  // drawn for every chain, one residue long too
+      for (int i = 2; i < m - 1; ++i) {
+        phi[i] = (2.0 * u() - 1.0) * pi;
+        theta[i] = u() * pi / 3.0;
+      }
+      for (int i = 0; i < m - 1; ++i) {
+        const double ct = std::cos(theta[i]), st = std::sin(theta[i]);
+        const double cp = std::cos(phi[i]), sp = std::sin(phi[i]);
+        double *M = &T[9 * (size_t)i];
+        M[0] = ct, M[1] = st, M[2] = 0.0;
+        M[3] = st * cp, M[4] = -ct * cp, M[5] = sp;
+        M[6] = st * sp, M[7] = -ct * sp, M[8] = -cp;
+      }
+      const double theta0 = std::acos(1.0 - 2.0 * u());
+      const double phi0 = 2.0 * pi * u();
+      for (int i = 0; i < m - 1; ++i) {
+        double v[3] = {o->bond * std::sin(theta0) * std::cos(phi0),
+                       o->bond * std::sin(theta0) * std::sin(phi0), o->bond * std::cos(theta0)};
+        for (int j = i; j >= 0; --j) {
+          const double *M = &T[9 * (size_t)j];
+          const double w0 = M[0] * v[0] + M[1] * v[1] + M[2] * v[2];
+          const double w1 = M[3] * v[0] + M[4] * v[1] + M[5] * v[2];
+          const double w2 = M[6] * v[0] + M[7] * v[1] + M[8] * v[2];
+          v[0] = w0, v[1] = w1, v[2] = w2;
+        }
+        R[3 * (size_t)i] = v[0], R[3 * (size_t)i + 1] = v[1], R[3 * (size_t)i + 2] = v[2];
+      }
+      double start[3];
+      for (int c = 0; c < 3; ++c) start[c] = span * u() + lo;
+      for (int i = 0; i < m; ++i)
+        for (int c = 0; c < 3; ++c) {
+          double x = start[c];
+          for (int j = 0; j < i; ++j) x += R[3 * (size_t)j + c];
+          pos[3 * (size_t)(a + i) + c] = x;
+        }
+      bool ok = true;
+      for (int i = 0; i + 3 < m && ok; ++i)
+        for (int j = i + 3; j < m && ok; ++j) {
+          const double *p1 = &pos[3 * (size_t)(a + i)], *p2 = &pos[3 * (size_t)(a + j)];
+          const double dx = wrap(p2[0] - p1[0]), dy = wrap(p2[1] - p1[1]), dz = wrap(p2[2] - p1[2]);
+          if (dx * dx + dy * dy + dz * dz < min_d2) ok = false;
+        }
+      if (ok) break;
+    }
+    a = b;
+  }
+  if (draws) *draws = used;
+  return PCG_OK;
+}
+
 } // extern "C"

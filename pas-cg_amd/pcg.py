@@ -87,6 +87,11 @@ class PcgBatchOptions(C.Structure):
                 ("steps_per_launch", I), ("use_graph", I), ("device", I)]
 
 
+class PcgSawOptions(C.Structure):
+    _fields_ = [("bond", D), ("intersection", D), ("box_size", D), ("density", D),
+                ("pbc", I), ("max_tries", I)]
+
+
 class PcgServerOptions(C.Structure):
     _fields_ = [("precision", I), ("hbm_budget_bytes", I64), ("max_resident", I),
                 ("chunk_steps", I), ("philox_seed", C.c_uint64), ("device", I)]
@@ -149,6 +154,8 @@ SIGNATURES = {
     "pcg_batch_sizes": [_V, I, C.POINTER(I)],
     "pcg_batch_autotune": [_V, I, C.POINTER(I), C.POINTER(D)],
     "pcg_stream_copy": [I64, I, I, C.POINTER(D), C.POINTER(D)],
+    "pcg_morph_saw": [C.POINTER(PcgSawOptions), I, C.POINTER(I), I, C.POINTER(D),
+                      C.POINTER(I64)],
 }
 
 
@@ -567,3 +574,16 @@ def stream_copy(nbytes: int = 1 << 30, iters: int = 20, device: int = 0):
     ms = C.c_double()
     check(L.pcg_stream_copy(nbytes, iters, device, C.byref(g), C.byref(ms)))
     return g.value, ms.value
+
+
+def morph_saw(chain_idx, seed: int, bond: float, intersection: float, box_size: float = 0.0,
+              density: float = 0.0, pbc: bool = False, max_tries: int = 0):
+    """Self-avoiding-walk positions [n, 3] of the chains given by chain_idx,
+    drawn from ran2(seed), and the number of draws used (pcg_morph_saw)."""
+    L = lib()
+    ch = np.ascontiguousarray(chain_idx, np.int32)
+    pos = np.zeros((len(ch), 3), np.float64)
+    o = PcgSawOptions(bond, intersection, box_size, density, 1 if pbc else 0, max_tries)
+    draws = I64()
+    check(L.pcg_morph_saw(C.byref(o), len(ch), _p(ch, I), seed, _p(pos, D), C.byref(draws)))
+    return pos, draws.value

@@ -14,6 +14,7 @@ import pytest
 
 from golden_cases import case
 from oracle_ffi import Oracle
+from pas_cg_amd import inputs as I
 from pas_cg_amd import model as M
 from pas_cg_amd import params as P
 from pas_cg_amd import pcg as G
@@ -147,14 +148,14 @@ def test_fp64_ran2_trajectory_matches_golden(name):
     oracle and the reference's own V trace step by step."""
     p, m, seed, saw, fx = case(name)
     o = Oracle(p, m, seed=seed, saw=saw)
-    if saw:  # the SAW is input preparation: hand the GPU its result
-        m = dict(m)
-        m["pos"] = o.pos0.copy()
-        if saw.get("update_orig"):
-            m["orig_pos"] = o.pos0.copy()
+    skip = 0
+    if saw:  # input preparation by the library's SAW (pcg_morph_saw)
+        m, skip = I.morph_into_saw(m, seed, bond=saw["bond"], intersection=saw["intersection"],
+                                   update_orig=saw["update_orig"])
+        assert np.array_equal(m["pos"], o.pos0) and skip == o.saw_draws()
     steps = 100
     ref = np.asarray(fx["trace"]["V_fort_7" if name == "start2" else "V_cxx"][:steps])
-    c = G.Context(p, m, precision=G.FP64, noise=G.RAN2, seed=seed, ran2_skip=o.saw_draws())
+    c = G.Context(p, m, precision=G.FP64, noise=G.RAN2, seed=seed, ran2_skip=skip)
     tr_o = o.step(steps)
     tr_g = np.zeros(steps)
     for k in range(steps):
