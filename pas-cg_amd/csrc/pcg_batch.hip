@@ -88,16 +88,17 @@ __device__ int block_exscan(int *cnt, int n) {
   return tot;
 }
 
-__device__ __forceinline__ double block_max(double v) {
-  __shared__ double part[16];
+// Maximum over the workgroup, for thread 0 only: each wave's maximum goes to
+// part[wave], the barrier, then thread 0 reads them. part is next written one
+// step later, after the barrier that ends this step, so one barrier suffices.
+__device__ __forceinline__ double block_max_to_first(double v, double *part) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) part[w] = v;
   __syncthreads();
   double m = 0.0;
-  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, part[k]);
-  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, part[k]);
   return m;
 }
 
@@ -581,6 +582,76 @@ __device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, SysDescC &s
   if (want_v && v != 0.0) atomicAdd(&vacc[PCG_QA_CAND], v);
 }
 
+// QA near pairs without workgroup barriers. Thread t of the workgroup takes
+// near pairs t, t + B, ...; every pair takes the cheap FP32 first test, and
+// the ones that pass (about two in five near pairs in a compact protein)
+// are stacked in the wave's own 128-entry LDS segment. Whenever 64 are
+// stacked, the wave takes the newest 64 through the exact path, one a lane,
+// so that path runs on full waves; the last few go after the loop. The next
+// pair is fetched before the stack is drained, so its load overlaps the
+// exact path. No wave waits for another: the force terms before and after
+// this one flow on. The exact path reads nothing from HBM (a pair's
+// formation distance bound comes from the LDS tables, its free-pair slot is
+// loaded only if it becomes a candidate), and the order in which candidates
+// are found does not matter: qa::finish ranks them by (distance, free-pair
+// index).
+template <typename R>
+__device__ __forceinline__ void qa_cand_waves(const DevBatch &b, SysDescC &sd, SysState &st,
+                                              const GeoL &g, AccL<double> &acc, const int *sizes,
+                                              uint2 *queue, double *vacc, bool want_v) {
+  constexpr int WCAP = 128;  // stacked pairs per wave: fewer than 64 + 64
+  static_assert(16 * WCAP * sizeof(uint2) <= LdsLayout::QA_QUEUE * sizeof(int),
+                "the QA queue holds every wave's stack");
+  const int n = sizes[PCG_QA_CAND + 1] - sizes[PCG_QA_CAND], B = blockDim.x;
+  if (n <= 0) return;
+  const int tid = threadIdx.x, lane = tid & 63, r0 = sd.res0;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint2 *wq = queue + (tid >> 6) * WCAP;
+  auto fetch = [&](int hk) { return hk < n ? qa_cand_fetch_lite(b, sd, hk) : QhLite{0u, -1.0f}; };
+  // the exact path's q: every field but the free-pair slot
+  auto full = [&](int hk, int i1, int i2) {
+    const int a1 = g.atype(b, i1), a2 = g.atype(b, i2);
+    double req = fmax(fmax(fmax(g.req_t[1], g.req_t[2]), g.req_t[3]), g.req_t[4 + a1 + 21 * a2]);
+    return QhFetch{hk, i1, i2, 0.f, 0, req, -1};
+  };
+  const int iters = (n + B - 1) / B;  // the same for every thread
+  int pend = 0;                       // stacked pairs of this wave (uniform)
+  double v = 0.0;
+  QhLite cur = fetch(tid);
+  for (int t = 0; t <= iters; ++t) {
+    if (t < iters) {
+      bool act = false;
+      if (cur.thr2 >= 0.0f) {  // a vacant pair has thr2 < 0
+        const v3<double> d = xwrapd(xsub(g.r(r0 + (int)(cur.ij >> 16)), g.r(r0 + (int)(cur.ij & 0xFFFFu))), sd);
+        const float dx = (float)d.x, dy = (float)d.y, dz = (float)d.z;
+        act = !(dx * dx + dy * dy + dz * dz > cur.thr2);
+      }
+      const unsigned long long m = __ballot(act);
+      if (act) wq[pend + __popcll(m & below)] = make_uint2((unsigned)(tid + t * B), cur.ij);
+      pend += __popcll(m);
+      cur = fetch(tid + (t + 1) * B);
+    }
+    if (pend >= 64 || (t == iters && pend > 0)) {
+      const int take = pend < 64 ? pend : 64;
+      pend -= take;
+      // the lanes' stack writes before the reads of other lanes' entries
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (lane < take) {
+        const uint2 w = wq[pend + lane];
+        v += qa_cand_exact<R>(b, sd, st, g, acc,
+                              full((int)w.x, r0 + (int)(w.y & 0xFFFFu), r0 + (int)(w.y >> 16)));
+      }
+      // and those reads before the slots take new pairs
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  if (want_v && v != 0.0) atomicAdd(&vacc[PCG_QA_CAND], v);
+}
+
 // The items of one term: item k is flat index sizes[T] + k, taken by thread
 // (sizes[T] + k) mod blockDim, as in one loop over all terms.
 template <int T, typename R>
@@ -636,6 +707,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
   __shared__ double vacc[NT];
   __shared__ int sizes[NT + 1];
   __shared__ int flag_rebuild, stop;
+  __shared__ double wave_max[16];
 
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     rx[k] = b.y[0][0][r0 + k];
@@ -730,7 +802,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     term_loop<PCG_PAULI, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_CONST_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_REL_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
-    if (P.p.qa_on) qa_cand_two_phase<R>(b, sd, st, g, acc, sizes, qa_queue, vacc, wv);
+    if (P.p.qa_on) qa_cand_waves<R>(b, sd, st, g, acc, sizes, qa_queue, vacc, wv);
     term_loop<PCG_QA_CONT, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_PID, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_VEL_AFM, R>(b, sd, st, g, acc, sizes, vacc, wv);
@@ -777,7 +849,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
       fy[k] = 0;
       fz[k] = 0;
     }
-    maxd = block_max(maxd);
+    maxd = block_max_to_first(maxd, wave_max);
     if (threadIdx.x == 0) {
       st.max_disp_bits = (unsigned long long)__double_as_longlong(__dsqrt_rn(maxd));
       if (want_v) {

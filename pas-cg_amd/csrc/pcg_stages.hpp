@@ -132,6 +132,10 @@ __device__ inline void build_qa_near(const DevBatch &b, SysDescC &sd, SysState &
 // sy[4 * (i - sy_off) + c] is sync budget c of residue i (HBM or LDS).
 __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState &st,
                                        int8_t *sy, int sy_off) {
+  // Nothing removed and no candidate: budgets, lists and counts stay as they
+  // are (the compress condition changes only with removals or acceptances).
+  // Every thread reads the same final counts, so the return is uniform.
+  if (st.cand_n == 0 && st.rem_n == 0) return;
   int rem_n = st.rem_n < sd.qccap ? st.rem_n : sd.qccap;
   int cand_n = st.cand_n < sd.nlcap ? st.cand_n : sd.nlcap;
   // rank of every candidate by (dist, free-pair index) -> permutation
