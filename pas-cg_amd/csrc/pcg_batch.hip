@@ -28,18 +28,25 @@ struct GeoL {
   __device__ __forceinline__ int ptype(const DevBatch &, int a) const { return ptype_t[a]; }
   __device__ __forceinline__ uint32_t sync4(const DevBatch &, int i) const { return sync[i - res0]; }
   __device__ __forceinline__ double type_req(const DevBatch &, int t) const { return req_t[t]; }
-  // n and h recomputed from the positions, with prepare_nh's operations (the
-  // chain neighbours of residue i are i - 1 and i + 1); zero at chain ends,
-  // where prepare_nh writes nothing
-  __device__ __forceinline__ void nh(const DevBatch &, int i, double out[6]) const {
-    const uint8_t f = at[i - res0];
-    if ((f & 0xC0) != 0xC0) {
-      for (int c = 0; c < 6; ++c) out[c] = 0.0;
-      return;
-    }
-    nh_from(r(i), r(i - 1), r(i + 1), out);
+  // n and h of residue i as wg_prepare_nh left them for this step
+  __device__ __forceinline__ void nh(const DevBatch &b, int i, double out[6]) const {
+    for (int c = 0; c < 6; ++c) out[c] = b.nh[c][i];
   }
 };
+
+// qa::prepare_nh (cg/src/cg/qa/prepare_nh.cpp:10-19) over the workgroup's
+// system, from the LDS positions, into the batch's n and h arrays (the chain
+// neighbours of residue i are i - 1 and i + 1; zero at chain ends, where
+// prepare_nh writes nothing). Run once per step after the positions move,
+// so that a QA pair reads its residues' vectors instead of recomputing them.
+__device__ __forceinline__ void wg_prepare_nh(const DevBatch &b, const GeoL &g, int n, int r0) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int i = r0 + k;
+    double o[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if ((g.at[k] & 0xC0) == 0xC0) nh_from(g.r(i), g.r(i - 1), g.r(i + 1), o);
+    for (int c = 0; c < 6; ++c) b.nh[c][i] = o[c];
+  }
+}
 
 template <typename RF> struct AccL {
   RF *fx, *fy, *fz;
@@ -767,12 +774,13 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     fy[k] = 0;
     fz[k] = 0;
   }
+  if (P.p.qa_on) wg_prepare_nh(b, g, n, r0);
   if (threadIdx.x == 0) prologue(0);
   __syncthreads();
   for (int it = 0; it < nsteps; ++it) {
     // a finished system keeps the forces of its last evaluation
     if (stop) break;
-    // (n and h of qa::prepare_nh are recomputed where a QA candidate needs them)
+    // (n and h of qa::prepare_nh were computed after the positions last moved)
     if (flag_rebuild) {
       wg_fix_nl(b, sd, st, g);
       if (threadIdx.x == 0) set_sizes();
@@ -850,6 +858,8 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
       fz[k] = 0;
     }
     maxd = block_max_to_first(maxd, wave_max);
+    // every position has moved: the next step's n and h
+    if (P.p.qa_on) wg_prepare_nh(b, g, n, r0);
     if (threadIdx.x == 0) {
       st.max_disp_bits = (unsigned long long)__double_as_longlong(__dsqrt_rn(maxd));
       if (want_v) {
