@@ -515,80 +515,6 @@ __device__ __forceinline__ void nat_cont_loop(const DevBatch &b, SysDescC &sd, S
   if (want_v && v != 0.0) atomicAdd(&vacc[PCG_NAT_CONT], v);
 }
 
-// QA near pairs in two phases per pass of QK items per thread: every pair
-// takes the cheap first test, and the few that pass are queued in LDS with
-// their residues; the queue is then evaluated exactly and densely. A wave no
-// longer waits, item after item, on the exact path of the one lane in it
-// that needs it, and the exact phase reads nothing from HBM (a pair's
-// formation distance bound is recomputed from the LDS tables, its free-pair
-// slot loaded only if it becomes a candidate). A pair that finds the queue
-// full is evaluated at once by its own lane.
-template <typename R>
-__device__ __forceinline__ void qa_cand_two_phase(const DevBatch &b, SysDescC &sd, SysState &st,
-                                                  const GeoL &g, AccL<double> &acc, const int *sizes,
-                                                  uint2 *queue, double *vacc, bool want_v) {
-  constexpr int QK = 8;
-  constexpr int QCAP = (int)(LdsLayout::QA_QUEUE * sizeof(int) / sizeof(uint2));
-  const int n = sizes[PCG_QA_CAND + 1] - sizes[PCG_QA_CAND], B = blockDim.x;
-  const int lane = threadIdx.x & 63, r0 = sd.res0;
-  __shared__ int qcount;
-  double v = 0.0;
-  // the exact path's q: every field but the free-pair slot
-  auto full = [&](int hk, int i1, int i2) {
-    const int a1 = g.atype(b, i1), a2 = g.atype(b, i2);
-    double req = fmax(fmax(fmax(g.req_t[1], g.req_t[2]), g.req_t[3]), g.req_t[4 + a1 + 21 * a2]);
-    return QhFetch{hk, i1, i2, 0.f, 0, req, -1};
-  };
-  for (int base = 0; base < n; base += QK * B) {
-    if (threadIdx.x == 0) qcount = 0;
-    __syncthreads();
-    QhLite q[QK];
-    for (int j = 0; j < QK; ++j) {
-      const int hk = base + j * B + (int)threadIdx.x;
-      q[j] = hk < n ? qa_cand_fetch_lite(b, sd, hk) : QhLite{0u, -1.0f};
-    }
-    for (int j = 0; j < QK; ++j) {
-      keep(q[j].ij);
-      keep(q[j].thr2);
-    }
-    unsigned ovf = 0;  // items of this thread that found the queue full
-    for (int j = 0; j < QK; ++j) {
-      // the first test of qa_cand_may_act; a vacant pair has thr2 < 0
-      bool act = false;
-      if (q[j].thr2 >= 0.0f) {
-        const v3<double> d = xwrapd(xsub(g.r(r0 + (int)(q[j].ij >> 16)), g.r(r0 + (int)(q[j].ij & 0xFFFFu))), sd);
-        const float dx = (float)d.x, dy = (float)d.y, dz = (float)d.z;
-        act = !(dx * dx + dy * dy + dz * dz > q[j].thr2);
-      }
-      unsigned long long m = __ballot(act);
-      int off = 0;
-      if (lane == 0 && m) off = atomicAdd(&qcount, __popcll(m));
-      off = __shfl(off, 0, 64);
-      if (act) {
-        int e = off + __popcll(m & ((1ull << lane) - 1ull));
-        if (e < QCAP)
-          queue[e] = make_uint2((unsigned)(base + j * B + (int)threadIdx.x), q[j].ij);
-        else
-          ovf |= 1u << j;
-      }
-    }
-    for (unsigned o = ovf; o; o &= o - 1) {
-      const int hk = base + (__ffs(o) - 1) * B + (int)threadIdx.x;
-      const QhFetch f = qa_cand_fetch(b, sd, hk);
-      v += qa_cand_exact<R>(b, sd, st, g, acc, full(hk, f.i1, f.i2));
-    }
-    __syncthreads();
-    const int m = qcount < QCAP ? qcount : QCAP;
-    for (int e = threadIdx.x; e < m; e += B) {
-      const uint2 w = queue[e];
-      v += qa_cand_exact<R>(b, sd, st, g, acc,
-                            full((int)w.x, r0 + (int)(w.y & 0xFFFFu), r0 + (int)(w.y >> 16)));
-    }
-    __syncthreads();
-  }
-  if (want_v && v != 0.0) atomicAdd(&vacc[PCG_QA_CAND], v);
-}
-
 // QA near pairs without workgroup barriers. Thread t of the workgroup takes
 // near pairs t, t + B, ...; every pair takes the cheap FP32 first test, and
 // the ones that pass (about two in five near pairs in a compact protein)

@@ -49,7 +49,8 @@ struct SysDesc {
 constexpr size_t LDS_DYN_MAX = 154 * 1024;
 
 struct LdsLayout {
-  // 32-bit words of the queue of QA pairs past the first test (8 bytes a pair)
+  // 32-bit words of the per-wave stacks of QA pairs past the first test
+  // (8 bytes a pair, 128 pairs for each of up to 16 waves)
   static constexpr size_t QA_QUEUE = 4 * 1024;
   size_t f, at, sy, bd, bu, g, q, total;
   __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged, bool gear) {
