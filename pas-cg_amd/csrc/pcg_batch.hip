@@ -517,16 +517,16 @@ __device__ __forceinline__ void nat_cont_loop(const DevBatch &b, SysDescC &sd, S
 
 // QA near pairs without workgroup barriers. Thread t of the workgroup takes
 // near pairs t, t + B, ...; every pair takes the cheap FP32 first test, and
-// the ones that pass (about two in five near pairs in a compact protein)
-// are stacked in the wave's own 128-entry LDS segment. Whenever 64 are
-// stacked, the wave takes the newest 64 through the exact path, one a lane,
-// so that path runs on full waves; the last few go after the loop. The next
-// pair is fetched before the stack is drained, so its load overlaps the
-// exact path. No wave waits for another: the force terms before and after
-// this one flow on. The exact path reads nothing from HBM (a pair's
-// formation distance bound comes from the LDS tables, its free-pair slot is
-// loaded only if it becomes a candidate), and the order in which candidates
-// are found does not matter: qa::finish ranks them by (distance, free-pair
+// the ones that pass (about one near pair in seven) are stacked in the
+// wave's own 128-entry LDS segment. Whenever 64 are stacked, the wave takes
+// the newest 64 through the exact path, one a lane, so that path runs on
+// full waves; the last few go after the loop. The next pair is fetched
+// before the stack is drained, so its load overlaps the exact path. No wave
+// waits for another: the force terms before and after this one flow on. The
+// exact path reads nothing from HBM but n and h (a pair's formation
+// distance bound comes from the LDS tables, its free-pair slot is loaded
+// only if it becomes a candidate), and the order in which candidates are
+// found does not matter: qa::finish ranks them by (distance, free-pair
 // index).
 template <typename R>
 __device__ __forceinline__ void qa_cand_waves(const DevBatch &b, SysDescC &sd, SysState &st,

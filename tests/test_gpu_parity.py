@@ -275,3 +275,42 @@ def test_plain_context_tiny_systems_follow_oracle():
         assert np.abs(c.r() - o.r()).max() < 1e-9, n
         assert abs(c.energy()[0] - o.V) <= 1e-9 * max(abs(o.V), 1e-12), (n, c.energy()[0], o.V)
         c.close()
+
+
+def _qa_survivors(m, prm):
+    """Free pairs (not native, |i - j| >= 3) within their QA first-test
+    radius min(max r_high, max(rep_cutoff, req)) at the starting positions."""
+    rh = np.asarray(prm["qa_lj_r_high"])
+    r, at = m["pos"], m["atype"]
+    i, j = np.triu_indices(len(r), k=3)
+    d = np.linalg.norm(r[i] - r[j], axis=1)
+    nat = set(map(tuple, np.asarray(m["cont"]).tolist()))
+    free = np.array([(a, b) not in nat for a, b in zip(i, j)])
+    req = np.maximum(rh[1:4].max(), rh[4 + at[i] + 21 * at[j]]) * (1 + prm["qa_formation_tol"])
+    thr = np.minimum(rh.max(), np.maximum(prm["rep_cutoff"], req))
+    return int((free & (d <= thr)).sum())
+
+
+@pytest.mark.parametrize("block", [1024, 256])
+def test_batch_large_qa_protein_follows_oracle(block):
+    """A 600-residue protein with QA on, in the persistent kernel at 16 and 4
+    waves. About 1000 pairs pass the first QA test at the start: with 4 waves
+    every wave fills and drains its 64-pair survivor stack several times a
+    step, with 16 waves about once. Noise-free FP64 positions, energy and the
+    QA contact count follow the oracle."""
+    prm = full_params(qa_on=1, pid_on=0)
+    m = M.synthetic_protein(600, M.job_seed(41))
+    assert _qa_survivors(m, prm) > 2 * 64 * 4
+    steps = 40
+    b = G.Batch(prm, [m], seeds=[5], precision=G.FP64, block=block, steps_per_launch=20)
+    b.set_noise_scale(0.0)
+    b.run(steps)
+    o = Oracle(prm, m, seed=5)
+    o.set_noise_scale(0.0)
+    tr = o.step(steps)
+    assert np.abs(b.get(0, 0) - o.r()).max() < 1e-8
+    V, _ = b.energy(0)
+    assert abs(V - tr[-1]) <= 1e-9 * abs(tr[-1]), (V, tr[-1])
+    rep = b.report()[0]
+    assert rep["INTRHC"] == o.num_qa_contacts() and o.num_qa_contacts() > 0
+    b.close()
