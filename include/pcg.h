@@ -50,6 +50,7 @@ enum pcg_term {
   PCG_CND,
   PCG_SND,
   PCG_VEL_AFM,
+  PCG_FORCE_AFM, /* constant-force AFM tips (afm/force/eval_forces.cpp:10-13) */
   PCG_NUM_TERMS
 };
 
@@ -134,6 +135,11 @@ typedef struct pcg_model {
   int32_t n_tips;
   const int32_t *tip_res;
   const double *tip_orig, *tip_vel;
+  /* constant-force AFM tips: residue and pulling force [3k]
+   * (afm::force::tip, state.cpp:563-566) */
+  int32_t n_ftips;
+  const int32_t *ftip_res;
+  const double *ftip_force;
 } pcg_model;
 
 enum { PCG_MIXED = 0, PCG_FP64 = 1 };               /* force arithmetic */

@@ -279,6 +279,7 @@ struct ora_sys {
   };
   std::vector<dih_t> nat_dih, heur_dih;
   std::vector<tip_t> tips;
+  std::vector<std::pair<int, v3>> ftips; /* residue, pulling force */
 
   /* dynamic lists */
   std::vector<nl_pair> nl;
@@ -885,6 +886,11 @@ struct ora_sys {
     }
   }
 
+  /* force AFM tips, cg/src/cg/afm/force/eval_forces.cpp:10-13 (no energy) */
+  void eval_force_afm(double *Fo) {
+    for (auto const &tp : ftips) add_F(Fo, tp.first, tp.second);
+  }
+
   bool term_enabled(int term) const {
     switch (term) {
     case ORA_CHIR: return p.chir_on;
@@ -903,6 +909,7 @@ struct ora_sys {
     case ORA_CND: return p.nat_dih_on && p.nat_dih_variant == 0;
     case ORA_SND: return p.nat_dih_on && p.nat_dih_variant == 1;
     case ORA_VEL_AFM: return p.afm_on;
+    case ORA_FORCE_AFM: return p.afm_on;
     }
     return false;
   }
@@ -925,6 +932,7 @@ struct ora_sys {
     case ORA_CND: eval_nat_dih(Fo, Vo, 0); break;
     case ORA_SND: eval_nat_dih(Fo, Vo, 1); break;
     case ORA_VEL_AFM: eval_vel_afm(Fo, Vo); break;
+    case ORA_FORCE_AFM: eval_force_afm(Fo); break;
     }
   }
 
@@ -1468,6 +1476,9 @@ ora_sys *ora_create(const ora_params *p, const ora_model *m, int32_t seed,
     tp.vel = mk(m->tip_vel[3 * k], m->tip_vel[3 * k + 1], m->tip_vel[3 * k + 2]);
     s->tips.push_back(tp);
   }
+  for (int k = 0; k < m->n_ftips; ++k)
+    s->ftips.emplace_back(m->ftip_res[k], mk(m->ftip_force[3 * k], m->ftip_force[3 * k + 1],
+                                              m->ftip_force[3 * k + 2]));
   return s;
 }
 

@@ -38,6 +38,7 @@ enum ora_term {
   ORA_CND,
   ORA_SND,
   ORA_VEL_AFM,
+  ORA_FORCE_AFM,
   ORA_NUM_TERMS
 };
 
@@ -137,6 +138,9 @@ typedef struct ora_model {
   int32_t n_tips;           /* velocity AFM tips */
   const int32_t *tip_res;
   const double *tip_orig, *tip_vel; /* [3 n_tips] */
+  int32_t n_ftips;                  /* constant-force AFM tips */
+  const int32_t *ftip_res;
+  const double *ftip_force;         /* [3 n_ftips] */
 } ora_model;
 
 typedef struct ora_saw {

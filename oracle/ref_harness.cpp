@@ -10,6 +10,7 @@
  * the kernel runs through its own for_slice (runtime.h:17-21) over the whole
  * list, as one thread does.
  */
+#include <cg/afm/force/eval_forces.h>
 #include <cg/afm/vel/eval_forces.h>
 #include <cg/angles/heur_ang/eval_forces.h>
 #include <cg/angles/heur_dih/rel.h>
@@ -416,6 +417,25 @@ void ref_pid_count(int n, const double *r, const double *cell, const uint8_t *at
   out[3] = cnt.inter;
   out[4] = cnt.inter_bb;
   out[5] = cnt.inter_ss;
+}
+
+// afm::force::eval_forces over the given tips (velocities only feed the
+// tips' statistics)
+double ref_force_afm(int n, double t, int k, const int *res, const double *force, double *F) {
+  vect::vector<vec3r> Fv(n), v(n);
+  vect::vector<afm::force::tip> tips;
+  for (int a = 0; a < k; ++a)
+    tips.emplace_back(res[a], vec3r(force[3 * a], force[3 * a + 1], force[3 * a + 2]),
+                      moving_avg<vec3r, real>(1.0));
+  real tt = t;
+  afm::force::eval_forces e;
+  e.F = Fv;
+  e.afm_tips = tips;
+  e.v = v;
+  e.t = &tt;
+  run(e);
+  from_vec(Fv, F);
+  return 0.0;
 }
 
 double ref_vel_afm(int n, const double *r, double t, int k, const int *res,

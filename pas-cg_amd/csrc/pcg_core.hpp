@@ -27,7 +27,8 @@ struct SysDesc {
   int res0, n;
   int teth0, nteth, lrep0, nlrep, chir0, nchir;
   int nang0, nnang, hang0, nhang, ndih0, nndih, hdih0, nhdih;
-  int nc0, nnc, tip0, ntip;
+  int nc0, nnc, tip0, ntip;  // velocity tips [tip0, tip0 + ntip)
+  int nftip;        // force tips after them: tip_vel holds the force
   int nl0, nlcap;   // NL and the NL-sized lists (pauli, dh, pid, QA candidates)
   int qf0, qfcap;   // QA free pairs
   int qc0, qccap;   // QA contacts and removed indices

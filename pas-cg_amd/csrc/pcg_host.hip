@@ -114,7 +114,8 @@ struct HostSys {
   std::vector<double> nc_nat;
   std::vector<uint8_t> nc_type, nc_active;
   std::vector<int> tip_res;
-  std::vector<double> tip_orig[3], tip_vel[3];
+  std::vector<double> tip_orig[3], tip_vel[3];  // velocity tips, then force tips
+  int nftip = 0;                                // (tip_vel = force, tip_orig = 0)
   std::vector<double> y1;  // initial velocities * dt
   double temperature = 0.0;
   int nlcap = 0, qfcap = 0, qccap = 0;
@@ -269,6 +270,17 @@ int build_system(const pcg_params &p, const pcg_model &m, double temperature, in
       h.tip_vel[c].push_back(m.tip_vel[3 * k + c]);
     }
   }
+  if (m.n_ftips > 0 && (!m.ftip_res || !m.ftip_force))
+    return fail(PCG_E_ARG, "force tips need ftip_res and ftip_force");
+  for (int k = 0; k < m.n_ftips; ++k) {
+    if (!chk(m.ftip_res[k])) return fail(PCG_E_ARG, "force tip residue out of range");
+    h.tip_res.push_back(m.ftip_res[k]);
+    for (int c = 0; c < 3; ++c) {
+      h.tip_orig[c].push_back(0.0);
+      h.tip_vel[c].push_back(m.ftip_force[3 * k + c]);
+    }
+    h.nftip += 1;
+  }
   long long pairs = (long long)n * (n - 1) / 2;
   long long cap = std::min<long long>(pairs, 1200LL * n);
   h.nlcap = round_up((int)std::max<long long>(cap, 1), BLK);
@@ -403,7 +415,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     d.ndih0 = (int)nndih, d.nndih = (int)h.ndih_ph.size();
     d.hdih0 = (int)nhdih, d.nhdih = (int)h.hdih_t.size();
     d.nc0 = (int)nnc, d.nnc = (int)h.nc_nat.size();
-    d.tip0 = (int)ntip, d.ntip = (int)h.tip_res.size();
+    d.tip0 = (int)ntip, d.ntip = (int)h.tip_res.size() - h.nftip, d.nftip = h.nftip;
     d.nl0 = (int)nl, d.nlcap = h.nlcap;
     d.qf0 = (int)qf, d.qfcap = h.qfcap;
     d.qc0 = (int)qc, d.qccap = h.qccap;
@@ -418,7 +430,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     d.steps_target = 0;
     ntot += h.n;
     nteth += d.nteth, nlrep += d.nlrep, nchir += d.nchir, nnang += d.nnang, nhang += d.nhang;
-    nndih += d.nndih, nhdih += d.nhdih, nnc += d.nnc, ntip += d.ntip;
+    nndih += d.nndih, nhdih += d.nhdih, nnc += d.nnc, ntip += d.ntip + d.nftip;
     nl += h.nlcap, qf += h.qfcap, qc += h.qccap;
     B->max_nlcap = std::max(B->max_nlcap, h.nlcap);
     B->max_n = std::max(B->max_n, h.n);
@@ -1040,6 +1052,7 @@ static int host_term_size(const SysDesc &s, const SysState &st, int term) {
   case PCG_HEUR_DIH: v = s.nhdih; break;
   case PCG_CND: case PCG_SND: v = s.nndih; break;
   case PCG_VEL_AFM: v = s.ntip; break;
+  case PCG_FORCE_AFM: v = s.nftip; break;
   }
   return v;
 }
@@ -1650,9 +1663,9 @@ double now_s() {
 
 struct OwnedModel {
   int n = 0;
-  std::vector<double> pos, orig, ang_theta, dih_phi, cont_len, tip_orig, tip_vel;
+  std::vector<double> pos, orig, ang_theta, dih_phi, cont_len, tip_orig, tip_vel, ftip_force;
   std::vector<uint8_t> atype, ang_has, dih_has;
-  std::vector<int32_t> chain, seq, teth, ang, dih, cont, cont_type, tip_res;
+  std::vector<int32_t> chain, seq, teth, ang, dih, cont, cont_type, tip_res, ftip_res;
   double cell[3] = {0, 0, 0};
   pcg_model view() const {
     pcg_model m{};
@@ -1681,6 +1694,9 @@ struct OwnedModel {
     m.tip_res = tip_res.data();
     m.tip_orig = tip_orig.data();
     m.tip_vel = tip_vel.data();
+    m.n_ftips = (int)ftip_res.size();
+    m.ftip_res = ftip_res.data();
+    m.ftip_force = ftip_force.data();
     return m;
   }
 };
@@ -1710,6 +1726,8 @@ OwnedModel own(const pcg_model &m) {
   o.tip_res = cp(m.tip_res, m.n_tips);
   o.tip_orig = cp(m.tip_orig, 3 * (size_t)m.n_tips);
   o.tip_vel = cp(m.tip_vel, 3 * (size_t)m.n_tips);
+  o.ftip_res = cp(m.ftip_res, m.n_ftips);
+  o.ftip_force = cp(m.ftip_force, 3 * (size_t)m.n_ftips);
   for (int c = 0; c < 3; ++c) o.cell[c] = m.cell[c];
   return o;
 }

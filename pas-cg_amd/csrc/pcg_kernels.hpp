@@ -108,6 +108,7 @@ __device__ __forceinline__ int term_size(SysDescC &s, const SysState &st, int te
   case PCG_CND: return s.nndih;
   case PCG_SND: return s.nndih;
   case PCG_VEL_AFM: return s.ntip;
+  case PCG_FORCE_AFM: return s.nftip;
   }
   return 0;
 }
@@ -131,6 +132,7 @@ __device__ __forceinline__ bool term_on(DevParamsC &P, int term) {
   case PCG_CND: return p.nat_dih_on && p.nat_dih_variant == 0;
   case PCG_SND: return p.nat_dih_on && p.nat_dih_variant == 1;
   case PCG_VEL_AFM: return p.afm_on;
+  case PCG_FORCE_AFM: return p.afm_on;
   }
   return false;
 }
@@ -159,6 +161,7 @@ __device__ __forceinline__ double eval_item(const DevBatch &b, SysDescC &s, SysS
   case PCG_CND: return nat_dih_item<double>(b, g, acc, s.ndih0 + k, 0);
   case PCG_SND: return nat_dih_item<double>(b, g, acc, s.ndih0 + k, 1);
   case PCG_VEL_AFM: return vel_afm_item<double>(b, g, acc, s.tip0 + k, st.t);
+  case PCG_FORCE_AFM: return force_afm_item(b, acc, s.tip0 + s.ntip + k);
   }
   return 0.0;
 }

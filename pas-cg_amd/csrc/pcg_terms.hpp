@@ -586,6 +586,15 @@ __device__ double vel_afm_item(const DevBatch &b, const G &g, A &acc, int k, dou
   return (double)V;
 }
 
+// afm::force::eval_forces::iter (cg/src/cg/afm/force/eval_forces.cpp:10-13):
+// the tip's constant pulling force on its residue, no energy. Force tips sit
+// after the velocity tips, their force in tip_vel.
+template <typename A>
+__device__ __forceinline__ double force_afm_item(const DevBatch &b, A &acc, int k) {
+  acc.add(b.tip_res[k], v3<double>{b.tip_vel[0][k], b.tip_vel[1][k], b.tip_vel[2][k]});
+  return 0.0;
+}
+
 // ------------------------------------------------------------------ QA ----
 // qa::prepare_nh::iter (cg/src/cg/qa/prepare_nh.cpp:10-19), exact FP64:
 // n and h of residue i from its own and its chain neighbours' positions.

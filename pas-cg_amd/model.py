@@ -33,6 +33,8 @@ def empty_tips(m: dict) -> dict:
     m["tip_res"] = np.zeros(0, np.int32)
     m["tip_orig"] = np.zeros((0, 3), np.float64)
     m["tip_vel"] = np.zeros((0, 3), np.float64)
+    m["ftip_res"] = np.zeros(0, np.int32)
+    m["ftip_force"] = np.zeros((0, 3), np.float64)
     return m
 
 
@@ -240,4 +242,21 @@ def add_pulling(m: dict, vel_A_per_tau: float = 0.005) -> dict:
     m["tip_res"] = np.array([first, last], np.int32)
     m["tip_orig"] = np.stack([r0, r1]).astype(np.float64)
     m["tip_vel"] = np.stack([np.zeros(3), vel_A_per_tau * ANGSTROM * u])
+    return m
+
+
+def add_force_pulling(m: dict, force_eps_per_A: float = 1.0) -> dict:
+    """Constant-force pulling (afm type "const force"): the first residue of
+    chain 0 held by a resting velocity tip, the last one pulled by a force of
+    the given size along the direction from the first to the last
+    (state::setup_afm, cg/src/cg/simul/state.cpp:548-568)."""
+    first = 0
+    last = int(np.nonzero(m["chain"] == m["chain"][0])[0][-1])
+    r0, r1 = m["pos"][first], m["pos"][last]
+    u = (r1 - r0) / np.linalg.norm(r1 - r0)
+    m["tip_res"] = np.array([first], np.int32)
+    m["tip_orig"] = r0[None].astype(np.float64)
+    m["tip_vel"] = np.zeros((1, 3), np.float64)
+    m["ftip_res"] = np.array([last], np.int32)
+    m["ftip_force"] = (force_eps_per_A / ANGSTROM * u)[None].astype(np.float64)
     return m

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libpcg.so")
 
 TERMS = ["chir", "tether", "lrep", "nat_cont", "pauli", "const_dh", "rel_dh",
          "qa_cand", "qa_cont", "pid", "nat_ang", "heur_ang", "heur_dih", "cnd",
-         "snd", "vel_afm"]
+         "snd", "vel_afm", "force_afm"]
 TERM_ID = {t: i for i, t in enumerate(TERMS)}
 # pcg_report_field order: the scalars of make_report::add_cur_scalars
 REPORT_FIELDS = ["TIME", "EPOT", "ETOT", "EKIN", "RG", "L", "RMSD", "W", "ICN", "ICNss",
@@ -74,6 +74,7 @@ class PcgModel(C.Structure):
         ("cell", D * 3),
         ("n_tips", I), ("tip_res", C.POINTER(I)), ("tip_orig", C.POINTER(D)),
         ("tip_vel", C.POINTER(D)),
+        ("n_ftips", I), ("ftip_res", C.POINTER(I)), ("ftip_force", C.POINTER(D)),
     ]
 
 
@@ -249,6 +250,10 @@ class _ModelRef:
         s.tip_res = _p(arr(m["tip_res"], np.int32), I)
         s.tip_orig = _p(arr(m["tip_orig"], np.float64), D)
         s.tip_vel = _p(arr(m["tip_vel"], np.float64), D)
+        ftip_res = m.get("ftip_res", np.zeros(0, np.int32))
+        s.n_ftips = len(ftip_res)
+        s.ftip_res = _p(arr(ftip_res, np.int32), I)
+        s.ftip_force = _p(arr(m.get("ftip_force", np.zeros((0, 3))), np.float64), D)
         self.s = s
 
 

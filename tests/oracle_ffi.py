@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "build", "liboracle.so")
 
 TERMS = ["chir", "tether", "lrep", "nat_cont", "pauli", "const_dh", "rel_dh",
          "qa_cand", "qa_cont", "pid", "nat_ang", "heur_ang", "heur_dih", "cnd",
-         "snd", "vel_afm"]
+         "snd", "vel_afm", "force_afm"]
 TERM_ID = {t: i for i, t in enumerate(TERMS)}
 
 D = C.c_double
@@ -69,6 +69,7 @@ class OraModel(C.Structure):
         ("cell", D * 3),
         ("n_tips", I), ("tip_res", C.POINTER(I)), ("tip_orig", C.POINTER(D)),
         ("tip_vel", C.POINTER(D)),
+        ("n_ftips", I), ("ftip_res", C.POINTER(I)), ("ftip_force", C.POINTER(D)),
     ]
 
 
@@ -205,6 +206,10 @@ class Oracle:
         m.tip_res = _ptr(arr(model["tip_res"], np.int32), I)
         m.tip_orig = _ptr(arr(model["tip_orig"], np.float64), D)
         m.tip_vel = _ptr(arr(model["tip_vel"], np.float64), D)
+        ftip_res = model.get("ftip_res", np.zeros(0, np.int32))
+        m.n_ftips = len(ftip_res)
+        m.ftip_res = _ptr(arr(ftip_res, np.int32), I)
+        m.ftip_force = _ptr(arr(model.get("ftip_force", np.zeros((0, 3))), np.float64), D)
         self._model = m
         p = fill_struct(OraParams(), params)
         s = OraSaw()

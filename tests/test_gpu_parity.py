@@ -59,6 +59,8 @@ CONFIGS = {
                           dh_perm=4.0 * P.EPS_0 / P.ANGSTROM)), protein(seed=6)),
     "multichain": lambda: (_sink(full_params(qa_on=0, nat_cont_on=0)), multichain()),
     "multichain_qa": lambda: (full_params(qa_on=1, pid_on=0), multichain(seed=3)),
+    "protein_force_afm": lambda: (full_params(qa_on=0),
+                                  M.add_force_pulling(protein(seed=8), force_eps_per_A=2.5)),
 }
 
 
@@ -96,7 +98,7 @@ def test_lists_bit_exact(cfg):
 
 
 @pytest.mark.parametrize("precision", [G.FP64, G.MIXED], ids=["fp64", "mixed"])
-@pytest.mark.parametrize("cfg", ["protein", "protein_sink_pid", "multichain"])
+@pytest.mark.parametrize("cfg", ["protein", "protein_sink_pid", "multichain", "protein_force_afm"])
 def test_terms_match_oracle(cfg, precision):
     prm, m, o, c = same_state(cfg, precision)
     o.fix_nl()
@@ -242,7 +244,7 @@ def test_c1_box_per_term_errors_mixed():
     assert checked >= 5
 
 
-@pytest.mark.parametrize("cfg", ["multichain", "multichain_qa"])
+@pytest.mark.parametrize("cfg", ["multichain", "multichain_qa", "protein_force_afm"])
 def test_batch_periodic_multichain_follows_oracle(cfg):
     """The persistent kernel on a periodic box of several chains (wrapped
     pairs; inter-chain Pauli, DH and PID, or QA with its sync budgets)

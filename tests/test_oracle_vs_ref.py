@@ -136,6 +136,10 @@ def term_ref(term, o, prm, m):
         vel = np.ascontiguousarray(m["tip_vel"], np.float64)
         V = L.ref_vel_afm(n, ptr(r, D), o.t, len(res), ptr(res, I), ptr(orig, D),
                           ptr(vel, D), prm["afm_H1"], prm["afm_H2"], ptr(F, D))
+    elif term == "force_afm":
+        res = np.ascontiguousarray(m["ftip_res"], np.int32)
+        frc = np.ascontiguousarray(m["ftip_force"], np.float64)
+        V = L.ref_force_afm(n, o.t, len(res), ptr(res, I), ptr(frc, D), ptr(F, D))
     else:
         raise KeyError(term)
     return F, V
@@ -390,3 +394,15 @@ def test_pid_activity_counts(variant):
     assert seen[1] > 0 and seen[2] > 0
     if variant == "lj-algebraic":
         assert seen[4] > 0 and seen[5] > 0
+
+
+def test_force_afm_tips():
+    """Constant-force AFM tips: the oracle equals afm::force::eval_forces."""
+    prm = full_params(qa_on=0)
+    m = M.add_force_pulling(perturbed_protein(), force_eps_per_A=2.5)
+    o = Oracle(prm, m, seed=8)
+    advance(o, 5)
+    F1, V1 = o.eval_term("force_afm")
+    F2, V2 = term_ref("force_afm", o, prm, m)
+    assert np.abs(F2).max() > 0 and V1 == 0.0 and V2 == 0.0
+    close(F1, V1, F2, V2, tol=1e-15)
