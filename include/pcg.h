@@ -212,6 +212,17 @@ int pcg_report(pcg_ctx *c, double *out /*[PCG_REPORT_NUM]*/);
 int pcg_get_time(pcg_ctx *c, double *t, int64_t *step);
 int pcg_set_time(pcg_ctx *c, double t);
 int pcg_set_noise_scale(pcg_ctx *c, double scale); /* 0: no noise */
+/* Protocol phases (thread_main.cpp:84-142): replaces the AFM tips, as
+ * state::setup_afm places them when a phase turns AFM on
+ * (state.cpp:548-568). n_tips velocity tips (residue, position, velocity
+ * [3k]) and n_ftips force tips (residue, force [3k]) fill the tip slots the
+ * model reserved at creation (n_tips + n_ftips of pcg_model); their
+ * reference time becomes the current time, so a velocity tip sits at
+ * orig + (t - t_now) vel. Zero tips turn the AFM terms off (a phase without
+ * AFM). The AFM terms must be enabled in pcg_params (afm_on). */
+int pcg_set_tips(pcg_ctx *c, int32_t n_tips, const int32_t *tip_res, const double *tip_orig,
+                 const double *tip_vel, int32_t n_ftips, const int32_t *ftip_res,
+                 const double *ftip_force);
 /* Lists. kind: 0 NL, 1 current native contacts, 2 pauli, 3 dh, 4 pid,
  * 5 qa free pairs (flag = vacant), 6 qa contacts (val = saturation,
  * flag = vacant). Returns the size; fills up to cap items. */
@@ -270,6 +281,10 @@ int pcg_batch_get(pcg_batch *b, int32_t sys, int32_t which, double *out);
 /* which: 0..5 Gear y0..y5 of one system; setting positions (0) makes its
  * next step rebuild the neighbour list */
 int pcg_batch_set(pcg_batch *b, int32_t sys, int32_t which, const double *in);
+/* pcg_set_tips for system sys of a batch (residue indices within the system) */
+int pcg_batch_set_tips(pcg_batch *b, int32_t sys, int32_t n_tips, const int32_t *tip_res,
+                       const double *tip_orig, const double *tip_vel, int32_t n_ftips,
+                       const int32_t *ftip_res, const double *ftip_force);
 int pcg_batch_energy(pcg_batch *b, int32_t sys, double *V, double *term_V);
 int pcg_batch_info(pcg_batch *b, int32_t *block, int32_t *steps_per_launch,
                    int64_t *device_bytes);

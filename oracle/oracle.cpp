@@ -1691,6 +1691,23 @@ void ora_sync(const ora_sys *s, int8_t *out) {
 }
 int32_t ora_nl_rebuilds(const ora_sys *s) { return s->nl_rebuilds; }
 void ora_pid_counts(const ora_sys *s, int32_t *out) { s->pid_counts(out); }
+void ora_set_tips(ora_sys *s, int32_t n_tips, const int32_t *tip_res, const double *tip_orig,
+                  const double *tip_vel, int32_t n_ftips, const int32_t *ftip_res,
+                  const double *ftip_force) {
+  s->tips.clear();
+  s->ftips.clear();
+  for (int k = 0; k < n_tips; ++k) {
+    tip_t tp;
+    tp.res = tip_res[k];
+    tp.orig = mk(tip_orig[3 * k], tip_orig[3 * k + 1], tip_orig[3 * k + 2]);
+    tp.ref_t = s->t; /* afm::vel::tip ref_t = t at setup_afm (state.cpp:555-561) */
+    tp.vel = mk(tip_vel[3 * k], tip_vel[3 * k + 1], tip_vel[3 * k + 2]);
+    s->tips.push_back(tp);
+  }
+  for (int k = 0; k < n_ftips; ++k)
+    s->ftips.emplace_back(ftip_res[k], mk(ftip_force[3 * k], ftip_force[3 * k + 1],
+                                          ftip_force[3 * k + 2]));
+}
 int64_t ora_saw_draws(const ora_sys *s) { return s->saw_draws; }
 
 struct ora_rng {

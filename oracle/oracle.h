@@ -213,6 +213,10 @@ int32_t ora_nl_rebuilds(const ora_sys *s);
 /* pid::count_active over the current PID bundles and positions:
  * {intra, intra_bb, intra_ss, inter, inter_bb, inter_ss} */
 void ora_pid_counts(const ora_sys *s, int32_t *out /*[6]*/);
+/* replaces the AFM tips; their reference time becomes the current time */
+void ora_set_tips(ora_sys *s, int32_t n_tips, const int32_t *tip_res, const double *tip_orig,
+                  const double *tip_vel, int32_t n_ftips, const int32_t *ftip_res,
+                  const double *ftip_force);
 int64_t ora_saw_draws(const ora_sys *s); /* ran2 draws the SAW consumed */
 
 /* The ran2 generator by itself (cg/include/cg/utils/random/nr.h). */

@@ -37,6 +37,7 @@ struct SysDesc {
   double temperature, const2; // const2 = sqrt(2 kB T gamma dt) dt (step.cpp:7-17)
   uint64_t philox_key;
   int64_t steps_target;       // batched/served: steps this system still owes
+  double afm_t0;              // time the tips were placed (afm::vel::tip ref_t)
 };
 
 // LDS layout of the batched kernel for a system of n residues: positions

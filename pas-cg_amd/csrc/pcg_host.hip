@@ -859,6 +859,55 @@ void report_scalars(const ReportSums &o, const SysState &st, double *out) {
   for (int k = 0; k < 6; ++k) out[PCG_REP_PID_INTRHC + k] = o.pid[k];
 }
 
+// Replaces the AFM tips of system s (state::setup_afm, state.cpp:548-568):
+// velocity tips, then force tips, in the slots the system's model reserved;
+// their reference time becomes the system's current time. Zero tips turn
+// the system's AFM off.
+int set_tips(DeviceBatch &B, int s, int32_t nv, const int32_t *vres, const double *vorig,
+             const double *vvel, int32_t nf, const int32_t *fres, const double *fforce) {
+  if (s < 0 || s >= (int)B.sys.size()) return fail(PCG_E_ARG, "system index out of range");
+  if (nv < 0 || nf < 0 || (nv > 0 && (!vres || !vorig || !vvel)) || (nf > 0 && (!fres || !fforce)))
+    return fail(PCG_E_ARG, "bad tip arrays");
+  SysDesc &d = B.sys[s];
+  const int cap = (int)B.hs[s].tip_res.size();
+  if (nv + nf > cap)
+    return fail(PCG_E_ARG, "more tips than the model reserved (" + std::to_string(cap) + ")");
+  std::vector<int32_t> res(nv + nf);
+  std::vector<double> orig[3], vel[3];
+  for (int c = 0; c < 3; ++c) orig[c].assign(nv + nf, 0.0), vel[c].assign(nv + nf, 0.0);
+  for (int k = 0; k < nv + nf; ++k) {
+    const bool v = k < nv;
+    const int j = v ? k : k - nv;
+    res[k] = v ? vres[j] : fres[j];
+    if (res[k] < 0 || res[k] >= d.n) return fail(PCG_E_ARG, "tip residue out of range");
+    res[k] += d.res0;
+    for (int c = 0; c < 3; ++c) {
+      orig[c][k] = v ? vorig[3 * j + c] : 0.0;
+      vel[c][k] = v ? vvel[3 * j + c] : fforce[3 * j + c];
+    }
+  }
+  SysState st;
+  int rc = read_state(B, s, st);
+  if (rc) return rc;
+  if (nv + nf > 0) {
+    // the device views are const for the kernels; the arena itself is writable
+    HIPCHK(hipMemcpyAsync((void *)(B.d.tip_res + d.tip0), res.data(), res.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice, B.stream));
+    for (int c = 0; c < 3; ++c) {
+      HIPCHK(hipMemcpyAsync((void *)(B.d.tip_orig[c] + d.tip0), orig[c].data(), res.size() * 8,
+                            hipMemcpyHostToDevice, B.stream));
+      HIPCHK(hipMemcpyAsync((void *)(B.d.tip_vel[c] + d.tip0), vel[c].data(), res.size() * 8,
+                            hipMemcpyHostToDevice, B.stream));
+    }
+  }
+  d.ntip = nv;
+  d.nftip = nf;
+  d.afm_t0 = st.t;
+  HIPCHK(hipMemcpyAsync((void *)(B.d.sys + s), &d, sizeof(SysDesc), hipMemcpyHostToDevice, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
 int report_into(DeviceBatch &B, double *out) {
   const size_t ns = B.sys.size();
   ReportSums *d = nullptr;
@@ -1158,6 +1207,13 @@ int pcg_set_time(pcg_ctx *c, double t) {
   if (rc) return rc;
   st.t = st.true_t = t;
   return write_state(*c->B, 0, st);
+}
+
+int pcg_set_tips(pcg_ctx *c, int32_t n_tips, const int32_t *tip_res, const double *tip_orig,
+                 const double *tip_vel, int32_t n_ftips, const int32_t *ftip_res,
+                 const double *ftip_force) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  return set_tips(*c->B, 0, n_tips, tip_res, tip_orig, tip_vel, n_ftips, ftip_res, ftip_force);
 }
 
 int pcg_set_noise_scale(pcg_ctx *c, double scale) {
@@ -1514,6 +1570,13 @@ int pcg_batch_energies(pcg_batch *b, int32_t on) {
     b->graph = nullptr;
   }
   return PCG_OK;
+}
+
+int pcg_batch_set_tips(pcg_batch *b, int32_t sys, int32_t n_tips, const int32_t *tip_res,
+                       const double *tip_orig, const double *tip_vel, int32_t n_ftips,
+                       const int32_t *ftip_res, const double *ftip_force) {
+  if (!b) return fail(PCG_E_ARG, "null batch");
+  return set_tips(*b->B, sys, n_tips, tip_res, tip_orig, tip_vel, n_ftips, ftip_res, ftip_force);
 }
 
 int pcg_batch_set_noise_scale(pcg_batch *b, double scale) {

@@ -160,7 +160,7 @@ __device__ __forceinline__ double eval_item(const DevBatch &b, SysDescC &s, SysS
   case PCG_HEUR_DIH: return heur_dih_item<double>(b, g, acc, s.hdih0 + k);
   case PCG_CND: return nat_dih_item<double>(b, g, acc, s.ndih0 + k, 0);
   case PCG_SND: return nat_dih_item<double>(b, g, acc, s.ndih0 + k, 1);
-  case PCG_VEL_AFM: return vel_afm_item<double>(b, g, acc, s.tip0 + k, st.t);
+  case PCG_VEL_AFM: return vel_afm_item<double>(b, g, acc, s.tip0 + k, st.t - s.afm_t0);
   case PCG_FORCE_AFM: return force_afm_item(b, acc, s.tip0 + s.ntip + k);
   }
   return 0.0;

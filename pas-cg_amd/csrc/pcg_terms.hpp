@@ -570,7 +570,8 @@ __device__ double pid_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, i
 }
 
 // ----------------------------------------------------------------- AFM ----
-// afm::vel::eval_forces::iter (cg/src/cg/afm/vel/eval_forces.cpp:10-31)
+// afm::vel::eval_forces::iter (cg/src/cg/afm/vel/eval_forces.cpp:10-31); t is
+// the time since the tips were placed (t - ref_t)
 template <typename R, typename G, typename A>
 __device__ double vel_afm_item(const DevBatch &b, const G &g, A &acc, int k, double t) {
   ParamsC &p = b.P->p;

@@ -155,6 +155,10 @@ SIGNATURES = {
     "pcg_batch_sizes": [_V, I, C.POINTER(I)],
     "pcg_batch_autotune": [_V, I, C.POINTER(I), C.POINTER(D)],
     "pcg_stream_copy": [I64, I, I, C.POINTER(D), C.POINTER(D)],
+    "pcg_set_tips": [C.c_void_p, I, C.POINTER(I), C.POINTER(D), C.POINTER(D), I, C.POINTER(I),
+                     C.POINTER(D)],
+    "pcg_batch_set_tips": [C.c_void_p, I, I, C.POINTER(I), C.POINTER(D), C.POINTER(D), I,
+                           C.POINTER(I), C.POINTER(D)],
     "pcg_morph_saw": [C.POINTER(PcgSawOptions), I, C.POINTER(I), I, C.POINTER(D),
                       C.POINTER(I64)],
 }
@@ -210,6 +214,19 @@ def fill_struct(st, p: dict):
 
 def _p(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _tip_args(tips: dict):
+    """pcg_set_tips arguments from a dict with tip_res/tip_orig/tip_vel and
+    ftip_res/ftip_force (missing keys: no tips of that kind); the arrays are
+    returned too, to keep them alive for the call."""
+    vr = np.ascontiguousarray(tips.get("tip_res", []), np.int32)
+    vo = np.ascontiguousarray(np.reshape(tips.get("tip_orig", np.zeros((0, 3))), (-1, 3)), np.float64)
+    vv = np.ascontiguousarray(np.reshape(tips.get("tip_vel", np.zeros((0, 3))), (-1, 3)), np.float64)
+    fr = np.ascontiguousarray(tips.get("ftip_res", []), np.int32)
+    ff = np.ascontiguousarray(np.reshape(tips.get("ftip_force", np.zeros((0, 3))), (-1, 3)), np.float64)
+    keep = (vr, vo, vv, fr, ff)
+    return keep, (len(vr), _p(vr, I), _p(vo, D), _p(vv, D), len(fr), _p(fr, I), _p(ff, D))
 
 
 class _ModelRef:
@@ -357,6 +374,11 @@ class Context:
     def set_noise_scale(self, s):
         check(self.L.pcg_set_noise_scale(self.h, s))
 
+    def set_tips(self, tips: dict):
+        """Replaces the AFM tips (pcg_set_tips); an empty dict turns AFM off."""
+        keep, args = _tip_args(tips)
+        check(self.L.pcg_set_tips(self.h, *args))
+
     def list(self, kind: int):
         size = C.c_int32()
         check(self.L.pcg_get_list(self.h, kind, 0, C.byref(size), None, None, None, None))
@@ -488,6 +510,11 @@ class Batch:
 
     def set_noise_scale(self, s: float):
         check(self.L.pcg_batch_set_noise_scale(self.h, s))
+
+    def set_tips(self, sys: int, tips: dict):
+        """Replaces the AFM tips of one system (pcg_batch_set_tips)."""
+        keep, args = _tip_args(tips)
+        check(self.L.pcg_batch_set_tips(self.h, sys, *args))
 
     def energies(self, on: bool = True):
         """Whether the last step of a launch evaluates energies (default on);

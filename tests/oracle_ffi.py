@@ -141,6 +141,8 @@ def lib():
             getattr(L, f).argtypes = [P]
         L.ora_sync.argtypes = [P, C.POINTER(C.c_int8)]
         L.ora_pid_counts.argtypes = [P, C.POINTER(I)]
+        L.ora_set_tips.argtypes = [P, I, C.POINTER(I), C.POINTER(D), C.POINTER(D), I,
+                                   C.POINTER(I), C.POINTER(D)]
         L.ora_saw_draws.restype = C.c_int64
         L.ora_saw_draws.argtypes = [P]
         L.ora_static_list.restype = I
@@ -356,6 +358,16 @@ class Oracle:
         out = np.zeros((self.n, 4), np.int8)
         self.L.ora_sync(self.h, _ptr(out, C.c_int8))
         return out
+
+    def set_tips(self, tips: dict):
+        """Replaces the AFM tips (reference time = now); {} removes them."""
+        vr = np.ascontiguousarray(tips.get("tip_res", []), np.int32)
+        vo = np.ascontiguousarray(np.reshape(tips.get("tip_orig", np.zeros((0, 3))), (-1, 3)), np.float64)
+        vv = np.ascontiguousarray(np.reshape(tips.get("tip_vel", np.zeros((0, 3))), (-1, 3)), np.float64)
+        fr = np.ascontiguousarray(tips.get("ftip_res", []), np.int32)
+        ff = np.ascontiguousarray(np.reshape(tips.get("ftip_force", np.zeros((0, 3))), (-1, 3)), np.float64)
+        self.L.ora_set_tips(self.h, len(vr), _ptr(vr, I), _ptr(vo, D), _ptr(vv, D), len(fr),
+                            _ptr(fr, I), _ptr(ff, D))
 
     def pid_counts(self) -> np.ndarray:
         """{intra, intra_bb, intra_ss, inter, inter_bb, inter_ss} of the
