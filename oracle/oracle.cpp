@@ -2,7 +2,7 @@
  * oracle.cpp -- FP64 CPU restatement of pas-cg's per-step simulation loop.
  *
  * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py as the checker. The product never uses it.
+ * cpu_baseline leg of bench.py, to check results against. The product never uses it.
  *
  * Build with -O2 -ffp-contract=off and without fast-math so that the
  * arithmetic follows the expression order written here, which in turn follows

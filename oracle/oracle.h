@@ -2,7 +2,7 @@
  * oracle.h -- C ABI of the FP64 CPU oracle for the pas-cg per-step loop.
  *
  * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker.
+ * cpu_baseline leg of bench.py may load this library, and only to check results.
  * The product (pas-cg_amd/) never links, loads or calls it.
  *
  * The oracle restates, in plain FP64 C++, the reference loop

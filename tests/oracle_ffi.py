@@ -1,7 +1,7 @@
 """ctypes binding of the FP64 CPU oracle (oracle/oracle.h).
 
 Test infrastructure: used only by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.
+cpu_baseline leg of bench.py, to check results against.
 """
 from __future__ import annotations
 
