@@ -3,7 +3,7 @@
  * from their sources where they lie under the reference tree (see
  * oracle/Makefile), on lists and positions handed in from the tests.
  *
- * TEST INFRASTRUCTURE ONLY (the checker of the checker): tests compare the
+ * TEST INFRASTRUCTURE ONLY (it pins the oracle): tests compare the
  * oracle with these calls. Nothing in the product loads it. Lists are built
  * the way cg/src/cg/simul/state.cpp and thread_setup.cpp build them, each
  * kernel object's parameters are bound as thread_setup.cpp binds them, and
