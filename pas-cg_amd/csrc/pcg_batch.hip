@@ -743,8 +743,20 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     term_loop<PCG_FORCE_AFM, R>(b, sd, st, g, acc, sizes, vacc, wv);
     __syncthreads();
     if (P.p.qa_on) qa_finish_block(b, sd, st, (int8_t *)syl, r0);
-    // lang::step (cg/src/cg/langevin/step.cpp:23-77)
+    // lang::step (cg/src/cg/langevin/step.cpp:23-77); each residue's force,
+    // once used, goes to HBM on the launch's last step and restarts at 0
     double maxd = 0.0;
+    const bool wl = last;
+    auto flush_force = [&](int k) {
+      if (wl) {
+        ((RF *)b.F[0])[r0 + k] = (RF)fx[k];
+        ((RF *)b.F[1])[r0 + k] = (RF)fy[k];
+        ((RF *)b.F[2])[r0 + k] = (RF)fz[k];
+      }
+      fx[k] = 0;
+      fy[k] = 0;
+      fz[k] = 0;
+    };
     if (P.p.lang_on) {
       for (int k = threadIdx.x; k < n; k += blockDim.x) {
         int i = r0 + k;
@@ -755,14 +767,16 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
         philox_normals(sd.philox_key, st.step, k, nz);
         double r[3], vel[3];
         const int aa = atl[k] & 63;  // the type, without the chain flags
+        const double Fx = fx[k], Fy = fy[k], Fz = fz[k];
+        flush_force(k);
         if (gear_l) {
-          gear_core(b, sd, st, minv_t[aa], sqminv_t[aa], fx[k], fy[k], fz[k], nz, r, vel,
+          gear_core(b, sd, st, minv_t[aa], sqminv_t[aa], Fx, Fy, Fz, nz, r, vel,
                     [&](int kk, int c) -> double & {
                       return kk == 0 ? (c == 0 ? rx[k] : c == 1 ? ry[k] : rz[k])
                                      : ys[((kk - 1) * 3 + c) * n + k];
                     });
         } else {
-          gear_core(b, sd, st, minv_t[aa], sqminv_t[aa], fx[k], fy[k], fz[k], nz, r, vel,
+          gear_core(b, sd, st, minv_t[aa], sqminv_t[aa], Fx, Fy, Fz, nz, r, vel,
                     [&](int kk, int c) -> double & { return b.y[kk][c][i]; });
           for (int c = 0; c < 3; ++c) b.vel[c][i] = vel[c];
         }
@@ -771,18 +785,8 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
         rz[k] = r[2];
         maxd = fmax(maxd, disp2_from(sd, o, r));  // squared; the root is taken once below
       }
-    }
-    // the forces of the last step go to HBM; then every force restarts at 0
-    const bool wl = last;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-      if (wl) {
-        ((RF *)b.F[0])[r0 + k] = (RF)fx[k];
-        ((RF *)b.F[1])[r0 + k] = (RF)fy[k];
-        ((RF *)b.F[2])[r0 + k] = (RF)fz[k];
-      }
-      fx[k] = 0;
-      fy[k] = 0;
-      fz[k] = 0;
+    } else {
+      for (int k = threadIdx.x; k < n; k += blockDim.x) flush_force(k);
     }
     maxd = block_max_to_first(maxd, wave_max);
     // every position has moved: the next step's n and h
