@@ -418,3 +418,32 @@ def test_batch_set_positions_then_forces_match_oracle():
         F2 = b.get(s, 6)
         err = np.abs(F2 - F1).max() / np.abs(F1).max()
         assert err <= 1e-5, (s, err)
+
+
+def test_server_force_tip_job_alone_equals_job_in_mix():
+    """A job pulled by a constant-force tip (the served form copies the force
+    tips with the job's model) gives the same result alone and among jobs
+    pulled by velocity tips or not pulled at all."""
+    prm = prm_full(qa_on=0)
+    models = ragged_models([70, 50, 110], seed0=60)
+    pulled = M.add_force_pulling(dict(models[0]), force_eps_per_A=2.0)
+    models[1] = M.add_pulling(models[1])
+    one = G.Server(prm, precision=G.FP64, chunk_steps=50)
+    j1 = one.submit(pulled, 250, seed=5)
+    _run_all(one)
+    mix = G.Server(prm, precision=G.FP64, chunk_steps=50, max_resident=2)
+    ids = [mix.submit(m, 100 + 80 * k, seed=30 + k) for k, m in enumerate(models[1:])]
+    j2 = mix.submit(pulled, 250, seed=5)
+    _run_all(mix)
+    assert one.status(j1) == ("done", 250) and mix.status(j2) == ("done", 250)
+    r1, V1 = one.result(j1)
+    r2, V2 = mix.result(j2)
+    assert np.abs(r1 - r2).max() < 1e-8
+    assert abs(V1 - V2) <= 1e-9 * abs(V1)
+    # the force tip acts: the same job without it ends elsewhere
+    free = M.add_force_pulling(dict(models[0]), force_eps_per_A=0.0)
+    ref = G.Server(prm, precision=G.FP64, chunk_steps=50)
+    j3 = ref.submit(free, 250, seed=5)
+    _run_all(ref)
+    r3, _ = ref.result(j3)
+    assert np.abs(r3 - r1).max() > 1e-6
