@@ -51,6 +51,7 @@ enum pcg_term {
   PCG_SND,
   PCG_VEL_AFM,
   PCG_FORCE_AFM, /* constant-force AFM tips (afm/force/eval_forces.cpp:10-13) */
+  PCG_SOLID_WALL, /* solid walls at fixed planes (wall/solid/eval_forces.cpp:4-17) */
   PCG_NUM_TERMS
 };
 
@@ -106,6 +107,13 @@ typedef struct pcg_params {
   double charge[21];
   int32_t ptype[21];
   int32_t base_sync[21][4];
+  /* solid walls (sbox.walls "solid", state.cpp:610-631): planes through
+   * origin with unit normal, potential depth d^-10 within min_dist
+   * (sbox.walls.threshold); n_walls 0 turns them off. Shared by every
+   * system of a batch. */
+  int32_t n_walls;
+  double wall_origin[6][3], wall_normal[6][3];
+  double wall_depth, wall_min_dist;
 } pcg_params;
 
 /* One protein system after state::compile_model

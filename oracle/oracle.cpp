@@ -891,6 +891,19 @@ struct ora_sys {
     for (auto const &tp : ftips) add_F(Fo, tp.first, tp.second);
   }
 
+  /* wall::solid::eval_forces::iter, cg/src/cg/wall/solid/eval_forces.cpp:4-17 */
+  void eval_solid_walls(double *Fo) {
+    for (int i = 0; i < n; ++i)
+      for (int w = 0; w < p.n_walls; ++w) {
+        v3 nrm{p.wall_normal[w][0], p.wall_normal[w][1], p.wall_normal[w][2]};
+        v3 org{p.wall_origin[w][0], p.wall_origin[w][1], p.wall_origin[w][2]};
+        double sd = dot(nrm, r[i] - org), dist = std::abs(sd);
+        if (dist >= p.wall_min_dist) continue;
+        double dV = p.wall_depth * std::pow(dist, -10.0) * (double)((0.0 < sd) - (sd < 0.0));
+        add_F(Fo, i, dV * nrm);
+      }
+  }
+
   bool term_enabled(int term) const {
     switch (term) {
     case ORA_CHIR: return p.chir_on;
@@ -910,6 +923,7 @@ struct ora_sys {
     case ORA_SND: return p.nat_dih_on && p.nat_dih_variant == 1;
     case ORA_VEL_AFM: return p.afm_on;
     case ORA_FORCE_AFM: return p.afm_on;
+    case ORA_SOLID_WALL: return p.n_walls > 0;
     }
     return false;
   }
@@ -933,6 +947,7 @@ struct ora_sys {
     case ORA_SND: eval_nat_dih(Fo, Vo, 1); break;
     case ORA_VEL_AFM: eval_vel_afm(Fo, Vo); break;
     case ORA_FORCE_AFM: eval_force_afm(Fo); break;
+    case ORA_SOLID_WALL: eval_solid_walls(Fo); break;
     }
   }
 

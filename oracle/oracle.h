@@ -39,6 +39,7 @@ enum ora_term {
   ORA_SND,
   ORA_VEL_AFM,
   ORA_FORCE_AFM,
+  ORA_SOLID_WALL,
   ORA_NUM_TERMS
 };
 
@@ -109,6 +110,10 @@ typedef struct ora_params {
   double charge[21];
   int32_t ptype[21];       /* 0 none, 1 polar, 2 hydrophobic */
   int32_t base_sync[21][4]; /* back, side_all, side_polar, side_hydrophobic */
+  /* solid walls: planes (origin, unit normal), depth, min dist */
+  int32_t n_walls;
+  double wall_origin[6][3], wall_normal[6][3];
+  double wall_depth, wall_min_dist;
 } ora_params;
 
 /* Input model, already compiled to residue indices (state::compile_model,

@@ -38,6 +38,7 @@
 #include <cg/qa/process_contacts.h>
 #include <cg/qa/update_free_pairs.h>
 #include <cg/tether/eval_forces.h>
+#include <cg/wall/solid/eval_forces.h>
 
 #include "oracle.h"
 
@@ -433,6 +434,28 @@ double ref_force_afm(int n, double t, int k, const int *res, const double *force
   e.afm_tips = tips;
   e.v = v;
   e.t = &tt;
+  run(e);
+  from_vec(Fv, F);
+  return 0.0;
+}
+
+// wall::solid::eval_forces over the walls of the parameters
+double ref_solid_wall(int n, const double *r, const ora_params *p, double *F) {
+  auto R = to_vec(n, r);
+  vect::vector<vec3r> Fv(n), wall_F(p->n_walls);
+  vect::vector<wall::solid::wall> walls;
+  for (int w = 0; w < p->n_walls; ++w) {
+    vec3r org(p->wall_origin[w][0], p->wall_origin[w][1], p->wall_origin[w][2]);
+    vec3r nrm(p->wall_normal[w][0], p->wall_normal[w][1], p->wall_normal[w][2]);
+    walls.emplace_back(cg::plane<real>(org, nrm), vect::iterator<vec3r>(wall_F[w]), (real)1.0);
+  }
+  wall::solid::eval_forces e;
+  e.min_dist = p->wall_min_dist;
+  e.walls = walls;
+  e.depth = p->wall_depth;
+  e.r = R;
+  e.F = Fv;
+  e.wall_F = wall_F;
   run(e);
   from_vec(Fv, F);
   return 0.0;

@@ -741,6 +741,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     term_loop<PCG_PID, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_VEL_AFM, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_FORCE_AFM, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    term_loop<PCG_SOLID_WALL, R>(b, sd, st, g, acc, sizes, vacc, wv);
     __syncthreads();
     if (P.p.qa_on) qa_finish_block(b, sd, st, (int8_t *)syl, r0);
     // lang::step (cg/src/cg/langevin/step.cpp:23-77); each residue's force,

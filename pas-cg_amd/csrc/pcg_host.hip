@@ -380,6 +380,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
                void *pool_block = nullptr, size_t pool_bytes = 0) {
   if (nsys <= 0 || !models) return fail(PCG_E_ARG, "no systems");
   if (prec != PCG_MIXED && prec != PCG_FP64) return fail(PCG_E_ARG, "unknown precision");
+  if (p.n_walls < 0 || p.n_walls > 6) return fail(PCG_E_ARG, "n_walls must be in [0, 6]");
   auto B = std::make_unique<DeviceBatch>();
   B->prec = prec;
   B->device = device;
@@ -1102,6 +1103,7 @@ static int host_term_size(const SysDesc &s, const SysState &st, int term) {
   case PCG_CND: case PCG_SND: v = s.nndih; break;
   case PCG_VEL_AFM: v = s.ntip; break;
   case PCG_FORCE_AFM: v = s.nftip; break;
+  case PCG_SOLID_WALL: v = s.n; break;
   }
   return v;
 }

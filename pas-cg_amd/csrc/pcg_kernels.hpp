@@ -109,6 +109,7 @@ __device__ __forceinline__ int term_size(SysDescC &s, const SysState &st, int te
   case PCG_SND: return s.nndih;
   case PCG_VEL_AFM: return s.ntip;
   case PCG_FORCE_AFM: return s.nftip;
+  case PCG_SOLID_WALL: return s.n;
   }
   return 0;
 }
@@ -133,6 +134,7 @@ __device__ __forceinline__ bool term_on(DevParamsC &P, int term) {
   case PCG_SND: return p.nat_dih_on && p.nat_dih_variant == 1;
   case PCG_VEL_AFM: return p.afm_on;
   case PCG_FORCE_AFM: return p.afm_on;
+  case PCG_SOLID_WALL: return p.n_walls > 0;
   }
   return false;
 }
@@ -162,6 +164,7 @@ __device__ __forceinline__ double eval_item(const DevBatch &b, SysDescC &s, SysS
   case PCG_SND: return nat_dih_item<double>(b, g, acc, s.ndih0 + k, 1);
   case PCG_VEL_AFM: return vel_afm_item<double>(b, g, acc, s.tip0 + k, st.t - s.afm_t0);
   case PCG_FORCE_AFM: return force_afm_item(b, acc, s.tip0 + s.ntip + k);
+  case PCG_SOLID_WALL: return solid_wall_item(b, g, acc, s.res0 + k);
   }
   return 0.0;
 }

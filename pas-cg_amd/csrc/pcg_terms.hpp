@@ -596,6 +596,33 @@ __device__ __forceinline__ double force_afm_item(const DevBatch &b, A &acc, int 
   return 0.0;
 }
 
+// wall::solid::eval_forces::iter (cg/src/cg/wall/solid/eval_forces.cpp:4-17):
+// residue i against every wall plane, d^-10 within min_dist, no energy; the
+// walls' own reaction forces (wall_F, statistics) are not kept.
+template <typename G, typename A>
+__device__ __forceinline__ double solid_wall_item(const DevBatch &b, const G &g, A &acc, int i) {
+#pragma clang fp contract(off)
+  ParamsC &p = b.P->p;
+  const v3<double> r = g.r(i);
+  v3<double> f{0.0, 0.0, 0.0};
+  bool hit = false;
+  for (int w = 0; w < p.n_walls; ++w) {
+    const double nx = p.wall_normal[w][0], ny = p.wall_normal[w][1], nz = p.wall_normal[w][2];
+    const double sd = nx * (r.x - p.wall_origin[w][0]) + ny * (r.y - p.wall_origin[w][1]) +
+                      nz * (r.z - p.wall_origin[w][2]);
+    const double dist = fabs(sd);
+    if (dist >= p.wall_min_dist) continue;
+    // dist^-10 by products of the reciprocal (a few ulp from pow(dist, -10),
+    // without pow's registers in the persistent kernel)
+    const double i1 = 1.0 / dist, i2 = i1 * i1, i4 = i2 * i2, i10 = i4 * i4 * i2;
+    const double dV = p.wall_depth * i10 * (double)((0.0 < sd) - (sd < 0.0));
+    f = f + v3<double>{dV * nx, dV * ny, dV * nz};
+    hit = true;
+  }
+  if (hit) acc.add(i, f);
+  return 0.0;
+}
+
 // ------------------------------------------------------------------ QA ----
 // qa::prepare_nh::iter (cg/src/cg/qa/prepare_nh.cpp:10-19), exact FP64:
 // n and h of residue i from its own and its chain neighbours' positions.

@@ -229,6 +229,13 @@ def default_params() -> dict:
     p["afm_on"] = 0
     p["afm_H1"] = 30.0 / (A ** 2)
     p["afm_H2"] = 0.0
+    # simulation box walls (sbox.walls): none by default; depth of the "solid"
+    # type and the wall threshold
+    p["n_walls"] = 0
+    p["wall_origin"] = [[0.0, 0.0, 0.0] for _ in range(6)]
+    p["wall_normal"] = [[0.0, 0.0, 0.0] for _ in range(6)]
+    p["wall_depth"] = 4.0 * EPS
+    p["wall_min_dist"] = 5.0 * A
     p["lang_on"] = 1
     p["gamma"] = f32(2.0)
     p["temperature"] = f32(0.35)
@@ -298,3 +305,24 @@ def set_pid_forces(p: dict, variant: str = "lj", bbp_depth=None, bbm_depth=None,
 
 def clone(p: dict) -> dict:
     return copy.deepcopy(p)
+
+
+def set_solid_walls(p: dict, lo, hi, axes="xyz") -> None:
+    """Solid walls on the faces of the box [lo, hi] along the given axes: on
+    each axis a wall at lo facing +axis and one at hi facing -axis, in that
+    order (state::setup_walls with the walls moved to the box faces,
+    cg/src/cg/simul/state.cpp:588-640)."""
+    origin, normal = [], []
+    for a in axes:
+        k = "xyz".index(a)
+        e = [0.0, 0.0, 0.0]
+        e[k] = 1.0
+        o_lo = [0.0, 0.0, 0.0]
+        o_lo[k] = float(lo[k])
+        o_hi = [0.0, 0.0, 0.0]
+        o_hi[k] = float(hi[k])
+        origin += [o_lo, o_hi]
+        normal += [e, [-x for x in e]]
+    p["n_walls"] = len(origin)
+    p["wall_origin"] = origin + [[0.0, 0.0, 0.0]] * (6 - len(origin))
+    p["wall_normal"] = normal + [[0.0, 0.0, 0.0]] * (6 - len(normal))

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libpcg.so")
 
 TERMS = ["chir", "tether", "lrep", "nat_cont", "pauli", "const_dh", "rel_dh",
          "qa_cand", "qa_cont", "pid", "nat_ang", "heur_ang", "heur_dih", "cnd",
-         "snd", "vel_afm", "force_afm"]
+         "snd", "vel_afm", "force_afm", "solid_wall"]
 TERM_ID = {t: i for i, t in enumerate(TERMS)}
 # pcg_report_field order: the scalars of make_report::add_cur_scalars
 REPORT_FIELDS = ["TIME", "EPOT", "ETOT", "EKIN", "RG", "L", "RMSD", "W", "ICN", "ICNss",
@@ -57,6 +57,8 @@ class PcgParams(C.Structure):
         ("lang_on", I), ("gamma", D), ("temperature", D), ("dt", D),
         ("mass", D * 21), ("charge", D * 21), ("ptype", I * 21),
         ("base_sync", (I * 4) * 21),
+        ("n_walls", I), ("wall_origin", (D * 3) * 6), ("wall_normal", (D * 3) * 6),
+        ("wall_depth", D), ("wall_min_dist", D),
     ]
 
 

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "build", "liboracle.so")
 
 TERMS = ["chir", "tether", "lrep", "nat_cont", "pauli", "const_dh", "rel_dh",
          "qa_cand", "qa_cont", "pid", "nat_ang", "heur_ang", "heur_dih", "cnd",
-         "snd", "vel_afm", "force_afm"]
+         "snd", "vel_afm", "force_afm", "solid_wall"]
 TERM_ID = {t: i for i, t in enumerate(TERMS)}
 
 D = C.c_double
@@ -51,6 +51,8 @@ class OraParams(C.Structure):
         ("lang_on", I), ("gamma", D), ("temperature", D), ("dt", D),
         ("mass", D * 21), ("charge", D * 21), ("ptype", I * 21),
         ("base_sync", (I * 4) * 21),
+        ("n_walls", I), ("wall_origin", (D * 3) * 6), ("wall_normal", (D * 3) * 6),
+        ("wall_depth", D), ("wall_min_dist", D),
     ]
 
 

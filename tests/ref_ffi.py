@@ -41,6 +41,7 @@ def lib():
             "ref_pid": [I, PD, PD, PU8, I, PI, PD, PI, PP, PD],
             "ref_vel_afm": [I, PD, D, I, PI, PD, PD, D, D, PD],
             "ref_force_afm": [I, D, I, PI, PD, PD],
+            "ref_solid_wall": [I, PD, PP, PD],
             "ref_qa_round": [I, PD, PD, PU8, PI, PP, D, PI8, PI, PI, PD, PU8, PI, PI, PD,
                              PI, PD, PI8, PU8, PI, I, PD],
         }

@@ -61,7 +61,16 @@ CONFIGS = {
     "multichain_qa": lambda: (full_params(qa_on=1, pid_on=0), multichain(seed=3)),
     "protein_force_afm": lambda: (full_params(qa_on=0),
                                   M.add_force_pulling(protein(seed=8), force_eps_per_A=2.5)),
+    "protein_walls": lambda: _walled(protein(seed=12)),
 }
+
+
+def _walled(m, gap_A=4.0):
+    """Solid walls on the faces of the model's bounding box widened by gap_A."""
+    prm = full_params(qa_on=0)
+    P.set_solid_walls(prm, m["pos"].min(axis=0) - gap_A * P.ANGSTROM,
+                      m["pos"].max(axis=0) + gap_A * P.ANGSTROM)
+    return prm, m
 
 
 def _sink(p):
@@ -98,7 +107,8 @@ def test_lists_bit_exact(cfg):
 
 
 @pytest.mark.parametrize("precision", [G.FP64, G.MIXED], ids=["fp64", "mixed"])
-@pytest.mark.parametrize("cfg", ["protein", "protein_sink_pid", "multichain", "protein_force_afm"])
+@pytest.mark.parametrize("cfg", ["protein", "protein_sink_pid", "multichain", "protein_force_afm",
+                                 "protein_walls"])
 def test_terms_match_oracle(cfg, precision):
     prm, m, o, c = same_state(cfg, precision)
     o.fix_nl()
@@ -244,7 +254,8 @@ def test_c1_box_per_term_errors_mixed():
     assert checked >= 5
 
 
-@pytest.mark.parametrize("cfg", ["multichain", "multichain_qa", "protein_force_afm"])
+@pytest.mark.parametrize("cfg", ["multichain", "multichain_qa", "protein_force_afm",
+                                 "protein_walls"])
 def test_batch_periodic_multichain_follows_oracle(cfg):
     """The persistent kernel on a periodic box of several chains (wrapped
     pairs; inter-chain Pauli, DH and PID, or QA with its sync budgets)

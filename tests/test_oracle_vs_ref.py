@@ -136,6 +136,8 @@ def term_ref(term, o, prm, m):
         vel = np.ascontiguousarray(m["tip_vel"], np.float64)
         V = L.ref_vel_afm(n, ptr(r, D), o.t, len(res), ptr(res, I), ptr(orig, D),
                           ptr(vel, D), prm["afm_H1"], prm["afm_H2"], ptr(F, D))
+    elif term == "solid_wall":
+        V = L.ref_solid_wall(n, ptr(r, D), C.byref(ps), ptr(F, D))
     elif term == "force_afm":
         res = np.ascontiguousarray(m["ftip_res"], np.int32)
         frc = np.ascontiguousarray(m["ftip_force"], np.float64)
@@ -406,3 +408,25 @@ def test_force_afm_tips():
     F2, V2 = term_ref("force_afm", o, prm, m)
     assert np.abs(F2).max() > 0 and V1 == 0.0 and V2 == 0.0
     close(F1, V1, F2, V2, tol=1e-15)
+
+
+def walled(m, gap_A=4.0):
+    """Solid walls on the six faces of the model's bounding box widened by
+    gap_A, so the outermost residues sit inside the wall threshold."""
+    prm = full_params(qa_on=0)
+    lo = m["pos"].min(axis=0) - gap_A * P.ANGSTROM
+    hi = m["pos"].max(axis=0) + gap_A * P.ANGSTROM
+    P.set_solid_walls(prm, lo, hi)
+    return prm
+
+
+def test_solid_walls():
+    """Solid walls: the oracle equals wall::solid::eval_forces."""
+    m = perturbed_protein()
+    prm = walled(m)
+    o = Oracle(prm, m, seed=9)
+    advance(o, 5)
+    F1, V1 = o.eval_term("solid_wall")
+    F2, V2 = term_ref("solid_wall", o, prm, m)
+    assert (np.abs(F2).max(axis=1) > 0).sum() >= 6
+    close(F1, V1, F2, V2, tol=1e-14)
