@@ -88,3 +88,37 @@ def test_out_file_reads_back_to_the_reports():
         assert float(vals["RG[A]"]) == pytest.approx(rep["RG"] / P.ANGSTROM, rel=1e-5)
         assert int(vals["ICN"]) == int(rep["ICN"])
     c.close()
+
+
+def test_pdb_snapshots_layout_and_round_trip():
+    """The `.pdb` snapshots of make_report::emit_pdb: fixed columns of the
+    reference's record writers (records.cpp), and a file the PDB reader
+    reads back to the same C-alpha positions (to the 3 printed decimals)."""
+    import numpy as np
+    from pas_cg_amd import inputs as I
+    from pas_cg_amd import model as M
+    m = M.replicate(M.synthetic_protein(12, 3), 2, 60 * P.ANGSTROM, 3)
+    snaps = [(0.0, m["pos"]), (2.5, m["pos"] + 0.01)]
+    txt = O.pdb_snapshots(snaps, m["chain"], m["atype"], m["cell"])
+    lines = txt.split("\n")
+    assert all(len(ln) == 80 for ln in lines) and not txt.endswith("\n")
+    assert lines[0].startswith("MODEL        1") and lines[-1].startswith("ENDMDL")
+    assert lines[1][:33] == "CRYST1   60.000   60.000   60.000"
+    assert lines[1][55:70] == "        P 1   1"
+    atom = next(ln for ln in lines if ln.startswith("ATOM"))
+    assert atom[6:11] == "    1" and atom[12:16] == "  CA" and atom[21] == "A" and atom[22:26] == "   1"
+    x = m["pos"][0] / P.ANGSTROM
+    assert atom[30:54] == "%8.3f%8.3f%8.3f" % tuple(x)
+    ter = [ln for ln in lines if ln.startswith("TER")]
+    assert len(ter) == 4 and ter[0][6:11] == "   13" and ter[0][22:26] == "  12"
+    rem = [ln for ln in lines if ln.startswith("REMARK")]
+    assert len(rem) == 8 and "T[TAU]" in rem[0] and "RG[A]" in rem[0]
+    st = I.read_pdb(txt)   # the first model
+    assert list(st.chains) == ["A", "B"]
+    got = np.array([a[2] for res in st.chains["A"] + st.chains["B"] for a in res["atoms"]])
+    assert np.abs(got - m["pos"]).max() < 6e-4 * P.ANGSTROM
+    # the REMARK row carries the chain's radius of gyration in A
+    vals = rem[1][11:].split()
+    r = m["pos"][m["chain"] == 0]
+    rg = np.sqrt(((r - r.mean(axis=0)) ** 2).sum() / 12.0) / P.ANGSTROM
+    assert abs(float(vals[1]) - rg) <= 1e-5 * rg
