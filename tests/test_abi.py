@@ -47,3 +47,20 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(G, "_lib", None)
     with pytest.raises(G.PcgError):
         G.lib()
+
+
+def _enum_names(first: str, last: str):
+    src = open(os.path.join(ROOT, "include", "pcg.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    body = src[src.index(first):src.index(last)]
+    return [n.split("=")[0].strip() for n in body.split(",") if n.strip()]
+
+
+def test_term_and_report_enums_match_the_python_mirror():
+    """The term ids and the report fields of pcg.h, in order, are the ones the
+    Python mirror indexes with (TERMS, REPORT_FIELDS)."""
+    terms = _enum_names("PCG_CHIR", "PCG_NUM_TERMS")
+    assert [t[4:].lower() for t in terms] == G.TERMS
+    fields = _enum_names("PCG_REP_TIME", "PCG_REPORT_NUM")
+    assert len(fields) == len(G.REPORT_FIELDS)
+    assert fields[0] == "PCG_REP_TIME" and fields[-1] == "PCG_REP_PID_S1S2"
