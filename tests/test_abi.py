@@ -64,3 +64,14 @@ def test_term_and_report_enums_match_the_python_mirror():
     fields = _enum_names("PCG_REP_TIME", "PCG_REPORT_NUM")
     assert len(fields) == len(G.REPORT_FIELDS)
     assert fields[0] == "PCG_REP_TIME" and fields[-1] == "PCG_REP_PID_S1S2"
+
+
+def test_oracle_terms_follow_the_library_order():
+    """The oracle's term ids (oracle/oracle.h) are the library's, so per-term
+    parity compares like with like."""
+    import oracle_ffi
+    assert oracle_ffi.TERMS == G.TERMS
+    src = open(os.path.join(ROOT, "oracle", "oracle.h")).read()
+    body = src[src.index("ORA_CHIR"):src.index("ORA_NUM_TERMS")]
+    names = [n.split("=")[0].strip()[4:].lower() for n in body.split(",") if n.strip()]
+    assert names == G.TERMS
