@@ -245,6 +245,16 @@ int pcg_stream(pcg_ctx *c, void **stream);
 /* Capture one full step as a hipGraph and replay it for later pcg_step
  * calls (steps without a NL rebuild); 0 turns it off. */
 int pcg_use_graph(pcg_ctx *c, int32_t on);
+/* Checkpoint and resume of the context (the fields of the reference's state
+ * serialisation, cg/src/cg/simul/state_io.cpp:117-156: positions, time,
+ * velocities, Gear y0..y5, QA contacts, sync budgets; here with the lists,
+ * the native-contact state and, in the RAN2 noise mode, the host ran2
+ * stream, which the reference does not save). The buffer is host memory of
+ * pcg_checkpoint_size bytes; restore needs a context created from the same
+ * model, parameters, precision and noise source. */
+int pcg_checkpoint_size(pcg_ctx *c, int64_t *bytes);
+int pcg_checkpoint(pcg_ctx *c, void *buf, int64_t bytes);
+int pcg_restore(pcg_ctx *c, const void *buf, int64_t bytes);
 
 /* ---- batched form: many systems, one persistent launch ---------------- */
 typedef struct pcg_batch pcg_batch;
@@ -337,6 +347,11 @@ int pcg_server_status(pcg_server *s, int64_t job, int32_t *status,
                       int64_t *steps_done);
 int pcg_server_result(pcg_server *s, int64_t job, double *pos /*[3n]*/,
                       double *V);
+/* one array [3n] of a finished job at its end: which 0 positions, 6 the forces
+ * of its last evaluation, 7 velocities (the reference's state::r, dyn.F, v) */
+int pcg_server_get(pcg_server *s, int64_t job, int32_t which, double *out);
+/* scales the Langevin noise of every resident and later job (0: no noise) */
+int pcg_server_set_noise_scale(pcg_server *s, double scale);
 int pcg_server_cancel(pcg_server *s, int64_t job);
 /* wall-clock seconds (steady clock) of submission, admission and end */
 int pcg_server_job_times(pcg_server *s, int64_t job, double *submit, double *start,

@@ -1497,55 +1497,122 @@ struct CkptHeader {
   int32_t nsys;
   int64_t ntot;
   int32_t prec;
-  int32_t pad;
+  int32_t nran2;  // host ran2 streams after the arena
   uint64_t arena_bytes;
   uint64_t fingerprint;
 };
+extern "C++" {
+// Digest of byte ranges and of the parameter fields one by one (the struct
+// has padding, whose bytes a caller need not have cleared).
+struct Digest {
+  uint64_t h = 0x5043474350434743ull;
+  void bytes(const void *p, size_t n) {
+    const unsigned char *c = (const unsigned char *)p;
+    for (size_t k = 0; k < n; ++k) h = (h ^ c[k]) * 0x100000001B3ull;
+    h = mix64(h ^ n);
+  }
+  template <typename T> void vec(const std::vector<T> &v) { bytes(v.data(), v.size() * sizeof(T)); }
+  template <typename T> void val(const T &x) { bytes(&x, sizeof x); }
+};
+void digest_params(Digest &d, const pcg_params &p) {
+#define PCG_D(f) d.val(p.f)
+  PCG_D(rep_cutoff); PCG_D(counting_factor); PCG_D(nl_cutoff); PCG_D(nl_pad);
+  PCG_D(tether_on); PCG_D(tether_H1); PCG_D(tether_H2); PCG_D(lrep_on); PCG_D(lrep_depth);
+  PCG_D(chir_on); PCG_D(e_chi); PCG_D(nat_ang_on); PCG_D(CBA); PCG_D(heur_ang_on);
+  PCG_D(heur_ang_poly); PCG_D(nat_dih_on); PCG_D(nat_dih_variant); PCG_D(CDA); PCG_D(CDB);
+  PCG_D(CDH); PCG_D(heur_dih_on); PCG_D(heur_dih_coeffs); PCG_D(pauli_on); PCG_D(pauli_depth);
+  PCG_D(nat_cont_on); PCG_D(nat_cont_depth); PCG_D(nc_ss_kind); PCG_D(nc_ss_H1); PCG_D(nc_ss_H2);
+  PCG_D(nc_ss_nat_r); PCG_D(nc_ss_depth); PCG_D(nc_ss_r_min); PCG_D(dh_on); PCG_D(dh_variant);
+  PCG_D(dh_screen_dist); PCG_D(dh_cutoff); PCG_D(dh_perm); PCG_D(qa_on); PCG_D(qa_include4);
+  PCG_D(qa_phase_dur); PCG_D(qa_formation_tol); PCG_D(qa_min_cos_hr); PCG_D(qa_min_cos_hh);
+  PCG_D(qa_max_cos_nr); PCG_D(qa_lj_depth); PCG_D(qa_lj_r_low); PCG_D(qa_lj_r_high);
+  PCG_D(pid_on); PCG_D(pid_include4); PCG_D(pid_lambda_variant); PCG_D(pid_bbp_alpha);
+  PCG_D(pid_bbp_psi0); PCG_D(pid_bbm_alpha); PCG_D(pid_bbm_psi0); PCG_D(pid_ss_alpha);
+  PCG_D(pid_ss_psi0); PCG_D(pid_bbp_lj); PCG_D(pid_bbm_lj); PCG_D(pid_ss_depth);
+  PCG_D(pid_ss_r_low); PCG_D(pid_ss_r_high); PCG_D(afm_on); PCG_D(afm_H1); PCG_D(afm_H2);
+  PCG_D(lang_on); PCG_D(gamma); PCG_D(temperature); PCG_D(dt); PCG_D(mass); PCG_D(charge);
+  PCG_D(ptype); PCG_D(base_sync); PCG_D(n_walls); PCG_D(wall_origin); PCG_D(wall_normal);
+  PCG_D(wall_depth); PCG_D(wall_min_dist);
+#undef PCG_D
+}
+}  // extern "C++"
+// What a checkpoint must be restored into: the same system layout, the same
+// parameters, the same models (types, chains, every static list and its
+// native values, tips) and the same precision.
 uint64_t ckpt_fingerprint(const DeviceBatch &B) {
-  uint64_t h = mix64((uint64_t)B.sys.size() ^ ((uint64_t)B.arena.total << 20));
-  for (const SysDesc &s : B.sys) h = mix64(h ^ (uint64_t)(uint32_t)s.n ^ ((uint64_t)(uint32_t)s.res0 << 32));
-  return mix64(h ^ (uint64_t)B.prec);
+  Digest d;
+  d.val(B.sys.size());
+  d.val(B.arena.total);
+  d.val(B.prec);
+  digest_params(d, B.hp.p);
+  for (size_t s = 0; s < B.sys.size(); ++s) {
+    const HostSys &h = B.hs[s];
+    d.val(B.sys[s].n);
+    d.val(B.sys[s].res0);
+    d.vec(h.atype); d.vec(h.chain); d.vec(h.seq);
+    for (auto &v : h.teth) d.vec(v);
+    d.vec(h.teth_nat);
+    for (auto &v : h.lrep) d.vec(v);
+    for (auto &v : h.chir) d.vec(v);
+    d.vec(h.chir_f0); d.vec(h.chir_f1);
+    for (auto &v : h.nang) d.vec(v);
+    d.vec(h.nang_th);
+    for (auto &v : h.hang) d.vec(v);
+    d.vec(h.hang_t);
+    for (auto &v : h.ndih) d.vec(v);
+    d.vec(h.ndih_ph);
+    for (auto &v : h.hdih) d.vec(v);
+    d.vec(h.hdih_t);
+    for (auto &v : h.nc) d.vec(v);
+    d.vec(h.nc_nat); d.vec(h.nc_type);
+    d.vec(h.tip_res);
+    d.val(h.nftip);
+    d.val(h.temperature);
+  }
+  return d.h;
 }
-}  // namespace
 
-int pcg_batch_checkpoint_size(pcg_batch *b, int64_t *bytes) {
-  if (!b || !bytes) return fail(PCG_E_ARG, "null argument");
-  *bytes = (int64_t)(sizeof(CkptHeader) + b->B->arena.total);
-  return PCG_OK;
+// A checkpoint is the header, the arena, then the host ran2 streams of the
+// RAN2 noise mode (one per system; none in the Philox mode, whose position is
+// the step counter in the arena).
+size_t ckpt_bytes(const DeviceBatch &B) {
+  return sizeof(CkptHeader) + B.arena.total + B.ran2.size() * sizeof(Ran2);
 }
 
-int pcg_batch_checkpoint(pcg_batch *b, void *buf, int64_t bytes) {
-  if (!b || !buf) return fail(PCG_E_ARG, "null argument");
-  DeviceBatch &B = *b->B;
-  if (bytes < (int64_t)(sizeof(CkptHeader) + B.arena.total))
-    return fail(PCG_E_ARG, "checkpoint buffer too small");
+int ckpt_write(DeviceBatch &B, const char magic[8], void *buf, int64_t bytes) {
+  if (!buf) return fail(PCG_E_ARG, "null argument");
+  if (bytes < (int64_t)ckpt_bytes(B)) return fail(PCG_E_ARG, "checkpoint buffer too small");
   HIPCHK(hipSetDevice(B.device));
   CkptHeader h{};
-  std::memcpy(h.magic, "PCGCKPT1", 8);
-  h.version = 1;
+  std::memcpy(h.magic, magic, 8);
+  h.version = 2;
   h.nsys = (int32_t)B.sys.size();
   h.ntot = B.d.ntot;
   h.prec = B.prec;
+  h.nran2 = (int32_t)B.ran2.size();
   h.arena_bytes = B.arena.total;
   h.fingerprint = ckpt_fingerprint(B);
   std::memcpy(buf, &h, sizeof h);
   HIPCHK(hipMemcpyAsync((char *)buf + sizeof h, B.arena.base, B.arena.total, hipMemcpyDeviceToHost,
                         B.stream));
   HIPCHK(hipStreamSynchronize(B.stream));
+  if (!B.ran2.empty())
+    std::memcpy((char *)buf + sizeof h + B.arena.total, B.ran2.data(), B.ran2.size() * sizeof(Ran2));
   return PCG_OK;
 }
 
-int pcg_batch_restore(pcg_batch *b, const void *buf, int64_t bytes) {
-  if (!b || !buf) return fail(PCG_E_ARG, "null argument");
-  DeviceBatch &B = *b->B;
+int ckpt_read(DeviceBatch &B, const char magic[8], const void *buf, int64_t bytes) {
+  if (!buf) return fail(PCG_E_ARG, "null argument");
   CkptHeader h;
   if (bytes < (int64_t)sizeof h) return fail(PCG_E_ARG, "not a checkpoint");
   std::memcpy(&h, buf, sizeof h);
-  if (std::memcmp(h.magic, "PCGCKPT1", 8) != 0 || h.version != 1) return fail(PCG_E_ARG, "not a checkpoint");
+  if (std::memcmp(h.magic, magic, 8) != 0 || h.version != 2) return fail(PCG_E_ARG, "not a checkpoint");
   if (h.nsys != (int32_t)B.sys.size() || h.ntot != B.d.ntot || h.prec != B.prec ||
       h.arena_bytes != B.arena.total || h.fingerprint != ckpt_fingerprint(B))
     return fail(PCG_E_ARG, "checkpoint of a different batch (models, parameters or precision)");
-  if (bytes < (int64_t)(sizeof h + h.arena_bytes)) return fail(PCG_E_ARG, "truncated checkpoint");
+  if (h.nran2 != (int32_t)B.ran2.size())
+    return fail(PCG_E_ARG, "checkpoint of another noise source (ran2 and Philox differ)");
+  if (bytes < (int64_t)ckpt_bytes(B)) return fail(PCG_E_ARG, "truncated checkpoint");
   HIPCHK(hipSetDevice(B.device));
   HIPCHK(hipMemcpyAsync(B.arena.base, (const char *)buf + sizeof h, B.arena.total,
                         hipMemcpyHostToDevice, B.stream));
@@ -1553,7 +1620,47 @@ int pcg_batch_restore(pcg_batch *b, const void *buf, int64_t bytes) {
   HIPCHK(hipMemcpyAsync(B.sys.data(), (const void *)B.d.sys, B.sys.size() * sizeof(SysDesc),
                         hipMemcpyDeviceToHost, B.stream));
   HIPCHK(hipStreamSynchronize(B.stream));
+  if (!B.ran2.empty())
+    std::memcpy(B.ran2.data(), (const char *)buf + sizeof h + B.arena.total,
+                B.ran2.size() * sizeof(Ran2));
   return PCG_OK;
+}
+
+}  // namespace
+
+int pcg_batch_checkpoint_size(pcg_batch *b, int64_t *bytes) {
+  if (!b || !bytes) return fail(PCG_E_ARG, "null argument");
+  *bytes = (int64_t)ckpt_bytes(*b->B);
+  return PCG_OK;
+}
+
+int pcg_batch_checkpoint(pcg_batch *b, void *buf, int64_t bytes) {
+  if (!b) return fail(PCG_E_ARG, "null argument");
+  return ckpt_write(*b->B, "PCGCKPTB", buf, bytes);
+}
+
+int pcg_batch_restore(pcg_batch *b, const void *buf, int64_t bytes) {
+  if (!b) return fail(PCG_E_ARG, "null argument");
+  return ckpt_read(*b->B, "PCGCKPTB", buf, bytes);
+}
+
+// The plain context: the same layout for its one system, with the ran2
+// stream of its RAN2 noise (state_io.cpp:117-156 saves the state but not
+// the generator; here a resumed run continues the same noise).
+int pcg_checkpoint_size(pcg_ctx *c, int64_t *bytes) {
+  if (!c || !bytes) return fail(PCG_E_ARG, "null argument");
+  *bytes = (int64_t)ckpt_bytes(*c->B);
+  return PCG_OK;
+}
+
+int pcg_checkpoint(pcg_ctx *c, void *buf, int64_t bytes) {
+  if (!c) return fail(PCG_E_ARG, "null argument");
+  return ckpt_write(*c->B, "PCGCKPTC", buf, bytes);
+}
+
+int pcg_restore(pcg_ctx *c, const void *buf, int64_t bytes) {
+  if (!c) return fail(PCG_E_ARG, "null argument");
+  return ckpt_read(*c->B, "PCGCKPTC", buf, bytes);
 }
 
 int pcg_batch_report(pcg_batch *b, double *out) {
@@ -1807,7 +1914,7 @@ struct Job {
   size_t bytes = 0;        // arena bytes (dry run)
   size_t block_bytes = 0;  // the pooled device block it runs in (>= bytes)
   std::unique_ptr<DeviceBatch> B;
-  std::vector<double> result;
+  std::vector<double> result, result_F, result_vel;  // [3n] each, at retirement
   double V = 0;
   double t_submit = 0, t_start = 0, t_end = 0;
 };
@@ -1830,6 +1937,7 @@ struct pcg_server {
   // (no hipMalloc/hipFree per job); sizes are multiples of 256 KiB
   std::multimap<size_t, void *> pool;
   size_t pooled_bytes = 0;
+  double noise_scale = 1.0;  // scales the Langevin noise of every job
   ~pcg_server() {
     for (auto &kv : jobs)
       if (kv.second->B && kv.second->B->arena.base)
@@ -1853,9 +1961,14 @@ int retire(pcg_server *s, Job &j, int status) {
   if (j.B) {
     SysState st;
     if (status == PCG_JOB_DONE) {
-      j.result.assign(3 * (size_t)j.model.n, 0.0);
+      const size_t n3 = 3 * (size_t)j.model.n;
+      j.result.assign(n3, 0.0);
+      j.result_F.assign(n3, 0.0);
+      j.result_vel.assign(n3, 0.0);
       int rc = get_cols(*j.B, 0, j.B->d.y[0], j.result.data());
       if (rc) return rc;
+      if ((rc = get_forces(*j.B, 0, j.result_F.data()))) return rc;
+      if ((rc = get_cols(*j.B, 0, j.B->d.vel, j.result_vel.data()))) return rc;
       if ((rc = read_state(*j.B, 0, st))) return rc;
       j.V = st.Vtot;
       j.done = st.step;
@@ -1976,6 +2089,12 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
       }
     }
     if (rc) return rc;
+    if (s->noise_scale != 1.0) {
+      SysDesc &d = j.B->sys[0];
+      d.const2 = const2_of(j.B->hp.p, d.temperature) * s->noise_scale;
+      HIPCHK(hipMemcpyAsync((void *)j.B->d.sys, &d, sizeof(SysDesc), hipMemcpyHostToDevice,
+                            s->stream));
+    }
     j.status = PCG_JOB_RUNNING;
     j.t_start = now_s();
     j.block_bytes = blk_bytes;
@@ -2065,6 +2184,33 @@ int pcg_server_result(pcg_server *s, int64_t job, double *pos, double *V) {
   if (j.status != PCG_JOB_DONE) return fail(PCG_E_STATE, "job not finished");
   if (pos) std::memcpy(pos, j.result.data(), j.result.size() * sizeof(double));
   if (V) *V = j.V;
+  return PCG_OK;
+}
+
+int pcg_server_get(pcg_server *s, int64_t job, int32_t which, double *out) {
+  if (!s || !out || (which != 0 && which != 6 && which != 7)) return fail(PCG_E_ARG, "bad arguments");
+  auto it = s->jobs.find(job);
+  if (it == s->jobs.end()) return fail(PCG_E_NOTFOUND, "no such job");
+  Job &j = *it->second;
+  if (j.status != PCG_JOB_DONE) return fail(PCG_E_STATE, "job not finished");
+  const std::vector<double> &v = which == 0 ? j.result : which == 6 ? j.result_F : j.result_vel;
+  std::memcpy(out, v.data(), v.size() * sizeof(double));
+  return PCG_OK;
+}
+
+int pcg_server_set_noise_scale(pcg_server *s, double scale) {
+  if (!s || !(scale >= 0)) return fail(PCG_E_ARG, "bad arguments");
+  HIPCHK(hipSetDevice(s->o.device));
+  s->noise_scale = scale;
+  // resident jobs change from their next round on
+  for (int64_t id : s->resident) {
+    Job &j = *s->jobs[id];
+    SysDesc &d = j.B->sys[0];
+    d.const2 = const2_of(j.B->hp.p, d.temperature) * scale;
+    HIPCHK(hipMemcpyAsync((void *)j.B->d.sys, &d, sizeof(SysDesc), hipMemcpyHostToDevice,
+                          s->stream));
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
   return PCG_OK;
 }
 

@@ -156,12 +156,18 @@ def _compact_saw(n: int, rng: np.random.Generator, bond: float, excl: float,
 
 
 def synthetic_protein(n: int, seed: int, native_cutoff_A: float = 7.5,
-                      bond_A: float = 3.8, charged: bool = True) -> dict:
+                      bond_A: float = 3.8, charged: bool = True,
+                      heuristic_frac: float = 0.0) -> dict:
     """Seeded synthetic C-alpha protein: residue types uniform over the 20
     standard amino acids, a compact self-avoiding native conformation with
     3.8 A bonds, native contacts |i-j| >= 3 with native distance < 7.5 A
     (type ANY), native bond angles and dihedrals from that conformation.
-    The simulation starts from the native conformation."""
+    The simulation starts from the native conformation.
+
+    heuristic_frac > 0 drops the native value of that (seeded) fraction of
+    the bond angles and dihedrals, which then take the heuristic potentials
+    of their residue pair (a model read from a sequence file without a map
+    has none, seq_file.cpp:103-136; state.cpp:337-352)."""
     rng = np.random.Generator(np.random.PCG64(seed & 0xFFFFFFFFFFFFFFFF))
     atype = rng.integers(0, 20, size=n).astype(np.uint8)
     bond = P.f32(bond_A) * ANGSTROM
@@ -183,7 +189,35 @@ def synthetic_protein(n: int, seed: int, native_cutoff_A: float = 7.5,
          "cont": cont, "cont_len": d[ii, jj].astype(np.float64),
          "cont_type": np.zeros(len(cont), np.int32),
          "cell": np.zeros(3)}
+    if heuristic_frac > 0:
+        hr = np.random.Generator(np.random.PCG64((seed ^ 0x4E55) & 0xFFFFFFFFFFFFFFFF))
+        m["ang_has"][hr.uniform(size=len(ang)) < heuristic_frac] = 0
+        m["dih_has"][hr.uniform(size=len(dih)) < heuristic_frac] = 0
     return empty_tips(m)
+
+
+def add_disulfides(m: dict, count: int, seed: int = 0) -> dict:
+    """Marks `count` native contacts (seeded choice among those whose residues
+    are not in another marked contact) as disulfide bonds: both residues
+    become CYS and the contact type SSBOND (5), as the PDB reader types the
+    contacts of SSBOND records (pdb_file.cpp:404-420)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    used: set = set()
+    picks = []
+    for k in rng.permutation(len(m["cont"])):
+        a, b = (int(x) for x in m["cont"][k])
+        if a in used or b in used:
+            continue
+        picks.append(int(k))
+        used.update((a, b))
+        if len(picks) == count:
+            break
+    m["cont_type"] = np.asarray(m["cont_type"], np.int32).copy()
+    m["atype"] = np.asarray(m["atype"], np.uint8).copy()
+    for k in picks:
+        m["cont_type"][k] = 5
+        m["atype"][m["cont"][k]] = P.AA_INDEX["CYS"]
+    return m
 
 
 def replicate(m: dict, copies: int, box_side: float, seed: int,

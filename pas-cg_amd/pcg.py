@@ -146,6 +146,11 @@ SIGNATURES = {
     "pcg_server_status": [_V, I64, C.POINTER(I), C.POINTER(I64)],
     "pcg_server_result": [_V, I64, C.POINTER(D), C.POINTER(D)],
     "pcg_server_cancel": [_V, I64],
+    "pcg_server_get": [_V, I64, I, C.POINTER(D)],
+    "pcg_server_set_noise_scale": [_V, D],
+    "pcg_checkpoint_size": [_V, C.POINTER(I64)],
+    "pcg_checkpoint": [_V, _V, I64],
+    "pcg_restore": [_V, _V, I64],
     "pcg_server_stats": [_V, C.POINTER(I64), C.POINTER(I), C.POINTER(I)],
     "pcg_server_job_times": [_V, I64, C.POINTER(D), C.POINTER(D), C.POINTER(D)],
     "pcg_batch_set_noise_scale": [_V, D],
@@ -406,6 +411,20 @@ class Context:
         check(self.L.pcg_qa_state(self.h, _p(sync, C.c_int8), C.byref(num)))
         return sync, num.value
 
+    def checkpoint(self) -> bytes:
+        """The context's whole state, with its ran2 stream (restore())."""
+        n = I64()
+        check(self.L.pcg_checkpoint_size(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(self.L.pcg_checkpoint(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, ck: bytes):
+        """Resumes from a checkpoint of a context of the same model,
+        parameters, precision and noise source."""
+        buf = C.create_string_buffer(ck, len(ck))
+        check(self.L.pcg_restore(self.h, buf, len(ck)))
+
     def stream(self) -> int:
         """The HIP stream every launch of this context is enqueued on."""
         st = C.c_void_p()
@@ -583,6 +602,15 @@ class Server:
         V = C.c_double()
         check(self.L.pcg_server_result(self.h, job, _p(pos, D), C.byref(V)))
         return pos, V.value
+
+    def get(self, job: int, which: int) -> np.ndarray:
+        """Positions (0), last forces (6) or velocities (7) of a finished job."""
+        out = np.zeros((self.ns[job], 3))
+        check(self.L.pcg_server_get(self.h, job, which, _p(out, D)))
+        return out
+
+    def set_noise_scale(self, s: float):
+        check(self.L.pcg_server_set_noise_scale(self.h, s))
 
     def cancel(self, job: int):
         check(self.L.pcg_server_cancel(self.h, job))

@@ -447,3 +447,66 @@ def test_server_force_tip_job_alone_equals_job_in_mix():
     _run_all(ref)
     r3, _ = ref.result(j3)
     assert np.abs(r3 - r1).max() > 1e-6
+
+
+def test_batch_checkpoint_resume_follows_oracle():
+    """A batch checkpointed mid-run and resumed in a new batch continues the
+    oracle's noise-free FP64 trajectory (QA contacts, sync budgets, lists and
+    native-contact state all come back with it); a batch with other
+    parameters or other models of the same sizes refuses the checkpoint."""
+    prm = prm_full(qa_on=1)
+    models = ragged_models([45, 130, 70], seed0=310)
+    b = G.Batch(prm, models, seeds=[1] * 3, precision=G.FP64, steps_per_launch=50)
+    b.set_noise_scale(0.0)
+    b.run(100)
+    ck = b.checkpoint()
+    b.close()
+    b2 = G.Batch(prm, models, seeds=[1] * 3, precision=G.FP64, steps_per_launch=50)
+    b2.restore(ck)
+    b2.run(100)
+    for s, m in enumerate(models):
+        o = Oracle(prm, m, seed=1)
+        o.set_noise_scale(0.0)
+        tr = o.step(200)
+        assert np.abs(b2.get(s, 0) - o.r()).max() < 1e-8, s
+        V, _ = b2.energy(s)
+        assert abs(V - tr[-1]) <= 1e-9 * abs(tr[-1]), (s, V, tr[-1])
+    b2.close()
+    other_p = prm_full(qa_on=1, temperature=0.4)
+    b3 = G.Batch(other_p, models, seeds=[1] * 3, precision=G.FP64, steps_per_launch=50)
+    with pytest.raises(G.PcgError):
+        b3.restore(ck)
+    b3.close()
+    other_m = ragged_models([45, 130, 70], seed0=410)
+    b4 = G.Batch(prm, other_m, seeds=[1] * 3, precision=G.FP64, steps_per_launch=50)
+    with pytest.raises(G.PcgError):
+        b4.restore(ck)
+    b4.close()
+
+
+def test_context_checkpoint_resume_follows_oracle():
+    """The plain context with the reference's ran2 noise: checkpointed after
+    60 steps and resumed in a new context, it continues the oracle's noisy
+    FP64 trajectory step for step (the ran2 stream is part of the
+    checkpoint); the same context without the restore would not."""
+    prm = prm_full(qa_on=1, pid_on=1)
+    m = ragged_models([80], seed0=330)[0]
+    c = G.Context(prm, m, precision=G.FP64, noise=G.RAN2, seed=7)
+    c.step(60)
+    ck = c.checkpoint()
+    c.close()
+    c2 = G.Context(prm, m, precision=G.FP64, noise=G.RAN2, seed=7)
+    c2.restore(ck)
+    o = Oracle(prm, m, seed=7)
+    o.step(60)
+    tr = o.step(60)
+    for k in range(60):
+        c2.step(1)
+        assert abs(c2.energy()[0] - tr[k]) <= 1e-9 * abs(tr[k]), (k, c2.energy()[0], tr[k])
+    assert np.abs(c2.r() - o.r()).max() < 1e-9
+    assert c2.time()[1] == 120
+    c3 = G.Context(prm, m, precision=G.FP64, noise=G.PHILOX, seed=7)
+    with pytest.raises(G.PcgError):
+        c3.restore(ck)  # a checkpoint of the ran2 noise source
+    c3.close()
+    c2.close()

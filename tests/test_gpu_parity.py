@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from golden_cases import case
+from lockstep import qa_lockstep
 from oracle_ffi import Oracle
 from pas_cg_amd import inputs as I
 from pas_cg_amd import model as M
@@ -221,13 +222,15 @@ def test_c1_box_per_term_errors_mixed():
     """64 copies of a 100-residue chain in a periodic box at 1 mM with Debye-
     Hueckel, QA and PID (the plain path, stage kernels): after a Langevin
     run, every term in mixed precision against the FP64 oracle on the same
-    positions, to 1e-5 in force and 1e-6 in energy."""
+    positions, to 1e-5 in force and 1e-6 in energy; then the QA terms with
+    the rest, step by step on the oracle's states through QA finishes."""
     from pas_cg_amd import workloads as W
     prm, m = W.c1()
     c = G.Context(prm, m, precision=G.MIXED, noise=G.PHILOX, seed=5)
     c.use_graph(True)
     c.step(300)
     r = c.r()
+    c_time = c.time()[0]
     o = Oracle(prm, m, seed=5)
     o.set_r(r)
     o.set_t(c.time()[0])
@@ -252,6 +255,11 @@ def test_c1_box_per_term_errors_mixed():
         assert np.abs(F2 - F1).max() / scale <= MIXED_F_TOL, term
         assert abs(V2 - V1) <= MIXED_V_TOL * abs(V1) + 1e-12, (term, V1, V2)
     assert checked >= 5
+    c.close()
+    c2.close()
+    worst, _ = qa_lockstep(prm, m, G.MIXED, 4, r0=r, t0=c_time, seed=5)
+    assert worst["total"] <= MIXED_F_TOL, worst
+    assert worst["V_qa_cand"] <= MIXED_V_TOL and worst["V_qa_cont"] <= MIXED_V_TOL, worst
 
 
 @pytest.mark.parametrize("cfg", ["multichain", "multichain_qa", "protein_force_afm",

@@ -1,0 +1,163 @@
+"""GPU parity of the served form (continuous batching) against the FP64 oracle.
+
+A mix of jobs like those of the served stream: lengths from 30 to 600,
+one pulled by a constant-velocity tip and one by a constant-force tip, the
+full force field with quasi-adiabatic contacts, temperatures drawn per job,
+step counts that differ so that jobs retire and others are admitted while
+the rest run. With the noise off:
+
+* FP64: every job's final positions follow its own oracle run to 1e-8 and
+  its energy to 1e-9 (relative);
+* mixed precision (the precision the stream is served in): a one-step job's
+  forces (evaluated at the model's starting positions) match the oracle's to
+  1e-5 of the largest component and its energy to 1e-6; after a few hundred
+  steps every job's positions stay within the bound written below of its
+  oracle run; and a job gives the same result alone and in the mix.
+"""
+import numpy as np
+import pytest
+
+from oracle_ffi import Oracle
+from pas_cg_amd import model as M
+from pas_cg_amd import params as P
+from pas_cg_amd import pcg as G
+from pas_cg_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (30, 75, 150, 300, 600)
+TEMPS = (0.30, 0.33, 0.38, 0.42, 0.45)
+
+
+def jobs():
+    out = []
+    for k, n in enumerate(SIZES):
+        m = M.synthetic_protein(n, M.job_seed(900 + k))
+        rng = np.random.default_rng(900 + k)
+        m["pos"] = m["pos"] + rng.normal(scale=0.3 * P.ANGSTROM, size=m["pos"].shape)
+        if k == 1:
+            m = M.add_pulling(m)
+        elif k == 2:
+            m = M.add_force_pulling(m, force_eps_per_A=2.0)
+        out.append(m)
+    return out
+
+
+def params():
+    p = W.full_params(pad_A=4.0)
+    p.update(chir_on=1)
+    return p
+
+
+def serve(prm, models, steps, precision, max_resident=8, chunk=40):
+    srv = G.Server(prm, precision=precision, chunk_steps=chunk, max_resident=max_resident,
+                   philox_seed=7)
+    srv.set_noise_scale(0.0)
+    ids = [srv.submit(m, s, temperature=T, seed=11 + k)
+           for k, (m, s, T) in enumerate(zip(models, steps, TEMPS))]
+    for _ in range(1000):
+        if srv.round() == 0:
+            break
+    else:
+        raise AssertionError("server did not drain")
+    for j, s in zip(ids, steps):
+        assert srv.status(j) == ("done", s), srv.status(j)
+    return srv, ids
+
+
+def oracle_run(prm, m, T, seed, steps):
+    o = Oracle(dict(prm, temperature=T), m, seed=seed)
+    o.set_noise_scale(0.0)
+    tr = o.step(steps)
+    return o, tr
+
+
+def test_served_jobs_follow_oracle_fp64():
+    prm = params()
+    models = jobs()
+    steps = [150, 90, 120, 60, 100]
+    # two slots: jobs queue, retire and are admitted mid-stream
+    srv, ids = serve(prm, models, steps, G.FP64, max_resident=2)
+    for k, (m, j) in enumerate(zip(models, ids)):
+        o, tr = oracle_run(prm, m, TEMPS[k], 11 + k, steps[k])
+        r, V = srv.result(j)
+        err = np.abs(r - o.r()).max()
+        assert err < 1e-8, (k, m["n"], err)
+        assert abs(V - tr[-1]) <= 1e-9 * abs(tr[-1]), (k, V, tr[-1])
+        F = srv.get(j, 6)
+        assert np.abs(F - o.F()).max() <= 1e-8 * np.abs(o.F()).max(), k
+    srv.close()
+
+
+def test_served_mixed_one_step_forces_match_oracle():
+    """A one-step job evaluates its forces at the model's starting positions:
+    the mixed kernel's forces against the oracle's there."""
+    prm = params()
+    models = jobs()
+    srv, ids = serve(prm, models, [1] * len(models), G.MIXED)
+    for k, (m, j) in enumerate(zip(models, ids)):
+        o = Oracle(dict(prm, temperature=TEMPS[k]), m, seed=11 + k)
+        o.set_noise_scale(0.0)
+        o.fix_nl()
+        o.prepare_nh()
+        o.eval_forces()
+        F1 = o.F()
+        F2 = srv.get(j, 6)
+        err = np.abs(F2 - F1).max() / np.abs(F1).max()
+        assert err <= 1e-5, (k, m["n"], err)
+        _, V = srv.result(j)
+        assert abs(V - o.V) <= 1e-6 * abs(o.V), (k, V, o.V)
+    srv.close()
+
+
+# Mixed-precision trajectory bound: positions after 200 noise-free steps,
+# internal units (1 = 5 A). Per step the mixed forces are within about 1e-6
+# of FP64 (relative to the largest component); the difference grows along
+# the trajectory, and 2e-6 (1e-5 A) leaves a margin over what 200 steps give.
+MIXED_TRAJ_TOL = 2e-6
+
+
+def test_served_mixed_jobs_follow_oracle():
+    prm = params()
+    models = jobs()
+    steps = [200] * len(models)
+    srv, ids = serve(prm, models, steps, G.MIXED, max_resident=3)
+    worst = 0.0
+    for k, (m, j) in enumerate(zip(models, ids)):
+        o, tr = oracle_run(prm, m, TEMPS[k], 11 + k, steps[k])
+        r, V = srv.result(j)
+        err = float(np.abs(r - o.r()).max())
+        worst = max(worst, err)
+        assert err < MIXED_TRAJ_TOL, (k, m["n"], err)
+        assert abs(V - tr[-1]) <= 1e-6 * abs(tr[-1]), (k, V, tr[-1])
+    print(f"mixed served: worst position error after 200 steps {worst:.3e}")
+    srv.close()
+
+
+def test_served_mixed_job_alone_equals_job_in_mix():
+    """Mixed precision: the 600-residue job alone and among the others (which
+    come and go around it) ends at the same positions, up to the order of
+    the FP64 force additions, which only the atomics' timing sets."""
+    prm = params()
+    models = jobs()
+    one = G.Server(prm, precision=G.MIXED, chunk_steps=50, philox_seed=7)
+    j1 = one.submit(models[4], 300, temperature=TEMPS[4], seed=15)
+    while one.round():
+        pass
+    mix = G.Server(prm, precision=G.MIXED, chunk_steps=50, max_resident=3, philox_seed=7)
+    others = [mix.submit(m, 80 + 60 * k, temperature=TEMPS[k], seed=11 + k)
+              for k, m in enumerate(models[:4])]
+    j2 = mix.submit(models[4], 300, temperature=TEMPS[4], seed=15)
+    while mix.round():
+        pass
+    assert one.status(j1) == ("done", 300) and mix.status(j2) == ("done", 300)
+    for k, j in enumerate(others):
+        assert mix.status(j) == ("done", 80 + 60 * k)
+    r1, V1 = one.result(j1)
+    r2, V2 = mix.result(j2)
+    # noise on (Philox, keyed by the job's seed): the trajectories coincide
+    # only if the noise follows the job and not its slot
+    assert np.abs(r1 - r2).max() < 1e-8
+    assert abs(V1 - V2) <= 1e-9 * abs(V1)
+    one.close()
+    mix.close()
