@@ -595,7 +595,7 @@ __device__ __forceinline__ void term_loop(const DevBatch &b, SysDescC &sd, SysSt
   if (n <= 0) return;
   int k = ((int)threadIdx.x - lo % B + B) % B;
   double v = 0.0;
-  for (; k < n; k += B) v += eval_item<R>(b, sd, st, g, acc, T, k);
+  for (; k < n; k += B) v += eval_item<R>(b, sd, st, g, acc, T, k, want_v);
   if (want_v && v != 0.0) atomicAdd(&vacc[T], v);
 }
 

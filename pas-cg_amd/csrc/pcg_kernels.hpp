@@ -145,15 +145,16 @@ __device__ __forceinline__ bool term_on(DevParamsC &P, int term) {
 // much in FP32).
 template <typename R, typename G, typename A>
 __device__ __forceinline__ double eval_item(const DevBatch &b, SysDescC &s, SysState &st,
-                                            const G &g, A &acc, int term, int k) {
+                                            const G &g, A &acc, int term, int k,
+                                            bool want_v = true) {
   switch (term) {
   case PCG_CHIR: return chir_item<double>(b, g, acc, s.chir0 + k);
   case PCG_TETHER: return tether_item<double>(b, g, acc, s.teth0 + k);
   case PCG_LREP: return lrep_item<double>(b, g, acc, s.lrep0 + k);
   case PCG_NAT_CONT: return nat_cont_item<R>(b, s, st, g, acc, s.nc0 + k, st.t);
   case PCG_PAULI: return pauli_item<R>(b, s, g, acc, s.nl0 + k);
-  case PCG_CONST_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 0);
-  case PCG_REL_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 1);
+  case PCG_CONST_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 0, want_v);
+  case PCG_REL_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 1, want_v);
   case PCG_QA_CAND: return qa_cand_item<R>(b, s, st, g, acc, k);
   case PCG_QA_CONT: return qa_cont_item<R>(b, s, st, g, acc, k);
   case PCG_PID: return pid_item<double>(b, s, g, acc, s.nl0 + k);

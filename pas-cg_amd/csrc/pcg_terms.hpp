@@ -213,9 +213,13 @@ __device__ __forceinline__ double nat_cont_item(const DevBatch &b, SysDescC &s, 
   return nat_cont_core<R>(b, s, st, g, acc, q, t, true);
 }
 
-// const_dh / rel_dh ::iter (cg/src/cg/dh/const.cpp:18-40, rel.cpp:17-40)
+// const_dh / rel_dh ::iter (cg/src/cg/dh/const.cpp:18-40, rel.cpp:17-40).
+// With want_v the FP32 form returns the pair's energy in FP64, in the
+// reference's order of operations: the charges' signs make the term's sum
+// cancel, so FP32 pair energies would not hold its relative error to 1e-6.
 template <typename R, typename G, typename A>
-__device__ double dh_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, int k, int variant) {
+__device__ double dh_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, int k, int variant,
+                          bool want_v = true) {
   ParamsC &p = b.P->p;
   int i1 = b.dh_i[0][k], i2 = b.dh_i[1][k];
   double q1q2 = b.dh_q[k];
@@ -238,6 +242,14 @@ __device__ double dh_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, in
   v3<R> f = (dV * rinv) * r12;
   acc.add(i1, f);
   acc.add(i2, -f);
+  if constexpr (sizeof(R) == 4) {
+    if (want_v) {
+      const double rn64 = 1.0 / rnd;
+      double V64 = b.P->V_factor * q1q2 * exp(-rnd * b.P->screen_inv) * rn64;
+      if (variant != 0) V64 = V64 * rn64;
+      return V64;
+    }
+  }
   return (double)Vij;
 }
 

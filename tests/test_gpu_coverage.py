@@ -185,10 +185,18 @@ def test_batch_variant_mixed_matches_fp64_energies_off(cfg):
     the FP64 kernel (relative to the largest component) on evolved
     positions."""
     prm, m = VARIANTS[cfg]()
-    b0 = G.Batch(prm, [m], precision=G.MIXED, steps_per_launch=100)
-    b0.run(200)
-    pos = b0.get(0, 0)
-    b0.close()
+    if cfg == "ss_lj":
+        # The reference hands the Lennard-Jones disulfide force the distance
+        # where it expects the inverse distance (nat_cont/eval_forces.cpp:27),
+        # which makes it grow as r^12 beyond 4.2 A: a trajectory blows up
+        # within a hundred steps, the oracle's as well. The model's own
+        # (perturbed) positions are the state here.
+        pos = m["pos"]
+    else:
+        b0 = G.Batch(prm, [m], precision=G.MIXED, steps_per_launch=100)
+        b0.run(200)
+        pos = b0.get(0, 0)
+        b0.close()
     F = {}
     for prec in (G.FP64, G.MIXED):
         b = G.Batch(prm, [m], precision=prec, steps_per_launch=1)
