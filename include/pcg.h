@@ -285,6 +285,13 @@ int pcg_batch_set_noise_scale(pcg_batch *b, double scale); /* 0: no noise */
 int pcg_batch_energies(pcg_batch *b, int32_t on);
 /* The report scalars of every system, [nsys][PCG_REPORT_NUM] (pcg_report). */
 int pcg_batch_report(pcg_batch *b, double *out);
+/* pcg_batch_report without any device or host allocation. No counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * per-system sums go to a device array of the library, one per GPU, held
+ * until the process ends; batches of at most 4096 systems (PCG_E_ARG above
+ * that). Calls that overlap on one GPU share that array and race. */
+int pcg_batch_report_static(pcg_batch *b, double *out);
 /* Checkpoint and resume of a whole batch (replaces the state serialisation of
  * cg/src/cg/simul/state_io.cpp:117-140: positions, time, velocities, Gear
  * y0..y5, QA contacts, sync budgets; here with the lists, the native-contact

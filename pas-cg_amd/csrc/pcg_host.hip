@@ -926,6 +926,31 @@ int report_into(DeviceBatch &B, double *out) {
   return PCG_OK;
 }
 
+// pcg_batch_report without allocation: the sums go to the library's device
+// array, and the host reads them back in chunks through buffers on its stack
+int report_into_static(DeviceBatch &B, double *out) {
+  const int ns = (int)B.sys.size();
+  if (ns > REPORT_STATIC_MAX)
+    return fail(PCG_E_ARG, "pcg_batch_report_static: " + std::to_string(ns) +
+                               " systems, more than its maximum of " +
+                               std::to_string(REPORT_STATIC_MAX));
+  ReportSums *d = nullptr;
+  int rc = report_static_sums(B.d, B.device, B.hp.p.pid_on, B.stream, &d);
+  if (rc == -2) return fail(PCG_E_ARG, "pcg_batch_report_static: device index beyond 63");
+  if (rc) return fail(PCG_E_HIP, "pcg_batch_report_static: report kernels or symbol lookup failed");
+  constexpr int CH = 16;
+  ReportSums h[CH];
+  SysState st[CH];
+  for (int s0 = 0; s0 < ns; s0 += CH) {
+    const int k = std::min(CH, ns - s0);
+    HIPCHK(hipMemcpyAsync(h, d + s0, k * sizeof(ReportSums), hipMemcpyDeviceToHost, B.stream));
+    HIPCHK(hipMemcpyAsync(st, B.d.st + s0, k * sizeof(SysState), hipMemcpyDeviceToHost, B.stream));
+    HIPCHK(hipStreamSynchronize(B.stream));
+    for (int j = 0; j < k; ++j) report_scalars(h[j], st[j], out + (size_t)(s0 + j) * PCG_REPORT_NUM);
+  }
+  return PCG_OK;
+}
+
 int check_overflow(DeviceBatch &B) {
   // every system's scalars in one copy
   std::vector<SysState> st(B.sys.size());
@@ -1667,6 +1692,12 @@ int pcg_batch_report(pcg_batch *b, double *out) {
   if (!b || !out) return fail(PCG_E_ARG, "null argument");
   HIPCHK(hipSetDevice(b->B->device));
   return report_into(*b->B, out);
+}
+
+int pcg_batch_report_static(pcg_batch *b, double *out) {
+  if (!b || !out) return fail(PCG_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(b->B->device));
+  return report_into_static(*b->B, out);
 }
 
 int pcg_batch_energies(pcg_batch *b, int32_t on) {

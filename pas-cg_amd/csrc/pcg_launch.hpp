@@ -20,4 +20,11 @@ void launch_persistent_jobs(const DevBatch *jobs, JobSlot *slots, int njobs, int
                             int nsteps, size_t lds_bytes, hipStream_t s);
 // report sums of every system (pcg_report.hip)
 void launch_report(const DevBatch &b, int pid_on, ReportSums *out, hipStream_t s);
+// the same into the library's own device array (no allocation), for batches
+// of at most REPORT_STATIC_MAX systems, with `device` current: 0, or -1 for
+// too many systems, -2 for a device index beyond the library's table, -3 for
+// a HIP failure
+constexpr int REPORT_STATIC_MAX = 4096;
+int report_static_sums(const DevBatch &b, int device, int pid_on, hipStream_t s,
+                       ReportSums **sums);
 } // namespace pcg

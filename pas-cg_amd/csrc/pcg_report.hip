@@ -1,6 +1,8 @@
 // pcg_report.hip -- the per-trajectory scalars of the reference's report
 // (cg/src/cg/output/make_report.cpp:160-247): one workgroup per system sums
 // what they need; the host turns the sums into the scalars (pcg_host.hip).
+#include <mutex>
+
 #include "pcg_launch.hpp"
 #include "pcg_stages.hpp"
 
@@ -234,6 +236,40 @@ void launch_report(const DevBatch &b, int pid_on, ReportSums *out, hipStream_t s
   if (pid_on && b.ntot > 0)
     hipLaunchKernelGGL(k_pid_count, dim3((unsigned)((b.ntot + RB / 64 - 1) / (RB / 64))), dim3(RB),
                        0, s, b, out);
+}
+
+// The report sums of the no-allocation report: a file-scope device array,
+// one per GPU for the life of the process. k_report writes every field of a
+// system's entry before k_pid_count adds into its counts, so nothing needs
+// clearing between calls. Calls that overlap on one GPU share it and race.
+namespace {
+__device__ __align__(256) ReportSums scratch[REPORT_STATIC_MAX];
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+ReportSums *scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+int report_static_sums(const DevBatch &b, int device, int pid_on, hipStream_t s,
+                       ReportSums **sums) {
+  if (b.nsys > REPORT_STATIC_MAX) return -1;
+  if (device < 0 || device >= MAX_DEVICES) return -2;
+  ReportSums *p;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    p = scratch_ptr[device];
+    if (!p) {
+      // the array's address on this device (the host-side name is not one)
+      int cur = 0;
+      if (hipGetDevice(&cur) != hipSuccess || cur != device) return -3;
+      void *ptr = nullptr;
+      if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -3;
+      p = scratch_ptr[device] = (ReportSums *)ptr;
+    }
+  }
+  launch_report(b, pid_on, p, s);
+  if (hipGetLastError() != hipSuccess) return -3;
+  *sums = p;
+  return 0;
 }
 
 } // namespace pcg

@@ -156,6 +156,7 @@ SIGNATURES = {
     "pcg_batch_set_noise_scale": [_V, D],
     "pcg_batch_energies": [_V, I],
     "pcg_batch_report": [_V, C.POINTER(D)],
+    "pcg_batch_report_static": [_V, C.POINTER(D)],
     "pcg_batch_checkpoint_size": [_V, C.POINTER(I64)],
     "pcg_batch_checkpoint": [_V, _V, I64],
     "pcg_batch_restore": [_V, _V, I64],
@@ -514,6 +515,12 @@ class Batch:
         """The report scalars (REPORT_FIELDS) of every system, one dict each."""
         out = np.zeros((len(self.ns), len(REPORT_FIELDS)))
         check(self.L.pcg_batch_report(self.h, _p(out, D)))
+        return [dict(zip(REPORT_FIELDS, row.tolist())) for row in out]
+
+    def report_static(self) -> list:
+        """report() through pcg_batch_report_static (no allocation)."""
+        out = np.zeros((len(self.ns), len(REPORT_FIELDS)))
+        check(self.L.pcg_batch_report_static(self.h, _p(out, D)))
         return [dict(zip(REPORT_FIELDS, row.tolist())) for row in out]
 
     def info(self):
