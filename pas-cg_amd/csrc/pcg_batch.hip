@@ -134,129 +134,307 @@ __device__ __forceinline__ void block_ranks5(int cls, int valid, int counts[5], 
 }
 
 
-// NL rebuild + ordered partition inside the workgroup.
+// Tallies of the list rebuild: NL pairs, then the pairs of each list the
+// ordered partition fills (native contacts, QA free pairs, the QA free pairs
+// that can become candidates, PID, DH, Pauli).
+enum { RB_NL = 0, RB_NC, RB_QA, RB_NEAR, RB_PID, RB_DH, RB_PAULI, RB_N };
+
+// First row of wave w's block when nw waves split rows 0..n-1 of the
+// upper-triangle pair rows (row k holds n - 1 - k pairs) into contiguous
+// blocks of about equal pair counts: the smallest k with
+// k (n - 1) - k (k - 1) / 2 >= w P / nw, P = n (n - 1) / 2. Monotone in w,
+// 0 for w = 0 and n for w = nw.
+__device__ __forceinline__ int rb_row_start(int w, int nw, int n) {
+  if (w <= 0) return 0;
+  if (w >= nw) return n;
+  const double P = 0.5 * (double)n * (double)(n - 1);
+  const double T = P * (double)w / (double)nw;
+  const double h = (double)n - 0.5;
+  const double disc = fmax(h * h - 2.0 * T, 0.0);
+  int k = (int)ceil(h - sqrt(disc));
+  k = max(0, min(n, k));
+  // exact integer correction of the rounding
+  auto C = [&](long long kk) { return kk * (long long)(n - 1) - kk * (kk - 1) / 2; };
+  const long long Ti = (long long)ceil(T);
+  while (k > 0 && C(k - 1) >= Ti) --k;
+  while (k < n && C(k) < Ti) ++k;
+  return k;
+}
+
+// nl::legacy_update (cg/src/cg/nl/legacy_update.cpp:11-55) fused with the
+// ordered list updates of thread_main.cpp:409-423 and the QA near list,
+// inside the workgroup and without writing the NL itself. The rows of the
+// sorted NL (pairs i < j) are split into contiguous blocks of about equal
+// pair counts, one per wave; a wave walks its rows in order, 64 columns j at
+// a time, and classifies every pair on the spot as the partition would
+// classify the NL pair: native contact (a taken pair of the row's native
+// contacts), QA free pair (taken), PID bundle, DH pair, Pauli pair. A first
+// pass counts each list's pairs per wave, a scan over the waves gives each
+// wave its offsets, and a second pass writes every pair at its wave's
+// offset plus its rank among the block's earlier pairs of that list (a
+// running count and a ballot): the lists come out exactly in the order of
+// partitioning the sorted NL. The counts are scalars; nothing per row goes
+// to memory.
+//
+// Every decision is the FP64 one of the reference's expressions. The squared
+// distance is formed in FP32 from the FP64 differences; when it is clear of
+// every threshold by a relative 2e-5 (far beyond FP32's rounding of a sum of
+// three squares) the FP32 comparison decides, else the exact FP64 distance
+// does. The FP64 distance itself is formed for the pairs whose lists keep it
+// (QA near pairs, PID bundles).
+//
+// QA free pairs beyond the QA maximum required distance are counted (qf_n,
+// which sizes the contact loop as the reference's free-pair count does) but
+// not stored: with total_disp reset after every verify
+// (cg/src/cg/nl/verify.cpp:31-32) loop_over_candidates skips them until the
+// next rebuild (qa/loop_over_candidates.cpp:20), and the stored pairs keep
+// their relative order, so the (distance, free-pair index) order of the QA
+// finish is unchanged.
+//
+// The chain and sequence indices of the residues, the row starts of the
+// static native contacts and the contacts' second residues sit in the LDS
+// region of the QA stacks (rb), which is free during a rebuild.
 // (inlined: as a call it would make the kernel save every live register)
-__device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysState &st, const GeoL &g) {
+__device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysState &st, const GeoL &g,
+                                          void *rb) {
 #pragma clang fp contract(off)
-  const double req = b.P->nl_eff_cutoff + b.P->p.nl_pad;
-  int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int end = sd.res0 + sd.n;
-  int *cnt = b.row_cnt + sd.res0;
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int row = sd.res0 + w; row < end; row += nw) {
-      int out = pass ? cnt[row - sd.res0] : 0, c = 0;
-      for (int j0 = row + 1; j0 < end; j0 += 64) {
-        int j = j0 + lane;
-        double dist = 0.0;
-        bool keep = false;
-        if (j < end) {
-          bool same = b.chain[row] == b.chain[j];
-          int dseq = abs(b.seq[row] - b.seq[j]);
-          if (!(same && dseq < 3)) {
-            v3<double> dd = xwrapd(xsub(g.r(j), g.r(row)), sd);
-            dist = xnorm3(dd);
-            keep = dist < req;
-          }
-        }
-        unsigned long long m = __ballot(keep);
-        if (pass && keep) {
-          int pos = out + c + __popcll(m & ((1ull << lane) - 1ull));
-          if (pos < sd.nlcap) {
-            int slot = sd.nl0 + pos;
-            b.nl_i[0][slot] = row;
-            b.nl_i[1][slot] = j;
-            b.nl_d[slot] = dist;
-          }
-        }
-        c += __popcll(m);
-      }
-      if (!pass && lane == 0) cnt[row - sd.res0] = c;
+  DevParamsC &P = *b.P;
+  ParamsC &p = P.p;
+  const int n = sd.n, r0 = sd.res0;
+  // the wave's index, known to the compiler as uniform: the row loops below
+  // are then scalar loops and the running counts scalar registers
+  const int nw = blockDim.x >> 6, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int nnc = p.nat_cont_on ? sd.nnc : 0;
+  int *chn = (int *)rb, *sqi = chn + n, *rs = sqi + n;  // rs[0..n]
+  uint16_t *pj = (uint16_t *)(rs + n + 1);               // [nnc]
+  __shared__ int wcnt[16][RB_N];
+  const bool cell = has_cell(sd);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    chn[k] = b.chain[r0 + k];
+    sqi[k] = b.seq[r0 + k];
+  }
+  // contacts sd.nc0 + [rs[k], rs[k + 1]) of the list sorted by (i1, i2) are
+  // those of residue r0 + k with later residues; pj holds their second
+  // residues within the system
+  if (nnc == 0) {
+    for (int k = threadIdx.x; k <= n; k += blockDim.x) rs[k] = 0;
+  } else {
+    for (int c = threadIdx.x; c < nnc; c += blockDim.x) {
+      const int a = b.nc_i[0][sd.nc0 + c] - r0;
+      const int prev = c > 0 ? b.nc_i[0][sd.nc0 + c - 1] - r0 : -1;
+      pj[c] = (uint16_t)(b.nc_i[1][sd.nc0 + c] - r0);
+      for (int r = prev + 1; r <= a; ++r) rs[r] = c;
+      if (c == nnc - 1)
+        for (int r = a + 1; r <= n; ++r) rs[r] = nnc;
     }
-    __syncthreads();
+  }
+  const double pad = p.nl_pad;
+  const double t_nl = __dadd_rn(P.nl_eff_cutoff, pad);
+  const double t_nc = __dadd_rn(p.nl_cutoff, pad);
+  const double t_dh = __dadd_rn(p.dh_cutoff, pad);
+  const double t_rep = __dadd_rn(p.rep_cutoff, pad);
+  const double t_near = P.qa_max_req;
+  const double rt_dh = __ddiv_rn(1.0, t_dh);
+  // squared thresholds in FP32, and their margins
+  const float s_nl = (float)(t_nl * t_nl), s_nc = (float)(t_nc * t_nc), s_dh = (float)(t_dh * t_dh),
+              s_rep = (float)(t_rep * t_rep), s_near = (float)(t_near * t_near);
+  const float mlo = 1.0f - 2e-5f, mhi = 1.0f + 2e-5f;
+  const float e_nl[2] = {s_nl * mlo, s_nl * mhi}, e_nc[2] = {s_nc * mlo, s_nc * mhi},
+              e_dh[2] = {s_dh * mlo, s_dh * mhi}, e_rep[2] = {s_rep * mlo, s_rep * mhi},
+              e_near[2] = {s_near * mlo, s_near * mhi};
+  const bool qa_on = p.qa_on, pid_on = p.pid_on, dh_on = p.dh_on, pauli_on = P.pauli_on;
+  const bool qa4 = p.qa_include4, pid4 = p.pid_include4;
+  // residue types with a charge, as bits (q1 q2 != 0 iff both are set)
+  uint32_t chg = 0u;
+  for (int t = 0; t < 21; ++t) chg |= (p.charge[t] != 0.0 ? 1u : 0u) << t;
+  const int kbeg = rb_row_start(w, nw, n), kend = rb_row_start(w + 1, nw, n);
+  __syncthreads();
+  for (int pass = 0; pass < 2; ++pass) {
+    int off[RB_N];
+    for (int c = 0; c < RB_N; ++c) off[c] = pass ? wcnt[w][c] : 0;
+    for (int k = kbeg; k < kend; ++k) {
+      const double xi = g.x[k], yi = g.y[k], zi = g.z[k];
+      const int ci = chn[k], si = sqi[k], ai = g.at[k] & 63;
+      const int c0 = rs[k], ce = rs[k + 1];
+      int tp = c0;  // the row's next native partner
+      for (int j0 = k + 1; j0 < n; j0 += 64) {
+        const int j = j0 + lane;
+        // native partners among columns j0 .. j0 + 63, as a lane mask
+        unsigned long long natm = 0ull;
+        const int tp0 = tp;
+        while (tp < ce) {
+          const int pt = pj[tp];
+          if (pt > j0 + 63) break;
+          natm |= 1ull << (pt - j0);
+          ++tp;
+        }
+        // straight-line for every lane (column n - 1 stands in for those
+        // past the row's end); the bitwise & and | keep it branch-free
+        const bool valid = j < n;
+        const int jj = valid ? j : n - 1;
+        const int dseq = abs(sqi[jj] - si);
+        const bool same = chn[jj] == ci;
+        const bool four = same & (dseq == 4);
+        const bool pair = valid & !(same & (dseq < 3));
+        v3<double> dd = xsub(v3<double>{g.x[jj], g.y[jj], g.z[jj]}, v3<double>{xi, yi, zi});
+        if (cell) dd = xwrapd(dd, sd);
+        const float fx = (float)dd.x, fy = (float)dd.y, fz = (float)dd.z;
+        const float d2 = fx * fx + fy * fy + fz * fz;
+        bool b_nl = d2 < s_nl, b_nc = d2 < s_nc, b_dh = d2 < s_dh, b_rep = d2 < s_rep;
+        bool b_near = d2 < s_near;
+        const bool edge = pair & ((d2 >= e_nl[0]) & (d2 <= e_nl[1]) | (d2 >= e_nc[0]) & (d2 <= e_nc[1]) |
+                                  (d2 >= e_dh[0]) & (d2 <= e_dh[1]) | (d2 >= e_rep[0]) & (d2 <= e_rep[1]) |
+                                  (d2 >= e_near[0]) & (d2 <= e_near[1]));
+        if (__ballot(edge)) {  // rare: the exact FP64 decisions of the pairs in a band
+          if (edge) {
+            const double dist = xnorm3(dd);
+            b_nl = dist < t_nl;
+            b_nc = dist < t_nc;
+            b_dh = __ddiv_rn(1.0, dist) > rt_dh;
+            b_rep = dist < t_rep;
+            b_near = !(dist > t_near);
+          }
+        }
+        const bool in = pair & b_nl;
+        const bool is_nc = in & b_nc & (bool)((natm >> lane) & 1ull);
+        const bool is_qa = in & !is_nc & qa_on & (qa4 | !four) & b_nc;
+        const bool near = is_qa & b_near;
+        const bool rest = in & !is_nc & !is_qa;
+        const bool is_pid = rest & pid_on & (pid4 | !four) & b_nc;
+        const int aj = g.at[jj] & 63;
+        const bool is_dh = rest & dh_on & (bool)((chg >> ai) & (chg >> aj) & 1u) & b_dh;
+        const bool is_pauli = rest & pauli_on & b_rep;
+        const unsigned long long m_nl = __ballot(in), m_nc = __ballot(is_nc),
+                                 m_qa = __ballot(is_qa), m_near = __ballot(near),
+                                 m_pid = __ballot(is_pid), m_dh = __ballot(is_dh),
+                                 m_pauli = __ballot(is_pauli);
+        if (pass) {
+          const int i = r0 + k, jg = r0 + j;
+          if (is_nc) {
+            // the contact: the row's partners before column j; its fields
+            // are copied after the pass
+            const int c = sd.nc0 + tp0 + __popcll(natm & below);
+            b.ncur[sd.nc0 + off[RB_NC] + __popcll(m_nc & below)] = c;
+          }
+          if (near) {
+            const int f = off[RB_NEAR] + __popcll(m_near & below);
+            if (f < sd.qfcap) {
+              const double d64 = xnorm3(dd);
+              const int q = sd.qf0 + f;
+              b.qf_i[0][q] = i;
+              b.qf_i[1][q] = jg;
+              b.qf_d[q] = d64;
+              b.qf_vac[q] = 0;
+              // near list entry f: the free pair in slot f
+              const double req = fmax(fmax(fmax(g.req_t[1], g.req_t[2]), g.req_t[3]),
+                                      g.req_t[4 + ai + 21 * aj]);
+              const double thr = fmin(P.qa_max_req, fmax(p.rep_cutoff, req)) * (1.0 + 1e-5);
+              b.qh_slot[q] = f;
+              b.qh_i[0][q] = i;
+              b.qh_i[1][q] = jg;
+              b.qh_vac[q] = 0;
+              b.qh_req[q] = req;
+              b.qh_thr2[q] = (float)(thr * thr);
+              b.qh_ij[q] = (uint32_t)k | ((uint32_t)j << 16);
+            }
+          }
+          if (is_pid) {
+            const int pos = off[RB_PID] + __popcll(m_pid & below);
+            if (pos < sd.nlcap) {
+              const int q = sd.nl0 + pos;
+              b.pid_i[0][q] = i;
+              b.pid_i[1][q] = jg;
+              b.pid_d[q] = xnorm3(dd);
+              b.pid_t[q] = (int16_t)(ai * 21 + aj);
+            }
+          }
+          if (is_dh) {
+            const int pos = off[RB_DH] + __popcll(m_dh & below);
+            if (pos < sd.nlcap) {
+              const int q = sd.nl0 + pos;
+              b.dh_i[0][q] = i;
+              b.dh_i[1][q] = jg;
+              b.dh_q[q] = p.charge[ai] * p.charge[aj];
+            }
+          }
+          if (is_pauli) {
+            const int pos = off[RB_PAULI] + __popcll(m_pauli & below);
+            if (pos < sd.nlcap) {
+              const int q = sd.nl0 + pos;
+              b.pauli_i[0][q] = i;
+              b.pauli_i[1][q] = jg;
+            }
+          }
+        }
+        off[RB_NL] += __popcll(m_nl);
+        off[RB_NC] += __popcll(m_nc);
+        off[RB_QA] += __popcll(m_qa);
+        off[RB_NEAR] += __popcll(m_near);
+        off[RB_PID] += __popcll(m_pid);
+        off[RB_DH] += __popcll(m_dh);
+        off[RB_PAULI] += __popcll(m_pauli);
+      }
+    }
     if (!pass) {
-      int tot = block_exscan(cnt, sd.n);
-      if (threadIdx.x == 0) {
-        st.nl_n = tot < sd.nlcap ? tot : sd.nlcap;
-        if (tot > sd.nlcap) st.overflow = 1;
+      if (lane == 0)
+        for (int c = 0; c < RB_N; ++c) wcnt[w][c] = off[c];
+      __syncthreads();
+      // the waves' offsets: an exclusive scan over the waves, in wave order
+      if (threadIdx.x < RB_N) {
+        const int c = threadIdx.x;
+        int run = 0;
+        for (int v = 0; v < nw; ++v) {
+          const int x = wcnt[v][c];
+          wcnt[v][c] = run;
+          run += x;
+        }
+        switch (c) {
+        case RB_NL: st.nl_n = run; break;
+        case RB_NC: st.nc_n = run; break;
+        case RB_QA: st.qf_n = run; break;
+        case RB_NEAR:
+          st.qf_store = run < sd.qfcap ? run : sd.qfcap;
+          st.qh_n = st.qf_store;
+          if (run > sd.qfcap) st.overflow = 1;
+          break;
+        case RB_PID:
+          st.pid_n = run < sd.nlcap ? run : sd.nlcap;
+          if (run > sd.nlcap) st.overflow = 1;
+          break;
+        case RB_DH:
+          st.dh_n = run < sd.nlcap ? run : sd.nlcap;
+          if (run > sd.nlcap) st.overflow = 1;
+          break;
+        case RB_PAULI:
+          st.pauli_n = run < sd.nlcap ? run : sd.nlcap;
+          if (run > sd.nlcap) st.overflow = 1;
+          break;
+        }
       }
       __syncthreads();
     }
   }
+  __syncthreads();
+  // the current native contacts' copied fields (set_current_contact), every
+  // slot at once
+  for (int q = threadIdx.x; q < st.nc_n; q += blockDim.x)
+    set_current_contact(b, sd.nc0 + q, b.ncur[sd.nc0 + q], r0);
   // snapshot (legacy_update.cpp:80-89)
-  for (int k = threadIdx.x; k < sd.n; k += blockDim.x) {
-    b.orig_r[0][sd.res0 + k] = g.x[k];
-    b.orig_r[1][sd.res0 + k] = g.y[k];
-    b.orig_r[2][sd.res0 + k] = g.z[k];
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    b.orig_r[0][r0 + k] = g.x[k];
+    b.orig_r[1][r0 + k] = g.y[k];
+    b.orig_r[2][r0 + k] = g.z[k];
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    st.orig_pad = b.P->p.nl_pad;
+    st.orig_pad = pad;
     for (int c = 0; c < 3; ++c) st.orig_cell[c] = sd.cell[c];
-  }
-  __syncthreads();
-  // ordered partition: chunks in order, running offsets
-  __shared__ int run[5];
-  if (threadIdx.x == 0)
-    for (int c = 0; c < 5; ++c) run[c] = 0;
-  __syncthreads();
-  ParamsC &p = b.P->p;
-  int nl_n = st.nl_n;
-  for (int base = 0; base < nl_n; base += blockDim.x) {
-    int k = base + threadIdx.x;
-    int valid = k < nl_n;
-    int slot = sd.nl0 + k;
-    int cls = valid ? classify(b, sd, st, slot) : 0;
-    int counts[5], ranks[5];
-    block_ranks5(cls, valid, counts, ranks);
-    if (valid && cls) {
-      int i1 = b.nl_i[0][slot], i2 = b.nl_i[1][slot];
-      double d = b.nl_d[slot];
-      if (cls & 1) set_current_contact(b, sd.nc0 + run[0] + ranks[0], find_contact(b, sd, i1, i2), sd.res0);
-      if (cls & 2) {
-        int pos = run[1] + ranks[1];
-        if (pos < sd.qfcap) {
-          int q = sd.qf0 + pos;
-          b.qf_i[0][q] = i1;
-          b.qf_i[1][q] = i2;
-          b.qf_d[q] = d;
-          b.qf_vac[q] = 0;
-        }
-      }
-      if (cls & 4) {
-        int q = sd.nl0 + run[2] + ranks[2];
-        b.pid_i[0][q] = i1;
-        b.pid_i[1][q] = i2;
-        b.pid_d[q] = d;
-        b.pid_t[q] = (int16_t)(b.atype[i1] * 21 + b.atype[i2]);
-      }
-      if (cls & 8) {
-        int q = sd.nl0 + run[3] + ranks[3];
-        b.dh_i[0][q] = i1;
-        b.dh_i[1][q] = i2;
-        b.dh_q[q] = p.charge[b.atype[i1]] * p.charge[b.atype[i2]];
-      }
-      if (cls & 16) {
-        int q = sd.nl0 + run[4] + ranks[4];
-        b.pauli_i[0][q] = i1;
-        b.pauli_i[1][q] = i2;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (int c = 0; c < 5; ++c) run[c] += counts[c];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    st.nc_n = run[0];
-    st.qf_n = run[1] < sd.qfcap ? run[1] : sd.qfcap;
-    if (run[1] > sd.qfcap) st.overflow = 1;
-    st.pid_n = run[2];
-    st.dh_n = run[3];
-    st.pauli_n = run[4];
     st.nl_invalid = 0;
     st.rebuilds += 1;
   }
   __syncthreads();
-  if (b.P->p.qa_on) build_qa_near(b, sd, st);
 }
 
 // Shares of one chain-local item: slot s receives the force on residue
@@ -708,7 +886,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     if (stop) break;
     // (n and h of qa::prepare_nh were computed after the positions last moved)
     if (flag_rebuild) {
-      wg_fix_nl(b, sd, st, g);
+      wg_fix_nl(b, sd, st, g, (void *)(lds + lay.q));
       if (threadIdx.x == 0) set_sizes();
       __syncthreads();
     }

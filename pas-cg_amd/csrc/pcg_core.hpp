@@ -55,7 +55,7 @@ struct LdsLayout {
   // (8 bytes a pair, 128 pairs for each of up to 16 waves)
   static constexpr size_t QA_QUEUE = 4 * 1024;
   size_t f, at, sy, bd, bu, g, q, total;
-  __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged, bool gear) {
+  __host__ __device__ LdsLayout(size_t n, size_t fsz, bool staged, bool gear, size_t nnc = 0) {
     f = 24 * n;
     at = f + 3 * n * fsz;
     sy = (at + n + 3) / 4 * 4;
@@ -63,7 +63,15 @@ struct LdsLayout {
     bu = bd + 48 * n;
     g = ((staged ? bu + 3 * n : bd) + 15) / 16 * 16;
     q = gear ? g + 120 * n : g;
-    total = q + 4 * QA_QUEUE + 64;
+    // the QA stacks' region also holds the list rebuild's tables (chain and
+    // sequence index of every residue, native-contact row starts and the
+    // contacts' second residues), which are never in use at the same time;
+    // nnc bounds the native contacts of a system
+    const size_t rb = rebuild_bytes(n, nnc);
+    total = q + (rb > 4 * QA_QUEUE ? rb : 4 * QA_QUEUE) + 64;
+  }
+  __host__ __device__ static size_t rebuild_bytes(size_t n, size_t nnc) {
+    return (12 * n + 4 + 2 * nnc + 15) / 16 * 16;
   }
 };
 
@@ -81,6 +89,11 @@ struct SysState {
   int nl_invalid, verify_first, num_changed, overflow;
   int rebuilds, active;
   int qh_n;                   // QA free pairs that can become candidates (qh_slot)
+  // free pairs held in the qf arrays: qf_n in the stage kernels; the
+  // persistent kernel's rebuild stores only those within the QA maximum
+  // required distance (the others cannot become candidates before the next
+  // rebuild) and counts the rest in qf_n
+  int qf_store;
   int64_t step;
   double t, true_t, orig_pad;
   double orig_cell[3];
@@ -191,6 +204,7 @@ struct DevBatch {
   int *cand_perm;
   int *rem;
   int *row_cnt;               // NL rebuild scratch: per-residue row counts / offsets
+  int *row_tally;             // the persistent kernel's rebuild: [7][ntot + 1] per-row list tallies
   int *blk_cnt;               // partition scratch: per-chunk counts
 };
 

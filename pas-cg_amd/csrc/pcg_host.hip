@@ -358,6 +358,7 @@ struct DeviceBatch {
   std::vector<HostSys> hs;
   Arena arena;
   int max_nlcap = 0, max_n = 0, max_items = 0;
+  int max_nnc = 0;  // most native contacts of one system
   size_t bytes = 0;
   std::vector<Ran2> ran2;  // per system, RAN2 noise mode
   std::vector<double> noise_h;
@@ -435,6 +436,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     nl += h.nlcap, qf += h.qfcap, qc += h.qccap;
     B->max_nlcap = std::max(B->max_nlcap, h.nlcap);
     B->max_n = std::max(B->max_n, h.n);
+    B->max_nnc = std::max(B->max_nnc, d.nnc);
     int items = std::max({d.nteth, d.nlrep, d.nchir, d.nnang, d.nhang, d.nndih, d.nhdih, d.nnc,
                           h.nlcap, h.qfcap});
     B->max_items = std::max(B->max_items, items);
@@ -471,6 +473,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&local, ntot);
   A.add(&D.sync, 4 * ntot);
   A.add(&D.row_cnt, ntot + 1);
+  A.add(&D.row_tally, 7 * (ntot + 1));
   int *teth_i[2], *lrep_i[2], *chir_i[4], *nang_i[3], *hang_i[3], *ndih_i[4], *hdih_i[4], *nc_i[2];
   double *teth_nat, *chir_f0, *chir_f1, *nang_th, *ndih_ph, *nc_nat;
   uint8_t *hang_t, *hdih_t, *nc_type;
@@ -1388,7 +1391,7 @@ struct pcg_batch {
 namespace {
 
 size_t lds_for(const DeviceBatch &B, bool staged) {
-  return LdsLayout((size_t)B.max_n, sizeof(double), staged, false).total;  // FP64 LDS forces
+  return LdsLayout((size_t)B.max_n, sizeof(double), staged, false, (size_t)B.max_nnc).total;  // FP64 LDS forces
 }
 
 // steps owed by every system = current + nsteps
@@ -1463,8 +1466,9 @@ int pcg_batch_create(const pcg_params *p, int32_t nsys, const pcg_model *models,
   // per-residue bonded data joins positions and forces in LDS when it fits;
   // systems whose positions and forces do not fit take the stage kernels
   bool staged = lds_for(B, true) <= LDS_DYN_MAX;
-  bool gear = staged && LdsLayout((size_t)B.max_n, sizeof(double), true, true).total <= LDS_DYN_MAX;
-  b->lds = LdsLayout((size_t)B.max_n, sizeof(double), staged, gear).total;
+  bool gear = staged && LdsLayout((size_t)B.max_n, sizeof(double), true, true, (size_t)B.max_nnc).total <=
+                            LDS_DYN_MAX;
+  b->lds = LdsLayout((size_t)B.max_n, sizeof(double), staged, gear, (size_t)B.max_nnc).total;
   B.d.bond_lds = staged ? 1 : 0;
   B.d.gear_lds = gear ? 1 : 0;
   b->persistent = b->lds <= LDS_DYN_MAX;
@@ -2058,7 +2062,8 @@ int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature, int
   if ((int64_t)block_size(bytes) > s->o.hbm_budget_bytes)
     return fail(PCG_E_NOMEM, "job needs " + std::to_string(block_size(bytes)) +
                                  " bytes, over the budget");
-  if (LdsLayout((size_t)m->n, sizeof(double), false, false).total > LDS_DYN_MAX)
+  const size_t nnc_bound = s->p.nat_cont_on ? (size_t)std::max(m->n_cont, 0) : 0;
+  if (LdsLayout((size_t)m->n, sizeof(double), false, false, nnc_bound).total > LDS_DYN_MAX)
     return fail(PCG_E_ARG, "job of " + std::to_string(m->n) +
                                " residues does not fit the persistent kernel's LDS");
   j->bytes = bytes;
@@ -2147,9 +2152,10 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
       slots[k] = JobSlot{std::min<int64_t>(j.done + s->o.chunk_steps, j.nsteps), 0, 0.0, 0, 0};
       // each job stages its bonded data in LDS when that fits on its own
       const size_t nj = (size_t)B.max_n;
-      B.d.bond_lds = LdsLayout(nj, fsz, true, false).total <= LDS_DYN_MAX ? 1 : 0;
-      B.d.gear_lds = B.d.bond_lds && LdsLayout(nj, fsz, true, true).total <= LDS_DYN_MAX ? 1 : 0;
-      lds = std::max(lds, LdsLayout(nj, fsz, B.d.bond_lds != 0, B.d.gear_lds != 0).total);
+      const size_t cj = (size_t)B.max_nnc;
+      B.d.bond_lds = LdsLayout(nj, fsz, true, false, cj).total <= LDS_DYN_MAX ? 1 : 0;
+      B.d.gear_lds = B.d.bond_lds && LdsLayout(nj, fsz, true, true, cj).total <= LDS_DYN_MAX ? 1 : 0;
+      lds = std::max(lds, LdsLayout(nj, fsz, B.d.bond_lds != 0, B.d.gear_lds != 0, cj).total);
       descs[k] = B.d;
       max_n = std::max(max_n, B.max_n);
     }

@@ -218,6 +218,7 @@ __global__ void k_part_scan(DevBatch b) {
   }
   st.nc_n = tot[0];
   st.qf_n = tot[1] < sd.qfcap ? tot[1] : sd.qfcap;
+  st.qf_store = st.qf_n;
   if (tot[1] > sd.qfcap) st.overflow = 1;
   st.pid_n = tot[2];
   st.dh_n = tot[3];

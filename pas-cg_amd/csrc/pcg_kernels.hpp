@@ -156,7 +156,7 @@ __device__ __forceinline__ double eval_item(const DevBatch &b, SysDescC &s, SysS
   case PCG_CONST_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 0, want_v);
   case PCG_REL_DH: return dh_item<R>(b, s, g, acc, s.nl0 + k, 1, want_v);
   case PCG_QA_CAND: return qa_cand_item<R>(b, s, st, g, acc, k);
-  case PCG_QA_CONT: return qa_cont_item<R>(b, s, st, g, acc, k);
+  case PCG_QA_CONT: return qa_cont_item<R>(b, s, st, g, acc, k, want_v);
   case PCG_PID: return pid_item<double>(b, s, g, acc, s.nl0 + k);
   case PCG_NAT_ANG: return nat_ang_item<double>(b, g, acc, s.nang0 + k);
   case PCG_HEUR_ANG: return heur_ang_item<double>(b, g, acc, s.hang0 + k);

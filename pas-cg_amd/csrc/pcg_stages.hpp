@@ -98,7 +98,7 @@ __device__ inline void build_qa_near(const DevBatch &b, SysDescC &sd, SysState &
   if (threadIdx.x == 0) run = 0;
   __syncthreads();
   const double req = b.P->qa_max_req;
-  const int n = st.qf_n;
+  const int n = st.qf_store;
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int base = 0; base < n; base += blockDim.x) {
     int k = base + threadIdx.x;
@@ -170,8 +170,8 @@ __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState
         sy[4 * (i1 - sy_off) + c] = (int8_t)(sy[4 * (i1 - sy_off) + c] + b.qc_sd[8 * q + c]);
         sy[4 * (i2 - sy_off) + c] = (int8_t)(sy[4 * (i2 - sy_off) + c] + b.qc_sd[8 * q + 4 + c]);
       }
-      if (st.qf_n < sd.qfcap) {
-        int f = sd.qf0 + st.qf_n;
+      if (st.qf_store < sd.qfcap) {
+        int f = sd.qf0 + st.qf_store;
         b.qf_i[0][f] = i1;
         b.qf_i[1][f] = i2;
         b.qf_d[f] = b.qc_d[q];
@@ -179,7 +179,7 @@ __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState
         // appended at the end, so the near list stays in slot order
         if (!(b.qf_d[f] > b.P->qa_max_req)) {
           int h = sd.qf0 + st.qh_n++;
-          b.qh_slot[h] = st.qf_n;
+          b.qh_slot[h] = st.qf_store;
           b.qh_i[0][h] = i1;
           b.qh_i[1][h] = i2;
           b.qh_vac[h] = 0;
@@ -187,6 +187,7 @@ __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState
           b.qh_ij[h] = (uint32_t)(i1 - sd.res0) | ((uint32_t)(i2 - sd.res0) << 16);
         }
         st.qf_n += 1;
+        st.qf_store += 1;
       } else {
         st.overflow = 1;
       }

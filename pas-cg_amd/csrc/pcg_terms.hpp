@@ -807,7 +807,7 @@ __device__ __forceinline__ double qa_cand_item(const DevBatch &b, SysDescC &s, S
 // qa::process_contacts::iter (cg/src/cg/qa/process_contacts.cpp:12-72)
 template <typename R, typename G, typename A>
 __device__ double qa_cont_item(const DevBatch &b, SysDescC &s, SysState &st, const G &g,
-                               A &acc, int k) {
+                               A &acc, int k, bool want_v = true) {
 #pragma clang fp contract(off)
   DevParamsC &P = *b.P;
   ParamsC &p = P.p;
@@ -847,7 +847,17 @@ __device__ double qa_cont_item(const DevBatch &b, SysDescC &s, SysState &st, con
     sat = fmax(__dadd_rn(sat, step), 0.0);
   }
   b.qc_sat[slot] = sat;
-  // the energy carries the saturation the force used (qa/process_contacts.cpp:40)
+  // the energy carries the saturation the force used (qa/process_contacts.cpp:40);
+  // with want_v the FP32 form takes it in FP64: contacts in the repulsive
+  // part of the well and in its tail have energies of both signs
+  if constexpr (sizeof(R) == 4) {
+    if (want_v) {
+      double V64, dV64;
+      sink_ljR<double>(p.qa_lj_depth[type], p.qa_lj_r_low[type], p.qa_lj_r_high[type], r12n, rn,
+                       V64, dV64);
+      return sat_used * V64;
+    }
+  }
   return (double)sat_used * (double)Vij;
 }
 

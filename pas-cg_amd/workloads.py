@@ -19,8 +19,13 @@ cg/default.yml:22):
           at T uniform in [0.3, 0.45].
 
 The native conformation is the compact self-avoiding walk of
-`synthetic_protein`; it is not annealed further. These generators prepare
-input; they are not part of the per-step loop.
+`synthetic_protein`, compacted by 2000 steps of Go-model annealing on the
+GPU (`anneal_natives`): Langevin dynamics in the Go model of the walk itself
+(tethers, native contacts of the walk, its bond angles and dihedrals, local
+repulsion, Pauli exclusion) with the temperature lowered geometrically from
+0.6 to 0.15 eps/kB over ten stages; the native map, angles and dihedrals are
+then taken from the annealed conformation. These generators prepare input;
+they are not part of the per-step loop.
 """
 from __future__ import annotations
 
@@ -57,20 +62,27 @@ def full_params(pad_A: float = 2.0) -> dict:
     return p
 
 
-def c0() -> tuple[dict, dict]:
-    return base_params(), M.synthetic_protein(56, M.job_seed(0))
+def c0(chain: dict | None = None) -> tuple[dict, dict]:
+    """The single chain (`chain`: the same chain after annealing)."""
+    return base_params(), chain if chain is not None else M.synthetic_protein(56, M.job_seed(0))
 
 
-def c1(copies: int = 64, n: int = 100) -> tuple[dict, dict]:
+def c1_chain(n: int = 100) -> dict:
+    return M.synthetic_protein(n, M.job_seed(1))
+
+
+def c1(copies: int = 64, n: int = 100, chain: dict | None = None) -> tuple[dict, dict]:
+    """The periodic box of `copies` copies of one chain (`chain`: c1_chain
+    after annealing)."""
     p = full_params()
     p.update(afm_on=0)
     side = (copies / (1e-3 * AVOGADRO * 1e3)) ** (1.0 / 3.0) * 1e10  # metres -> A
-    one = M.synthetic_protein(n, M.job_seed(1))
+    one = chain if chain is not None else c1_chain(n)
     return p, M.replicate(one, copies, side * A, seed=1)
 
 
-def c2(copies: int = 1024) -> tuple[dict, list[dict]]:
-    p, m = c0()
+def c2(copies: int = 1024, chain: dict | None = None) -> tuple[dict, list[dict]]:
+    p, m = c0(chain)
     return p, [m] * copies
 
 
@@ -87,10 +99,12 @@ def ragged_lengths(count: int = 256, lo: int = 30, hi: int = 600, seed: int = 44
     return rng.integers(lo, hi + 1, size=count)
 
 
-def c3(count: int = 256, workers: int | None = None) -> tuple[dict, list[dict]]:
-    """The ragged batch; every 10th protein is pulled."""
+def c3(count: int = 256, workers: int | None = None, pulled: bool = True
+       ) -> tuple[dict, list[dict]]:
+    """The ragged batch on the walks' own natives; every 10th protein is pulled
+    (pulled=False: none)."""
     ns = ragged_lengths(count)
-    jobs = [(j, ns[j], j % 10 == 0) for j in range(count)]
+    jobs = [(j, ns[j], pulled and j % 10 == 0) for j in range(count)]
     workers = workers or min(8, os.cpu_count() or 1)
     if workers > 1 and count > 8:
         with ProcessPoolExecutor(workers) as ex:
@@ -115,8 +129,71 @@ def c4_jobs(count: int, rate_per_s: float, steps_lo: int, steps_hi: int,
 
 
 def c4_model(job: dict) -> dict:
-    m = M.synthetic_protein(job["n"], M.job_seed(10_000 + job["id"]))
-    return M.add_pulling(m) if job["pulled"] else m
+    """Job j's protein (not yet annealed, no tips: c4_finish adds them)."""
+    return M.synthetic_protein(job["n"], M.job_seed(10_000 + job["id"]))
+
+
+def c4_finish(jobs: list, models: list) -> list:
+    """The stream's proteins on Go-annealed natives, pulled jobs with their
+    tips. Needs the GPU."""
+    models = anneal_natives(models)
+    return [M.add_pulling(m) if j["pulled"] else m for j, m in zip(jobs, models)]
+
+
+def go_params() -> dict:
+    """Go model of the annealing: tether, native contacts, native angles and
+    dihedrals, local repulsion and Pauli exclusion; no QA, PID, DH or AFM."""
+    p = base_params()
+    p.update(chir_on=0, heur_ang_on=0, heur_dih_on=0, pauli_on=1)
+    return p
+
+
+def renative(m: dict, pos: np.ndarray, native_cutoff_A: float = 7.5) -> dict:
+    """The model with `pos` as its native (and starting) conformation: native
+    contacts |i-j| >= 3 closer than the cutoff, bond angles and dihedrals
+    from `pos` (one chain, as synthetic_protein makes them)."""
+    out = dict(m)
+    pos = np.asarray(pos, np.float64) - np.asarray(pos, np.float64).mean(axis=0)
+    out["pos"] = pos
+    out["orig_pos"] = pos.copy()
+    out["ang_theta"] = np.array([M._bond_angle(pos[a], pos[b], pos[c]) for a, b, c in m["ang"]])
+    out["dih_phi"] = np.array([M._dihedral(pos[a], pos[b], pos[c], pos[d])
+                               for a, b, c, d in m["dih"]])
+    d = np.linalg.norm(pos[:, None, :] - pos[None, :, :], axis=2)
+    ii, jj = np.nonzero(np.triu(d < native_cutoff_A * A, k=3))
+    out["cont"] = np.stack([ii, jj], axis=1).astype(np.int32)
+    out["cont_len"] = d[ii, jj].astype(np.float64)
+    out["cont_type"] = np.zeros(len(ii), np.int32)
+    return M.empty_tips(out)
+
+
+def anneal_natives(models: list, steps: int = 2000, t_hot: float = 0.6, t_cold: float = 0.15,
+                   stages: int = 10, seed: int = 448) -> list:
+    """Go-model annealing of the synthetic natives on the GPU: all models in
+    one batch (mixed precision), `stages` equal stages of `steps` in all with
+    the temperature lowered geometrically from t_hot to t_cold (by the noise
+    scale sqrt(T / t_hot)); returns the models rebuilt on the annealed
+    conformations (tips dropped: add them after)."""
+    from . import pcg as G
+    p = go_params()
+    b = G.Batch(p, models, temps=[t_hot] * len(models), seeds=[seed + k for k in range(len(models))],
+                precision=G.MIXED, philox_seed=seed, steps_per_launch=max(1, steps // stages))
+    per = max(1, steps // stages)
+    for k in range(stages):
+        T = t_hot * (t_cold / t_hot) ** (k / max(1, stages - 1))
+        b.set_noise_scale(math.sqrt(T / t_hot))
+        b.run(per)
+    out = [renative(m, b.get(s, 0)) for s, m in enumerate(models)]
+    b.close()
+    return out
+
+
+def c3_annealed(count: int = 256, workers: int | None = None) -> tuple[dict, list[dict]]:
+    """The ragged batch on Go-annealed natives; every 10th protein is pulled
+    (its tips placed on the annealed conformation). Needs the GPU."""
+    p, models = c3(count, workers, pulled=False)
+    models = anneal_natives(models)
+    return p, [M.add_pulling(m) if j % 10 == 0 else m for j, m in enumerate(models)]
 
 
 # algorithmic bytes per list item of each term (index ints plus parameters)
