@@ -347,6 +347,44 @@ def served_one(p, pool, rate, window_s, per_plain, chunk):
             "plain_fifo_model_p99_latency_s": None if math.isnan(f99) else round(float(f99), 1)}
 
 
+def served_backlog(p, pool, jobs_n=256, steps=4000, chunk=200):
+    """Capacity of the served form against the batched form on the same job
+    set: `jobs_n` jobs of the stream (their lengths, temperatures, pulling),
+    `steps` steps each, all submitted at once; served residue-steps/s over
+    the time to drain them, against one batch of the same proteins run for
+    the same steps."""
+    from pas_cg_amd import pcg as G
+    jobs, models = pool[0][:jobs_n], pool[1][:jobs_n]
+    work = sum(j["n"] for j in jobs) * steps
+    srv = G.Server(p, precision=G.MIXED, hbm_budget_bytes=64 << 30, max_resident=1024,
+                   chunk_steps=chunk, philox_seed=448)
+    t0 = time.perf_counter()
+    ids = [srv.submit(m, steps, temperature=j["T"], seed=j["id"]) for j, m in zip(jobs, models)]
+    while srv.round():
+        pass
+    span = time.perf_counter() - t0
+    ok = sum(srv.status(i)[0] == "done" for i in ids)
+    lat = [srv.times(i)[2] - srv.times(i)[0] for i in ids]
+    srv.close()
+    b = G.Batch(p, models, temps=[j["T"] for j in jobs], seeds=[j["id"] for j in jobs],
+                precision=G.MIXED, philox_seed=448, steps_per_launch=500)
+    b.run(1)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.sync()
+    bspan = time.perf_counter() - t0
+    b.close()
+    out = {"jobs": jobs_n, "steps_each": steps, "done": ok,
+           "served_residue_steps_per_s": work / span,
+           "batched_residue_steps_per_s": work / bspan,
+           "served_over_batched": round(bspan / span, 3),
+           "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+           "p99_latency_s": round(float(np.percentile(lat, 99)), 3)}
+    log(f"served backlog: {out}")
+    return out
+
+
 def served(p, capacity_rs, pool, per_plain, window_s, chunk):
     """The served stream as configured (lengths lognormal with median 150 and
     at most 2000 residues, steps log-uniform in [1e4, 1e6], 20 % pulled, T
@@ -591,8 +629,9 @@ def main():
                                   "hipGraph-captured step, 200 steps each"}
         out["speedup_vs_plain"] = round(value / prate, 2)
     if not args.no_serve:
-        out["served"] = served(W.full_params(args.pad), value, pool, per_plain,
-                               args.serve_window, args.serve_chunk)
+        sp = W.full_params(args.pad)
+        out["served"] = served(sp, value, pool, per_plain, args.serve_window, args.serve_chunk)
+        out["served"]["backlog"] = served_backlog(sp, pool, chunk=args.serve_chunk)
     if extra is not None:
         out["other_configs"] = other_configs(*extra)
     if not args.no_cpu:

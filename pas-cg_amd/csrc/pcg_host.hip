@@ -17,6 +17,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pcg_launch.hpp"
@@ -1956,31 +1957,57 @@ struct Job {
 
 } // namespace
 
+// Size classes of the served form: each runs its own launches on its own
+// stream, with a workgroup size and an LDS layout of its own, so that small
+// jobs never wait for the largest one (at most four streams: the hardware
+// queues a process gets).
+constexpr int SERVE_BUCKETS = 4;
+constexpr int SERVE_BUCKET_MAX_N[SERVE_BUCKETS] = {128, 320, 640, 1 << 30};
+constexpr int SERVE_BUCKET_BLOCK[SERVE_BUCKETS] = {256, 512, 1024, 1024};
+
+struct ServeBucket {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<int64_t> resident;  // jobs resident in this class
+  std::vector<int64_t> inflight;  // the jobs of the launch in flight, in slot order
+  bool busy = false;
+  DevBatch *d_jobs = nullptr;
+  JobSlot *d_slots = nullptr;
+  DevBatch *h_jobs = nullptr;  // pinned: the copies run asynchronously
+  JobSlot *h_slots = nullptr;
+  int cap = 0;
+};
+
 struct pcg_server {
   pcg_params p{};
   pcg_server_options o{};
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // size checks at submission
   std::map<int64_t, std::unique_ptr<Job>> jobs;
   std::deque<int64_t> queue;
-  std::vector<int64_t> resident;
+  ServeBucket bucket[SERVE_BUCKETS];
   int64_t next_id = 1;
   size_t resident_bytes = 0;
-  DevBatch *d_jobs = nullptr;
-  JobSlot *d_slots = nullptr;
-  int d_jobs_cap = 0;
   // device blocks of retired jobs, by size, reused for the next admissions
   // (no hipMalloc/hipFree per job); sizes are multiples of 256 KiB
   std::multimap<size_t, void *> pool;
   size_t pooled_bytes = 0;
   double noise_scale = 1.0;  // scales the Langevin noise of every job
   ~pcg_server() {
+    for (auto &bk : bucket)
+      if (bk.stream) (void)hipStreamSynchronize(bk.stream);
     for (auto &kv : jobs)
       if (kv.second->B && kv.second->B->arena.base)
         pool.emplace(kv.second->block_bytes, kv.second->B->arena.base);
     jobs.clear();
     for (auto &kv : pool) (void)hipFree(kv.second);
-    if (d_jobs) (void)hipFree(d_jobs);
-    if (d_slots) (void)hipFree(d_slots);
+    for (auto &bk : bucket) {
+      if (bk.d_jobs) (void)hipFree(bk.d_jobs);
+      if (bk.d_slots) (void)hipFree(bk.d_slots);
+      if (bk.h_jobs) (void)hipHostFree(bk.h_jobs);
+      if (bk.h_slots) (void)hipHostFree(bk.h_slots);
+      if (bk.done) (void)hipEventDestroy(bk.done);
+      if (bk.stream) (void)hipStreamDestroy(bk.stream);
+    }
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -2033,6 +2060,10 @@ int pcg_server_create(const pcg_params *p, const pcg_server_options *o, pcg_serv
   if (s->o.hbm_budget_bytes <= 0) s->o.hbm_budget_bytes = (int64_t)64 << 30;
   HIPCHK(hipSetDevice(s->o.device));
   HIPCHK(hipStreamCreate(&s->stream));
+  for (auto &bk : s->bucket) {
+    HIPCHK(hipStreamCreate(&bk.stream));
+    HIPCHK(hipEventCreateWithFlags(&bk.done, hipEventDisableTiming));
+  }
   *out = s.release();
   return PCG_OK;
 }
@@ -2073,11 +2104,105 @@ int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature, int
   return PCG_OK;
 }
 
-int pcg_server_round(pcg_server *s, int32_t *active) {
-  if (!s) return fail(PCG_E_ARG, "null server");
-  HIPCHK(hipSetDevice(s->o.device));
-  // admission: first fit in arrival order
-  for (auto it = s->queue.begin(); it != s->queue.end() && (int)s->resident.size() < s->o.max_resident;) {
+namespace {
+
+int bucket_of(int n) {
+  int k = 0;
+  while (k + 1 < SERVE_BUCKETS && n > SERVE_BUCKET_MAX_N[k]) ++k;
+  return k;
+}
+
+// A finished launch of bucket k: every job's progress from its slot; jobs
+// that are done, failed (non-finite energy or a list overflow) or cancelled
+// leave the bucket.
+int serve_collect(pcg_server *s, int k) {
+  ServeBucket &bk = s->bucket[k];
+  bk.busy = false;
+  std::vector<int64_t> keep;
+  for (size_t q = 0; q < bk.inflight.size(); ++q) {
+    Job &j = *s->jobs[bk.inflight[q]];
+    const JobSlot &sl = bk.h_slots[q];
+    j.done = sl.step;
+    int rc = PCG_OK;
+    if (j.status == PCG_JOB_CANCELLED) {
+      rc = retire(s, j, PCG_JOB_CANCELLED);
+    } else if (sl.overflow || !std::isfinite(sl.Vtot)) {
+      rc = retire(s, j, PCG_JOB_FAILED);
+    } else if (j.done >= j.nsteps) {
+      rc = retire(s, j, PCG_JOB_DONE);
+    } else {
+      keep.push_back(j.id);
+    }
+    if (rc) return rc;
+  }
+  // jobs admitted while the launch ran stay resident too
+  for (int64_t id : bk.resident)
+    if (std::find(bk.inflight.begin(), bk.inflight.end(), id) == bk.inflight.end())
+      keep.push_back(id);
+  bk.resident.swap(keep);
+  bk.inflight.clear();
+  return PCG_OK;
+}
+
+// The next launch of bucket k (idle, with resident jobs): chunk_steps steps of
+// every resident job, the job descriptors and step targets copied in and the
+// slots copied out on the bucket's stream, its event recorded after them.
+int serve_launch(pcg_server *s, int k) {
+  ServeBucket &bk = s->bucket[k];
+  const int nres = (int)bk.resident.size();
+  if (nres == 0 || bk.busy) return PCG_OK;
+  if (nres > bk.cap) {
+    if (bk.d_jobs) HIPCHK(hipFree(bk.d_jobs));
+    if (bk.d_slots) HIPCHK(hipFree(bk.d_slots));
+    if (bk.h_jobs) HIPCHK(hipHostFree(bk.h_jobs));
+    if (bk.h_slots) HIPCHK(hipHostFree(bk.h_slots));
+    bk.d_jobs = nullptr;
+    bk.d_slots = nullptr;
+    bk.h_jobs = nullptr;
+    bk.h_slots = nullptr;
+    bk.cap = std::max(nres, 2 * bk.cap);
+    HIPCHK(hipMalloc(&bk.d_jobs, bk.cap * sizeof(DevBatch)));
+    HIPCHK(hipMalloc(&bk.d_slots, bk.cap * sizeof(JobSlot)));
+    HIPCHK(hipHostMalloc(&bk.h_jobs, bk.cap * sizeof(DevBatch)));
+    HIPCHK(hipHostMalloc(&bk.h_slots, bk.cap * sizeof(JobSlot)));
+  }
+  const size_t fsz = sizeof(double);  // the persistent kernel's LDS forces are FP64
+  size_t lds = 0;
+  for (int q = 0; q < nres; ++q) {
+    Job &j = *s->jobs[bk.resident[q]];
+    DeviceBatch &B = *j.B;
+    bk.h_slots[q] = JobSlot{std::min<int64_t>(j.done + s->o.chunk_steps, j.nsteps), 0, 0.0, 0, 0};
+    // each job stages its bonded data and its Gear state in LDS when that
+    // fits on its own
+    const size_t nj = (size_t)B.max_n, cj = (size_t)B.max_nnc;
+    B.d.bond_lds = LdsLayout(nj, fsz, true, false, cj).total <= LDS_DYN_MAX ? 1 : 0;
+    B.d.gear_lds = B.d.bond_lds && LdsLayout(nj, fsz, true, true, cj).total <= LDS_DYN_MAX ? 1 : 0;
+    lds = std::max(lds, LdsLayout(nj, fsz, B.d.bond_lds != 0, B.d.gear_lds != 0, cj).total);
+    bk.h_jobs[q] = B.d;
+  }
+  if (lds > LDS_DYN_MAX) return fail(PCG_E_ARG, "job too large for the persistent kernel");
+  HIPCHK(hipMemcpyAsync(bk.d_jobs, bk.h_jobs, nres * sizeof(DevBatch), hipMemcpyHostToDevice,
+                        bk.stream));
+  HIPCHK(hipMemcpyAsync(bk.d_slots, bk.h_slots, nres * sizeof(JobSlot), hipMemcpyHostToDevice,
+                        bk.stream));
+  launch_persistent_jobs(bk.d_jobs, bk.d_slots, nres, s->o.precision, SERVE_BUCKET_BLOCK[k],
+                         s->o.chunk_steps, lds, bk.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(bk.h_slots, bk.d_slots, nres * sizeof(JobSlot), hipMemcpyDeviceToHost,
+                        bk.stream));
+  HIPCHK(hipEventRecord(bk.done, bk.stream));
+  bk.inflight = bk.resident;
+  bk.busy = true;
+  return PCG_OK;
+}
+
+// Admission, first fit in arrival order, within the HBM budget and the slot
+// count; each job goes to the size class of its length, its state set up on
+// that class's stream.
+int serve_admit(pcg_server *s) {
+  int nres = 0;
+  for (auto &bk : s->bucket) nres += (int)bk.resident.size();
+  for (auto it = s->queue.begin(); it != s->queue.end() && nres < s->o.max_resident;) {
     Job &j = *s->jobs[*it];
     if (j.status == PCG_JOB_CANCELLED) {
       it = s->queue.erase(it);
@@ -2112,10 +2237,12 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
         continue;
       }
     }
+    const int k = bucket_of(j.model.n);
+    ServeBucket &bk = s->bucket[k];
     pcg_model v = j.model.view();
     uint64_t key = mix64(s->o.philox_seed ^ mix64((uint64_t)(uint32_t)j.seed));
     int rc = make_batch(s->p, 1, &v, &j.T, &j.seed, s->o.precision, key, s->o.device, j.B,
-                        s->stream, nullptr, 0, blk, blk_bytes);
+                        bk.stream, nullptr, 0, blk, blk_bytes);
     if (rc) {
       s->pool.emplace(blk_bytes, blk);
       s->pooled_bytes += blk_bytes;
@@ -2123,84 +2250,75 @@ int pcg_server_round(pcg_server *s, int32_t *active) {
         ++it;
         continue;
       }
+      return rc;
     }
-    if (rc) return rc;
     if (s->noise_scale != 1.0) {
       SysDesc &d = j.B->sys[0];
       d.const2 = const2_of(j.B->hp.p, d.temperature) * s->noise_scale;
       HIPCHK(hipMemcpyAsync((void *)j.B->d.sys, &d, sizeof(SysDesc), hipMemcpyHostToDevice,
-                            s->stream));
+                            bk.stream));
     }
     j.status = PCG_JOB_RUNNING;
     j.t_start = now_s();
     j.block_bytes = blk_bytes;
     s->resident_bytes += blk_bytes;
-    s->resident.push_back(j.id);
+    bk.resident.push_back(j.id);
+    ++nres;
     it = s->queue.erase(it);
   }
-  int nres = (int)s->resident.size();
-  if (nres > 0) {
-    // per-job step budget for this round
-    std::vector<DevBatch> descs(nres);
-    std::vector<JobSlot> slots(nres);
-    int max_n = 0;
-    const size_t fsz = sizeof(double);  // the persistent kernel's LDS forces are FP64
-    size_t lds = 0;
-    for (int k = 0; k < nres; ++k) {
-      Job &j = *s->jobs[s->resident[k]];
-      DeviceBatch &B = *j.B;
-      slots[k] = JobSlot{std::min<int64_t>(j.done + s->o.chunk_steps, j.nsteps), 0, 0.0, 0, 0};
-      // each job stages its bonded data in LDS when that fits on its own
-      const size_t nj = (size_t)B.max_n;
-      const size_t cj = (size_t)B.max_nnc;
-      B.d.bond_lds = LdsLayout(nj, fsz, true, false, cj).total <= LDS_DYN_MAX ? 1 : 0;
-      B.d.gear_lds = B.d.bond_lds && LdsLayout(nj, fsz, true, true, cj).total <= LDS_DYN_MAX ? 1 : 0;
-      lds = std::max(lds, LdsLayout(nj, fsz, B.d.bond_lds != 0, B.d.gear_lds != 0, cj).total);
-      descs[k] = B.d;
-      max_n = std::max(max_n, B.max_n);
-    }
-    if (nres > s->d_jobs_cap) {
-      if (s->d_jobs) HIPCHK(hipFree(s->d_jobs));
-      if (s->d_slots) HIPCHK(hipFree(s->d_slots));
-      s->d_jobs = nullptr;
-      s->d_slots = nullptr;
-      s->d_jobs_cap = std::max(nres, 2 * s->d_jobs_cap);
-      HIPCHK(hipMalloc(&s->d_jobs, s->d_jobs_cap * sizeof(DevBatch)));
-      HIPCHK(hipMalloc(&s->d_slots, s->d_jobs_cap * sizeof(JobSlot)));
-    }
-    HIPCHK(hipMemcpyAsync(s->d_jobs, descs.data(), nres * sizeof(DevBatch), hipMemcpyHostToDevice,
-                          s->stream));
-    HIPCHK(hipMemcpyAsync(s->d_slots, slots.data(), nres * sizeof(JobSlot), hipMemcpyHostToDevice,
-                          s->stream));
-    if (lds > LDS_DYN_MAX) return fail(PCG_E_ARG, "job too large for the persistent kernel");
-    int block = 64;
-    while (block < 1024 && block < 2 * max_n) block *= 2;
-    block = std::max(block, 128);
-    launch_persistent_jobs(s->d_jobs, s->d_slots, nres, s->o.precision, block, s->o.chunk_steps,
-                           lds, s->stream);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(slots.data(), s->d_slots, nres * sizeof(JobSlot), hipMemcpyDeviceToHost,
-                          s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    // retire finished / failed jobs (their progress from the round's slots)
-    std::vector<int64_t> keep;
-    for (int k = 0; k < nres; ++k) {
-      int64_t id = s->resident[k];
-      Job &j = *s->jobs[id];
-      int rc;
-      j.done = slots[k].step;
-      bool bad = slots[k].overflow || !std::isfinite(slots[k].Vtot);
-      if (bad) {
-        if ((rc = retire(s, j, PCG_JOB_FAILED))) return rc;
-      } else if (j.done >= j.nsteps) {
-        if ((rc = retire(s, j, PCG_JOB_DONE))) return rc;
-      } else {
-        keep.push_back(id);
-      }
-    }
-    s->resident.swap(keep);
+  return PCG_OK;
+}
+
+} // namespace
+
+int pcg_server_round(pcg_server *s, int32_t *active) {
+  if (!s) return fail(PCG_E_ARG, "null server");
+  HIPCHK(hipSetDevice(s->o.device));
+  int rc;
+  // launches that have finished
+  bool progressed = false;
+  for (int k = 0; k < SERVE_BUCKETS; ++k) {
+    ServeBucket &bk = s->bucket[k];
+    if (!bk.busy) continue;
+    hipError_t e = hipEventQuery(bk.done);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) return fail(PCG_E_HIP, std::string("HIP: ") + hipGetErrorString(e));
+    if ((rc = serve_collect(s, k))) return rc;
+    progressed = true;
   }
-  if (active) *active = (int)(s->resident.size() + s->queue.size());
+  if ((rc = serve_admit(s))) return rc;
+  for (int k = 0; k < SERVE_BUCKETS; ++k)
+    if (!s->bucket[k].busy && !s->bucket[k].resident.empty()) {
+      if ((rc = serve_launch(s, k))) return rc;
+      progressed = true;
+    }
+  // nothing finished and nothing to start: wait until some launch finishes
+  // (whichever does first), so that a caller's loop of rounds does not spin
+  while (!progressed) {
+    bool any_busy = false;
+    for (int k = 0; k < SERVE_BUCKETS; ++k) {
+      ServeBucket &bk = s->bucket[k];
+      if (!bk.busy) continue;
+      any_busy = true;
+      hipError_t e = hipEventQuery(bk.done);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess) return fail(PCG_E_HIP, std::string("HIP: ") + hipGetErrorString(e));
+      if ((rc = serve_collect(s, k))) return rc;
+      progressed = true;
+    }
+    if (!any_busy) break;
+    if (!progressed) {
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      continue;
+    }
+    if ((rc = serve_admit(s))) return rc;
+    for (int k = 0; k < SERVE_BUCKETS; ++k)
+      if (!s->bucket[k].busy && !s->bucket[k].resident.empty() && (rc = serve_launch(s, k)))
+        return rc;
+  }
+  int n = (int)s->queue.size();
+  for (auto &bk : s->bucket) n += (int)bk.resident.size();
+  if (active) *active = n;
   return PCG_OK;
 }
 
@@ -2239,15 +2357,18 @@ int pcg_server_set_noise_scale(pcg_server *s, double scale) {
   if (!s || !(scale >= 0)) return fail(PCG_E_ARG, "bad arguments");
   HIPCHK(hipSetDevice(s->o.device));
   s->noise_scale = scale;
-  // resident jobs change from their next round on
-  for (int64_t id : s->resident) {
-    Job &j = *s->jobs[id];
-    SysDesc &d = j.B->sys[0];
-    d.const2 = const2_of(j.B->hp.p, d.temperature) * scale;
-    HIPCHK(hipMemcpyAsync((void *)j.B->d.sys, &d, sizeof(SysDesc), hipMemcpyHostToDevice,
-                          s->stream));
+  // resident jobs change from their next launch on (queued on their class's
+  // stream, after any launch in flight)
+  for (auto &bk : s->bucket) {
+    for (int64_t id : bk.resident) {
+      Job &j = *s->jobs[id];
+      SysDesc &d = j.B->sys[0];
+      d.const2 = const2_of(j.B->hp.p, d.temperature) * scale;
+      HIPCHK(hipMemcpyAsync((void *)j.B->d.sys, &d, sizeof(SysDesc), hipMemcpyHostToDevice,
+                            bk.stream));
+    }
+    HIPCHK(hipStreamSynchronize(bk.stream));
   }
-  HIPCHK(hipStreamSynchronize(s->stream));
   return PCG_OK;
 }
 
@@ -2260,9 +2381,19 @@ int pcg_server_cancel(pcg_server *s, int64_t job) {
     j.status = PCG_JOB_CANCELLED;
     j.t_end = now_s();
   } else if (j.status == PCG_JOB_RUNNING) {
-    int rc = retire(s, j, PCG_JOB_CANCELLED);
-    if (rc) return rc;
-    s->resident.erase(std::remove(s->resident.begin(), s->resident.end(), job), s->resident.end());
+    // its memory may be in use by a launch in flight: the job leaves its
+    // class once the class's launch has finished
+    ServeBucket &bk = s->bucket[bucket_of(j.model.n)];
+    if (bk.busy) HIPCHK(hipEventSynchronize(bk.done));
+    if (bk.busy) {
+      int rc = serve_collect(s, bucket_of(j.model.n));
+      if (rc) return rc;
+    }
+    if (j.status == PCG_JOB_RUNNING) {
+      int rc = retire(s, j, PCG_JOB_CANCELLED);
+      if (rc) return rc;
+      bk.resident.erase(std::remove(bk.resident.begin(), bk.resident.end(), job), bk.resident.end());
+    }
   }
   return PCG_OK;
 }
@@ -2270,7 +2401,11 @@ int pcg_server_cancel(pcg_server *s, int64_t job) {
 int pcg_server_stats(pcg_server *s, int64_t *resident_bytes, int32_t *resident, int32_t *queued) {
   if (!s) return fail(PCG_E_ARG, "null server");
   if (resident_bytes) *resident_bytes = (int64_t)s->resident_bytes;
-  if (resident) *resident = (int)s->resident.size();
+  if (resident) {
+    int n = 0;
+    for (auto &bk : s->bucket) n += (int)bk.resident.size();
+    *resident = n;
+  }
   if (queued) *queued = (int)s->queue.size();
   return PCG_OK;
 }
