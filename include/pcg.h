@@ -51,7 +51,11 @@ enum pcg_term {
   PCG_SND,
   PCG_VEL_AFM,
   PCG_FORCE_AFM, /* constant-force AFM tips (afm/force/eval_forces.cpp:10-13) */
-  PCG_SOLID_WALL, /* solid walls at fixed planes (wall/solid/eval_forces.cpp:4-17) */
+  PCG_SOLID_WALL, /* solid walls (wall/solid/eval_forces.cpp:4-17) */
+  PCG_HW_FREE,    /* harmonic walls, free residues (wall/harmonic/eval_free.cpp:15-31) */
+  PCG_HW_CONN,    /* harmonic wall connections (wall/harmonic/eval_connected.cpp:17-29) */
+  PCG_LJW_FREE,   /* LJ walls, free residues (wall/lj/sift_free.cpp:15-37) */
+  PCG_LJW_CONN,   /* LJ wall connections (wall/lj/eval_connected.cpp:15-41) */
   PCG_NUM_TERMS
 };
 
@@ -107,13 +111,30 @@ typedef struct pcg_params {
   double charge[21];
   int32_t ptype[21];
   int32_t base_sync[21][4];
-  /* solid walls (sbox.walls "solid", state.cpp:610-631): planes through
-   * origin with unit normal, potential depth d^-10 within min_dist
-   * (sbox.walls.threshold); n_walls 0 turns them off. Shared by every
-   * system of a batch. */
+  /* walls (state::setup_walls, state.cpp:570-700): n_walls planes through
+   * origin with normal; n_walls 0 turns them off. wall_depth is
+   * sbox.walls.solid wall.depth (the d^-10 force of solid walls and of
+   * residues free of harmonic walls) and wall_min_dist sbox.walls.threshold.
+   * Batches and servers take solid walls only, at fixed planes, shared by
+   * every system. */
   int32_t n_walls;
   double wall_origin[6][3], wall_normal[6][3];
   double wall_depth, wall_min_dist;
+  /* per plane: kind (0 solid, 1 harmonic, 2 Lennard-Jones), the side of the
+   * simulation box it stands on (0..5: -x, +x, -y, +y, -z, +z; -1 none, a
+   * plane the box protocol leaves in place) and its connection limit
+   * (sbox.walls.<kind> wall.connection limit, "auto" resolved as
+   * state.cpp:613-620 does). The planes of one axis come as the reference
+   * creates them: the -axis side, then the +axis side. */
+  int32_t wall_kind[6], wall_side[6], wall_limit[6];
+  /* state::harmonic_walls_enabled and lj_walls_enabled: the reference's
+   * trajectory set-up clears both right after setup_walls (state.cpp:111),
+   * so its trajectories evaluate neither; 1 evaluates them */
+  int32_t hw_on, ljw_on;
+  double wall_hh1;          /* sbox.walls.harmonic wall.HH1 */
+  double wall_lj_depth;     /* sbox.walls.lj wall.depth */
+  double ljw_breaking_dist; /* breaking dist factor * 2^(-1/6) * threshold */
+  double ljw_saturation_diff; /* dt / lj wall.cycle duration (thread_setup.cpp:787-790) */
 } pcg_params;
 
 /* One protein system after state::compile_model
@@ -255,6 +276,77 @@ int pcg_use_graph(pcg_ctx *c, int32_t on);
 int pcg_checkpoint_size(pcg_ctx *c, int64_t *bytes);
 int pcg_checkpoint(pcg_ctx *c, void *buf, int64_t bytes);
 int pcg_restore(pcg_ctx *c, const void *buf, int64_t bytes);
+
+/* ---- simulation box, walls and the box protocol (plain context) --------
+ * sbox parameters (cg/include/cg/sbox/parameters.h, the "simulation box"
+ * section of cg/default.yml) that the protocol phases of
+ * thread_main.cpp:144-330 read, in internal units. */
+typedef struct pcg_box {
+  double box_min[3], box_max[3]; /* state::box at the trajectory start (state.cpp:173-225) */
+  int32_t pbc[3];                /* axes of type "periodic": cell = box extent */
+  double total_time;             /* general.total time */
+  double rest_period, target_vel, accel_dist, avg_forces_over; /* sbox.common */
+  int32_t squeezing;             /* sbox.squeezing */
+  double target_density, vel_above_2V;
+  int32_t force_min;             /* sbox.finding force minimum */
+  double force_for_max_vel;
+  int32_t oscillations;          /* sbox.oscillations */
+  int32_t osc_shear;             /* type "shear" (else "squeeze") */
+  int32_t num_of_cycles;
+  double angular_freq;
+  int32_t amp_relative;          /* amplitude variant "relative" (else "absolute") */
+  double amp_abs, amp_rel;
+} pcg_box;
+
+/* phases of the box protocol (simul::phase, thread_main.cpp:32-54); NONE:
+ * the walls stay where they are */
+enum {
+  PCG_PHASE_NONE = 0, PCG_PHASE_SQUEEZING, PCG_PHASE_REST_AFTER_SQUEEZING,
+  PCG_PHASE_FIND_FORCE_MIN, PCG_PHASE_REST_AFTER_FORCE_MIN, PCG_PHASE_MAX_AMPLITUDE,
+  PCG_PHASE_REST_AFTER_MAX_AMP, PCG_PHASE_OSCILLATIONS, PCG_PHASE_REST_AFTER_OSCILLATIONS,
+  PCG_PHASE_FREEFORM
+};
+
+/* pcg_box_state layout (doubles): phase; since; until; box min [3], max [3];
+ * amplitude; omega; displacement; shear; whether the averaged z force has a
+ * value, and the value; per plane [6][3]: origin, normal, reaction force of
+ * the last evaluation (the wall's dyn.*_wall_F), shift of the last move;
+ * per plane [6]: work; whether avg_F has a value; avg_F [6][3]; the PBC
+ * cell [3]. */
+enum {
+  PCG_BOX_PHASE = 0, PCG_BOX_SINCE = 1, PCG_BOX_UNTIL = 2, PCG_BOX_MIN = 3, PCG_BOX_MAX = 6,
+  PCG_BOX_AMPLITUDE = 9, PCG_BOX_OMEGA = 10, PCG_BOX_DISPLACEMENT = 11, PCG_BOX_SHEAR = 12,
+  PCG_BOX_AVG_Z_HAS = 13, PCG_BOX_AVG_Z = 14, PCG_BOX_ORIGIN = 15, PCG_BOX_NORMAL = 33,
+  PCG_BOX_WALL_F = 51, PCG_BOX_SHIFT = 69, PCG_BOX_WORK = 87, PCG_BOX_AVG_F_HAS = 93,
+  PCG_BOX_AVG_F = 99, PCG_BOX_CELL = 117, PCG_BOX_STATE_NUM = 120
+};
+
+/* The box of the trajectory (state::traj_init's box and setup_walls,
+ * state.cpp:570-730): the planes with a box side move to its faces
+ * (reinit_wall_values), periodic axes take the box extent as their PBC
+ * cell, and, when both z sides have a plane, every step logs the wall forces
+ * (wall/log_forces.cpp:4-17) averaged over avg_forces_over. The phase is
+ * NONE until pcg_box_start. */
+int pcg_box_setup(pcg_ctx *c, const pcg_box *box);
+/* equil_step's hand-over to the box protocol (thread_main.cpp:130-135):
+ * from the current time, SQUEEZING until total_time. Each later pcg_step
+ * step first runs the phase logic of thread_main.cpp:144-330 on the device
+ * (wall moves, phase changes), then advances. */
+int pcg_box_start(pcg_ctx *c);
+int pcg_box_state(pcg_ctx *c, double *out /*[PCG_BOX_STATE_NUM]*/);
+/* wall::lj::process_candidates (wall/lj/process_candidates.cpp:5-31): removes
+ * the connections eval_connected released, then connects the candidates
+ * sift_free found, nearest first. The reference never clears its candidate
+ * list (state.cpp:580 is its only clear); this call consumes the candidates
+ * it processes. */
+int pcg_ljw_process(pcg_ctx *c);
+/* Wall connections: kind 0 harmonic (made at creation from the starting
+ * positions, state.cpp:653-670), 1 Lennard-Jones (live slots in creation
+ * order). Per connection: residue, wall (plane index), bead offset [3] and
+ * saturation (harmonic: the natural distance). Returns the count in *size;
+ * fills up to cap. */
+int pcg_wall_conns(pcg_ctx *c, int32_t kind, int32_t cap, int32_t *size, int32_t *res,
+                   int32_t *wall, double *offset, double *sat);
 
 /* ---- batched form: many systems, one persistent launch ---------------- */
 typedef struct pcg_batch pcg_batch;

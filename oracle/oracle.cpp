@@ -304,6 +304,67 @@ struct ora_sys {
   std::vector<sync_t> sync;
   int num_qa_contacts = 0;
 
+  /* walls (state::setup_walls, state.cpp:570-700): reaction force of each
+   * plane in the last evaluation, residues connected to a wall, harmonic
+   * connections (fixed at set-up), LJ connections in slot order (vect::set:
+   * removal leaves a vacant slot, emplace appends), the LJ candidates and
+   * releases since the last processing */
+  v3 wall_F[6]{};
+  std::vector<uint8_t> is_conn;
+  struct hw_conn_t {
+    int wall, res;
+    v3 off;
+    double nat;
+  };
+  std::vector<hw_conn_t> hw_conns;
+  struct ljw_conn_t {
+    int res, wall;
+    v3 off;
+    double sat;
+    bool vacant;
+  };
+  std::vector<ljw_conn_t> ljw_conns;
+  struct ljw_cand_t {
+    int res, wall;
+    double dist;
+  };
+  std::vector<ljw_cand_t> ljw_cands;
+  std::vector<int> ljw_removed;
+  int ljw_num_conns[6] = {0, 0, 0, 0, 0, 0};
+
+  /* simulation box and its protocol (state.cpp:702-730, thread_main.cpp:144-330) */
+  template <typename T> struct mavg_t { /* moving_avg, cg/include/cg/types/avg.h:5-38 */
+    bool has = false;
+    T avg{}, sum{};
+    double init_time = 0.0, window = 0.0;
+    int num = 0;
+    void add(double time, T sample) {
+      if (num == 0) {
+        init_time = time;
+        sum = sample;
+        num = 1;
+      } else {
+        sum += sample;
+        ++num;
+      }
+      if (init_time + window <= time) {
+        avg = sum / (double)num;
+        has = true;
+        num = 0;
+      }
+    }
+  };
+  bool box_set = false, log_on = false, shear = false;
+  ora_box bx{};
+  v3 box_min{0, 0, 0}, box_max{0, 0, 0};
+  int side_wall[6] = {-1, -1, -1, -1, -1, -1};
+  int phase = 0;
+  double since = 0.0, until = 0.0, amplitude = 0.0, omega = 0.0, displacement = 0.0;
+  v3 shift[6]{};
+  double work[6] = {0, 0, 0, 0, 0, 0};
+  mavg_t<v3> avg_F[6];
+  mavg_t<double> avg_z;
+
   /* derived parameters */
   double V_factor = 0.0, screen_inv = 0.0;
   double gamma2 = 0, const2 = 0, dt = 0, dt_inv = 0, deltsq = 0;
@@ -891,17 +952,326 @@ struct ora_sys {
     for (auto const &tp : ftips) add_F(Fo, tp.first, tp.second);
   }
 
+  v3 plane_n(int w) const { return v3{p.wall_normal[w][0], p.wall_normal[w][1], p.wall_normal[w][2]}; }
+  v3 plane_o(int w) const { return v3{p.wall_origin[w][0], p.wall_origin[w][1], p.wall_origin[w][2]}; }
+  static double sgn(double x) { return (double)((0.0 < x) - (x < 0.0)); } /* math.h:119-121 */
+
   /* wall::solid::eval_forces::iter, cg/src/cg/wall/solid/eval_forces.cpp:4-17 */
   void eval_solid_walls(double *Fo) {
     for (int i = 0; i < n; ++i)
       for (int w = 0; w < p.n_walls; ++w) {
-        v3 nrm{p.wall_normal[w][0], p.wall_normal[w][1], p.wall_normal[w][2]};
-        v3 org{p.wall_origin[w][0], p.wall_origin[w][1], p.wall_origin[w][2]};
+        if (p.wall_kind[w] != 0) continue;
+        v3 nrm = plane_n(w), org = plane_o(w);
         double sd = dot(nrm, r[i] - org), dist = std::abs(sd);
         if (dist >= p.wall_min_dist) continue;
-        double dV = p.wall_depth * std::pow(dist, -10.0) * (double)((0.0 < sd) - (sd < 0.0));
+        double dV = p.wall_depth * std::pow(dist, -10.0) * sgn(sd);
         add_F(Fo, i, dV * nrm);
+        wall_F[w] -= dV * nrm;
       }
+  }
+
+  /* wall::harmonic::eval_free::iter, cg/src/cg/wall/harmonic/eval_free.cpp:15-31:
+   * the sign is the reference's (the opposite of the solid wall's) */
+  void eval_hw_free(double *Fo) {
+    for (int i = 0; i < n; ++i) {
+      if (is_conn[i]) continue;
+      for (int w = 0; w < p.n_walls; ++w) {
+        if (p.wall_kind[w] != 1) continue;
+        v3 nrm = plane_n(w), org = plane_o(w);
+        double sd = dot(nrm, r[i] - org), dist = std::abs(sd);
+        if (dist >= p.wall_min_dist) continue;
+        double dV = p.wall_depth * std::pow(dist, -10.0) * sgn(sd);
+        v3 force = dV * nrm;
+        sub_F(Fo, i, force);
+        wall_F[w] += force;
+      }
+    }
+  }
+
+  /* wall::harmonic::eval_connected::iter,
+   * cg/src/cg/wall/harmonic/eval_connected.cpp:17-29 (the force sign is the
+   * reference's: F[res] -= dV_dr * u with u pointing from the residue to the bead) */
+  void eval_hw_conn(double *Fo, double &Vo) {
+    for (auto const &c : hw_conns) {
+      v3 bead = c.off + plane_o(c.wall);
+      v3 r12 = bead - r[c.res];
+      double r12_n = norm(r12), r12_rn = 1.0 / r12_n;
+      v3 u = r12 * r12_rn;
+      double V_, dV;
+      harmonic(p.wall_hh1, 0.0, c.nat, r12_n, V_, dV);
+      Vo += V_;
+      sub_F(Fo, c.res, dV * u);
+      wall_F[c.wall] += dV * u;
+    }
+  }
+
+  /* wall::lj::sift_free::iter, cg/src/cg/wall/lj/sift_free.cpp:15-37 */
+  void eval_ljw_free(double *Fo, double &Vo) {
+    for (int i = 0; i < n; ++i) {
+      if (is_conn[i]) continue;
+      for (int w = 0; w < p.n_walls; ++w) {
+        if (p.wall_kind[w] != 2) continue;
+        v3 nrm = plane_n(w), org = plane_o(w);
+        double sd = dot(nrm, r[i] - org), dist = std::abs(sd);
+        if (dist > p.wall_min_dist) continue;
+        double V_, dV;
+        shifted_lj(p.wall_lj_depth, p.wall_min_dist, 1.0 / dist, V_, dV);
+        Vo += V_;
+        v3 F_ = (dV * nrm) * sgn(sd);
+        sub_F(Fo, i, F_);
+        wall_F[w] += F_;
+        if (ljw_num_conns[w] < p.wall_limit[w]) ljw_cands.push_back(ljw_cand_t{i, w, dist});
+      }
+    }
+  }
+
+  /* wall::lj::eval_connected::iter, cg/src/cg/wall/lj/eval_connected.cpp:15-41
+   * (force sign as the reference's) */
+  void eval_ljw_conn(double *Fo, double &Vo) {
+    for (int k = 0; k < (int)ljw_conns.size(); ++k) {
+      auto &c = ljw_conns[k];
+      if (c.vacant) continue;
+      v3 bead = plane_o(c.wall) + c.off;
+      v3 r12 = bead - r[c.res];
+      double r12_n = norm(r12), r12_rn = 1.0 / r12_n;
+      v3 u = r12 * r12_rn;
+      double V_, dV;
+      lj(p.wall_lj_depth, p.wall_min_dist, r12_rn, V_, dV);
+      Vo += c.sat * V_;
+      v3 F_ = (c.sat * dV) * u;
+      sub_F(Fo, c.res, F_);
+      wall_F[c.wall] += F_;
+      if (r12_n < p.ljw_breaking_dist) {
+        c.sat = std::min(c.sat + p.ljw_saturation_diff, 1.0);
+      } else {
+        c.sat = std::max(c.sat - p.ljw_saturation_diff, 0.0);
+        if (c.sat == 0.0) ljw_removed.push_back(k);
+      }
+    }
+  }
+
+  /* wall::lj::process_candidates, cg/src/cg/wall/lj/process_candidates.cpp:5-31:
+   * releases first, then the candidates nearest first; a new connection does
+   * not mark its residue connected (as in the reference). Candidates with
+   * equal distances keep their (residue, wall) order. The reference keeps its
+   * candidates for ever; here each call consumes its own. */
+  void ljw_process() {
+    for (int k : ljw_removed) {
+      auto &c = ljw_conns[k];
+      is_conn[c.res] = 0;
+      --ljw_num_conns[c.wall];
+      c.vacant = true;
+    }
+    ljw_removed.clear();
+    std::stable_sort(ljw_cands.begin(), ljw_cands.end(),
+                     [](ljw_cand_t const &a, ljw_cand_t const &b) { return a.dist < b.dist; });
+    for (auto const &cd : ljw_cands) {
+      int w = cd.wall;
+      if (is_conn[cd.res] || ljw_num_conns[w] >= p.wall_limit[w]) continue;
+      v3 nrm = plane_n(w), org = plane_o(w), rr = r[cd.res];
+      v3 proj = rr - dot(nrm, rr - org) * nrm;
+      ljw_conns.push_back(ljw_conn_t{cd.res, w, proj - org, 0.0, false});
+      ++ljw_num_conns[w];
+    }
+    ljw_cands.clear();
+  }
+
+  /* state::setup_walls' harmonic connections (state.cpp:653-670): after each
+   * axis with harmonic walls, every harmonic plane so far takes up to its
+   * limit of the unconnected residues nearest to it */
+  void setup_harmonic() {
+    for (int w = 0; w < p.n_walls; ++w) {
+      if (p.wall_kind[w] != 1) continue;
+      if (p.wall_side[w] >= 0 && p.wall_side[w] % 2 == 0) continue; /* the axis' second plane ends it */
+      for (int hw = 0; hw <= w; ++hw) {
+        if (p.wall_kind[hw] != 1) continue;
+        v3 nrm = plane_n(hw), org = plane_o(hw);
+        std::vector<std::pair<double, int>> dist(n);
+        for (int i = 0; i < n; ++i) dist[i] = std::make_pair(std::abs(dot(nrm, r[i] - org)), i);
+        std::sort(dist.begin(), dist.end());
+        for (int idx = 0, conn = 0; conn < p.wall_limit[hw] && idx < n; ++idx) {
+          int res = dist[idx].second;
+          if (is_conn[res]) continue;
+          v3 proj = r[res] - dot(nrm, r[res] - org) * nrm;
+          hw_conns.push_back(hw_conn_t{hw, res, proj - org, 0.0});
+          is_conn[res] = 1;
+          ++conn;
+        }
+      }
+    }
+  }
+
+  /* state::reinit_wall_values, state.cpp:708-729 */
+  void reinit_walls() {
+    v3 center = (box_max + box_min) / 2.0, ext = box_max - box_min;
+    for (int side = 0; side < 6; ++side) {
+      int w = side_wall[side];
+      if (w < 0) continue;
+      v3 prev = plane_o(w), nrm = plane_n(w);
+      v3 d = side % 2 == 1 ? box_max - center : box_min - center;
+      v3 next = center + nrm * dot(d, nrm);
+      p.wall_origin[w][0] = next.x, p.wall_origin[w][1] = next.y, p.wall_origin[w][2] = next.z;
+      shift[w] = next - prev;
+    }
+    pbc.set(v3{bx.pbc[0] ? ext.x : 0.0, bx.pbc[1] ? ext.y : 0.0, bx.pbc[2] ? ext.z : 0.0});
+  }
+  /* state::adjust_wall_pos, state.cpp:702-706 */
+  void adjust_walls(v3 size_change, v3 translation) {
+    box_min += -size_change + translation;
+    box_max += size_change + translation;
+    reinit_walls();
+  }
+  void shear_walls(double sh) {
+    for (int side : {4, 5})
+      if (side_wall[side] >= 0) p.wall_normal[side_wall[side]][0] += sh;
+  }
+
+  /* The protocol phases of thread_main.cpp:144-330 up to the next step that
+   * advances (a step() that only changes the phase takes no time). */
+  void box_phase() {
+    const double dt_ = p.dt;
+    const double num_res = (double)n;
+    for (;;) {
+      switch (phase) {
+      case 1: { /* squeezing_step, :144-181 */
+        if (!bx.squeezing) {
+          since = t;
+          phase = 3;
+          continue;
+        }
+        v3 ext = box_max - box_min;
+        double cur_vol = ext.x * ext.y * ext.z;
+        double target_vol = num_res / bx.target_density;
+        if (cur_vol <= target_vol) {
+          until = std::min(t + bx.rest_period, bx.total_time);
+          phase = 2;
+          continue;
+        }
+        double vel;
+        if (cur_vol > 2.0 * target_vol) {
+          double accel_time = bx.accel_dist / bx.vel_above_2V;
+          double time_frac = std::min((t - since) / accel_time, 1.0);
+          vel = time_frac * bx.vel_above_2V;
+        } else {
+          double accel_time = bx.accel_dist / bx.target_vel;
+          double time_frac = std::min((t - since) / accel_time, 1.0);
+          vel = time_frac * bx.target_vel;
+        }
+        double sh = 0.5 * vel * dt_;
+        adjust_walls(v3{-sh, -sh, -sh}, v3{0, 0, 0});
+        return;
+      }
+      case 2: /* rest_after_squeezing_step, :183-191 */
+        if (t < until) return;
+        phase = 3;
+        continue;
+      case 3: { /* find_force_min_step, :193-222 */
+        if (!bx.force_min) {
+          phase = 9;
+          continue;
+        }
+        if (avg_z.has && std::abs(avg_z.avg) < 5e-2 + 5e-5 * num_res) {
+          until = t + bx.rest_period;
+          phase = 4;
+          continue;
+        }
+        double accel_time = bx.accel_dist / bx.target_vel;
+        double time_vel_frac = std::min((t - since) / accel_time, 1.0);
+        double fvf = (avg_z.has ? avg_z.avg : 0.0) / bx.force_for_max_vel;
+        fvf = std::copysign(std::min(std::abs(fvf), 1.0), fvf);
+        double vel = bx.target_vel * time_vel_frac * fvf;
+        double sh = 0.5 * vel * dt_;
+        adjust_walls(v3{0, 0, sh}, v3{0, 0, 0});
+        return;
+      }
+      case 4: /* rest_after_force_min_step, :224-243 */
+        if (t < until) return;
+        if (!bx.amp_relative) {
+          amplitude = bx.amp_abs;
+        } else {
+          shear = bx.osc_shear != 0;
+          v3 ext = box_max - box_min;
+          double base_val = shear ? ext.x : ext.z;
+          amplitude = bx.amp_rel * base_val;
+        }
+        since = t;
+        phase = 5;
+        continue;
+      case 5: { /* max_amplitude_step, :245-276 */
+        if (!bx.oscillations) {
+          phase = 9;
+          continue;
+        }
+        double accel_time = bx.accel_dist / bx.target_vel;
+        double time_frac = std::min((t - since) / accel_time, 1.0);
+        double vel = time_frac * bx.target_vel;
+        double sh = vel * dt_;
+        if (displacement > 0.5 * amplitude) {
+          until = t + bx.rest_period;
+          phase = 6;
+          continue;
+        }
+        displacement += sh;
+        if (shear)
+          shear_walls(sh);
+        else
+          adjust_walls(v3{0, 0, -sh}, v3{0, 0, 0});
+        return;
+      }
+      case 6: /* rest_after_max_amp_step, :278-293 */
+        if (t < until) return;
+        since = t;
+        omega = bx.angular_freq;
+        until = t + (M_PI * 2 * bx.num_of_cycles + M_PI_2) / omega;
+        phase = 7;
+        continue;
+      case 7: { /* oscillations_step, :295-320 */
+        if (t >= until) {
+          until = t + bx.rest_period;
+          phase = 8;
+          continue;
+        }
+        double vel = 0.5 * amplitude * omega * std::sin(omega * (t - since));
+        double sh = vel * dt_;
+        if (shear)
+          shear_walls(sh);
+        else
+          adjust_walls(v3{0, 0, sh}, v3{0, 0, 0});
+        return;
+      }
+      case 8: /* rest_after_oscillations_step, :322-330 */
+        if (t < until) return;
+        phase = 9;
+        continue;
+      default: /* NONE, FREEFORM */
+        return;
+      }
+    }
+  }
+
+  /* wall::log_forces, cg/src/cg/wall/log_forces.cpp:4-17, over the
+   * reference's wall pointer list: the six sides, then once more the -axis
+   * plane of every LJ axis (setup_walls pushes it twice, state.cpp:680-681) */
+  void log_forces() {
+    auto one = [&](int w) {
+      avg_F[w].add(t, wall_F[w]);
+      work[w] += dot(wall_F[w], shift[w]);
+    };
+    for (int side = 0; side < 6; ++side)
+      if (side_wall[side] >= 0) one(side_wall[side]);
+    for (int side = 0; side < 6; side += 2) {
+      int w = side_wall[side];
+      if (w >= 0 && p.wall_kind[w] == 2) one(w);
+    }
+    int wn = side_wall[4], wp = side_wall[5];
+    double z_force = dot(wall_F[wn], plane_n(wn)) + dot(wall_F[wp], plane_n(wp));
+    avg_z.add(t, z_force);
+  }
+
+  bool has_kind(int kind) const {
+    for (int w = 0; w < p.n_walls; ++w)
+      if (p.wall_kind[w] == kind) return true;
+    return false;
   }
 
   bool term_enabled(int term) const {
@@ -923,7 +1293,11 @@ struct ora_sys {
     case ORA_SND: return p.nat_dih_on && p.nat_dih_variant == 1;
     case ORA_VEL_AFM: return p.afm_on;
     case ORA_FORCE_AFM: return p.afm_on;
-    case ORA_SOLID_WALL: return p.n_walls > 0;
+    case ORA_SOLID_WALL: return has_kind(0);
+    case ORA_HW_FREE: return p.hw_on && has_kind(1);
+    case ORA_HW_CONN: return p.hw_on && has_kind(1);
+    case ORA_LJW_FREE: return p.ljw_on && has_kind(2);
+    case ORA_LJW_CONN: return p.ljw_on && has_kind(2);
     }
     return false;
   }
@@ -948,6 +1322,10 @@ struct ora_sys {
     case ORA_VEL_AFM: eval_vel_afm(Fo, Vo); break;
     case ORA_FORCE_AFM: eval_force_afm(Fo); break;
     case ORA_SOLID_WALL: eval_solid_walls(Fo); break;
+    case ORA_HW_FREE: eval_hw_free(Fo); break;
+    case ORA_HW_CONN: eval_hw_conn(Fo, Vo); break;
+    case ORA_LJW_FREE: eval_ljw_free(Fo, Vo); break;
+    case ORA_LJW_CONN: eval_ljw_conn(Fo, Vo); break;
     }
   }
 
@@ -956,6 +1334,7 @@ struct ora_sys {
   void eval_forces() {
     std::vector<double> Fb(3 * (size_t)n, 0.0);
     V = 0.0;
+    for (auto &f : wall_F) f = v3{0, 0, 0}; /* dynamics::reset, dynamics.cpp:11-19 */
     for (int k = 0; k < ORA_NUM_TERMS; ++k) {
       term_V[k] = 0.0;
       if (!term_enabled(k)) continue;
@@ -1183,6 +1562,8 @@ struct ora_sys {
     if (nl_invalid) fix_nl();
     eval_forces();
     if (p.qa_on) qa_finish();
+    if (p.ljw_on && has_kind(2)) ljw_process();
+    if (log_on) log_forces();
     if (p.lang_on) lang_step();
   }
 };
@@ -1494,6 +1875,9 @@ ora_sys *ora_create(const ora_params *p, const ora_model *m, int32_t seed,
   for (int k = 0; k < m->n_ftips; ++k)
     s->ftips.emplace_back(m->ftip_res[k], mk(m->ftip_force[3 * k], m->ftip_force[3 * k + 1],
                                               m->ftip_force[3 * k + 2]));
+  /* walls: harmonic connections from the starting positions */
+  s->is_conn.assign(n, 0);
+  s->setup_harmonic();
   return s;
 }
 
@@ -1501,6 +1885,7 @@ void ora_destroy(ora_sys *s) { delete s; }
 
 void ora_step(ora_sys *s, int32_t nsteps, double *V_trace) {
   for (int k = 0; k < nsteps; ++k) {
+    if (s->phase) s->box_phase();
     s->step();
     if (V_trace) V_trace[k] = s->V;
   }
@@ -1518,6 +1903,7 @@ void ora_eval_forces(ora_sys *s) { s->eval_forces(); }
 void ora_eval_term(ora_sys *s, int32_t term, double *F, double *V) {
   for (int i = 0; i < 3 * s->n; ++i) F[i] = 0.0;
   *V = 0.0;
+  for (auto &f : s->wall_F) f = v3{0, 0, 0};
   if (term < 0 || term >= ORA_NUM_TERMS) return;
   s->run_term(term, F, *V);
 }
@@ -1724,6 +2110,102 @@ void ora_set_tips(ora_sys *s, int32_t n_tips, const int32_t *tip_res, const doub
                                           ftip_force[3 * k + 2]));
 }
 int64_t ora_saw_draws(const ora_sys *s) { return s->saw_draws; }
+
+/* state::setup_walls' box part and reinit_wall_values (state.cpp:570-730) */
+void ora_box_setup(ora_sys *s, const ora_box *b) {
+  s->bx = *b;
+  s->box_set = true;
+  s->box_min = mk(b->box_min[0], b->box_min[1], b->box_min[2]);
+  s->box_max = mk(b->box_max[0], b->box_max[1], b->box_max[2]);
+  for (int side = 0; side < 6; ++side) s->side_wall[side] = -1;
+  for (int w = 0; w < s->p.n_walls; ++w)
+    if (s->p.wall_side[w] >= 0 && s->p.wall_side[w] < 6) s->side_wall[s->p.wall_side[w]] = w;
+  for (int w = 0; w < 6; ++w) {
+    s->avg_F[w] = {};
+    s->avg_F[w].window = b->avg_forces_over;
+    s->work[w] = 0.0;
+  }
+  s->avg_z = {};
+  s->avg_z.window = b->avg_forces_over;
+  s->log_on = s->side_wall[4] >= 0 && s->side_wall[5] >= 0;
+  s->phase = 0;
+  s->reinit_walls();
+}
+/* equil_step's hand-over (thread_main.cpp:130-135) */
+void ora_box_start(ora_sys *s) {
+  s->until = s->bx.total_time;
+  s->since = s->t;
+  s->phase = 1;
+}
+void ora_box_state(const ora_sys *s, double *out) {
+  for (int k = 0; k < 120; ++k) out[k] = 0.0;
+  out[0] = s->phase;
+  out[1] = s->since;
+  out[2] = s->until;
+  out[3] = s->box_min.x, out[4] = s->box_min.y, out[5] = s->box_min.z;
+  out[6] = s->box_max.x, out[7] = s->box_max.y, out[8] = s->box_max.z;
+  out[9] = s->amplitude;
+  out[10] = s->omega;
+  out[11] = s->displacement;
+  out[12] = s->shear ? 1.0 : 0.0;
+  out[13] = s->avg_z.has ? 1.0 : 0.0;
+  out[14] = s->avg_z.has ? s->avg_z.avg : 0.0;
+  for (int w = 0; w < 6; ++w) {
+    for (int c = 0; c < 3; ++c) {
+      out[15 + 3 * w + c] = s->p.wall_origin[w][c];
+      out[33 + 3 * w + c] = s->p.wall_normal[w][c];
+    }
+    const v3 f = s->wall_F[w], sh = s->shift[w], a = s->avg_F[w].avg;
+    out[51 + 3 * w] = f.x, out[52 + 3 * w] = f.y, out[53 + 3 * w] = f.z;
+    out[69 + 3 * w] = sh.x, out[70 + 3 * w] = sh.y, out[71 + 3 * w] = sh.z;
+    out[87 + w] = s->work[w];
+    out[93 + w] = s->avg_F[w].has ? 1.0 : 0.0;
+    out[99 + 3 * w] = a.x, out[100 + 3 * w] = a.y, out[101 + 3 * w] = a.z;
+  }
+  out[117] = s->pbc.cell.x, out[118] = s->pbc.cell.y, out[119] = s->pbc.cell.z;
+}
+void ora_ljw_process(ora_sys *s) { s->ljw_process(); }
+int32_t ora_ljw_slots(const ora_sys *s, int32_t cap, int32_t *res, int32_t *wall, double *off,
+                      double *sat, uint8_t *vac) {
+  int k = 0;
+  for (auto const &c : s->ljw_conns) {
+    if (k < cap) {
+      res[k] = c.res, wall[k] = c.wall, sat[k] = c.sat, vac[k] = c.vacant ? 1 : 0;
+      off[3 * k] = c.off.x, off[3 * k + 1] = c.off.y, off[3 * k + 2] = c.off.z;
+    }
+    ++k;
+  }
+  return k;
+}
+void ora_ljw_pending(const ora_sys *s, int32_t cap, int32_t *ncand, int32_t *cand_res,
+                     int32_t *cand_wall, double *cand_d, int32_t *nrem, int32_t *rem,
+                     uint8_t *is_conn, int32_t *num_conns) {
+  *ncand = (int)s->ljw_cands.size();
+  for (int a = 0; a < *ncand && a < cap; ++a)
+    cand_res[a] = s->ljw_cands[a].res, cand_wall[a] = s->ljw_cands[a].wall,
+    cand_d[a] = s->ljw_cands[a].dist;
+  *nrem = (int)s->ljw_removed.size();
+  for (int a = 0; a < *nrem && a < cap; ++a) rem[a] = s->ljw_removed[a];
+  for (int i = 0; i < s->n; ++i) is_conn[i] = s->is_conn[i];
+  for (int w = 0; w < 6; ++w) num_conns[w] = s->ljw_num_conns[w];
+}
+int32_t ora_wall_conns(const ora_sys *s, int32_t kind, int32_t cap, int32_t *res,
+                       int32_t *wall, double *off, double *sat) {
+  int k = 0;
+  auto put = [&](int r_, int w_, v3 o, double x) {
+    if (k < cap) {
+      res[k] = r_, wall[k] = w_, sat[k] = x;
+      off[3 * k] = o.x, off[3 * k + 1] = o.y, off[3 * k + 2] = o.z;
+    }
+    ++k;
+  };
+  if (kind == 0)
+    for (auto const &c : s->hw_conns) put(c.res, c.wall, c.off, c.nat);
+  else
+    for (auto const &c : s->ljw_conns)
+      if (!c.vacant) put(c.res, c.wall, c.off, c.sat);
+  return k;
+}
 
 struct ora_rng {
   ran2_t g;

@@ -20,7 +20,10 @@
 extern "C" {
 #endif
 
-/* Term ids, in the reference's evaluation order (thread_setup.cpp:27-48). */
+/* Term ids, in the reference's evaluation order (thread_setup.cpp:27-48);
+ * the wall terms: solid, harmonic free and connected (wall::harmonic::
+ * eval_free, eval_connected), LJ free and connected (wall::lj::sift_free,
+ * eval_connected). */
 enum ora_term {
   ORA_CHIR = 0,
   ORA_TETHER,
@@ -40,6 +43,10 @@ enum ora_term {
   ORA_VEL_AFM,
   ORA_FORCE_AFM,
   ORA_SOLID_WALL,
+  ORA_HW_FREE,
+  ORA_HW_CONN,
+  ORA_LJW_FREE,
+  ORA_LJW_CONN,
   ORA_NUM_TERMS
 };
 
@@ -110,11 +117,34 @@ typedef struct ora_params {
   double charge[21];
   int32_t ptype[21];       /* 0 none, 1 polar, 2 hydrophobic */
   int32_t base_sync[21][4]; /* back, side_all, side_polar, side_hydrophobic */
-  /* solid walls: planes (origin, unit normal), depth, min dist */
+  /* walls: planes (origin, normal), solid depth, threshold */
   int32_t n_walls;
   double wall_origin[6][3], wall_normal[6][3];
   double wall_depth, wall_min_dist;
+  /* per plane: kind (0 solid, 1 harmonic, 2 lj), box side (0..5 = -x +x -y
+   * +y -z +z, -1 none), connection limit */
+  int32_t wall_kind[6], wall_side[6], wall_limit[6];
+  int32_t hw_on, ljw_on; /* harmonic_walls_enabled, lj_walls_enabled */
+  double wall_hh1, wall_lj_depth, ljw_breaking_dist, ljw_saturation_diff;
 } ora_params;
+
+/* sbox parameters of the box protocol (the same fields as pcg_box). */
+typedef struct ora_box {
+  double box_min[3], box_max[3];
+  int32_t pbc[3];
+  double total_time;
+  double rest_period, target_vel, accel_dist, avg_forces_over;
+  int32_t squeezing;
+  double target_density, vel_above_2V;
+  int32_t force_min;
+  double force_for_max_vel;
+  int32_t oscillations;
+  int32_t osc_shear;
+  int32_t num_of_cycles;
+  double angular_freq;
+  int32_t amp_relative;
+  double amp_abs, amp_rel;
+} ora_box;
 
 /* Input model, already compiled to residue indices (state::compile_model,
  * cg/src/cg/simul/state.cpp:144-249). Angles and dihedrals carry an optional
@@ -223,6 +253,28 @@ void ora_set_tips(ora_sys *s, int32_t n_tips, const int32_t *tip_res, const doub
                   const double *tip_vel, int32_t n_ftips, const int32_t *ftip_res,
                   const double *ftip_force);
 int64_t ora_saw_draws(const ora_sys *s); /* ran2 draws the SAW consumed */
+
+/* The simulation box and its protocol (state::setup_walls and
+ * reinit_wall_values, state.cpp:570-730; thread_main.cpp:144-330). After
+ * ora_box_start every ora_step first runs the phase logic, then advances.
+ * ora_box_state fills the layout of pcg_box_state (120 doubles). */
+void ora_box_setup(ora_sys *s, const ora_box *b);
+void ora_box_start(ora_sys *s);
+void ora_box_state(const ora_sys *s, double *out);
+/* wall::lj::process_candidates over the candidates and releases gathered
+ * since the last call (then cleared) */
+void ora_ljw_process(ora_sys *s);
+/* every LJ connection slot (vacant ones too), and the LJ bookkeeping:
+ * pending candidates and releases, the connected flags [n], the connection
+ * counts by plane [6] */
+int32_t ora_ljw_slots(const ora_sys *s, int32_t cap, int32_t *res, int32_t *wall, double *off,
+                      double *sat, uint8_t *vac);
+void ora_ljw_pending(const ora_sys *s, int32_t cap, int32_t *ncand, int32_t *cand_res,
+                     int32_t *cand_wall, double *cand_d, int32_t *nrem, int32_t *rem,
+                     uint8_t *is_conn, int32_t *num_conns);
+/* wall connections, kind 0 harmonic, 1 lj (live ones in slot order) */
+int32_t ora_wall_conns(const ora_sys *s, int32_t kind, int32_t cap, int32_t *res,
+                       int32_t *wall, double *off, double *sat);
 
 /* The ran2 generator by itself (cg/include/cg/utils/random/nr.h). */
 typedef struct ora_rng ora_rng;

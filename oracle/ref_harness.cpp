@@ -38,6 +38,12 @@
 #include <cg/qa/process_contacts.h>
 #include <cg/qa/update_free_pairs.h>
 #include <cg/tether/eval_forces.h>
+#include <cg/wall/harmonic/eval_connected.h>
+#include <cg/wall/harmonic/eval_free.h>
+#include <cg/wall/lj/eval_connected.h>
+#include <cg/wall/lj/process_candidates.h>
+#include <cg/wall/lj/sift_free.h>
+#include <cg/wall/log_forces.h>
 #include <cg/wall/solid/eval_forces.h>
 
 #include "oracle.h"
@@ -67,6 +73,30 @@ vect::vector<amino_acid> to_aa(int n, const uint8_t *atype) {
   vect::vector<amino_acid> v(n);
   for (int i = 0; i < n; ++i) v[i] = amino_acid((aa_code)atype[i]);
   return v;
+}
+cg::plane<real> plane_of(const ora_params *p, int w) {
+  vec3r org(p->wall_origin[w][0], p->wall_origin[w][1], p->wall_origin[w][2]);
+  vec3r nrm(p->wall_normal[w][0], p->wall_normal[w][1], p->wall_normal[w][2]);
+  return cg::plane<real>(org, nrm);
+}
+// the planes of one kind in plane order (the reference keeps one vector per
+// kind, state.h:174-183): plane index of each, and each plane's kind index
+std::vector<int> planes_of_kind(const ora_params *p, int kind, int *kind_idx) {
+  std::vector<int> v;
+  for (int w = 0; w < p->n_walls; ++w) {
+    kind_idx[w] = -1;
+    if (p->wall_kind[w] == kind) {
+      kind_idx[w] = (int)v.size();
+      v.push_back(w);
+    }
+  }
+  return v;
+}
+void put_wall_F(vect::vector<vec3r> const &wf, std::vector<int> const &planes, double *out) {
+  for (int k = 0; k < (int)planes.size(); ++k) {
+    vec3r f = wf[k];
+    out[3 * planes[k]] = f.x(), out[3 * planes[k] + 1] = f.y(), out[3 * planes[k] + 2] = f.z();
+  }
 }
 template <typename K> double run(K &k) {
   k.for_slice(0, k.total_size());
@@ -445,9 +475,8 @@ double ref_solid_wall(int n, const double *r, const ora_params *p, double *F) {
   vect::vector<vec3r> Fv(n), wall_F(p->n_walls);
   vect::vector<wall::solid::wall> walls;
   for (int w = 0; w < p->n_walls; ++w) {
-    vec3r org(p->wall_origin[w][0], p->wall_origin[w][1], p->wall_origin[w][2]);
-    vec3r nrm(p->wall_normal[w][0], p->wall_normal[w][1], p->wall_normal[w][2]);
-    walls.emplace_back(cg::plane<real>(org, nrm), vect::iterator<vec3r>(wall_F[w]), (real)1.0);
+    if (p->wall_kind[w] != 0) continue;
+    walls.emplace_back(plane_of(p, w), vect::iterator<vec3r>(wall_F[w]), (real)1.0);
   }
   wall::solid::eval_forces e;
   e.min_dist = p->wall_min_dist;
@@ -766,6 +795,258 @@ double ref_qa_round(int n, const double *r, const double *cell,
   }
   from_vec(Fv, F);
   return V;
+}
+
+
+// wall::harmonic::eval_free over the harmonic planes of the parameters;
+// wall_F [6][3] by plane index
+double ref_hw_free(int n, const double *r, const ora_params *p, const uint8_t *is_conn,
+                   double *F, double *wall_F) {
+  auto R = to_vec(n, r);
+  int kidx[6];
+  auto planes = planes_of_kind(p, 1, kidx);
+  vect::vector<vec3r> Fv(n), wf((int)planes.size());
+  vect::vector<bool> conn(n);
+  for (int i = 0; i < n; ++i) conn[i] = is_conn[i] != 0;
+  vect::vector<wall::harmonic::wall> walls;
+  for (int k = 0; k < (int)planes.size(); ++k)
+    walls.emplace_back(plane_of(p, planes[k]), vect::iterator<vec3r>(wf[k]), (real)1.0,
+                       p->wall_limit[planes[k]]);
+  real V = 0.0;
+  wall::harmonic::eval_free e;
+  e.depth = p->wall_depth;
+  e.min_dist = p->wall_min_dist;
+  e.is_connected = conn;
+  e.walls = walls;
+  e.r = R;
+  e.F = Fv;
+  e.wall_F = wf;
+  e.V = &V;
+  run(e);
+  from_vec(Fv, F);
+  put_wall_F(wf, planes, wall_F);
+  return V;
+}
+
+// wall::harmonic::eval_connected over k connections (plane index per connection)
+double ref_hw_conn(int n, const double *r, const ora_params *p, int k, const int *res,
+                   const int *wall_plane, const double *off, const double *nat, double *F,
+                   double *wall_F) {
+  auto R = to_vec(n, r);
+  int kidx[6];
+  auto planes = planes_of_kind(p, 1, kidx);
+  vect::vector<vec3r> Fv(n), wf((int)planes.size());
+  vect::vector<wall::harmonic::wall> walls;
+  for (int a = 0; a < (int)planes.size(); ++a)
+    walls.emplace_back(plane_of(p, planes[a]), vect::iterator<vec3r>(wf[a]), (real)1.0,
+                       p->wall_limit[planes[a]]);
+  vect::vector<wall::harmonic::connection> conns;
+  for (int a = 0; a < k; ++a)
+    conns.emplace_back(kidx[wall_plane[a]], res[a],
+                       vec3r(off[3 * a], off[3 * a + 1], off[3 * a + 2]), nat[a]);
+  real V = 0.0;
+  wall::harmonic::eval_connected e;
+  e.HH1 = p->wall_hh1;
+  e.walls = walls;
+  e.conns = conns;
+  e.r = R;
+  e.F = Fv;
+  e.wall_F = wf;
+  e.V = &V;
+  run(e);
+  from_vec(Fv, F);
+  put_wall_F(wf, planes, wall_F);
+  return V;
+}
+
+// wall::lj::sift_free: forces, wall_F and the candidates it emits
+// (residue, plane index, distance; at most cap)
+double ref_ljw_sift(int n, const double *r, const ora_params *p, const uint8_t *is_conn,
+                    const int *num_conns, double *F, double *wall_F, int cap, int *cand_res,
+                    int *cand_plane, double *cand_d, int *ncand) {
+  auto R = to_vec(n, r);
+  int kidx[6];
+  auto planes = planes_of_kind(p, 2, kidx);
+  vect::vector<vec3r> Fv(n), wf((int)planes.size());
+  vect::vector<bool> conn(n);
+  for (int i = 0; i < n; ++i) conn[i] = is_conn[i] != 0;
+  vect::vector<wall::lj::wall> walls;
+  for (int a = 0; a < (int)planes.size(); ++a) {
+    auto &w = walls.emplace_back(plane_of(p, planes[a]), vect::iterator<vec3r>(wf[a]),
+                                 (real)1.0, p->wall_limit[planes[a]]);
+    w.num_conns = num_conns[planes[a]];
+  }
+  vect::vector<wall::lj::candidate> cands;
+  vect::vector<int> removed;
+  real V = 0.0;
+  wall::lj::sift_free e;
+  e.min_dist = p->wall_min_dist;
+  e.force.depth = p->wall_lj_depth;
+  e.force.r_min = p->wall_min_dist;
+  e.walls = walls;
+  e.is_connected = conn;
+  e.candidates = &cands;
+  e.removed = &removed;
+  e.r = R;
+  e.F = Fv;
+  e.wall_F = wf;
+  e.V = &V;
+  run(e);
+  from_vec(Fv, F);
+  put_wall_F(wf, planes, wall_F);
+  *ncand = cands.size();
+  for (int a = 0; a < cands.size() && a < cap; ++a) {
+    auto c = cands[a];
+    cand_res[a] = c.res_idx();
+    cand_plane[a] = planes[c.wall_idx()];
+    cand_d[a] = c.dist();
+  }
+  return V;
+}
+
+// wall::lj::eval_connected over k connection slots (vacant ones included):
+// forces, wall_F, the updated saturations (in place) and the released slots
+double ref_ljw_conn(int n, const double *r, const ora_params *p, int k, const int *res,
+                    const int *wall_plane, const double *off, double *sat, const uint8_t *vac,
+                    double *F, double *wall_F, int cap, int *removed_out, int *nremoved) {
+  auto R = to_vec(n, r);
+  int kidx[6];
+  auto planes = planes_of_kind(p, 2, kidx);
+  vect::vector<vec3r> Fv(n), wf((int)planes.size());
+  vect::vector<bool> conn(n, false);
+  vect::vector<wall::lj::wall> walls;
+  for (int a = 0; a < (int)planes.size(); ++a)
+    walls.emplace_back(plane_of(p, planes[a]), vect::iterator<vec3r>(wf[a]), (real)1.0,
+                       p->wall_limit[planes[a]]);
+  vect::set<wall::lj::connection> conns;
+  for (int a = 0; a < k; ++a) {
+    conns.emplace(res[a], kidx[wall_plane[a]], vec3r(off[3 * a], off[3 * a + 1], off[3 * a + 2]),
+                  sat[a]);
+    if (vac[a]) conns.remove(a);
+  }
+  vect::vector<int> removed;
+  real V = 0.0;
+  wall::lj::eval_connected e;
+  e.min_dist = p->wall_min_dist;
+  e.force.depth() = p->wall_lj_depth;
+  e.force.r_min() = p->wall_min_dist;
+  e.breaking_dist = p->ljw_breaking_dist;
+  e.saturation_diff = p->ljw_saturation_diff;
+  e.walls = walls;
+  e.conns = &conns;
+  e.r = R;
+  e.F = Fv;
+  e.wall_F = wf;
+  e.V = &V;
+  e.is_connected = conn;
+  e.removed = &removed;
+  run(e);
+  from_vec(Fv, F);
+  put_wall_F(wf, planes, wall_F);
+  for (int a = 0; a < k; ++a) sat[a] = conns[a].item().saturation();
+  *nremoved = removed.size();
+  for (int a = 0; a < removed.size() && a < cap; ++a) removed_out[a] = removed[a];
+  return V;
+}
+
+// wall::lj::process_candidates on the given slots, releases and candidates;
+// the slots after it go to the out arrays (at most cap), is_conn and
+// num_conns (by plane) are updated in place
+void ref_ljw_process(int n, const double *r, const ora_params *p, int k, const int *res,
+                     const int *wall_plane, const double *off, const double *sat,
+                     const uint8_t *vac, int nrem, const int *rem, int ncand,
+                     const int *cand_res, const int *cand_plane, const double *cand_d,
+                     uint8_t *is_conn, int *num_conns, int cap, int *nslots, int *res_out,
+                     int *plane_out, double *off_out, double *sat_out, uint8_t *vac_out) {
+  auto R = to_vec(n, r);
+  int kidx[6];
+  auto planes = planes_of_kind(p, 2, kidx);
+  vect::vector<vec3r> wf((int)planes.size());
+  vect::vector<bool> conn(n);
+  for (int i = 0; i < n; ++i) conn[i] = is_conn[i] != 0;
+  vect::vector<wall::lj::wall> walls;
+  for (int a = 0; a < (int)planes.size(); ++a) {
+    auto &w = walls.emplace_back(plane_of(p, planes[a]), vect::iterator<vec3r>(wf[a]),
+                                 (real)1.0, p->wall_limit[planes[a]]);
+    w.num_conns = num_conns[planes[a]];
+  }
+  vect::set<wall::lj::connection> conns;
+  for (int a = 0; a < k; ++a) {
+    conns.emplace(res[a], kidx[wall_plane[a]], vec3r(off[3 * a], off[3 * a + 1], off[3 * a + 2]),
+                  sat[a]);
+    if (vac[a]) conns.remove(a);
+  }
+  vect::vector<int> removed;
+  for (int a = 0; a < nrem; ++a) removed.push_back(rem[a]);
+  vect::vector<wall::lj::candidate> cands;
+  for (int a = 0; a < ncand; ++a) cands.emplace_back(cand_res[a], kidx[cand_plane[a]], cand_d[a]);
+  wall::lj::process_candidates e;
+  e.r = R;
+  e.walls = walls;
+  e.conns = &conns;
+  e.removed = &removed;
+  e.candidates = &cands;
+  e.is_connected = conn;
+  e();
+  for (int i = 0; i < n; ++i) is_conn[i] = conn[i] ? 1 : 0;
+  for (int a = 0; a < (int)planes.size(); ++a) num_conns[planes[a]] = walls[a].num_conns;
+  *nslots = conns.size();
+  for (int a = 0; a < conns.size() && a < cap; ++a) {
+    auto c = conns[a].item();
+    res_out[a] = c.res_idx();
+    plane_out[a] = planes[c.wall_idx()];
+    vec3r o = c.bead_offset();
+    off_out[3 * a] = o.x(), off_out[3 * a + 1] = o.y(), off_out[3 * a + 2] = o.z();
+    sat_out[a] = c.saturation();
+    vac_out[a] = conns[a].is_vacant() ? 1 : 0;
+  }
+}
+
+// wall::log_forces over nsteps samples: walls on the sides present[6] (the
+// plane list of the reference's state::walls, then the extra sides appended
+// by setup_walls for LJ axes), with the given normals, shifts and per-step
+// reaction forces F[nsteps][6][3]; returns avg_F [6][3], its has-flags, work
+// [6] and the z-force average and flag
+void ref_log_forces(int nsteps, const double *t, const double *F, const double *normal,
+                    const double *shift, const int *present, int nextra, const int *extra,
+                    double window, double *avg_F, double *avg_has, double *work,
+                    double *avg_z, double *avg_z_has) {
+  vect::vector<vec3r> wf(6);
+  std::vector<wall::gen_wall> walls;
+  walls.reserve(6);
+  for (int w = 0; w < 6; ++w) {
+    vec3r nrm(normal[3 * w], normal[3 * w + 1], normal[3 * w + 2]);
+    walls.emplace_back(cg::plane<real>(vec3r::Zero(), nrm), vect::iterator<vec3r>(wf[w]),
+                       (real)window);
+    walls[w].shift = vec3r(shift[3 * w], shift[3 * w + 1], shift[3 * w + 2]);
+  }
+  vect::vector<wall::gen_wall *> ptrs(6, nullptr);
+  for (int w = 0; w < 6; ++w)
+    if (present[w]) ptrs[w] = &walls[w];
+  for (int a = 0; a < nextra; ++a) ptrs.push_back(&walls[extra[a]]);
+  moving_avg<real, real> avg_zf((real)window);
+  real tt = 0.0;
+  wall::log_forces e;
+  e.t = &tt;
+  e.walls = ptrs;
+  e.neg_z = &walls[4];
+  e.pos_z = &walls[5];
+  e.avg_z_force = &avg_zf;
+  for (int k = 0; k < nsteps; ++k) {
+    tt = t[k];
+    for (int w = 0; w < 6; ++w)
+      wf[w] = vec3r(F[18 * k + 3 * w], F[18 * k + 3 * w + 1], F[18 * k + 3 * w + 2]);
+    e();
+  }
+  for (int w = 0; w < 6; ++w) {
+    auto const &a = walls[w].avg_F;
+    avg_has[w] = a->has_value() ? 1.0 : 0.0;
+    vec3r v = a->has_value() ? a->value() : vec3r::Zero();
+    avg_F[3 * w] = v.x(), avg_F[3 * w + 1] = v.y(), avg_F[3 * w + 2] = v.z();
+    work[w] = walls[w].work;
+  }
+  *avg_z_has = avg_zf->has_value() ? 1.0 : 0.0;
+  *avg_z = avg_zf->has_value() ? avg_zf->value() : 0.0;
 }
 
 } // extern "C"

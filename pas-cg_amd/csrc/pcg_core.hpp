@@ -94,12 +94,52 @@ struct SysState {
   // required distance (the others cannot become candidates before the next
   // rebuild) and counts the rest in qf_n
   int qf_store;
+  int n_hw, n_lj;             // harmonic wall connections, LJ wall connection slots
   int64_t step;
   double t, true_t, orig_pad;
   double orig_cell[3];
   double V[NT];
   double Vtot;
   unsigned long long max_disp_bits; // FP64 max displacement since rebuild (as bits)
+};
+
+// Moving average of cg/include/cg/types/avg.h:5-38 (moving_avg<T, real>).
+struct MAvg {
+  double sum[3], avg[3], init_time, window;
+  int num, has;
+};
+
+// Walls and the simulation box of a plain context (one system): the
+// reaction force of every plane in the last evaluation, the wall
+// connections, the LJ candidates and releases of the current step, and the
+// state of the box protocol (thread_main.cpp:144-330). The planes themselves
+// live in the parameters (wall_origin, wall_normal), which the box kernel
+// moves between steps.
+struct WallDev {
+  double wall_F[6][3];
+  // harmonic connections, fixed at creation (state.cpp:653-670)
+  const int *hw_res, *hw_wall;
+  const double *hw_off[3], *hw_nat;
+  // LJ connection slots in creation order (a vect::set: releases leave
+  // vacant slots, which the processing compacts in order)
+  int *lj_res, *lj_wall;
+  double *lj_off[3], *lj_sat;
+  uint8_t *lj_gone;           // released by eval_connected this step
+  int lj_cap;
+  int *cand_res, *cand_wall;  // candidates of sift_free this step
+  double *cand_d;
+  int n_cand, cand_cap, n_gone;
+  int num_conns[6];           // LJ connections per plane
+  uint8_t *is_conn;           // state::is_connected_to_wall [n]
+  int overflow;               // a candidate or slot beyond capacity
+  // box protocol
+  int box_on, log_on, phase, shear;
+  int side_wall[6];           // plane on each box side, or -1
+  double box_min[3], box_max[3];
+  double since, until, amplitude, omega, displacement;
+  double shift[6][3], work[6];
+  MAvg avg_F[6], avg_z;
+  pcg_box bx;
 };
 
 // Force-field parameters on the device (a copy of pcg_params plus derived
@@ -112,6 +152,8 @@ struct DevParams {
   double qa_max_req, qa_factor, qa_sat_speed;
   int pauli_on;              // pauli.enabled && !qa.enabled (state.cpp:473)
   int nl_required;
+  // wall terms on: a solid plane; harmonic planes with hw_on; LJ planes with ljw_on
+  int solid_on, hw_terms, ljw_terms;
 };
 
 // Device code reads the parameters through the constant address space: they
@@ -204,8 +246,8 @@ struct DevBatch {
   int *cand_perm;
   int *rem;
   int *row_cnt;               // NL rebuild scratch: per-residue row counts / offsets
-  int *row_tally;             // the persistent kernel's rebuild: [7][ntot + 1] per-row list tallies
   int *blk_cnt;               // partition scratch: per-chunk counts
+  WallDev *wall;              // plain context: walls and the box protocol (else null)
 };
 
 // ------------------------------------------------------------- helpers ----

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -306,6 +307,12 @@ void fill_dev_params(const pcg_params &p, DevParams &d) {
   d.qa_sat_speed = 1.0 / p.qa_phase_dur;
   d.pauli_on = p.pauli_on && !p.qa_on;
   d.nl_required = d.pauli_on || p.dh_on || p.qa_on || p.pid_on;
+  int kinds[3] = {0, 0, 0};
+  for (int w = 0; w < p.n_walls && w < 6; ++w)
+    if (p.wall_kind[w] >= 0 && p.wall_kind[w] < 3) kinds[p.wall_kind[w]] = 1;
+  d.solid_on = kinds[0];
+  d.hw_terms = p.hw_on && kinds[1];
+  d.ljw_terms = p.ljw_on && kinds[2];
 }
 
 double const2_of(const pcg_params &p, double T) {
@@ -365,6 +372,12 @@ struct DeviceBatch {
   std::vector<double> noise_h;
   double *noise_pinned = nullptr;
   bool own_stream = true;
+  // plain context: walls and the box protocol (device copy dW; hW holds its
+  // array pointers); box_set once pcg_box_setup ran (the step then runs the
+  // box kernels)
+  WallDev *dW = nullptr;
+  WallDev hW{};
+  bool box_set = false;
 
   ~DeviceBatch() {
     if (arena.base && !arena.borrowed) (void)hipFree(arena.base);
@@ -379,10 +392,28 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
                const int32_t *seeds, int prec, uint64_t philox_seed, int device,
                std::unique_ptr<DeviceBatch> &out, hipStream_t use_stream = nullptr,
                size_t *dry_run_bytes = nullptr, int64_t ran2_skip = 0,
-               void *pool_block = nullptr, size_t pool_bytes = 0) {
+               void *pool_block = nullptr, size_t pool_bytes = 0, bool plain = false) {
   if (nsys <= 0 || !models) return fail(PCG_E_ARG, "no systems");
   if (prec != PCG_MIXED && prec != PCG_FP64) return fail(PCG_E_ARG, "unknown precision");
   if (p.n_walls < 0 || p.n_walls > 6) return fail(PCG_E_ARG, "n_walls must be in [0, 6]");
+  {
+    bool side_used[6] = {false, false, false, false, false, false};
+    for (int w = 0; w < p.n_walls; ++w) {
+      if (p.wall_kind[w] < 0 || p.wall_kind[w] > 2)
+        return fail(PCG_E_ARG, "wall_kind must be 0 (solid), 1 (harmonic) or 2 (LJ)");
+      if (p.wall_side[w] < -1 || p.wall_side[w] > 5)
+        return fail(PCG_E_ARG, "wall_side must be -1 or a box side 0..5");
+      if (p.wall_side[w] >= 0) {
+        if (side_used[p.wall_side[w]]) return fail(PCG_E_ARG, "two walls on one box side");
+        side_used[p.wall_side[w]] = true;
+      }
+      if (p.wall_kind[w] != 0 && p.wall_limit[w] < 0)
+        return fail(PCG_E_ARG, "wall_limit must not be negative");
+      if (!plain && p.wall_kind[w] != 0)
+        return fail(PCG_E_ARG, "harmonic and LJ walls run in the plain context (pcg_create) only");
+    }
+    if (plain && nsys != 1) return fail(PCG_E_ARG, "a plain context holds one system");
+  }
   auto B = std::make_unique<DeviceBatch>();
   B->prec = prec;
   B->device = device;
@@ -474,7 +505,6 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&local, ntot);
   A.add(&D.sync, 4 * ntot);
   A.add(&D.row_cnt, ntot + 1);
-  A.add(&D.row_tally, 7 * (ntot + 1));
   int *teth_i[2], *lrep_i[2], *chir_i[4], *nang_i[3], *hang_i[3], *ndih_i[4], *hdih_i[4], *nc_i[2];
   double *teth_nat, *chir_f0, *chir_f1, *nang_th, *ndih_ph, *nc_nat;
   uint8_t *hang_t, *hdih_t, *nc_type;
@@ -576,6 +606,65 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.qc_vac, qc);
   A.add(&D.rem, qc);
   A.add(&D.blk_cnt, (nl / BLK + 1) * 5);
+  // walls of a plain context: harmonic connections (state::setup_walls,
+  // state.cpp:653-670, from the starting positions), LJ slots and candidates
+  struct HwConn {
+    int wall, res;
+    double off[3];
+  };
+  std::vector<HwConn> hw;
+  std::vector<uint8_t> conn;
+  WallDev &W = B->hW;
+  std::memset(&W, 0, sizeof(W));
+  if (plain) {
+    const HostSys &h = B->hs[0];
+    const int n = h.n;
+    conn.assign(n, 0);
+    auto plane_dist = [&](int w, int i) {
+      const double *o = p.wall_origin[w], *nv = p.wall_normal[w], *r = &h.pos[3 * i];
+      return nv[0] * (r[0] - o[0]) + nv[1] * (r[1] - o[1]) + nv[2] * (r[2] - o[2]);
+    };
+    for (int w = 0; w < p.n_walls; ++w) {
+      if (p.wall_kind[w] != 1) continue;
+      if (p.wall_side[w] >= 0 && p.wall_side[w] % 2 == 0) continue;  // the axis' second plane ends it
+      for (int hw_i = 0; hw_i <= w; ++hw_i) {
+        if (p.wall_kind[hw_i] != 1) continue;
+        std::vector<std::pair<double, int>> dist(n);
+        for (int i = 0; i < n; ++i) dist[i] = std::make_pair(std::abs(plane_dist(hw_i, i)), i);
+        std::sort(dist.begin(), dist.end());
+        for (int idx = 0, made = 0; made < p.wall_limit[hw_i] && idx < n; ++idx) {
+          const int res = dist[idx].second;
+          if (conn[res]) continue;
+          const double sd = plane_dist(hw_i, res), *o = p.wall_origin[hw_i], *nv = p.wall_normal[hw_i];
+          HwConn c{hw_i, res, {0, 0, 0}};
+          for (int k = 0; k < 3; ++k) c.off[k] = (h.pos[3 * res + k] - sd * nv[k]) - o[k];
+          hw.push_back(c);
+          conn[res] = 1;
+          ++made;
+        }
+      }
+    }
+    long long lj_lim = 0;
+    int lj_planes = 0;
+    for (int w = 0; w < p.n_walls; ++w)
+      if (p.wall_kind[w] == 2) lj_lim += p.wall_limit[w], ++lj_planes;
+    W.lj_cap = (int)std::min<long long>(lj_lim, 16LL * n) + 1;
+    W.cand_cap = std::max(1, lj_planes * n);
+    A.add(&B->dW, 1);
+    A.add((int **)&W.hw_res, std::max<size_t>(hw.size(), 1));
+    A.add((int **)&W.hw_wall, std::max<size_t>(hw.size(), 1));
+    for (int k = 0; k < 3; ++k) A.add((double **)&W.hw_off[k], std::max<size_t>(hw.size(), 1));
+    A.add((double **)&W.hw_nat, std::max<size_t>(hw.size(), 1));
+    A.add(&W.lj_res, W.lj_cap);
+    A.add(&W.lj_wall, W.lj_cap);
+    for (int k = 0; k < 3; ++k) A.add(&W.lj_off[k], W.lj_cap);
+    A.add(&W.lj_sat, W.lj_cap);
+    A.add(&W.lj_gone, W.lj_cap);
+    A.add(&W.cand_res, W.cand_cap);
+    A.add(&W.cand_wall, W.cand_cap);
+    A.add(&W.cand_d, W.cand_cap);
+    A.add(&W.is_conn, std::max(n, 1));
+  }
   if (dry_run_bytes) {
     *dry_run_bytes = A.total;
     return PCG_OK;
@@ -586,6 +675,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   D.sys = (SysDescC *)dsys;
   D.st = dst;
   D.P = (DevParamsC *)B->dP;
+  D.wall = B->dW;
   D.atype = atype, D.chain = chain, D.seq = seq, D.prev = prev, D.next = next;
   D.sysid = sysid, D.local = local;
   for (int k = 0; k < 2; ++k) D.teth_i[k] = teth_i[k], D.lrep_i[k] = lrep_i[k], D.nc_i[k] = nc_i[k];
@@ -605,6 +695,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     st[s].verify_first = 1;
     st[s].active = 1;
   }
+  if (plain) st[0].n_hw = (int)hw.size();
   auto up = [&](void *dst_, const void *src, size_t bytes) -> int {
     if (bytes) HIPCHK(hipMemcpyAsync(dst_, src, bytes, hipMemcpyHostToDevice, B->stream));
     return PCG_OK;
@@ -721,6 +812,28 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
 #undef CATI
 #undef CATD
 #undef CATU
+  if (plain) {
+    const size_t nh = hw.size();
+    std::vector<int> hr(nh), hwl(nh);
+    std::vector<double> ho[3], hn(nh, 0.0);  // the natural distance: 0 (state.cpp:664-665)
+    for (int k = 0; k < 3; ++k) ho[k].resize(nh);
+    for (size_t a = 0; a < nh; ++a) {
+      hr[a] = hw[a].res, hwl[a] = hw[a].wall;
+      for (int k = 0; k < 3; ++k) ho[k][a] = hw[a].off[k];
+    }
+    if ((rc = up((void *)W.hw_res, hr.data(), nh * 4))) return rc;
+    if ((rc = up((void *)W.hw_wall, hwl.data(), nh * 4))) return rc;
+    for (int k = 0; k < 3; ++k)
+      if ((rc = up((void *)W.hw_off[k], ho[k].data(), nh * 8))) return rc;
+    if ((rc = up((void *)W.hw_nat, hn.data(), nh * 8))) return rc;
+    if ((rc = up(W.is_conn, conn.data(), conn.size()))) return rc;
+    std::vector<uint8_t> zero8(W.lj_cap, 0);
+    if ((rc = up(W.lj_gone, zero8.data(), zero8.size()))) return rc;
+    for (int side = 0; side < 6; ++side) W.side_wall[side] = -1;
+    for (int w = 0; w < p.n_walls; ++w)
+      if (p.wall_side[w] >= 0) W.side_wall[p.wall_side[w]] = w;
+    if ((rc = up(B->dW, &W, sizeof(WallDev)))) return rc;
+  }
   if ((rc = up(dsys, B->sys.data(), nsys * sizeof(SysDesc)))) return rc;
   if ((rc = up(dst, st.data(), nsys * sizeof(SysState)))) return rc;
   if ((rc = up(B->dP, &B->hp, sizeof(DevParams)))) return rc;
@@ -965,6 +1078,15 @@ int check_overflow(DeviceBatch &B) {
     if (st[s].overflow)
       return fail(PCG_E_OVERFLOW, "a neighbour or QA list of system " + std::to_string(s) +
                                       " outgrew its capacity");
+  if (B.dW && B.hp.ljw_terms) {
+    int ovf = 0;
+    HIPCHK(hipMemcpyAsync(&ovf, (char *)B.dW + offsetof(WallDev, overflow), sizeof(int),
+                          hipMemcpyDeviceToHost, B.stream));
+    HIPCHK(hipStreamSynchronize(B.stream));
+    if (ovf)
+      return fail(PCG_E_OVERFLOW, "the LJ wall candidates (at most 4096 a step) or connections "
+                                  "outgrew their capacity");
+  }
   return PCG_OK;
 }
 
@@ -985,17 +1107,59 @@ struct pcg_ctx {
 
 namespace {
 
+SysDesc *sys_rw(DeviceBatch &B) { return const_cast<SysDesc *>((const SysDesc *)B.d.sys); }
+
 void enqueue_step(DeviceBatch &B) {
   const DevBatch &d = B.d;
   const pcg_params &p = B.hp.p;
+  if (B.box_set) launch_box_phase(d, B.dP, sys_rw(B), B.stream);
   launch_begin(d, B.stream);
   launch_zero_forces(d, B.prec, B.stream);
   if (p.qa_on) launch_prepare_nh(d, B.stream);
   launch_fix_nl(d, B.max_nlcap, B.stream);
   launch_terms(d, B.prec, B.max_items, -1, 0, -1, B.stream);
   if (p.qa_on) launch_qa_finish(d, B.stream);
+  if (B.hp.ljw_terms) launch_ljw_process(d, B.stream);
+  if (B.box_set) launch_box_log(d, B.stream);
   launch_lang(d, B.prec, B.noise, B.stream);
   launch_tick(d, 1, B.stream);
+}
+
+// the reaction forces of the walls start a term evaluation at zero
+int zero_wall_F(DeviceBatch &B) {
+  if (B.dW)
+    HIPCHK(hipMemsetAsync((char *)B.dW + offsetof(WallDev, wall_F), 0, sizeof(B.hW.wall_F),
+                          B.stream));
+  return PCG_OK;
+}
+
+// the host copy of the system descriptor after the box protocol moved the
+// PBC cell on the device
+int refresh_sys(DeviceBatch &B) {
+  if (!B.box_set) return PCG_OK;
+  HIPCHK(hipMemcpyAsync(B.sys.data(), (const void *)B.d.sys, B.sys.size() * sizeof(SysDesc),
+                        hipMemcpyDeviceToHost, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+int read_wall(DeviceBatch &B, WallDev &W) {
+  HIPCHK(hipMemcpyAsync(&W, B.dW, sizeof(WallDev), hipMemcpyDeviceToHost, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+// writes the state part of W, with this context's array pointers
+int write_wall(DeviceBatch &B, WallDev W) {
+  const WallDev &H = B.hW;
+  W.hw_res = H.hw_res, W.hw_wall = H.hw_wall, W.hw_nat = H.hw_nat;
+  W.lj_res = H.lj_res, W.lj_wall = H.lj_wall, W.lj_sat = H.lj_sat, W.lj_gone = H.lj_gone;
+  for (int k = 0; k < 3; ++k) W.hw_off[k] = H.hw_off[k], W.lj_off[k] = H.lj_off[k];
+  W.cand_res = H.cand_res, W.cand_wall = H.cand_wall, W.cand_d = H.cand_d;
+  W.is_conn = H.is_conn;
+  W.lj_cap = H.lj_cap, W.cand_cap = H.cand_cap;
+  HIPCHK(hipMemcpyAsync(B.dW, &W, sizeof(WallDev), hipMemcpyHostToDevice, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
 }
 
 } // namespace
@@ -1018,7 +1182,7 @@ int pcg_create(const pcg_params *p, const pcg_model *m, const pcg_options *o, pc
   int32_t seed = opt.seed;
   auto c = std::make_unique<pcg_ctx>();
   int rc = make_batch(*p, 1, m, &T, &seed, opt.precision, opt.philox_seed, opt.device, c->B,
-                      nullptr, nullptr, opt.ran2_skip);
+                      nullptr, nullptr, opt.ran2_skip, nullptr, 0, true);
   if (rc) return rc;
   c->B->noise = opt.noise;
   if (opt.noise == PCG_NOISE_RAN2) {
@@ -1107,6 +1271,7 @@ int pcg_eval_forces(pcg_ctx *c) {
   if (rc) return rc;
   for (int k = 0; k < NT; ++k) st.V[k] = 0.0;
   if ((rc = write_state(B, 0, st))) return rc;
+  if ((rc = zero_wall_F(B))) return rc;
   launch_zero_forces(B.d, B.prec, B.stream);
   launch_terms(B.d, B.prec, B.max_items, -1, 0, -1, B.stream);
   launch_tick(B.d, 0, B.stream);
@@ -1133,6 +1298,9 @@ static int host_term_size(const SysDesc &s, const SysState &st, int term) {
   case PCG_VEL_AFM: v = s.ntip; break;
   case PCG_FORCE_AFM: v = s.nftip; break;
   case PCG_SOLID_WALL: v = s.n; break;
+  case PCG_HW_FREE: case PCG_LJW_FREE: v = s.n; break;
+  case PCG_HW_CONN: v = st.n_hw; break;
+  case PCG_LJW_CONN: v = st.n_lj; break;
   }
   return v;
 }
@@ -1155,6 +1323,7 @@ int pcg_eval_term(pcg_ctx *c, int32_t term, int32_t from, int32_t to, double *F,
   if (rc) return rc;
   for (int k = 0; k < NT; ++k) st.V[k] = 0.0;
   if ((rc = write_state(B, 0, st))) return rc;
+  if ((rc = zero_wall_F(B))) return rc;
   launch_zero_forces(B.d, B.prec, B.stream);
   launch_terms(B.d, B.prec, B.max_items, term, from, to, B.stream);
   HIPCHK(hipStreamSynchronize(B.stream));
@@ -1244,12 +1413,16 @@ int pcg_set_tips(pcg_ctx *c, int32_t n_tips, const int32_t *tip_res, const doubl
                  const double *tip_vel, int32_t n_ftips, const int32_t *ftip_res,
                  const double *ftip_force) {
   if (!c) return fail(PCG_E_ARG, "null ctx");
+  int rc = refresh_sys(*c->B);
+  if (rc) return rc;
   return set_tips(*c->B, 0, n_tips, tip_res, tip_orig, tip_vel, n_ftips, ftip_res, ftip_force);
 }
 
 int pcg_set_noise_scale(pcg_ctx *c, double scale) {
   if (!c || scale < 0) return fail(PCG_E_ARG, "bad arguments");
   DeviceBatch &B = *c->B;
+  int rc = refresh_sys(B);
+  if (rc) return rc;
   B.sys[0].const2 = const2_of(B.hp.p, B.sys[0].temperature) * scale;
   HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), sizeof(SysDesc), hipMemcpyHostToDevice,
                         B.stream));
@@ -1562,7 +1735,9 @@ void digest_params(Digest &d, const pcg_params &p) {
   PCG_D(pid_ss_r_low); PCG_D(pid_ss_r_high); PCG_D(afm_on); PCG_D(afm_H1); PCG_D(afm_H2);
   PCG_D(lang_on); PCG_D(gamma); PCG_D(temperature); PCG_D(dt); PCG_D(mass); PCG_D(charge);
   PCG_D(ptype); PCG_D(base_sync); PCG_D(n_walls); PCG_D(wall_origin); PCG_D(wall_normal);
-  PCG_D(wall_depth); PCG_D(wall_min_dist);
+  PCG_D(wall_depth); PCG_D(wall_min_dist); PCG_D(wall_kind); PCG_D(wall_side); PCG_D(wall_limit);
+  PCG_D(hw_on); PCG_D(ljw_on); PCG_D(wall_hh1); PCG_D(wall_lj_depth); PCG_D(ljw_breaking_dist);
+  PCG_D(ljw_saturation_diff);
 #undef PCG_D
 }
 }  // extern "C++"
@@ -1615,7 +1790,7 @@ int ckpt_write(DeviceBatch &B, const char magic[8], void *buf, int64_t bytes) {
   HIPCHK(hipSetDevice(B.device));
   CkptHeader h{};
   std::memcpy(h.magic, magic, 8);
-  h.version = 2;
+  h.version = 3;
   h.nsys = (int32_t)B.sys.size();
   h.ntot = B.d.ntot;
   h.prec = B.prec;
@@ -1636,7 +1811,7 @@ int ckpt_read(DeviceBatch &B, const char magic[8], const void *buf, int64_t byte
   CkptHeader h;
   if (bytes < (int64_t)sizeof h) return fail(PCG_E_ARG, "not a checkpoint");
   std::memcpy(&h, buf, sizeof h);
-  if (std::memcmp(h.magic, magic, 8) != 0 || h.version != 2) return fail(PCG_E_ARG, "not a checkpoint");
+  if (std::memcmp(h.magic, magic, 8) != 0 || h.version != 3) return fail(PCG_E_ARG, "not a checkpoint");
   if (h.nsys != (int32_t)B.sys.size() || h.ntot != B.d.ntot || h.prec != B.prec ||
       h.arena_bytes != B.arena.total || h.fingerprint != ckpt_fingerprint(B))
     return fail(PCG_E_ARG, "checkpoint of a different batch (models, parameters or precision)");
@@ -1690,7 +1865,159 @@ int pcg_checkpoint(pcg_ctx *c, void *buf, int64_t bytes) {
 
 int pcg_restore(pcg_ctx *c, const void *buf, int64_t bytes) {
   if (!c) return fail(PCG_E_ARG, "null argument");
-  return ckpt_read(*c->B, "PCGCKPTC", buf, bytes);
+  DeviceBatch &B = *c->B;
+  int rc = ckpt_read(B, "PCGCKPTC", buf, bytes);
+  if (rc || !B.dW) return rc;
+  // the wall state came with the arena; its array pointers are this context's
+  WallDev W;
+  if ((rc = read_wall(B, W)) || (rc = write_wall(B, W))) return rc;
+  B.box_set = W.box_on != 0;
+  if (c->gexec) {  // the captured step may lack (or hold) the box kernels
+    HIPCHK(hipGraphExecDestroy(c->gexec));
+    HIPCHK(hipGraphDestroy(c->graph));
+    c->gexec = nullptr;
+    c->graph = nullptr;
+  }
+  return PCG_OK;
+}
+
+int pcg_box_setup(pcg_ctx *c, const pcg_box *box) {
+  if (!c || !box) return fail(PCG_E_ARG, "null argument");
+  DeviceBatch &B = *c->B;
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(box->box_min[k]) || !std::isfinite(box->box_max[k]) ||
+        box->box_max[k] < box->box_min[k])
+      return fail(PCG_E_ARG, "pcg_box_setup: box_max must not be below box_min");
+  if (!(box->avg_forces_over >= 0.0)) return fail(PCG_E_ARG, "pcg_box_setup: negative averaging time");
+  HIPCHK(hipSetDevice(B.device));
+  WallDev W;
+  int rc = read_wall(B, W);
+  if (rc) return rc;
+  W.bx = *box;
+  for (int k = 0; k < 3; ++k) W.box_min[k] = box->box_min[k], W.box_max[k] = box->box_max[k];
+  W.box_on = 1;
+  W.log_on = W.side_wall[4] >= 0 && W.side_wall[5] >= 0;
+  W.phase = PCG_PHASE_NONE;
+  W.shear = 0;
+  W.since = W.until = W.amplitude = W.omega = W.displacement = 0.0;
+  if ((rc = write_wall(B, W))) return rc;
+  launch_box_setup(B.d, B.dP, sys_rw(B), B.stream);
+  HIPCHK(hipStreamSynchronize(B.stream));
+  B.box_set = true;
+  if ((rc = refresh_sys(B))) return rc;
+  if (c->gexec) {  // the next step captures the box kernels
+    HIPCHK(hipGraphExecDestroy(c->gexec));
+    HIPCHK(hipGraphDestroy(c->graph));
+    c->gexec = nullptr;
+    c->graph = nullptr;
+  }
+  return PCG_OK;
+}
+
+int pcg_box_start(pcg_ctx *c) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  DeviceBatch &B = *c->B;
+  if (!B.box_set) return fail(PCG_E_STATE, "pcg_box_start: no box (pcg_box_setup)");
+  HIPCHK(hipSetDevice(B.device));
+  WallDev W;
+  SysState st;
+  int rc = read_wall(B, W);
+  if (rc || (rc = read_state(B, 0, st))) return rc;
+  W.until = W.bx.total_time;
+  W.since = st.t;
+  W.phase = PCG_PHASE_SQUEEZING;
+  return write_wall(B, W);
+}
+
+int pcg_box_state(pcg_ctx *c, double *out) {
+  if (!c || !out) return fail(PCG_E_ARG, "null argument");
+  DeviceBatch &B = *c->B;
+  HIPCHK(hipSetDevice(B.device));
+  WallDev W;
+  DevParams P;
+  SysDesc sd;
+  int rc = read_wall(B, W);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(&P, B.dP, sizeof(DevParams), hipMemcpyDeviceToHost, B.stream));
+  HIPCHK(hipMemcpyAsync(&sd, (const void *)B.d.sys, sizeof(SysDesc), hipMemcpyDeviceToHost,
+                        B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  for (int k = 0; k < PCG_BOX_STATE_NUM; ++k) out[k] = 0.0;
+  out[PCG_BOX_PHASE] = W.phase;
+  out[PCG_BOX_SINCE] = W.since;
+  out[PCG_BOX_UNTIL] = W.until;
+  for (int k = 0; k < 3; ++k) {
+    out[PCG_BOX_MIN + k] = W.box_min[k];
+    out[PCG_BOX_MAX + k] = W.box_max[k];
+    out[PCG_BOX_CELL + k] = sd.cell[k];
+  }
+  out[PCG_BOX_AMPLITUDE] = W.amplitude;
+  out[PCG_BOX_OMEGA] = W.omega;
+  out[PCG_BOX_DISPLACEMENT] = W.displacement;
+  out[PCG_BOX_SHEAR] = W.shear;
+  out[PCG_BOX_AVG_Z_HAS] = W.avg_z.has;
+  out[PCG_BOX_AVG_Z] = W.avg_z.has ? W.avg_z.avg[0] : 0.0;
+  for (int w = 0; w < 6; ++w) {
+    for (int k = 0; k < 3; ++k) {
+      out[PCG_BOX_ORIGIN + 3 * w + k] = P.p.wall_origin[w][k];
+      out[PCG_BOX_NORMAL + 3 * w + k] = P.p.wall_normal[w][k];
+      out[PCG_BOX_WALL_F + 3 * w + k] = W.wall_F[w][k];
+      out[PCG_BOX_SHIFT + 3 * w + k] = W.shift[w][k];
+      out[PCG_BOX_AVG_F + 3 * w + k] = W.avg_F[w].has ? W.avg_F[w].avg[k] : 0.0;
+    }
+    out[PCG_BOX_WORK + w] = W.work[w];
+    out[PCG_BOX_AVG_F_HAS + w] = W.avg_F[w].has;
+  }
+  return PCG_OK;
+}
+
+int pcg_ljw_process(pcg_ctx *c) {
+  if (!c) return fail(PCG_E_ARG, "null ctx");
+  DeviceBatch &B = *c->B;
+  if (!B.hp.ljw_terms) return fail(PCG_E_STATE, "pcg_ljw_process: no LJ walls (ljw_on and a plane of kind 2)");
+  HIPCHK(hipSetDevice(B.device));
+  launch_ljw_process(B.d, B.stream);
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return check_overflow(B);
+}
+
+int pcg_wall_conns(pcg_ctx *c, int32_t kind, int32_t cap, int32_t *size, int32_t *res,
+                   int32_t *wall, double *offset, double *sat) {
+  if (!c || !size || (kind != 0 && kind != 1) || cap < 0) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  HIPCHK(hipSetDevice(B.device));
+  SysState st;
+  int rc = read_state(B, 0, st);
+  if (rc) return rc;
+  const int k = kind == 0 ? st.n_hw : st.n_lj;
+  *size = k;
+  const int m = std::min(k, cap);
+  if (m == 0) return PCG_OK;
+  const WallDev &H = B.hW;
+  std::vector<double> o[3];
+  for (int a = 0; a < 3; ++a) o[a].resize(m);
+  std::vector<int> r(m), w(m);
+  std::vector<double> sv(m);
+  auto get = [&](void *dst, const void *src, size_t bytes) -> int {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, B.stream));
+    return PCG_OK;
+  };
+  if ((rc = get(r.data(), kind == 0 ? (const void *)H.hw_res : H.lj_res, m * 4)) ||
+      (rc = get(w.data(), kind == 0 ? (const void *)H.hw_wall : H.lj_wall, m * 4)) ||
+      (rc = get(sv.data(), kind == 0 ? (const void *)H.hw_nat : H.lj_sat, m * 8)))
+    return rc;
+  for (int a = 0; a < 3; ++a)
+    if ((rc = get(o[a].data(), kind == 0 ? (const void *)H.hw_off[a] : H.lj_off[a], m * 8)))
+      return rc;
+  HIPCHK(hipStreamSynchronize(B.stream));
+  for (int a = 0; a < m; ++a) {
+    if (res) res[a] = r[a];
+    if (wall) wall[a] = w[a];
+    if (sat) sat[a] = sv[a];
+    if (offset)
+      for (int q = 0; q < 3; ++q) offset[3 * a + q] = o[q][a];
+  }
+  return PCG_OK;
 }
 
 int pcg_batch_report(pcg_batch *b, double *out) {

@@ -20,6 +20,9 @@ __global__ void k_begin(DevBatch b) {
   SysDescC &d = b.sys[s];
   if (!st.active) return;
   for (int k = 0; k < NT; ++k) st.V[k] = 0.0;
+  if (b.wall)  // dynamics::reset (dynamics.cpp:11-19) of the wall forces
+    for (int w = 0; w < 6; ++w)
+      for (int c = 0; c < 3; ++c) b.wall->wall_F[w][c] = 0.0;
   if (b.P->nl_required) {
     if (st.verify_first) {
       st.verify_first = 0;
@@ -369,6 +372,366 @@ __global__ void k_tick(DevBatch b, int advance) {
   }
 }
 
+// ------------------------------------------------------------------ walls ----
+// wall::lj::process_candidates (cg/src/cg/wall/lj/process_candidates.cpp:5-31)
+// in one workgroup: the released slots leave (their residue unconnected, the
+// plane's count down) and the live slots close up in order; then the
+// candidates, sorted by distance (ties by residue, then plane: the order
+// sift_free emits them in), connect one by one while their plane is below its
+// limit and their residue is not connected, with the bead at the residue's
+// projection on the plane. As in the reference a new connection does not
+// mark its residue connected. The candidates are consumed.
+constexpr int LJW_SORT_MAX = 4096;
+__global__ void __launch_bounds__(1024) k_ljw_process(DevBatch b) {
+#pragma clang fp contract(off)
+  WallDev &W = *b.wall;
+  SysState &st = b.st[0];
+  ParamsC &p = b.P->p;
+  const int tid = threadIdx.x, B = blockDim.x;
+  __shared__ double key[LJW_SORT_MAX];
+  __shared__ int val[LJW_SORT_MAX];
+  __shared__ int scan[1024];
+  __shared__ int base;
+  const int n_lj = st.n_lj;
+  // releases
+  for (int k = tid; k < n_lj; k += B)
+    if (W.lj_gone[k]) {
+      W.is_conn[W.lj_res[k]] = 0;
+      atomicSub(&W.num_conns[W.lj_wall[k]], 1);
+    }
+  if (tid == 0) base = 0;
+  __syncthreads();
+  // ordered compaction, one chunk of B slots at a time (reads before writes)
+  for (int c0 = 0; c0 < n_lj; c0 += B) {
+    const int k = c0 + tid;
+    const bool keep = k < n_lj && !W.lj_gone[k];
+    int res = 0, wl = 0;
+    double o0 = 0, o1 = 0, o2 = 0, sat = 0;
+    if (keep) {
+      res = W.lj_res[k], wl = W.lj_wall[k], sat = W.lj_sat[k];
+      o0 = W.lj_off[0][k], o1 = W.lj_off[1][k], o2 = W.lj_off[2][k];
+    }
+    scan[tid] = keep ? 1 : 0;
+    __syncthreads();
+    for (int off = 1; off < B; off <<= 1) {
+      int v = tid >= off ? scan[tid - off] : 0;
+      __syncthreads();
+      scan[tid] += v;
+      __syncthreads();
+    }
+    const int pos = base + scan[tid] - (keep ? 1 : 0);
+    const int total = scan[B - 1];
+    __syncthreads();
+    if (keep) {
+      W.lj_res[pos] = res, W.lj_wall[pos] = wl, W.lj_sat[pos] = sat;
+      W.lj_off[0][pos] = o0, W.lj_off[1][pos] = o1, W.lj_off[2][pos] = o2;
+      W.lj_gone[pos] = 0;
+    }
+    __syncthreads();
+    if (tid == 0) base += total;
+    __syncthreads();
+  }
+  int live = base;
+  for (int k = live + tid; k < n_lj; k += B) W.lj_gone[k] = 0;
+  // candidates by (distance, residue, plane)
+  int nc = W.n_cand < W.cand_cap ? W.n_cand : W.cand_cap;
+  if (nc > LJW_SORT_MAX) {
+    if (tid == 0) W.overflow = 1;
+    nc = LJW_SORT_MAX;
+  }
+  int m = 1;
+  while (m < nc) m <<= 1;
+  for (int k = tid; k < m; k += B) {
+    key[k] = k < nc ? W.cand_d[k] : __builtin_inf();
+    val[k] = k < nc ? k : -1;
+  }
+  __syncthreads();
+  auto before = [&](int a, int c) {  // entry a sorts before entry c
+    if (key[a] != key[c]) return key[a] < key[c];
+    const int va = val[a], vc = val[c];
+    if (va < 0 || vc < 0) return vc < 0 && va >= 0;
+    const int ra = W.cand_res[va], rc = W.cand_res[vc];
+    if (ra != rc) return ra < rc;
+    return W.cand_wall[va] < W.cand_wall[vc];
+  };
+  for (int size = 2; size <= m; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < m; i += B) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          if (before(j, i) == up) {
+            const double tk = key[i];
+            key[i] = key[j], key[j] = tk;
+            const int tv = val[i];
+            val[i] = val[j], val[j] = tv;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  if (tid == 0) {
+    for (int a = 0; a < nc; ++a) {
+      const int c = val[a], res = W.cand_res[c], w = W.cand_wall[c];
+      if (W.is_conn[res] || W.num_conns[w] >= p.wall_limit[w]) continue;
+      if (live >= W.lj_cap) {
+        W.overflow = 1;
+        break;
+      }
+      const double nx = p.wall_normal[w][0], ny = p.wall_normal[w][1], nz = p.wall_normal[w][2];
+      const double ox = p.wall_origin[w][0], oy = p.wall_origin[w][1], oz = p.wall_origin[w][2];
+      const double rx = b.y[0][0][res], ry = b.y[0][1][res], rz = b.y[0][2][res];
+      const double sd = nx * (rx - ox) + ny * (ry - oy) + nz * (rz - oz);
+      W.lj_res[live] = res, W.lj_wall[live] = w, W.lj_sat[live] = 0.0, W.lj_gone[live] = 0;
+      W.lj_off[0][live] = (rx - sd * nx) - ox;
+      W.lj_off[1][live] = (ry - sd * ny) - oy;
+      W.lj_off[2][live] = (rz - sd * nz) - oz;
+      ++live;
+      ++W.num_conns[w];
+    }
+    W.n_cand = 0;
+    W.n_gone = 0;
+    st.n_lj = live;
+  }
+}
+
+// ------------------------------------------------------------- box protocol ----
+__device__ __forceinline__ void mavg_add(MAvg &a, double time, const double *sample, int dim) {
+#pragma clang fp contract(off)
+  if (a.num == 0) {
+    a.init_time = time;
+    for (int c = 0; c < dim; ++c) a.sum[c] = sample[c];
+    a.num = 1;
+  } else {
+    for (int c = 0; c < dim; ++c) a.sum[c] += sample[c];
+    ++a.num;
+  }
+  if (a.init_time + a.window <= time) {
+    for (int c = 0; c < dim; ++c) a.avg[c] = a.sum[c] / (double)a.num;
+    a.has = 1;
+    a.num = 0;
+  }
+}
+
+// state::reinit_wall_values (cg/src/cg/simul/state.cpp:708-729): each plane
+// on a box side goes to the centre plus the projection on its normal of the
+// vector to its face; shift = new origin - old; periodic axes take the box
+// extent as their PBC cell
+__device__ void box_reinit(WallDev &W, DevParams *P, SysDesc *sd) {
+#pragma clang fp contract(off)
+  double center[3], ext[3];
+  for (int c = 0; c < 3; ++c) {
+    center[c] = (W.box_max[c] + W.box_min[c]) / 2.0;
+    ext[c] = W.box_max[c] - W.box_min[c];
+  }
+  for (int side = 0; side < 6; ++side) {
+    const int w = W.side_wall[side];
+    if (w < 0) continue;
+    double d[3], *o = P->p.wall_origin[w];
+    const double *n = P->p.wall_normal[w];
+    for (int c = 0; c < 3; ++c) d[c] = (side % 2 == 1 ? W.box_max[c] : W.box_min[c]) - center[c];
+    const double proj = d[0] * n[0] + d[1] * n[1] + d[2] * n[2];
+    for (int c = 0; c < 3; ++c) {
+      const double next = center[c] + n[c] * proj;
+      W.shift[w][c] = next - o[c];
+      o[c] = next;
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double cell = W.bx.pbc[c] ? ext[c] : 0.0;
+    sd->cell[c] = cell;
+    sd->cell_inv[c] = cell != 0.0 ? 1.0 / cell : 0.0;
+  }
+}
+
+// state::adjust_wall_pos (state.cpp:702-706) along z, or with the same
+// change on every axis
+__device__ __forceinline__ void box_adjust(WallDev &W, DevParams *P, SysDesc *sd, const double *size_change) {
+#pragma clang fp contract(off)
+  for (int c = 0; c < 3; ++c) {
+    W.box_min[c] += -size_change[c] + 0.0;
+    W.box_max[c] += size_change[c] + 0.0;
+  }
+  box_reinit(W, P, sd);
+}
+
+// state::setup_walls' box part: averages reset, planes to the faces
+__global__ void k_box_setup(DevBatch b, DevParams *P, SysDesc *sd) {
+  if (threadIdx.x != 0) return;
+  WallDev &W = *b.wall;
+  for (int w = 0; w < 6; ++w) {
+    W.avg_F[w] = MAvg{};
+    W.avg_F[w].window = W.bx.avg_forces_over;
+    W.work[w] = 0.0;
+  }
+  W.avg_z = MAvg{};
+  W.avg_z.window = W.bx.avg_forces_over;
+  box_reinit(W, P, sd);
+}
+
+// The phase logic of thread_main.cpp:144-330 before a step: the phase
+// changes of this time (which take no step in the reference) and the wall
+// move of the phase that then advances.
+__global__ void k_box_phase(DevBatch b, DevParams *P, SysDesc *sd) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  WallDev &W = *b.wall;
+  const SysState &st = b.st[0];
+  if (!st.active || W.phase == PCG_PHASE_NONE) return;
+  const pcg_box &bx = W.bx;
+  const double t = st.t, dt = P->p.dt, num_res = (double)sd->n;
+  const double pi = 3.14159265358979323846, pi_2 = 1.57079632679489661923;
+  for (int guard = 0; guard < 16; ++guard) {
+    switch (W.phase) {
+    case PCG_PHASE_SQUEEZING: {  // squeezing_step, thread_main.cpp:144-181
+      if (!bx.squeezing) {
+        W.since = t;
+        W.phase = PCG_PHASE_FIND_FORCE_MIN;
+        continue;
+      }
+      const double ex = W.box_max[0] - W.box_min[0], ey = W.box_max[1] - W.box_min[1],
+                   ez = W.box_max[2] - W.box_min[2];
+      const double cur_vol = ex * ey * ez, target_vol = num_res / bx.target_density;
+      if (cur_vol <= target_vol) {
+        W.until = fmin(t + bx.rest_period, bx.total_time);
+        W.phase = PCG_PHASE_REST_AFTER_SQUEEZING;
+        continue;
+      }
+      double vel;
+      if (cur_vol > 2.0 * target_vol) {
+        const double accel_time = bx.accel_dist / bx.vel_above_2V;
+        vel = fmin((t - W.since) / accel_time, 1.0) * bx.vel_above_2V;
+      } else {
+        const double accel_time = bx.accel_dist / bx.target_vel;
+        vel = fmin((t - W.since) / accel_time, 1.0) * bx.target_vel;
+      }
+      const double sh = 0.5 * vel * dt, ch[3] = {-sh, -sh, -sh};
+      box_adjust(W, P, sd, ch);
+      return;
+    }
+    case PCG_PHASE_REST_AFTER_SQUEEZING:  // :183-191
+      if (t < W.until) return;
+      W.phase = PCG_PHASE_FIND_FORCE_MIN;
+      continue;
+    case PCG_PHASE_FIND_FORCE_MIN: {  // :193-222
+      if (!bx.force_min) {
+        W.phase = PCG_PHASE_FREEFORM;
+        continue;
+      }
+      if (W.avg_z.has && fabs(W.avg_z.avg[0]) < 5e-2 + 5e-5 * num_res) {
+        W.until = t + bx.rest_period;
+        W.phase = PCG_PHASE_REST_AFTER_FORCE_MIN;
+        continue;
+      }
+      const double accel_time = bx.accel_dist / bx.target_vel;
+      const double time_vel_frac = fmin((t - W.since) / accel_time, 1.0);
+      double fvf = (W.avg_z.has ? W.avg_z.avg[0] : 0.0) / bx.force_for_max_vel;
+      fvf = copysign(fmin(fabs(fvf), 1.0), fvf);
+      const double vel = bx.target_vel * time_vel_frac * fvf;
+      const double sh = 0.5 * vel * dt, ch[3] = {0.0, 0.0, sh};
+      box_adjust(W, P, sd, ch);
+      return;
+    }
+    case PCG_PHASE_REST_AFTER_FORCE_MIN:  // :224-243
+      if (t < W.until) return;
+      if (!bx.amp_relative) {
+        W.amplitude = bx.amp_abs;
+      } else {
+        W.shear = bx.osc_shear != 0;
+        const double base_val = W.shear ? W.box_max[0] - W.box_min[0] : W.box_max[2] - W.box_min[2];
+        W.amplitude = bx.amp_rel * base_val;
+      }
+      W.since = t;
+      W.phase = PCG_PHASE_MAX_AMPLITUDE;
+      continue;
+    case PCG_PHASE_MAX_AMPLITUDE: {  // :245-276
+      if (!bx.oscillations) {
+        W.phase = PCG_PHASE_FREEFORM;
+        continue;
+      }
+      const double accel_time = bx.accel_dist / bx.target_vel;
+      const double vel = fmin((t - W.since) / accel_time, 1.0) * bx.target_vel;
+      const double sh = vel * dt;
+      if (W.displacement > 0.5 * W.amplitude) {
+        W.until = t + bx.rest_period;
+        W.phase = PCG_PHASE_REST_AFTER_MAX_AMP;
+        continue;
+      }
+      W.displacement += sh;
+      if (W.shear) {
+        for (int side = 4; side < 6; ++side)
+          if (W.side_wall[side] >= 0) P->p.wall_normal[W.side_wall[side]][0] += sh;
+      } else {
+        const double ch[3] = {0.0, 0.0, -sh};
+        box_adjust(W, P, sd, ch);
+      }
+      return;
+    }
+    case PCG_PHASE_REST_AFTER_MAX_AMP:  // :278-293
+      if (t < W.until) return;
+      W.since = t;
+      W.omega = bx.angular_freq;
+      W.until = t + (pi * 2 * bx.num_of_cycles + pi_2) / W.omega;
+      W.phase = PCG_PHASE_OSCILLATIONS;
+      continue;
+    case PCG_PHASE_OSCILLATIONS: {  // :295-320
+      if (t >= W.until) {
+        W.until = t + bx.rest_period;
+        W.phase = PCG_PHASE_REST_AFTER_OSCILLATIONS;
+        continue;
+      }
+      const double vel = 0.5 * W.amplitude * W.omega * sin(W.omega * (t - W.since));
+      const double sh = vel * dt;
+      if (W.shear) {
+        for (int side = 4; side < 6; ++side)
+          if (W.side_wall[side] >= 0) P->p.wall_normal[W.side_wall[side]][0] += sh;
+      } else {
+        const double ch[3] = {0.0, 0.0, sh};
+        box_adjust(W, P, sd, ch);
+      }
+      return;
+    }
+    case PCG_PHASE_REST_AFTER_OSCILLATIONS:  // :322-330
+      if (t < W.until) return;
+      W.phase = PCG_PHASE_FREEFORM;
+      continue;
+    default:
+      return;
+    }
+  }
+}
+
+// wall::log_forces (cg/src/cg/wall/log_forces.cpp:4-17) after the forces of
+// a step: every plane on a box side, then once more the -axis plane of each
+// LJ axis (the reference's wall list holds it twice, state.cpp:680-681);
+// work += F . shift; the z force averaged
+__global__ void k_box_log(DevBatch b) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  WallDev &W = *b.wall;
+  const SysState &st = b.st[0];
+  if (!st.active || !W.log_on) return;
+  ParamsC &p = b.P->p;
+  const double t = st.t;
+  auto one = [&](int w) {
+    mavg_add(W.avg_F[w], t, W.wall_F[w], 3);
+    W.work[w] += W.wall_F[w][0] * W.shift[w][0] + W.wall_F[w][1] * W.shift[w][1] +
+                 W.wall_F[w][2] * W.shift[w][2];
+  };
+  for (int side = 0; side < 6; ++side)
+    if (W.side_wall[side] >= 0) one(W.side_wall[side]);
+  for (int side = 0; side < 6; side += 2) {
+    const int w = W.side_wall[side];
+    if (w >= 0 && p.wall_kind[w] == 2) one(w);
+  }
+  const int wn = W.side_wall[4], wp = W.side_wall[5];
+  const double zf = (W.wall_F[wn][0] * p.wall_normal[wn][0] + W.wall_F[wn][1] * p.wall_normal[wn][1] +
+                     W.wall_F[wn][2] * p.wall_normal[wn][2]) +
+                    (W.wall_F[wp][0] * p.wall_normal[wp][0] + W.wall_F[wp][1] * p.wall_normal[wp][1] +
+                     W.wall_F[wp][2] * p.wall_normal[wp][2]);
+  mavg_add(W.avg_z, t, &zf, 1);
+}
+
 // ------------------------------------------------------------- launchers ----
 static inline dim3 grid1(int n, int blk = BLK) { return dim3((unsigned)((n + blk - 1) / blk)); }
 
@@ -420,6 +783,18 @@ void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s) {
 }
 void launch_tick(const DevBatch &b, int advance, hipStream_t s) {
   hipLaunchKernelGGL(k_tick, grid1(b.nsys, 64), dim3(64), 0, s, b, advance);
+}
+void launch_ljw_process(const DevBatch &b, hipStream_t s) {
+  hipLaunchKernelGGL(k_ljw_process, dim3(1), dim3(1024), 0, s, b);
+}
+void launch_box_setup(const DevBatch &b, DevParams *P, SysDesc *sd, hipStream_t s) {
+  hipLaunchKernelGGL(k_box_setup, dim3(1), dim3(64), 0, s, b, P, sd);
+}
+void launch_box_phase(const DevBatch &b, DevParams *P, SysDesc *sd, hipStream_t s) {
+  hipLaunchKernelGGL(k_box_phase, dim3(1), dim3(64), 0, s, b, P, sd);
+}
+void launch_box_log(const DevBatch &b, hipStream_t s) {
+  hipLaunchKernelGGL(k_box_log, dim3(1), dim3(64), 0, s, b);
 }
 
 } // namespace pcg

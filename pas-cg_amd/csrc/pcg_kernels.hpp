@@ -110,6 +110,10 @@ __device__ __forceinline__ int term_size(SysDescC &s, const SysState &st, int te
   case PCG_VEL_AFM: return s.ntip;
   case PCG_FORCE_AFM: return s.nftip;
   case PCG_SOLID_WALL: return s.n;
+  case PCG_HW_FREE: return s.n;
+  case PCG_HW_CONN: return st.n_hw;
+  case PCG_LJW_FREE: return s.n;
+  case PCG_LJW_CONN: return st.n_lj;
   }
   return 0;
 }
@@ -134,7 +138,11 @@ __device__ __forceinline__ bool term_on(DevParamsC &P, int term) {
   case PCG_SND: return p.nat_dih_on && p.nat_dih_variant == 1;
   case PCG_VEL_AFM: return p.afm_on;
   case PCG_FORCE_AFM: return p.afm_on;
-  case PCG_SOLID_WALL: return p.n_walls > 0;
+  case PCG_SOLID_WALL: return P.solid_on;
+  case PCG_HW_FREE: return P.hw_terms;
+  case PCG_HW_CONN: return P.hw_terms;
+  case PCG_LJW_FREE: return P.ljw_terms;
+  case PCG_LJW_CONN: return P.ljw_terms;
   }
   return false;
 }
@@ -166,6 +174,10 @@ __device__ __forceinline__ double eval_item(const DevBatch &b, SysDescC &s, SysS
   case PCG_VEL_AFM: return vel_afm_item<double>(b, g, acc, s.tip0 + k, st.t - s.afm_t0);
   case PCG_FORCE_AFM: return force_afm_item(b, acc, s.tip0 + s.ntip + k);
   case PCG_SOLID_WALL: return solid_wall_item(b, g, acc, s.res0 + k);
+  case PCG_HW_FREE: return hw_free_item(b, g, acc, s.res0 + k);
+  case PCG_HW_CONN: return hw_conn_item(b, g, acc, k);
+  case PCG_LJW_FREE: return ljw_free_item(b, g, acc, s.res0 + k);
+  case PCG_LJW_CONN: return ljw_conn_item(b, g, acc, k);
   }
   return 0.0;
 }

@@ -13,6 +13,13 @@ void launch_terms(const DevBatch &b, int prec, int max_items, int only_term, int
 void launch_qa_finish(const DevBatch &b, hipStream_t s);
 void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s);
 void launch_tick(const DevBatch &b, int advance, hipStream_t s);
+// walls and the box protocol of a plain context (b.wall set): LJ candidate
+// processing; box set-up, the phase logic before a step, the force log after
+// it. P and sd are the writable device parameters and system descriptor.
+void launch_ljw_process(const DevBatch &b, hipStream_t s);
+void launch_box_setup(const DevBatch &b, DevParams *P, SysDesc *sd, hipStream_t s);
+void launch_box_phase(const DevBatch &b, DevParams *P, SysDesc *sd, hipStream_t s);
+void launch_box_log(const DevBatch &b, hipStream_t s);
 // persistent form: every system of the batch advances nsteps in one launch
 void launch_persistent(const DevBatch &b, int prec, int block, int nsteps, size_t lds_bytes,
                        hipStream_t s);

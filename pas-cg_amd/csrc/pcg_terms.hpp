@@ -608,9 +608,24 @@ __device__ __forceinline__ double force_afm_item(const DevBatch &b, A &acc, int 
   return 0.0;
 }
 
+// ---------------------------------------------------------------- walls ----
+// Walls are evaluated in FP64 in every precision (a few items per step).
+__device__ __forceinline__ double sgnd(double x) { return (double)((0.0 < x) - (x < 0.0)); }
+// dist^-10 by products of the reciprocal (a few ulp from pow(dist, -10),
+// without pow's registers in the persistent kernel)
+__device__ __forceinline__ double inv10(double d) {
+  const double i1 = 1.0 / d, i2 = i1 * i1, i4 = i2 * i2;
+  return i4 * i4 * i2;
+}
+__device__ __forceinline__ void wall_react(const DevBatch &b, int w, v3<double> f) {
+  atomicAdd(&b.wall->wall_F[w][0], f.x);
+  atomicAdd(&b.wall->wall_F[w][1], f.y);
+  atomicAdd(&b.wall->wall_F[w][2], f.z);
+}
+
 // wall::solid::eval_forces::iter (cg/src/cg/wall/solid/eval_forces.cpp:4-17):
-// residue i against every wall plane, d^-10 within min_dist, no energy; the
-// walls' own reaction forces (wall_F, statistics) are not kept.
+// residue i against every solid plane, d^-10 within min_dist, no energy; the
+// planes' reaction forces are kept in a plain context (b.wall).
 template <typename G, typename A>
 __device__ __forceinline__ double solid_wall_item(const DevBatch &b, const G &g, A &acc, int i) {
 #pragma clang fp contract(off)
@@ -619,20 +634,145 @@ __device__ __forceinline__ double solid_wall_item(const DevBatch &b, const G &g,
   v3<double> f{0.0, 0.0, 0.0};
   bool hit = false;
   for (int w = 0; w < p.n_walls; ++w) {
+    if (p.wall_kind[w] != 0) continue;
     const double nx = p.wall_normal[w][0], ny = p.wall_normal[w][1], nz = p.wall_normal[w][2];
     const double sd = nx * (r.x - p.wall_origin[w][0]) + ny * (r.y - p.wall_origin[w][1]) +
                       nz * (r.z - p.wall_origin[w][2]);
     const double dist = fabs(sd);
     if (dist >= p.wall_min_dist) continue;
-    // dist^-10 by products of the reciprocal (a few ulp from pow(dist, -10),
-    // without pow's registers in the persistent kernel)
-    const double i1 = 1.0 / dist, i2 = i1 * i1, i4 = i2 * i2, i10 = i4 * i4 * i2;
-    const double dV = p.wall_depth * i10 * (double)((0.0 < sd) - (sd < 0.0));
-    f = f + v3<double>{dV * nx, dV * ny, dV * nz};
+    const double dV = p.wall_depth * inv10(dist) * sgnd(sd);
+    const v3<double> fw{dV * nx, dV * ny, dV * nz};
+    f = f + fw;
     hit = true;
+    if (b.wall) wall_react(b, w, -fw);
   }
   if (hit) acc.add(i, f);
   return 0.0;
+}
+
+// wall::harmonic::eval_free::iter (cg/src/cg/wall/harmonic/eval_free.cpp:15-31):
+// a residue not connected to a wall against every harmonic plane, the solid
+// wall's d^-10 with the reference's opposite sign (F -= dV n), no energy.
+template <typename G, typename A>
+__device__ __forceinline__ double hw_free_item(const DevBatch &b, const G &g, A &acc, int i) {
+#pragma clang fp contract(off)
+  ParamsC &p = b.P->p;
+  if (b.wall->is_conn[i]) return 0.0;
+  const v3<double> r = g.r(i);
+  for (int w = 0; w < p.n_walls; ++w) {
+    if (p.wall_kind[w] != 1) continue;
+    const v3<double> n{p.wall_normal[w][0], p.wall_normal[w][1], p.wall_normal[w][2]};
+    const double sd = n.x * (r.x - p.wall_origin[w][0]) + n.y * (r.y - p.wall_origin[w][1]) +
+                      n.z * (r.z - p.wall_origin[w][2]);
+    const double dist = fabs(sd);
+    if (dist >= p.wall_min_dist) continue;
+    const double dV = p.wall_depth * inv10(dist) * sgnd(sd);
+    const v3<double> force{dV * n.x, dV * n.y, dV * n.z};
+    acc.add(i, -force);
+    wall_react(b, w, force);
+  }
+  return 0.0;
+}
+
+// wall::harmonic::eval_connected::iter
+// (cg/src/cg/wall/harmonic/eval_connected.cpp:17-29): connection k pulls its
+// residue by harmonic(HH1, 0, nat) towards its bead (offset + plane origin),
+// with the reference's sign F[res] -= dV_dr u, u from the residue to the bead.
+template <typename G, typename A>
+__device__ __forceinline__ double hw_conn_item(const DevBatch &b, const G &g, A &acc, int k) {
+#pragma clang fp contract(off)
+  ParamsC &p = b.P->p;
+  const WallDev &W = *b.wall;
+  const int res = W.hw_res[k], w = W.hw_wall[k];
+  const v3<double> bead{W.hw_off[0][k] + p.wall_origin[w][0], W.hw_off[1][k] + p.wall_origin[w][1],
+                        W.hw_off[2][k] + p.wall_origin[w][2]};
+  const v3<double> r12 = bead - g.r(res);
+  const double r12_n = sqrt(nsq(r12)), r12_rn = 1.0 / r12_n;
+  const v3<double> u = r12 * r12_rn;
+  const double dr = r12_n - W.hw_nat[k], dr2 = dr * dr;
+  const double V = 0.5 * dr2 * (p.wall_hh1 + 0.0 * dr2);
+  const double dV = clampR(dr * (p.wall_hh1 + 2.0 * 0.0 * dr2), -1.0e3, 1.0e3);
+  const v3<double> f = dV * u;
+  acc.add(res, -f);
+  wall_react(b, w, f);
+  return V;
+}
+
+// wall::lj::sift_free::iter (cg/src/cg/wall/lj/sift_free.cpp:15-37): a
+// residue not connected to a wall against every LJ plane within min_dist:
+// shifted LJ with r_min = min_dist, and a connection candidate while the
+// plane is below its limit.
+template <typename G, typename A>
+__device__ __forceinline__ double ljw_free_item(const DevBatch &b, const G &g, A &acc, int i) {
+#pragma clang fp contract(off)
+  ParamsC &p = b.P->p;
+  WallDev &W = *b.wall;
+  if (W.is_conn[i]) return 0.0;
+  const v3<double> r = g.r(i);
+  double V = 0.0;
+  for (int w = 0; w < p.n_walls; ++w) {
+    if (p.wall_kind[w] != 2) continue;
+    const v3<double> n{p.wall_normal[w][0], p.wall_normal[w][1], p.wall_normal[w][2]};
+    const double sd = n.x * (r.x - p.wall_origin[w][0]) + n.y * (r.y - p.wall_origin[w][1]) +
+                      n.z * (r.z - p.wall_origin[w][2]);
+    const double dist = fabs(sd);
+    if (dist > p.wall_min_dist) continue;
+    const double rinv = 1.0 / dist, s = rinv * p.wall_min_dist, s6 = ipow6(s), s12 = s6 * s6;
+    V += p.wall_lj_depth * (s12 - 2.0 * s6 + 1.0);
+    const double dV = clampR(12.0 * p.wall_lj_depth * rinv * (s6 - s12), -1.0e3, 1.0e3);
+    const double sg = sgnd(sd);
+    const v3<double> F_{(dV * n.x) * sg, (dV * n.y) * sg, (dV * n.z) * sg};
+    acc.add(i, -F_);
+    wall_react(b, w, F_);
+    if (W.num_conns[w] < p.wall_limit[w]) {
+      const int slot = atomicAdd(&W.n_cand, 1);
+      if (slot < W.cand_cap) {
+        W.cand_res[slot] = i;
+        W.cand_wall[slot] = w;
+        W.cand_d[slot] = dist;
+      } else {
+        W.overflow = 1;
+      }
+    }
+  }
+  return V;
+}
+
+// wall::lj::eval_connected::iter (cg/src/cg/wall/lj/eval_connected.cpp:15-41):
+// connection k, saturation-weighted LJ (r_min = min_dist) to its bead with the
+// reference's sign (F[res] -= F_), then its saturation moves by
+// saturation_diff towards 1 within the breaking distance, else towards 0;
+// at 0 the connection is released (processed by k_ljw_process).
+template <typename G, typename A>
+__device__ __forceinline__ double ljw_conn_item(const DevBatch &b, const G &g, A &acc, int k) {
+#pragma clang fp contract(off)
+  ParamsC &p = b.P->p;
+  WallDev &W = *b.wall;
+  const int res = W.lj_res[k], w = W.lj_wall[k];
+  const v3<double> bead{p.wall_origin[w][0] + W.lj_off[0][k], p.wall_origin[w][1] + W.lj_off[1][k],
+                        p.wall_origin[w][2] + W.lj_off[2][k]};
+  const v3<double> r12 = bead - g.r(res);
+  const double r12_n = sqrt(nsq(r12)), r12_rn = 1.0 / r12_n;
+  const v3<double> u = r12 * r12_rn;
+  const double s = r12_rn * p.wall_min_dist, s6 = ipow6(s), s12 = s6 * s6;
+  const double V_ = p.wall_lj_depth * (s12 - 2.0 * s6);
+  const double dV = 12.0 * p.wall_lj_depth * r12_rn * (s6 - s12);
+  double sat = W.lj_sat[k];
+  const v3<double> F_ = (sat * dV) * u;
+  acc.add(res, -F_);
+  wall_react(b, w, F_);
+  const double V = sat * V_;
+  if (r12_n < p.ljw_breaking_dist) {
+    sat = fmin(sat + p.ljw_saturation_diff, 1.0);
+  } else {
+    sat = fmax(sat - p.ljw_saturation_diff, 0.0);
+    if (sat == 0.0 && !W.lj_gone[k]) {
+      W.lj_gone[k] = 1;
+      atomicAdd(&W.n_gone, 1);
+    }
+  }
+  W.lj_sat[k] = sat;
+  return V;
 }
 
 // ------------------------------------------------------------------ QA ----

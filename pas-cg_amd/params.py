@@ -236,10 +236,21 @@ def default_params() -> dict:
     p["wall_normal"] = [[0.0, 0.0, 0.0] for _ in range(6)]
     p["wall_depth"] = 4.0 * EPS
     p["wall_min_dist"] = 5.0 * A
+    # per plane: kind (0 solid, 1 harmonic, 2 lj), box side (-1 none) and
+    # connection limit; harmonic and LJ walls off, as state::traj_init leaves
+    # them (state.cpp:111)
+    p["wall_kind"] = [0] * 6
+    p["wall_side"] = [-1] * 6
+    p["wall_limit"] = [0] * 6
+    p["hw_on"] = 0
+    p["ljw_on"] = 0
+    p["wall_hh1"] = 30.0 * EPS / (A ** 2)
+    p["wall_lj_depth"] = 4.0 * EPS
     p["lang_on"] = 1
     p["gamma"] = f32(2.0)
     p["temperature"] = f32(0.35)
     p["dt"] = f32(5e-3)
+    set_lj_wall_timing(p)
     p["mass"] = [1.0] * 21  # input.normalize mass: true
     p["charge"] = [float(_AA_TABLE[n][2]) for n in AA_NAMES]
     p["ptype"] = [_PTYPE[_AA_TABLE[n][1]] for n in AA_NAMES]
@@ -326,3 +337,125 @@ def set_solid_walls(p: dict, lo, hi, axes="xyz") -> None:
     p["n_walls"] = len(origin)
     p["wall_origin"] = origin + [[0.0, 0.0, 0.0]] * (6 - len(origin))
     p["wall_normal"] = normal + [[0.0, 0.0, 0.0]] * (6 - len(normal))
+
+
+def set_lj_wall_timing(p: dict, breaking_dist_factor: float = 1.5, cycle_dur: float = 0.0) -> None:
+    """The LJ wall constants thread::setup_lj_walls binds
+    (cg/src/cg/simul/thread_setup.cpp:787-790): breaking distance
+    factor * 2^(-1/6) * threshold, saturation step dt / cycle duration (a zero
+    cycle duration gives an infinite step: connections saturate, or break, at
+    once). Call again after changing dt or the threshold."""
+    p["ljw_breaking_dist"] = breaking_dist_factor * math.pow(0.5, 1.0 / 6.0) * p["wall_min_dist"]
+    p["ljw_saturation_diff"] = p["dt"] / cycle_dur if cycle_dur != 0.0 else math.inf
+
+
+def auto_wall_limit(num_res: int, threshold: float, target_density: float) -> int:
+    """The "auto" connection limit of harmonic and LJ walls
+    (cg/src/cg/simul/state.cpp:613-620): C int conversion of a double
+    quotient, then an integer halving, both toward zero."""
+    import numpy as np
+    x = float(np.cbrt(float(num_res))) - threshold * float(np.cbrt(float(target_density)))
+    lim = int((num_res - float(round(math.pow(x, 3.0)))) / 3)
+    return int(lim / 2)
+
+
+AXIS_TYPES = ("void", "periodic", "solid", "harmonic", "lj")
+
+
+def set_box_walls(p: dict, axes: dict, num_res: int, target_density: float = 1e-3 / ANGSTROM ** 3,
+                  limits: dict | None = None) -> list:
+    """Walls of state::setup_walls (cg/src/cg/simul/state.cpp:570-700) from the
+    per-axis types of sbox.walls ({"x": "solid", "y": "periodic", ...};
+    missing axes are "void"): for each walled axis, in x, y, z order, a plane
+    facing +axis on the -axis side and one facing -axis on the +axis side,
+    through the origin (pcg_box_setup moves them to the box faces). limits
+    gives the connection limit of "harmonic" and "lj" ({"harmonic": 10}); the
+    default is the auto limit of the target density. Returns the periodic
+    flag of each axis (the pbc field of the box)."""
+    limits = limits or {}
+    origin, normal, kind, side, limit = [], [], [], [], []
+    pbc = [0, 0, 0]
+    for k, a in enumerate("xyz"):
+        t = axes.get(a, "void")
+        if t not in AXIS_TYPES:
+            raise ValueError(f"unknown wall type {t!r}")
+        if t == "periodic":
+            pbc[k] = 1
+        if t in ("void", "periodic"):
+            continue
+        e = [0.0, 0.0, 0.0]
+        e[k] = 1.0
+        lim = limits.get(t)
+        if lim is None:
+            lim = auto_wall_limit(num_res, p["wall_min_dist"], target_density) if t != "solid" else 0
+        for s_, nrm in ((2 * k, e), (2 * k + 1, [-x for x in e])):
+            origin.append([0.0, 0.0, 0.0])
+            normal.append(list(nrm))
+            kind.append({"solid": 0, "harmonic": 1, "lj": 2}[t])
+            side.append(s_)
+            limit.append(int(lim))
+    pad = 6 - len(origin)
+    p["n_walls"] = len(origin)
+    p["wall_origin"] = origin + [[0.0, 0.0, 0.0]] * pad
+    p["wall_normal"] = normal + [[0.0, 0.0, 0.0]] * pad
+    p["wall_kind"] = kind + [0] * pad
+    p["wall_side"] = side + [-1] * pad
+    p["wall_limit"] = limit + [0] * pad
+    return pbc
+
+
+def box_params(box_min, box_max, pbc=(0, 0, 0), total_time: float = 3e6 * TAU, **kw) -> dict:
+    """The simulation-box protocol parameters (pcg_box) with the defaults of
+    the "simulation box" section of cg/default.yml in internal units;
+    keyword arguments override fields by name."""
+    A = ANGSTROM
+    b = {
+        "box_min": [float(x) for x in box_min], "box_max": [float(x) for x in box_max],
+        "pbc": [int(x) for x in pbc], "total_time": total_time,
+        "rest_period": 10e3 * TAU, "target_vel": 5e-3 * A / TAU,
+        "accel_dist": 12.0 * A, "avg_forces_over": 100.0 * TAU,
+        "squeezing": 0, "target_density": 1e-3 / A ** 3, "vel_above_2V": 2e-2 * A / TAU,
+        "force_min": 0, "force_for_max_vel": 0.4 * EPS / A,
+        "oscillations": 0, "osc_shear": 0, "num_of_cycles": 6, "angular_freq": 1e-4 / TAU,
+        "amp_relative": 0, "amp_abs": 10.0 * A, "amp_rel": 0.1,
+    }
+    for k, v in kw.items():
+        if k not in b:
+            raise KeyError(k)
+        b[k] = v
+    return b
+
+
+def initial_box(pos, kind: str = "sufficient", bond: float = 3.8 * ANGSTROM, pad_x_bond: float = 2.0,
+                max_density: float | None = 1e-4 / ANGSTROM ** 3, cubic: bool = False,
+                squeezing: bool = False, cryst1=None):
+    """state::compile_model's box (cg/src/cg/simul/state.cpp:173-225):
+    "from CRYST1 record" [0, cryst1]; "sufficient": the cube of the maximum
+    density (if given) widened to every residue + pad (pad = pad_x_bond *
+    bond, the mean tether length), as a cube when cubic or squeezing.
+    Returns (box_min, box_max). As in the reference, the non-cubic case starts
+    its residue loop at index 1."""
+    import numpy as np
+    pos = np.asarray(pos, np.float64)
+    n = len(pos)
+    if kind == "from CRYST1 record":
+        return [0.0, 0.0, 0.0], [float(x) for x in cryst1]
+    if kind != "sufficient":
+        raise ValueError(kind)
+    pad = pad_x_bond * bond
+    side = 0.0
+    lo, hi = [1.0, 1.0, 1.0], [-1.0, -1.0, -1.0]
+    if max_density is not None:
+        vol = n / max_density
+        side = float(np.cbrt(vol)) / 2.0
+        lo, hi = [-side] * 3, [side] * 3
+    if cubic or squeezing:
+        for i in range(n):
+            for c in range(3):
+                side = max(side, abs(float(pos[i, c])) + pad)
+        return [-side] * 3, [side] * 3
+    for i in range(1, n):
+        for c in range(3):
+            lo[c] = min(lo[c], float(pos[i, c]) - pad)
+            hi[c] = max(hi[c], float(pos[i, c]) + pad)
+    return lo, hi

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libpcg.so")
 
 TERMS = ["chir", "tether", "lrep", "nat_cont", "pauli", "const_dh", "rel_dh",
          "qa_cand", "qa_cont", "pid", "nat_ang", "heur_ang", "heur_dih", "cnd",
-         "snd", "vel_afm", "force_afm", "solid_wall"]
+         "snd", "vel_afm", "force_afm", "solid_wall", "hw_free", "hw_conn", "ljw_free",
+         "ljw_conn"]
 TERM_ID = {t: i for i, t in enumerate(TERMS)}
 # pcg_report_field order: the scalars of make_report::add_cur_scalars
 REPORT_FIELDS = ["TIME", "EPOT", "ETOT", "EKIN", "RG", "L", "RMSD", "W", "ICN", "ICNss",
@@ -59,7 +60,54 @@ class PcgParams(C.Structure):
         ("base_sync", (I * 4) * 21),
         ("n_walls", I), ("wall_origin", (D * 3) * 6), ("wall_normal", (D * 3) * 6),
         ("wall_depth", D), ("wall_min_dist", D),
+        ("wall_kind", I * 6), ("wall_side", I * 6), ("wall_limit", I * 6),
+        ("hw_on", I), ("ljw_on", I),
+        ("wall_hh1", D), ("wall_lj_depth", D), ("ljw_breaking_dist", D),
+        ("ljw_saturation_diff", D),
     ]
+
+
+class PcgBox(C.Structure):
+    """pcg_box: the sbox parameters of the box protocol (params.box_params)."""
+    _fields_ = [
+        ("box_min", D * 3), ("box_max", D * 3), ("pbc", I * 3), ("total_time", D),
+        ("rest_period", D), ("target_vel", D), ("accel_dist", D), ("avg_forces_over", D),
+        ("squeezing", I), ("target_density", D), ("vel_above_2V", D),
+        ("force_min", I), ("force_for_max_vel", D),
+        ("oscillations", I), ("osc_shear", I), ("num_of_cycles", I), ("angular_freq", D),
+        ("amp_relative", I), ("amp_abs", D), ("amp_rel", D),
+    ]
+
+
+# pcg_box_state layout and phase ids
+BOX_STATE = {"phase": 0, "since": 1, "until": 2, "min": 3, "max": 6, "amplitude": 9,
+             "omega": 10, "displacement": 11, "shear": 12, "avg_z_has": 13, "avg_z": 14,
+             "origin": 15, "normal": 33, "wall_F": 51, "shift": 69, "work": 87,
+             "avg_F_has": 93, "avg_F": 99, "cell": 117}
+BOX_STATE_NUM = 120
+PHASES = ["NONE", "SQUEEZING", "REST_AFTER_SQUEEZING", "FIND_FORCE_MIN",
+          "REST_AFTER_FORCE_MIN", "MAX_AMPLITUDE", "REST_AFTER_MAX_AMP", "OSCILLATIONS",
+          "REST_AFTER_OSCILLATIONS", "FREEFORM"]
+
+
+def box_state_dict(v: np.ndarray) -> dict:
+    """The pcg_box_state array as named fields."""
+    S = BOX_STATE
+    return {
+        "phase": PHASES[int(v[S["phase"]])], "since": v[S["since"]], "until": v[S["until"]],
+        "min": v[S["min"]:S["min"] + 3], "max": v[S["max"]:S["max"] + 3],
+        "amplitude": v[S["amplitude"]], "omega": v[S["omega"]],
+        "displacement": v[S["displacement"]], "shear": bool(v[S["shear"]]),
+        "avg_z": v[S["avg_z"]] if v[S["avg_z_has"]] else None,
+        "origin": v[S["origin"]:S["origin"] + 18].reshape(6, 3),
+        "normal": v[S["normal"]:S["normal"] + 18].reshape(6, 3),
+        "wall_F": v[S["wall_F"]:S["wall_F"] + 18].reshape(6, 3),
+        "shift": v[S["shift"]:S["shift"] + 18].reshape(6, 3),
+        "work": v[S["work"]:S["work"] + 6],
+        "avg_F_has": v[S["avg_F_has"]:S["avg_F_has"] + 6].astype(bool),
+        "avg_F": v[S["avg_F"]:S["avg_F"] + 18].reshape(6, 3),
+        "cell": v[S["cell"]:S["cell"] + 3],
+    }
 
 
 class PcgModel(C.Structure):
@@ -169,6 +217,12 @@ SIGNATURES = {
                            C.POINTER(I), C.POINTER(D)],
     "pcg_morph_saw": [C.POINTER(PcgSawOptions), I, C.POINTER(I), I, C.POINTER(D),
                       C.POINTER(I64)],
+    "pcg_box_setup": [_V, C.POINTER(PcgBox)],
+    "pcg_box_start": [_V],
+    "pcg_box_state": [_V, C.POINTER(D)],
+    "pcg_ljw_process": [_V],
+    "pcg_wall_conns": [_V, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(D),
+                       C.POINTER(D)],
 }
 
 
@@ -425,6 +479,36 @@ class Context:
         parameters, precision and noise source."""
         buf = C.create_string_buffer(ck, len(ck))
         check(self.L.pcg_restore(self.h, buf, len(ck)))
+
+    def box_setup(self, box: dict):
+        """The simulation box and its walls (pcg_box_setup); box from
+        params.box_params."""
+        self._box = fill_struct(PcgBox(), box)
+        check(self.L.pcg_box_setup(self.h, C.byref(self._box)))
+
+    def box_start(self):
+        """Hands the trajectory over to the box protocol (pcg_box_start)."""
+        check(self.L.pcg_box_start(self.h))
+
+    def box_state(self, raw: bool = False):
+        out = np.zeros(BOX_STATE_NUM)
+        check(self.L.pcg_box_state(self.h, _p(out, D)))
+        return out if raw else box_state_dict(out)
+
+    def ljw_process(self):
+        check(self.L.pcg_ljw_process(self.h))
+
+    def wall_conns(self, kind: int) -> dict:
+        """kind 0 harmonic, 1 LJ: residue, plane, bead offset, saturation."""
+        size = C.c_int32()
+        check(self.L.pcg_wall_conns(self.h, kind, 0, C.byref(size), None, None, None, None))
+        k = size.value
+        res, wall = np.zeros(k, np.int32), np.zeros(k, np.int32)
+        off, sat = np.zeros((k, 3)), np.zeros(k)
+        if k:
+            check(self.L.pcg_wall_conns(self.h, kind, k, C.byref(size), _p(res, I), _p(wall, I),
+                                        _p(off, D), _p(sat, D)))
+        return {"res": res, "wall": wall, "off": off, "sat": sat}
 
     def stream(self) -> int:
         """The HIP stream every launch of this context is enqueued on."""

@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(ROOT, "oracle", "build", "liboracle.so")
 
 TERMS = ["chir", "tether", "lrep", "nat_cont", "pauli", "const_dh", "rel_dh",
          "qa_cand", "qa_cont", "pid", "nat_ang", "heur_ang", "heur_dih", "cnd",
-         "snd", "vel_afm", "force_afm", "solid_wall"]
+         "snd", "vel_afm", "force_afm", "solid_wall", "hw_free", "hw_conn", "ljw_free",
+         "ljw_conn"]
 TERM_ID = {t: i for i, t in enumerate(TERMS)}
 
 D = C.c_double
@@ -53,6 +54,22 @@ class OraParams(C.Structure):
         ("base_sync", (I * 4) * 21),
         ("n_walls", I), ("wall_origin", (D * 3) * 6), ("wall_normal", (D * 3) * 6),
         ("wall_depth", D), ("wall_min_dist", D),
+        ("wall_kind", I * 6), ("wall_side", I * 6), ("wall_limit", I * 6),
+        ("hw_on", I), ("ljw_on", I),
+        ("wall_hh1", D), ("wall_lj_depth", D), ("ljw_breaking_dist", D),
+        ("ljw_saturation_diff", D),
+    ]
+
+
+class OraBox(C.Structure):
+    """The sbox parameters of the box protocol (ora_box, the fields of pcg_box)."""
+    _fields_ = [
+        ("box_min", D * 3), ("box_max", D * 3), ("pbc", I * 3), ("total_time", D),
+        ("rest_period", D), ("target_vel", D), ("accel_dist", D), ("avg_forces_over", D),
+        ("squeezing", I), ("target_density", D), ("vel_above_2V", D),
+        ("force_min", I), ("force_for_max_vel", D),
+        ("oscillations", I), ("osc_shear", I), ("num_of_cycles", I), ("angular_freq", D),
+        ("amp_relative", I), ("amp_abs", D), ("amp_rel", D),
     ]
 
 
@@ -154,6 +171,19 @@ def lib():
         L.ora_qa_contacts.argtypes = [P, I, C.POINTER(I), C.POINTER(D), C.POINTER(I),
                                       C.POINTER(D), C.POINTER(C.c_int8),
                                       C.POINTER(C.c_uint8)]
+        L.ora_box_setup.argtypes = [P, C.POINTER(OraBox)]
+        L.ora_box_start.argtypes = [P]
+        L.ora_box_state.argtypes = [P, C.POINTER(D)]
+        L.ora_ljw_process.argtypes = [P]
+        L.ora_wall_conns.restype = I
+        L.ora_wall_conns.argtypes = [P, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(D),
+                                     C.POINTER(D)]
+        L.ora_ljw_slots.restype = I
+        L.ora_ljw_slots.argtypes = [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(D),
+                                    C.POINTER(D), C.POINTER(C.c_uint8)]
+        L.ora_ljw_pending.argtypes = [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(I),
+                                      C.POINTER(D), C.POINTER(I), C.POINTER(I),
+                                      C.POINTER(C.c_uint8), C.POINTER(I)]
         L.ora_rng_new.restype = P
         L.ora_rng_new.argtypes = [I]
         L.ora_rng_free.argtypes = [P]
@@ -377,6 +407,52 @@ class Oracle:
         out = np.zeros(6, np.int32)
         self.L.ora_pid_counts(self.h, _ptr(out, I))
         return out
+
+    def box_setup(self, box: dict):
+        self._box = fill_struct(OraBox(), box)
+        self.L.ora_box_setup(self.h, C.byref(self._box))
+
+    def box_start(self):
+        self.L.ora_box_start(self.h)
+
+    def box_state(self) -> np.ndarray:
+        out = np.zeros(120, np.float64)
+        self.L.ora_box_state(self.h, _ptr(out, D))
+        return out
+
+    def ljw_process(self):
+        self.L.ora_ljw_process(self.h)
+
+    def wall_conns(self, kind: int) -> dict:
+        """kind 0 harmonic, 1 LJ (live slots): residue, plane, offset, saturation."""
+        k = self.L.ora_wall_conns(self.h, kind, 0, None, None, None, None)
+        res, wall = np.zeros(k, np.int32), np.zeros(k, np.int32)
+        off, sat = np.zeros((k, 3), np.float64), np.zeros(k, np.float64)
+        if k:
+            self.L.ora_wall_conns(self.h, kind, k, _ptr(res, I), _ptr(wall, I), _ptr(off, D),
+                                  _ptr(sat, D))
+        return {"res": res, "wall": wall, "off": off, "sat": sat}
+
+    def ljw_slots(self) -> dict:
+        k = self.L.ora_ljw_slots(self.h, 0, None, None, None, None, None)
+        res, wall = np.zeros(k, np.int32), np.zeros(k, np.int32)
+        off, sat, vac = np.zeros((k, 3)), np.zeros(k), np.zeros(k, np.uint8)
+        if k:
+            self.L.ora_ljw_slots(self.h, k, _ptr(res, I), _ptr(wall, I), _ptr(off, D),
+                                 _ptr(sat, D), _ptr(vac, C.c_uint8))
+        return {"res": res, "wall": wall, "off": off, "sat": sat, "vac": vac}
+
+    def ljw_pending(self, cap: int = 100000) -> dict:
+        nc, nr = I(0), I(0)
+        cr, cw, cd = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap)
+        rem = np.zeros(cap, np.int32)
+        conn = np.zeros(self.n, np.uint8)
+        num = np.zeros(6, np.int32)
+        self.L.ora_ljw_pending(self.h, cap, C.byref(nc), _ptr(cr, I), _ptr(cw, I), _ptr(cd, D),
+                               C.byref(nr), _ptr(rem, I), _ptr(conn, C.c_uint8), _ptr(num, I))
+        k, m = nc.value, nr.value
+        return {"cand_res": cr[:k], "cand_wall": cw[:k], "cand_d": cd[:k], "removed": rem[:m],
+                "is_conn": conn, "num_conns": num}
 
 
 class Ran2:

@@ -42,6 +42,10 @@ def lib():
             "ref_vel_afm": [I, PD, D, I, PI, PD, PD, D, D, PD],
             "ref_force_afm": [I, D, I, PI, PD, PD],
             "ref_solid_wall": [I, PD, PP, PD],
+            "ref_hw_free": [I, PD, PP, PU8, PD, PD],
+            "ref_hw_conn": [I, PD, PP, I, PI, PI, PD, PD, PD, PD],
+            "ref_ljw_sift": [I, PD, PP, PU8, PI, PD, PD, I, PI, PI, PD, PI],
+            "ref_ljw_conn": [I, PD, PP, I, PI, PI, PD, PD, PU8, PD, PD, I, PI, PI],
             "ref_qa_round": [I, PD, PD, PU8, PI, PP, D, PI8, PI, PI, PD, PU8, PI, PI, PD,
                              PI, PD, PI8, PU8, PI, I, PD],
         }
@@ -54,6 +58,11 @@ def lib():
         L.ref_fix_nl.restype = None
         L.ref_pid_count.argtypes = [I, PD, PD, PU8, PI, I, PI, PD, PI, PP, PI]
         L.ref_pid_count.restype = None
+        L.ref_ljw_process.argtypes = [I, PD, PP, I, PI, PI, PD, PD, PU8, I, PI, I, PI, PI, PD,
+                                      PU8, PI, I, PI, PI, PI, PD, PD, PU8]
+        L.ref_ljw_process.restype = None
+        L.ref_log_forces.argtypes = [I, PD, PD, PD, PD, PI, I, PI, D, PD, PD, PD, PD, PD]
+        L.ref_log_forces.restype = None
         _lib = L
     return _lib
 
