@@ -209,6 +209,11 @@ int pcg_eval_forces(pcg_ctx *c);                     /* thread::eval_forces */
  * term's forces alone, V its energy. to < 0 means the whole list. */
 int pcg_eval_term(pcg_ctx *c, int32_t term, int32_t from, int32_t to,
                   double *F, double *V);
+/* pcg_eval_term with the forces in three component arrays (see pcg_get3):
+ * for_slice writing a reference dyn.F view directly. Fx, Fy, Fz NULL: energy
+ * only. */
+int pcg_eval_term3(pcg_ctx *c, int32_t term, int32_t from, int32_t to, double *Fx, double *Fy,
+                   double *Fz, int64_t stride, double *V);
 /* total_size(); for PCG_QA_CAND the free pairs whose NL distance is within
  * the QA maximum required distance, the only ones loop_over_candidates
  * evaluates past its first test (qa/loop_over_candidates.cpp:20) */
@@ -220,6 +225,15 @@ int pcg_lang_step(pcg_ctx *c);                       /* lang::step */
  * 7 velocities. */
 int pcg_get(pcg_ctx *c, int32_t which, double *out);
 int pcg_set(pcg_ctx *c, int32_t which, const double *in);
+/* The same with the three components in separate arrays, element i of
+ * component c at comp[c][i * stride]: stride 1 takes the reference's own
+ * vect::vector<vec3r> storage, a structure of arrays of x, y, z
+ * (cg/include/cg/types/vec3_def.h:53 with INST/FIELD), without a transposing
+ * copy (the device keeps the same layout); x, x + 1, x + 2 with stride 3 is
+ * the interleaved layout. */
+int pcg_get3(pcg_ctx *c, int32_t which, double *x, double *y, double *z, int64_t stride);
+int pcg_set3(pcg_ctx *c, int32_t which, const double *x, const double *y, const double *z,
+             int64_t stride);
 int pcg_get_energy(pcg_ctx *c, double *V, double *term_V /*[PCG_NUM_TERMS]*/);
 /* The scalars make_report::add_cur_scalars writes for the trajectory
  * (cg/src/cg/output/make_report.cpp:196-247), of the current state, indexed

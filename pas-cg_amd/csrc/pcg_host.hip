@@ -884,6 +884,50 @@ int get_forces(DeviceBatch &B, int s, double *out) {
   return PCG_OK;
 }
 
+// The same three columns to or from caller arrays of one component each,
+// element i at comp[c][i * stride]; unit stride copies straight between the
+// device columns and the caller's arrays.
+int get_cols3(DeviceBatch &B, int s, const void *const cols[3], bool f32, double *const out[3],
+              int64_t stride) {
+  const SysDesc &d = B.sys[s];
+  const size_t esz = f32 ? 4 : 8;
+  if (stride == 1 && !f32) {
+    for (int c = 0; c < 3; ++c)
+      HIPCHK(hipMemcpyAsync(out[c], (const char *)cols[c] + esz * d.res0, d.n * esz,
+                            hipMemcpyDeviceToHost, B.stream));
+    HIPCHK(hipStreamSynchronize(B.stream));
+    return PCG_OK;
+  }
+  std::vector<char> tmp(3 * esz * d.n);
+  for (int c = 0; c < 3; ++c)
+    HIPCHK(hipMemcpyAsync(tmp.data() + c * esz * d.n, (const char *)cols[c] + esz * d.res0,
+                          esz * d.n, hipMemcpyDeviceToHost, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  for (int c = 0; c < 3; ++c)
+    for (int i = 0; i < d.n; ++i)
+      out[c][(size_t)i * stride] = f32 ? (double)((const float *)tmp.data())[c * d.n + i]
+                                       : ((const double *)tmp.data())[c * d.n + i];
+  return PCG_OK;
+}
+int set_cols3(DeviceBatch &B, int s, double *const cols[3], const double *const in[3],
+              int64_t stride) {
+  const SysDesc &d = B.sys[s];
+  if (stride == 1) {
+    for (int c = 0; c < 3; ++c)
+      HIPCHK(hipMemcpyAsync(cols[c] + d.res0, in[c], d.n * 8, hipMemcpyHostToDevice, B.stream));
+  } else {
+    std::vector<double> tmp(3 * (size_t)d.n);
+    for (int c = 0; c < 3; ++c)
+      for (int i = 0; i < d.n; ++i) tmp[c * d.n + i] = in[c][(size_t)i * stride];
+    for (int c = 0; c < 3; ++c)
+      HIPCHK(hipMemcpyAsync(cols[c] + d.res0, tmp.data() + c * d.n, d.n * 8,
+                            hipMemcpyHostToDevice, B.stream));
+    HIPCHK(hipStreamSynchronize(B.stream));
+  }
+  HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
 // Reference-order noise for RAN2 mode: all x, then all y, then all z per
 // system (cg/src/cg/langevin/step.cpp:119-127).
 int upload_ran2_noise(DeviceBatch &B) {
@@ -1315,7 +1359,7 @@ int pcg_term_size(pcg_ctx *c, int32_t term, int32_t *size) {
   return PCG_OK;
 }
 
-int pcg_eval_term(pcg_ctx *c, int32_t term, int32_t from, int32_t to, double *F, double *V) {
+static int eval_term_only(pcg_ctx *c, int32_t term, int32_t from, int32_t to, double *V) {
   if (!c || term < 0 || term >= NT) return fail(PCG_E_ARG, "bad arguments");
   DeviceBatch &B = *c->B;
   SysState st;
@@ -1329,7 +1373,28 @@ int pcg_eval_term(pcg_ctx *c, int32_t term, int32_t from, int32_t to, double *F,
   HIPCHK(hipStreamSynchronize(B.stream));
   if ((rc = read_state(B, 0, st))) return rc;
   if (V) *V = st.V[term];
-  if (F && (rc = get_forces(B, 0, F))) return rc;
+  return PCG_OK;
+}
+
+int pcg_eval_term(pcg_ctx *c, int32_t term, int32_t from, int32_t to, double *F, double *V) {
+  int rc = eval_term_only(c, term, from, to, V);
+  if (rc) return rc;
+  if (F && (rc = get_forces(*c->B, 0, F))) return rc;
+  return check_overflow(*c->B);
+}
+
+int pcg_eval_term3(pcg_ctx *c, int32_t term, int32_t from, int32_t to, double *Fx, double *Fy,
+                   double *Fz, int64_t stride, double *V) {
+  if ((Fx || Fy || Fz) && (!Fx || !Fy || !Fz || stride < 1))
+    return fail(PCG_E_ARG, "pcg_eval_term3: all three components and a positive stride");
+  int rc = eval_term_only(c, term, from, to, V);
+  if (rc) return rc;
+  DeviceBatch &B = *c->B;
+  if (Fx) {
+    const void *cols[3] = {B.d.F[0], B.d.F[1], B.d.F[2]};
+    double *const out[3] = {Fx, Fy, Fz};
+    if ((rc = get_cols3(B, 0, cols, B.prec != PCG_FP64, out, stride))) return rc;
+  }
   return check_overflow(B);
 }
 
@@ -1357,6 +1422,28 @@ int pcg_get(pcg_ctx *c, int32_t which, double *out) {
   if (which <= 5) return get_cols(B, 0, B.d.y[which], out);
   if (which == 6) return get_forces(B, 0, out);
   return get_cols(B, 0, B.d.vel, out);
+}
+
+int pcg_get3(pcg_ctx *c, int32_t which, double *x, double *y, double *z, int64_t stride) {
+  if (!c || !x || !y || !z || which < 0 || which > 7 || stride < 1)
+    return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  double *const out[3] = {x, y, z};
+  if (which == 6) {
+    const void *cols[3] = {B.d.F[0], B.d.F[1], B.d.F[2]};
+    return get_cols3(B, 0, cols, B.prec != PCG_FP64, out, stride);
+  }
+  double *const *src = which <= 5 ? B.d.y[which] : B.d.vel;
+  const void *cols[3] = {src[0], src[1], src[2]};
+  return get_cols3(B, 0, cols, false, out, stride);
+}
+
+int pcg_set3(pcg_ctx *c, int32_t which, const double *x, const double *y, const double *z,
+             int64_t stride) {
+  if (!c || !x || !y || !z || which < 0 || which > 5 || stride < 1)
+    return fail(PCG_E_ARG, "bad arguments");
+  const double *const in[3] = {x, y, z};
+  return set_cols3(*c->B, 0, c->B->d.y[which], in, stride);
 }
 
 int pcg_set(pcg_ctx *c, int32_t which, const double *in) {

@@ -217,6 +217,10 @@ SIGNATURES = {
                            C.POINTER(I), C.POINTER(D)],
     "pcg_morph_saw": [C.POINTER(PcgSawOptions), I, C.POINTER(I), I, C.POINTER(D),
                       C.POINTER(I64)],
+    "pcg_get3": [_V, I, C.POINTER(D), C.POINTER(D), C.POINTER(D), I64],
+    "pcg_set3": [_V, I, C.POINTER(D), C.POINTER(D), C.POINTER(D), I64],
+    "pcg_eval_term3": [_V, I, I, I, C.POINTER(D), C.POINTER(D), C.POINTER(D), I64,
+                       C.POINTER(D)],
     "pcg_box_setup": [_V, C.POINTER(PcgBox)],
     "pcg_box_start": [_V],
     "pcg_box_state": [_V, C.POINTER(D)],
@@ -401,6 +405,25 @@ class Context:
         out = np.zeros((self.n, 3))
         check(self.L.pcg_get(self.h, which, _p(out, D)))
         return out
+
+    def get3(self, which: int) -> np.ndarray:
+        """pcg_get3 into a structure of arrays: [3, n] (x, y, z rows)."""
+        out = np.zeros((3, self.n))
+        check(self.L.pcg_get3(self.h, which, _p(out[0], D), _p(out[1], D), _p(out[2], D), 1))
+        return out
+
+    def set3(self, which: int, soa):
+        a = np.ascontiguousarray(np.asarray(soa, np.float64))
+        check(self.L.pcg_set3(self.h, which, _p(a[0], D), _p(a[1], D), _p(a[2], D), 1))
+
+    def eval_term3(self, term, frm=0, to=-1):
+        """pcg_eval_term3: the term's forces as [3, n] and its energy."""
+        F = np.zeros((3, self.n))
+        V = C.c_double()
+        tid = TERM_ID[term] if isinstance(term, str) else int(term)
+        check(self.L.pcg_eval_term3(self.h, tid, frm, to, _p(F[0], D), _p(F[1], D), _p(F[2], D),
+                                    1, C.byref(V)))
+        return F, V.value
 
     def r(self):
         return self.get(0)

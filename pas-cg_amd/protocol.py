@@ -10,7 +10,14 @@ library accumulates it.
 
 The context must have AFM enabled in its parameters (afm_on) and its model
 must reserve two tip slots (pcg_model n_tips + n_ftips >= 2), for example
-with model.add_pulling. Walls and squeezing are not part of this module.
+with model.add_pulling.
+
+Without AFM after equilibration the reference hands the trajectory to the
+simulation-box protocol (squeezing, force-minimum search, oscillations,
+thread_main.cpp:130-135 and 144-330). Here that protocol runs on the device
+inside every step once pcg_box_start has been called (the box itself comes
+from Context.box_setup before the trajectory starts); the host only counts
+steps to the total time.
 
 Deviation: after pull-then-release and equilibration the reference turns
 AFM back on and appends a resting tip to the two old ones (setup_afm only
@@ -54,14 +61,20 @@ def steps_until(t: float, until: float, dt: float) -> int:
 
 def run_phases(ctx, chain, dt: float, total_time: float, equil_time: float = 0.0,
                afm: str | None = None, magnitude: float = 0.0,
-               pull_release_time: float | None = None, pull_speed: float = 0.0) -> list:
+               pull_release_time: float | None = None, pull_speed: float = 0.0,
+               box: bool = False) -> list:
     """Runs one trajectory through the reference's phases (traj_init_step,
     pull_release_step, equil_step): returns [(phase, steps)].
 
     afm: None (equilibrate, then run freely), "const velocity" or "const
     force" (equilibrate, then pull with tips placed at that moment), or
     "pull, then release" (pull at pull_speed for pull_release_time, then
-    equilibrate without AFM, then run freely)."""
+    equilibrate without AFM, then run freely).
+
+    box: the context has a simulation box (Context.box_setup); without AFM
+    pulling the run after equilibration is the box protocol ("BOX": the
+    device's phases from SQUEEZING to FREEFORM, Context.box_state tells
+    which one it is in)."""
     phases = []
 
     def advance(name, until):
@@ -82,5 +95,9 @@ def run_phases(ctx, chain, dt: float, total_time: float, equil_time: float = 0.0
         advance("EQUIL", min(total_time, equil_time))
     if afm in ("const velocity", "const force"):
         ctx.set_tips(setup_afm_tips(ctx.r(), chain, afm, magnitude))
+    elif box:
+        ctx.box_start()
+        advance("BOX", total_time)
+        return phases
     advance("FREEFORM", total_time)
     return phases

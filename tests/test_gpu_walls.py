@@ -188,9 +188,13 @@ def test_box_protocol_trajectory(variant, graph):
 
 
 def test_lj_walls_under_squeezing():
+    """LJ walls closing in: connections form and saturate slowly (a long
+    cycle: with the reference's sign a saturated connection pulls its residue
+    onto the bead, which then blows up, so a lockstep comparison stays short
+    of saturation)."""
     m = protein(60, seed=14)
     prm, pbc = walled({"x": "lj", "y": "solid", "z": "lj"}, m["n"], limits={"lj": 4},
-                      cycle_dur=0.05, ljw_on=1)
+                      cycle_dur=5.0, ljw_on=1)
     dt = prm["dt"]
     box = protocol_box(m, pbc, dt, force_min=0, oscillations=0)
     o, c = pair(prm, m, box)
@@ -230,7 +234,7 @@ def test_harmonic_walls_in_the_step():
 def test_checkpoint_mid_protocol_resumes():
     m = protein(60, seed=12)
     prm, pbc = walled({"x": "solid", "y": "lj", "z": "solid"}, m["n"], limits={"lj": 3},
-                      cycle_dur=0.05, ljw_on=1)
+                      cycle_dur=5.0, ljw_on=1)
     dt = prm["dt"]
     box = protocol_box(m, pbc, dt)
     o, c = pair(prm, m, box, graph=True)
@@ -246,7 +250,27 @@ def test_checkpoint_mid_protocol_resumes():
         o.step(50)
         c.step(50)
         c2.step(50)
-        assert np.array_equal(c2.r(), c.r())
-        np.testing.assert_array_equal(c2.box_state(raw=True), c.box_state(raw=True))
+        # the plain path sums forces with atomics: equal to rounding, not bits
+        assert np.isfinite(c.r()).all() and np.abs(c2.r() - c.r()).max() < 1e-10
+        box_close(c.box_state(raw=True), c2.box_state(raw=True))
         assert np.abs(c.r() - o.r()).max() < 1e-8
         box_close(o.box_state(), c.box_state(raw=True))
+
+
+def test_run_phases_hands_over_to_the_box():
+    """protocol.run_phases without AFM: equilibration, then the box
+    protocol on the device to the total time, as the oracle does it."""
+    from pas_cg_amd import protocol as X
+    m = protein(60, seed=12)
+    prm, pbc = walled({"x": "solid", "y": "solid", "z": "solid"}, m["n"])
+    dt = prm["dt"]
+    o, c = pair(prm, m, protocol_box(m, pbc, dt))
+    ph = X.run_phases(c, m["chain"], dt, 300 * dt, equil_time=40 * dt, box=True)
+    assert [p for p, _ in ph] == ["EQUIL", "BOX"]
+    o.step(ph[0][1])
+    o.box_start()
+    o.step(ph[1][1])
+    assert ph[0][1] + ph[1][1] == X.steps_until(0.0, 300 * dt, dt)
+    assert np.abs(c.r() - o.r()).max() < 1e-8
+    box_close(o.box_state(), c.box_state(raw=True))
+    assert int(o.box_state()[0]) >= 5
