@@ -195,12 +195,16 @@ def batched_force_error(p, models, bat, picks):
     return max(float(np.abs(a - c).max() / np.abs(c).max()) for a, c in zip(F[G.MIXED], F[G.FP64]))
 
 
-def plain(p, models, picks, steps=200):
+def plain(p, models, picks, steps=200, graph=True, form=None):
+    """The drop-in path: one protein per plain context through pcg_step, in
+    the context's own choice of form (the persistent single-system kernel for
+    these proteins) or in the stage kernels (form G.FORM_STAGES)."""
     from pas_cg_amd import pcg as G
     rs, tt, per = 0, 0.0, []
     for s in picks:
         c = G.Context(p, models[s], precision=G.MIXED, noise=G.PHILOX, seed=s)
-        c.use_graph(True)
+        c.use_graph(graph)
+        c.step_form(G.FORM_AUTO if form is None else form)
         c.step(20)
         t0 = time.perf_counter()
         c.step(steps)
@@ -214,7 +218,7 @@ def plain(p, models, picks, steps=200):
     return rs / tt, per
 
 
-def other_configs(c2, c1):
+def other_configs(c2, c1, graph=True):
     """The other configurations, briefly: 1024 replicas of the 56-residue
     chain in one batch, and the 6400-residue periodic box through the plain
     (stage-kernel) path."""
@@ -236,7 +240,7 @@ def other_configs(c2, c1):
     b.close()
     p, m = c1
     c = G.Context(p, m, precision=G.MIXED, noise=G.PHILOX, seed=1)
-    c.use_graph(True)
+    c.use_graph(graph)
     c.step(50)
     t0 = time.perf_counter()
     c.step(300)
@@ -505,7 +509,12 @@ def main():
     ap.add_argument("--no-full", action="store_true",
                     help="skip the configured 5e4-step run when --steps is shorter")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-stages", action="store_true",
+                    help="plain contexts launch each step's stage kernels without a hipGraph "
+                         "(for kernel tracers)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-backlog-only", action="store_true",
+                    help="of the served legs only the saturated backlog (for counter profiles)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-other", action="store_true",
@@ -623,17 +632,28 @@ def main():
         chk["b"].close()
     per_plain = None
     if not args.no_plain:
-        prate, per_plain = plain(p, models, [int(i) for i in order[8::16]])
+        from pas_cg_amd import pcg as G
+        picks = [int(i) for i in order[8::16]]
+        prate, per_plain = plain(p, models, picks, graph=not args.plain_stages)
+        srate, _ = plain(p, models, picks, graph=not args.plain_stages, form=G.FORM_STAGES)
         out["plain"] = {"value": round(prate, 1), "unit": "residue-steps/s",
-                        "sample": "16 proteins (every 16th by length), one at a time, "
-                                  "hipGraph-captured step, 200 steps each"}
+                        "sample": "16 proteins (every 16th by length), one plain context at a "
+                                  "time, 200 steps each through pcg_step in its own form: the "
+                                  "persistent kernel for one system (one workgroup)",
+                        "stage_kernels": round(srate, 1),
+                        "stage_kernels_sample": "the same, forced to the stage kernels (one "
+                                                "launch per stage, " +
+                                                ("plain launches" if args.plain_stages else
+                                                 "hipGraph-captured step") + ")"}
         out["speedup_vs_plain"] = round(value / prate, 2)
+        out["speedup_vs_plain_stages"] = round(value / srate, 2)
     if not args.no_serve:
         sp = W.full_params(args.pad)
-        out["served"] = served(sp, value, pool, per_plain, args.serve_window, args.serve_chunk)
+        out["served"] = ({} if args.serve_backlog_only else
+                         served(sp, value, pool, per_plain, args.serve_window, args.serve_chunk))
         out["served"]["backlog"] = served_backlog(sp, pool, chunk=args.serve_chunk)
     if extra is not None:
-        out["other_configs"] = other_configs(*extra)
+        out["other_configs"] = other_configs(*extra, graph=not args.plain_stages)
     if not args.no_cpu:
         k = cpu_ex._max_workers
         spread = [int(order[(2 * j + 1) * len(order) // (2 * k)]) for j in range(k)]

@@ -280,6 +280,20 @@ int pcg_stream(pcg_ctx *c, void **stream);
 /* Capture one full step as a hipGraph and replay it for later pcg_step
  * calls (steps without a NL rebuild); 0 turns it off. */
 int pcg_use_graph(pcg_ctx *c, int32_t on);
+/* How pcg_step advances: PCG_FORM_STAGES, one launch per stage of the
+ * reference's step (what the per-call entry points above run); or
+ * PCG_FORM_PERSISTENT, the batched form's persistent kernel for this one
+ * system (one workgroup, many steps a launch, positions, forces and lists
+ * in LDS), possible when the system fits LDS, the noise is Philox and there
+ * are no walls or box (PCG_E_STATE otherwise). PCG_FORM_AUTO (the default)
+ * takes the persistent form whenever it is possible. The first step in one
+ * form after steps in the other rebuilds the neighbour list, and so does the
+ * first nl_verify after persistent steps; per-term calls right after
+ * persistent steps see the lists of the persistent kernel's last rebuild,
+ * which keeps every per-term list but not the NL itself. *active receives
+ * the form the next pcg_step takes. No counterpart in the reference. */
+enum { PCG_FORM_AUTO = 0, PCG_FORM_STAGES = 1, PCG_FORM_PERSISTENT = 2 };
+int pcg_step_form(pcg_ctx *c, int32_t form, int32_t *active);
 /* Checkpoint and resume of the context (the fields of the reference's state
  * serialisation, cg/src/cg/simul/state_io.cpp:117-156: positions, time,
  * velocities, Gear y0..y5, QA contacts, sync budgets; here with the lists,

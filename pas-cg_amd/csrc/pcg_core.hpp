@@ -95,6 +95,7 @@ struct SysState {
   // rebuild) and counts the rest in qf_n
   int qf_store;
   int n_hw, n_lj;             // harmonic wall connections, LJ wall connection slots
+  int form;                   // plain context: the form (stages / persistent) whose lists these are
   int64_t step;
   double t, true_t, orig_pad;
   double orig_cell[3];

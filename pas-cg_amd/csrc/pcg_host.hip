@@ -1134,15 +1134,58 @@ int check_overflow(DeviceBatch &B) {
   return PCG_OK;
 }
 
+size_t lds_for(const DeviceBatch &B, bool staged) {
+  return LdsLayout((size_t)B.max_n, sizeof(double), staged, false, (size_t)B.max_nnc).total;  // FP64 LDS forces
+}
+
+// Launch configuration of the persistent kernel for a batch: per-residue
+// bonded data joins positions and forces in LDS when it fits, the Gear
+// derivatives too when they also fit; systems whose positions and forces do
+// not fit take the stage kernels (ok false). The default workgroup is the
+// smallest power of two of at least twice the largest system, in [128, 512].
+struct PersistentCfg {
+  bool ok;
+  int block;
+  size_t lds;
+};
+PersistentCfg persistent_cfg(DeviceBatch &B) {
+  bool staged = lds_for(B, true) <= LDS_DYN_MAX;
+  bool gear = staged && LdsLayout((size_t)B.max_n, sizeof(double), true, true,
+                                  (size_t)B.max_nnc).total <= LDS_DYN_MAX;
+  PersistentCfg c;
+  c.lds = LdsLayout((size_t)B.max_n, sizeof(double), staged, gear, (size_t)B.max_nnc).total;
+  B.d.bond_lds = staged ? 1 : 0;
+  B.d.gear_lds = gear ? 1 : 0;
+  c.ok = c.lds <= LDS_DYN_MAX;
+  int want = 64;
+  while (want < 1024 && want < B.max_n * 2) want *= 2;
+  c.block = std::min(std::max(want, 128), 512);
+  return c;
+}
+
 } // namespace
 
 // ------------------------------------------------------------------ ctx ----
+// A plain context steps in one of two forms: the stage kernels (one launch
+// per stage of the reference's step, optionally replayed as a hipGraph), or
+// the persistent kernel of the batched form for this one system (one
+// workgroup, many steps a launch, everything in LDS). The persistent form is
+// taken when the system fits its LDS, the noise is Philox (the reference's
+// ran2 noise is drawn on the host every step) and there are no walls or box
+// protocol (which the stage path carries); a switch between the forms makes
+// the next step rebuild the neighbour list, so that each form works on lists
+// it built itself.
+enum { FORM_NONE = 0, FORM_STAGES = 1, FORM_PERSISTENT = 2 };
 struct pcg_ctx {
   std::unique_ptr<DeviceBatch> B;
   bool graph_on = false;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int64_t steps_since_check = 0;
+  int form = PCG_FORM_AUTO;  // requested
+  int last_form = FORM_NONE;
+  PersistentCfg pcfg{false, 0, 0};
+  int spl = 500;             // persistent form: steps per launch
   ~pcg_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1184,6 +1227,31 @@ int refresh_sys(DeviceBatch &B) {
   HIPCHK(hipMemcpyAsync(B.sys.data(), (const void *)B.d.sys, B.sys.size() * sizeof(SysDesc),
                         hipMemcpyDeviceToHost, B.stream));
   HIPCHK(hipStreamSynchronize(B.stream));
+  return PCG_OK;
+}
+
+bool persistent_eligible(const pcg_ctx &c) {
+  const DeviceBatch &B = *c.B;
+  return c.pcfg.ok && B.noise == PCG_NOISE_PHILOX && B.hp.p.n_walls == 0 && !B.box_set;
+}
+bool use_persistent(const pcg_ctx &c) {
+  return c.form == PCG_FORM_PERSISTENT || (c.form == PCG_FORM_AUTO && persistent_eligible(c));
+}
+// the first step in a form after steps in the other rebuilds the lists (the
+// device state records the form, so a restored checkpoint carries it)
+int switch_form(pcg_ctx &c, int to) {
+  DeviceBatch &B = *c.B;
+  if (c.last_form == to) return PCG_OK;
+  SysState st;
+  int rc = read_state(B, 0, st);
+  if (rc) return rc;
+  if (st.form != FORM_NONE && st.form != to) {
+    st.verify_first = 1;
+    st.nl_invalid = 1;
+  }
+  st.form = to;
+  if ((rc = write_state(B, 0, st))) return rc;
+  c.last_form = to;
   return PCG_OK;
 }
 
@@ -1229,6 +1297,15 @@ int pcg_create(const pcg_params *p, const pcg_model *m, const pcg_options *o, pc
                       nullptr, nullptr, opt.ran2_skip, nullptr, 0, true);
   if (rc) return rc;
   c->B->noise = opt.noise;
+  {
+    DeviceBatch &B = *c->B;
+    c->pcfg = persistent_cfg(B);
+    // the persistent kernel stops a system at its step target: never here
+    B.sys[0].steps_target = INT64_MAX;
+    HIPCHK(hipMemcpyAsync((void *)B.d.sys, B.sys.data(), sizeof(SysDesc), hipMemcpyHostToDevice,
+                          B.stream));
+    HIPCHK(hipStreamSynchronize(B.stream));
+  }
   if (opt.noise == PCG_NOISE_RAN2) {
     // the noise stream continues the one the initial velocities came from
     Ran2 g(seed);
@@ -1250,8 +1327,18 @@ int pcg_step(pcg_ctx *c, int32_t nsteps) {
   if (!c || nsteps < 0) return fail(PCG_E_ARG, "bad arguments");
   DeviceBatch &B = *c->B;
   HIPCHK(hipSetDevice(B.device));
+  int rc;
+  if (use_persistent(*c)) {
+    if ((rc = switch_form(*c, FORM_PERSISTENT))) return rc;
+    for (int done = 0; done < nsteps; done += c->spl)
+      launch_persistent(B.d, B.prec, c->pcfg.block, std::min(c->spl, nsteps - done),
+                        c->pcfg.lds, B.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(B.stream));
+    return check_overflow(B);
+  }
+  if ((rc = switch_form(*c, FORM_STAGES))) return rc;
   for (int k = 0; k < nsteps; ++k) {
-    int rc;
     if (B.noise == PCG_NOISE_RAN2 && (rc = upload_ran2_noise(B))) return rc;
     if (c->graph_on) {
       if (!c->gexec) {
@@ -1275,12 +1362,25 @@ int pcg_use_graph(pcg_ctx *c, int32_t on) {
   return PCG_OK;
 }
 
+int pcg_step_form(pcg_ctx *c, int32_t form, int32_t *active) {
+  if (!c || form < PCG_FORM_AUTO || form > PCG_FORM_PERSISTENT)
+    return fail(PCG_E_ARG, "bad arguments");
+  if (form == PCG_FORM_PERSISTENT && !persistent_eligible(*c))
+    return fail(PCG_E_STATE, "pcg_step_form: the persistent form needs a system that fits its "
+                             "LDS, Philox noise and no walls or box");
+  c->form = form;
+  if (active) *active = use_persistent(*c) ? PCG_FORM_PERSISTENT : PCG_FORM_STAGES;
+  return PCG_OK;
+}
+
 int pcg_nl_verify(pcg_ctx *c, int32_t *invalid) {
   if (!c || !invalid) return fail(PCG_E_ARG, "null argument");
   DeviceBatch &B = *c->B;
+  int rc = switch_form(*c, FORM_STAGES);
+  if (rc) return rc;
   launch_begin(B.d, B.stream);
   SysState st;
-  int rc = read_state(B, 0, st);
+  rc = read_state(B, 0, st);
   if (rc) return rc;
   *invalid = st.nl_invalid;
   return PCG_OK;
@@ -1290,7 +1390,8 @@ int pcg_fix_nl(pcg_ctx *c) {
   if (!c) return fail(PCG_E_ARG, "null ctx");
   DeviceBatch &B = *c->B;
   SysState st;
-  int rc = read_state(B, 0, st);
+  int rc = switch_form(*c, FORM_STAGES);
+  if (rc || (rc = read_state(B, 0, st))) return rc;
   if (rc) return rc;
   st.nl_invalid = 1;
   st.verify_first = 0;
@@ -1302,6 +1403,8 @@ int pcg_fix_nl(pcg_ctx *c) {
 
 int pcg_prepare_nh(pcg_ctx *c) {
   if (!c) return fail(PCG_E_ARG, "null ctx");
+  int rc = switch_form(*c, FORM_STAGES);
+  if (rc) return rc;
   launch_prepare_nh(c->B->d, c->B->stream);
   HIPCHK(hipStreamSynchronize(c->B->stream));
   return PCG_OK;
@@ -1311,7 +1414,8 @@ int pcg_eval_forces(pcg_ctx *c) {
   if (!c) return fail(PCG_E_ARG, "null ctx");
   DeviceBatch &B = *c->B;
   SysState st;
-  int rc = read_state(B, 0, st);
+  int rc = switch_form(*c, FORM_STAGES);
+  if (rc || (rc = read_state(B, 0, st))) return rc;
   if (rc) return rc;
   for (int k = 0; k < NT; ++k) st.V[k] = 0.0;
   if ((rc = write_state(B, 0, st))) return rc;
@@ -1363,7 +1467,8 @@ static int eval_term_only(pcg_ctx *c, int32_t term, int32_t from, int32_t to, do
   if (!c || term < 0 || term >= NT) return fail(PCG_E_ARG, "bad arguments");
   DeviceBatch &B = *c->B;
   SysState st;
-  int rc = read_state(B, 0, st);
+  int rc = switch_form(*c, FORM_STAGES);
+  if (rc || (rc = read_state(B, 0, st))) return rc;
   if (rc) return rc;
   for (int k = 0; k < NT; ++k) st.V[k] = 0.0;
   if ((rc = write_state(B, 0, st))) return rc;
@@ -1400,6 +1505,8 @@ int pcg_eval_term3(pcg_ctx *c, int32_t term, int32_t from, int32_t to, double *F
 
 int pcg_qa_finish(pcg_ctx *c) {
   if (!c) return fail(PCG_E_ARG, "null ctx");
+  int rc = switch_form(*c, FORM_STAGES);
+  if (rc) return rc;
   launch_qa_finish(c->B->d, c->B->stream);
   HIPCHK(hipStreamSynchronize(c->B->stream));
   return check_overflow(*c->B);
@@ -1408,7 +1515,8 @@ int pcg_qa_finish(pcg_ctx *c) {
 int pcg_lang_step(pcg_ctx *c) {
   if (!c) return fail(PCG_E_ARG, "null ctx");
   DeviceBatch &B = *c->B;
-  int rc;
+  int rc = switch_form(*c, FORM_STAGES);
+  if (rc) return rc;
   if (B.noise == PCG_NOISE_RAN2 && (rc = upload_ran2_noise(B))) return rc;
   launch_lang(B.d, B.prec, B.noise, B.stream);
   launch_tick(B.d, 1, B.stream);
@@ -1651,10 +1759,6 @@ struct pcg_batch {
 
 namespace {
 
-size_t lds_for(const DeviceBatch &B, bool staged) {
-  return LdsLayout((size_t)B.max_n, sizeof(double), staged, false, (size_t)B.max_nnc).total;  // FP64 LDS forces
-}
-
 // steps owed by every system = current + nsteps
 // The batch's systems never stop on their own (steps_target is INT64_MAX), so
 // a run is a fixed sequence of launches: captured once per (nsteps, block,
@@ -1726,22 +1830,13 @@ int pcg_batch_create(const pcg_params *p, int32_t nsys, const pcg_model *models,
   DeviceBatch &B = *b->B;
   // per-residue bonded data joins positions and forces in LDS when it fits;
   // systems whose positions and forces do not fit take the stage kernels
-  bool staged = lds_for(B, true) <= LDS_DYN_MAX;
-  bool gear = staged && LdsLayout((size_t)B.max_n, sizeof(double), true, true, (size_t)B.max_nnc).total <=
-                            LDS_DYN_MAX;
-  b->lds = LdsLayout((size_t)B.max_n, sizeof(double), staged, gear, (size_t)B.max_nnc).total;
-  B.d.bond_lds = staged ? 1 : 0;
-  B.d.gear_lds = gear ? 1 : 0;
-  b->persistent = b->lds <= LDS_DYN_MAX;
+  const PersistentCfg pc = persistent_cfg(B);
+  b->lds = pc.lds;
+  b->persistent = pc.ok;
   if (opt.block != 0 && (opt.block < 64 || opt.block > 1024 || opt.block % 64 != 0))
     return fail(PCG_E_ARG, "block must be a multiple of 64 in [64, 1024]");
   if (opt.steps_per_launch < 0) return fail(PCG_E_ARG, "steps_per_launch must be >= 0");
-  b->block = opt.block;
-  if (b->block <= 0) {
-    int want = 64;
-    while (want < 1024 && want < B.max_n * 2) want *= 2;
-    b->block = std::min(std::max(want, 128), 512);
-  }
+  b->block = opt.block > 0 ? opt.block : pc.block;
   b->spl = opt.steps_per_launch > 0 ? opt.steps_per_launch : 100;
   b->use_graph = opt.use_graph != 0;
   // batch systems run until told otherwise: each launch makes its steps
@@ -1954,6 +2049,7 @@ int pcg_restore(pcg_ctx *c, const void *buf, int64_t bytes) {
   if (!c) return fail(PCG_E_ARG, "null argument");
   DeviceBatch &B = *c->B;
   int rc = ckpt_read(B, "PCGCKPTC", buf, bytes);
+  c->last_form = FORM_NONE;  // the restored state says which form built its lists
   if (rc || !B.dW) return rc;
   // the wall state came with the arena; its array pointers are this context's
   WallDev W;

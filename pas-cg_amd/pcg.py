@@ -25,6 +25,7 @@ REPORT_FIELDS = ["TIME", "EPOT", "ETOT", "EKIN", "RG", "L", "RMSD", "W", "ICN", 
                  "INTEHC", "INTRHC", "PID-INTRHC", "PID-B1-B1", "PID-B1-S1", "PID-INTEHC",
                  "PID-B1-B2", "PID-S1-S2"]
 MIXED, FP64 = 0, 1
+FORM_AUTO, FORM_STAGES, FORM_PERSISTENT = 0, 1, 2
 PHILOX, RAN2 = 0, 1
 
 D, I, I64, U8 = C.c_double, C.c_int32, C.c_int64, C.c_uint8
@@ -217,6 +218,7 @@ SIGNATURES = {
                            C.POINTER(I), C.POINTER(D)],
     "pcg_morph_saw": [C.POINTER(PcgSawOptions), I, C.POINTER(I), I, C.POINTER(D),
                       C.POINTER(I64)],
+    "pcg_step_form": [_V, I, C.POINTER(I)],
     "pcg_get3": [_V, I, C.POINTER(D), C.POINTER(D), C.POINTER(D), I64],
     "pcg_set3": [_V, I, C.POINTER(D), C.POINTER(D), C.POINTER(D), I64],
     "pcg_eval_term3": [_V, I, I, I, C.POINTER(D), C.POINTER(D), C.POINTER(D), I64,
@@ -368,6 +370,13 @@ class Context:
 
     def use_graph(self, on=True):
         check(self.L.pcg_use_graph(self.h, int(on)))
+
+    def step_form(self, form: int = FORM_AUTO) -> int:
+        """pcg_step_form: FORM_AUTO, FORM_STAGES or FORM_PERSISTENT; returns
+        the form the next step takes."""
+        v = C.c_int32()
+        check(self.L.pcg_step_form(self.h, form, C.byref(v)))
+        return v.value
 
     def nl_verify(self) -> bool:
         v = C.c_int32()
