@@ -15,6 +15,8 @@
 namespace pcg {
 
 constexpr int NT = PCG_NUM_TERMS;
+// partition classes of an NL pair in the stage kernels (per-chunk counts)
+constexpr int NCL = 6;
 // Static bonded lists whose items are chain-local: item of base residue r
 // spans r, r+1 (tether), r, r+2 (local repulsion), r..r+2 (angles) or
 // r..r+3 (dihedrals, chiral quads). The batched kernel evaluates them per

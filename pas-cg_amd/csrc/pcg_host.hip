@@ -605,7 +605,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&D.qc_sd, 8 * qc);
   A.add(&D.qc_vac, qc);
   A.add(&D.rem, qc);
-  A.add(&D.blk_cnt, (nl / BLK + 1) * 5);
+  A.add(&D.blk_cnt, (nl / BLK + 1) * NCL);
   // walls of a plain context: harmonic connections (state::setup_walls,
   // state.cpp:653-670, from the starting positions), LJ slots and candidates
   struct HwConn {

@@ -159,19 +159,19 @@ __global__ void k_nl_snapshot(DevBatch b) {
 // Pauli takes last (pauli/update_pairs.cpp:4-20). Each NL pair's destination
 // depends only on the pair, so the partition is one classification pass plus
 // an order-preserving compaction per list.
-__device__ __forceinline__ void block_counts5(int cls, int valid, int counts[5], int ranks[5]) {
+__device__ __forceinline__ void block_counts6(int cls, int valid, int counts[NCL], int ranks[NCL]) {
   // ordered rank of this thread within the block for each class, and totals
-  __shared__ int wc[BLK / 64][5];
+  __shared__ int wc[BLK / 64][NCL];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   unsigned long long lt = (1ull << lane) - 1ull;
-  int mine[5];
-  for (int c = 0; c < 5; ++c) {
+  int mine[NCL];
+  for (int c = 0; c < NCL; ++c) {
     unsigned long long m = __ballot(valid && (cls & (1 << c)));
     mine[c] = __popcll(m & lt);
     if (lane == 0) wc[w][c] = __popcll(m);
   }
   __syncthreads();
-  for (int c = 0; c < 5; ++c) {
+  for (int c = 0; c < NCL; ++c) {
     int off = 0, tot = 0;
     for (int k = 0; k < nw; ++k) {
       if (k < w) off += wc[k][c];
@@ -195,11 +195,11 @@ __global__ void k_part_count(DevBatch b) {
   int valid = k < st.nl_n;
   int cls = valid ? classify(b, sd, st, sd.nl0 + k) : 0;
   if (valid) b.nl_cls[sd.nl0 + k] = (uint8_t)cls;
-  int counts[5], ranks[5];
-  block_counts5(cls, valid, counts, ranks);
+  int counts[NCL], ranks[NCL];
+  block_counts6(cls, valid, counts, ranks);
   if (threadIdx.x == 0) {
-    int base = (sd.nl0 / BLK + blockIdx.x) * 5;
-    for (int c = 0; c < 5; ++c) b.blk_cnt[base + c] = counts[c];
+    int base = (sd.nl0 / BLK + blockIdx.x) * NCL;
+    for (int c = 0; c < NCL; ++c) b.blk_cnt[base + c] = counts[c];
   }
 }
 
@@ -210,10 +210,10 @@ __global__ void k_part_scan(DevBatch b) {
   if (!st.active || !st.nl_invalid) return;
   SysDescC &sd = b.sys[s];
   int nchunks = (st.nl_n + BLK - 1) / BLK;
-  int tot[5] = {0, 0, 0, 0, 0};
+  int tot[NCL] = {0, 0, 0, 0, 0, 0};
   for (int ch = 0; ch < nchunks; ++ch) {
-    int base = (sd.nl0 / BLK + ch) * 5;
-    for (int c = 0; c < 5; ++c) {
+    int base = (sd.nl0 / BLK + ch) * NCL;
+    for (int c = 0; c < NCL; ++c) {
       int v = b.blk_cnt[base + c];
       b.blk_cnt[base + c] = tot[c];
       tot[c] += v;
@@ -226,6 +226,7 @@ __global__ void k_part_scan(DevBatch b) {
   st.pid_n = tot[2];
   st.dh_n = tot[3];
   st.pauli_n = tot[4];
+  st.qh_n = tot[5] < sd.qfcap ? tot[5] : sd.qfcap;
 }
 
 __global__ void k_part_write(DevBatch b) {
@@ -239,10 +240,10 @@ __global__ void k_part_write(DevBatch b) {
   int valid = k < st.nl_n;
   int slot = sd.nl0 + k;
   int cls = valid ? b.nl_cls[slot] : 0;
-  int counts[5], ranks[5];
-  block_counts5(cls, valid, counts, ranks);
+  int counts[NCL], ranks[NCL];
+  block_counts6(cls, valid, counts, ranks);
   if (!valid || !cls) return;
-  int base = (sd.nl0 / BLK + blockIdx.x) * 5;
+  int base = (sd.nl0 / BLK + blockIdx.x) * NCL;
   int i1 = b.nl_i[0][slot], i2 = b.nl_i[1][slot];
   double d = b.nl_d[slot];
   ParamsC &p = b.P->p;
@@ -258,6 +259,17 @@ __global__ void k_part_write(DevBatch b) {
       b.qf_i[1][q] = i2;
       b.qf_d[q] = d;
       b.qf_vac[q] = 0;
+      if (cls & CL_NEAR) {
+        // near list entry: the free pair in slot pos (its order is the
+        // free pairs' order)
+        const int h = sd.qf0 + b.blk_cnt[base + 5] + ranks[5];
+        b.qh_slot[h] = pos;
+        b.qh_i[0][h] = i1;
+        b.qh_i[1][h] = i2;
+        b.qh_vac[h] = 0;
+        qa_near_filters(b, i1, i2, b.qh_req[h], b.qh_thr2[h]);
+        b.qh_ij[h] = (uint32_t)(i1 - sd.res0) | ((uint32_t)(i2 - sd.res0) << 16);
+      }
     }
   }
   if (cls & CL_PID) {
@@ -278,14 +290,6 @@ __global__ void k_part_write(DevBatch b) {
     b.pauli_i[0][q] = i1;
     b.pauli_i[1][q] = i2;
   }
-}
-
-// QA free pairs within the maximum required distance (one block per system).
-__global__ void k_qa_near(DevBatch b) {
-  int s = blockIdx.x;
-  SysState &st = b.st[s];
-  if (!st.active || !st.nl_invalid || !b.P->p.qa_on) return;
-  build_qa_near(b, b.sys[s], st);
 }
 
 // The rebuild is complete: record it and clear nl_invalid.
@@ -334,26 +338,44 @@ __global__ void k_qa_finish(DevBatch b) {
   qa_finish_block(b, b.sys[s], st, b.sync, 0);
 }
 
+// The displacement maximum goes to the system's state with one atomic per
+// wave when the wave's residues belong to one system (a system of the plain
+// context has thousands of residues: one atomic per residue on one address
+// serialised the whole kernel), else one per residue.
 template <typename RF> __global__ void k_lang(DevBatch b, int noise_mode) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.ntot) return;
-  int s = b.sysid[i];
-  SysState &st = b.st[s];
-  if (!st.active || !b.P->p.lang_on) return;
-  SysDescC &sd = b.sys[s];
-  double nz[3];
-  if (noise_mode == PCG_NOISE_RAN2) {
-    nz[0] = b.noise[0][i];
-    nz[1] = b.noise[1][i];
-    nz[2] = b.noise[2][i];
-  } else {
-    philox_normals(sd.philox_key, st.step, b.local[i], nz);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = -1;
+  double disp = -1.0;  // below every displacement: lanes without a residue
+  if (i < b.ntot) {
+    s = b.sysid[i];
+    SysState &st = b.st[s];
+    if (st.active && b.P->p.lang_on) {
+      SysDescC &sd = b.sys[s];
+      double nz[3];
+      if (noise_mode == PCG_NOISE_RAN2) {
+        nz[0] = b.noise[0][i];
+        nz[1] = b.noise[1][i];
+        nz[2] = b.noise[2][i];
+      } else {
+        philox_normals(sd.philox_key, st.step, b.local[i], nz);
+      }
+      double r[3];
+      gear_update(b, sd, st, i, (double)((RF *)b.F[0])[i], (double)((RF *)b.F[1])[i],
+                  (double)((RF *)b.F[2])[i], nz, r);
+      disp = disp_from_orig(b, sd, i, r);
+    } else {
+      s = -1;
+    }
   }
-  double r[3];
-  gear_update(b, sd, st, i, (double)((RF *)b.F[0])[i], (double)((RF *)b.F[1])[i],
-              (double)((RF *)b.F[2])[i], nz, r);
-  double disp = disp_from_orig(b, sd, i, r);
-  atomicMax(&st.max_disp_bits, (unsigned long long)__double_as_longlong(disp));
+  const int s0 = __shfl(s, 0, 64);
+  if (__all(s == s0)) {
+    double m = disp;
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && s0 >= 0)
+      atomicMax(&b.st[s0].max_disp_bits, (unsigned long long)__double_as_longlong(m));
+  } else if (s >= 0) {
+    atomicMax(&b.st[s].max_disp_bits, (unsigned long long)__double_as_longlong(disp));
+  }
 }
 
 // time and energy bookkeeping at the end of a step (step.cpp:157-162)
@@ -758,7 +780,6 @@ void launch_fix_nl(const DevBatch &b, int max_nlcap, hipStream_t s) {
   hipLaunchKernelGGL(k_part_count, pg, dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_part_scan, grid1(b.nsys, 64), dim3(64), 0, s, b);
   hipLaunchKernelGGL(k_part_write, pg, dim3(BLK), 0, s, b);
-  hipLaunchKernelGGL(k_qa_near, dim3(b.nsys), dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_nl_done, grid1(b.nsys, 64), dim3(64), 0, s, b);
 }
 void launch_terms(const DevBatch &b, int prec, int max_items, int only_term, int from, int to,

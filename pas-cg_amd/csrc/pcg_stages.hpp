@@ -6,7 +6,9 @@
 
 namespace pcg {
 
-enum { CL_NC = 1, CL_QA = 2, CL_PID = 4, CL_DH = 8, CL_PAULI = 16 };
+// partition classes of an NL pair; CL_NEAR marks a QA free pair within the
+// QA maximum required distance (its near-list entry is written beside it)
+enum { CL_NC = 1, CL_QA = 2, CL_PID = 4, CL_DH = 8, CL_PAULI = 16, CL_NEAR = 32 };
 
 __device__ inline int find_contact(const DevBatch &b, SysDescC &s, int i1, int i2) {
   int lo = s.nc0, hi = s.nc0 + s.nnc;
@@ -45,7 +47,8 @@ __device__ inline int classify(const DevBatch &b, SysDescC &s, const SysState &s
   bool four = b.chain[i1] == b.chain[i2] && abs(b.seq[i1] - b.seq[i2]) == 4;
   if (p.nat_cont_on && s.nnc > 0 && d < __dadd_rn(p.nl_cutoff, pad) && find_contact(b, s, i1, i2) >= 0)
     return CL_NC;
-  if (p.qa_on && !(!p.qa_include4 && four) && d < __dadd_rn(p.nl_cutoff, pad)) return CL_QA;
+  if (p.qa_on && !(!p.qa_include4 && four) && d < __dadd_rn(p.nl_cutoff, pad))
+    return CL_QA | (!(d > P.qa_max_req) ? CL_NEAR : 0);
   int cls = 0;
   if (p.pid_on && !(!p.pid_include4 && four) && d < __dadd_rn(p.nl_cutoff, pad)) cls |= CL_PID;
   if (p.dh_on) {
@@ -86,47 +89,6 @@ __device__ inline void qa_near_filters(const DevBatch &b, int i1, int i2, double
   req = fmax(req, __dmul_rn(p.qa_lj_r_high[ss], one_tol));
   double thr = fmin(b.P->qa_max_req, fmax(p.rep_cutoff, req)) * (1.0 + 1e-5);
   thr2 = (float)(thr * thr);
-}
-
-// The QA free pairs that can become candidates, in slot order (an ordered
-// compaction of the free list by !(orig_dist > max_req_dist), the test that
-// qa_cand_item makes first). Run by one workgroup per system after the free
-// list is rebuilt.
-__device__ inline void build_qa_near(const DevBatch &b, SysDescC &sd, SysState &st) {
-  __shared__ int wc[16];
-  __shared__ int run;
-  if (threadIdx.x == 0) run = 0;
-  __syncthreads();
-  const double req = b.P->qa_max_req;
-  const int n = st.qf_store;
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int base = 0; base < n; base += blockDim.x) {
-    int k = base + threadIdx.x;
-    bool near = k < n && !(b.qf_d[sd.qf0 + k] > req);
-    unsigned long long m = __ballot(near);
-    if (lane == 0) wc[w] = __popcll(m);
-    __syncthreads();
-    int off = run, tot = 0;
-    for (int q = 0; q < nw; ++q) {
-      if (q < w) off += wc[q];
-      tot += wc[q];
-    }
-    if (near) {
-      int h = sd.qf0 + off + __popcll(m & ((1ull << lane) - 1ull));
-      b.qh_slot[h] = k;
-      b.qh_i[0][h] = b.qf_i[0][sd.qf0 + k];
-      b.qh_i[1][h] = b.qf_i[1][sd.qf0 + k];
-      b.qh_vac[h] = b.qf_vac[sd.qf0 + k];
-      qa_near_filters(b, b.qh_i[0][h], b.qh_i[1][h], b.qh_req[h], b.qh_thr2[h]);
-      b.qh_ij[h] = (uint32_t)(b.qh_i[0][h] - sd.res0) | ((uint32_t)(b.qh_i[1][h] - sd.res0) << 16);
-      if (b.qh_vac[h]) b.qh_thr2[h] = -1.0f;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) run += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) st.qh_n = run;
-  __syncthreads();
 }
 
 // sy[4 * (i - sy_off) + c] is sync budget c of residue i (HBM or LDS).
