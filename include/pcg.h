@@ -135,6 +135,7 @@ typedef struct pcg_params {
   double wall_lj_depth;     /* sbox.walls.lj wall.depth */
   double ljw_breaking_dist; /* breaking dist factor * 2^(-1/6) * threshold */
   double ljw_saturation_diff; /* dt / lj wall.cycle duration (thread_setup.cpp:787-790) */
+  double afm_stats_window;    /* afm.average stats over: the tips' moving-average window */
 } pcg_params;
 
 /* One protein system after state::compile_model
@@ -266,9 +267,25 @@ int pcg_set_noise_scale(pcg_ctx *c, double scale); /* 0: no noise */
 int pcg_set_tips(pcg_ctx *c, int32_t n_tips, const int32_t *tip_res, const double *tip_orig,
                  const double *tip_vel, int32_t n_ftips, const int32_t *ftip_res,
                  const double *ftip_force);
+/* The AFM tips' statistics (moving_avg over afm.average stats over,
+ * cg/include/cg/types/avg.h:14-29), as make_report::add_afm_stuff reads
+ * them (make_report.cpp:264-310): entry k < n_tips of pcg_set_tips is a
+ * velocity tip, avg [3k] the average force on the tip (afm/vel/
+ * eval_forces.cpp:23) and perp [k] the average of its part perpendicular to
+ * the pulling direction (:25-29); the force tips follow, avg the average
+ * velocity of the pulled residue (afm/force/eval_forces.cpp:12) and perp 0.
+ * has [k] = the average has a value (a window has closed). Every force
+ * evaluation with the AFM terms on adds a sample; new tips start over.
+ * Writes up to cap entries, *size = the number of tips. */
+int pcg_afm_stats(pcg_ctx *c, int32_t cap, int32_t *size, double *avg, double *perp,
+                  uint8_t *has);
 /* Lists. kind: 0 NL, 1 current native contacts, 2 pauli, 3 dh, 4 pid,
  * 5 qa free pairs (flag = vacant), 6 qa contacts (val = saturation,
- * flag = vacant). Returns the size; fills up to cap items. */
+ * flag = vacant), 7 qa contacts (val = the qa::contact_type value: 1
+ * back-back, 2 back-side, 3 side-back, 4 + a1 + 21 a2 side-side,
+ * cg/include/cg/qa/contact_type.h:9-24; flag = vacant). Returns the size;
+ * fills up to cap items. The persistent step form keeps no NL: kind 0 after
+ * its steps is PCG_E_STATE until pcg_fix_nl rebuilds in the stage form. */
 int pcg_get_list(pcg_ctx *c, int32_t kind, int32_t cap, int32_t *size,
                  int32_t *i1, int32_t *i2, double *val, uint8_t *flag);
 int pcg_nat_cont_state(pcg_ctx *c, uint8_t *active, double *change_t,

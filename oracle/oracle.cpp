@@ -354,6 +354,20 @@ struct ora_sys {
       }
     }
   };
+  /* the tips' moving averages (afm::vel::tip avg_force and avg_perp_force,
+   * afm::force::tip avg_vel), window afm.average stats over, fresh with every
+   * set of tips (state::setup_afm, state.cpp:552-565) */
+  std::vector<mavg_t<v3>> tip_avg_F, ftip_avg_v;
+  std::vector<mavg_t<double>> tip_avg_perp;
+  void reset_tip_stats() {
+    mavg_t<v3> a;
+    a.window = p.afm_stats_window;
+    mavg_t<double> b;
+    b.window = p.afm_stats_window;
+    tip_avg_F.assign(tips.size(), a);
+    tip_avg_perp.assign(tips.size(), b);
+    ftip_avg_v.assign(ftips.size(), a);
+  }
   bool box_set = false, log_on = false, shear = false;
   ora_box bx{};
   v3 box_min{0, 0, 0}, box_max{0, 0, 0};
@@ -933,7 +947,8 @@ struct ora_sys {
 
   /* vel AFM tips, cg/src/cg/afm/vel/eval_forces.cpp:10-31 */
   void eval_vel_afm(double *Fo, double &Vo) {
-    for (auto const &tp : tips) {
+    for (size_t k = 0; k < tips.size(); ++k) {
+      tip_t const &tp = tips[k];
       v3 cur = tp.orig + (t - tp.ref_t) * tp.vel;
       v3 ra = cur - r[tp.res];
       double ra_n = norm(ra);
@@ -942,14 +957,29 @@ struct ora_sys {
         double V_, dV;
         harmonic(p.afm_H1, p.afm_H2, 0.0, ra_n, V_, dV);
         Vo += V_;
-        add_F(Fo, tp.res, dV * u);
+        v3 force = dV * u;
+        add_F(Fo, tp.res, force);
+        /* the force on the tip, and its part perpendicular to the pulling
+         * direction in the x-z plane (eval_forces.cpp:23-29) */
+        tip_avg_F[k].add(t, -force);
+        v3 w = unit(tp.vel);
+        double perp = force.x * w.z / (w.x + w.z * w.z / w.x) -
+                      force.z / (1.0 + (w.z / w.x) * (w.z / w.x));
+        tip_avg_perp[k].add(t, perp);
       }
     }
   }
 
-  /* force AFM tips, cg/src/cg/afm/force/eval_forces.cpp:10-13 (no energy) */
+  /* force AFM tips, cg/src/cg/afm/force/eval_forces.cpp:10-13 (no energy).
+   * The average velocity reads the state's velocities (v = y1 / dt as
+   * lang::step leaves them, step.cpp:67, zero before the first step) at
+   * the current time: thread_setup.cpp:711-713 binds neither v nor t of
+   * this task, so the reference reads unset views there. */
   void eval_force_afm(double *Fo) {
-    for (auto const &tp : ftips) add_F(Fo, tp.first, tp.second);
+    for (size_t k = 0; k < ftips.size(); ++k) {
+      add_F(Fo, ftips[k].first, ftips[k].second);
+      ftip_avg_v[k].add(t, v[ftips[k].first]);
+    }
   }
 
   v3 plane_n(int w) const { return v3{p.wall_normal[w][0], p.wall_normal[w][1], p.wall_normal[w][2]}; }
@@ -1875,6 +1905,7 @@ ora_sys *ora_create(const ora_params *p, const ora_model *m, int32_t seed,
   for (int k = 0; k < m->n_ftips; ++k)
     s->ftips.emplace_back(m->ftip_res[k], mk(m->ftip_force[3 * k], m->ftip_force[3 * k + 1],
                                               m->ftip_force[3 * k + 2]));
+  s->reset_tip_stats();
   /* walls: harmonic connections from the starting positions */
   s->is_conn.assign(n, 0);
   s->setup_harmonic();
@@ -2108,6 +2139,22 @@ void ora_set_tips(ora_sys *s, int32_t n_tips, const int32_t *tip_res, const doub
   for (int k = 0; k < n_ftips; ++k)
     s->ftips.emplace_back(ftip_res[k], mk(ftip_force[3 * k], ftip_force[3 * k + 1],
                                           ftip_force[3 * k + 2]));
+  s->reset_tip_stats();
+}
+int32_t ora_afm_stats(const ora_sys *s, double *avg, double *perp, uint8_t *has) {
+  const size_t nv = s->tips.size(), nf = s->ftips.size();
+  for (size_t k = 0; k < nv + nf; ++k) {
+    const bool vt = k < nv;
+    const auto &a = vt ? s->tip_avg_F[k] : s->ftip_avg_v[k - nv];
+    if (avg) {
+      avg[3 * k] = a.avg.x;
+      avg[3 * k + 1] = a.avg.y;
+      avg[3 * k + 2] = a.avg.z;
+    }
+    if (perp) perp[k] = vt ? s->tip_avg_perp[k].avg : 0.0;
+    if (has) has[k] = a.has ? 1 : 0;
+  }
+  return (int32_t)(nv + nf);
 }
 int64_t ora_saw_draws(const ora_sys *s) { return s->saw_draws; }
 

@@ -126,6 +126,7 @@ typedef struct ora_params {
   int32_t wall_kind[6], wall_side[6], wall_limit[6];
   int32_t hw_on, ljw_on; /* harmonic_walls_enabled, lj_walls_enabled */
   double wall_hh1, wall_lj_depth, ljw_breaking_dist, ljw_saturation_diff;
+  double afm_stats_window; /* afm.average stats over */
 } ora_params;
 
 /* sbox parameters of the box protocol (the same fields as pcg_box). */
@@ -272,6 +273,11 @@ int32_t ora_ljw_slots(const ora_sys *s, int32_t cap, int32_t *res, int32_t *wall
 void ora_ljw_pending(const ora_sys *s, int32_t cap, int32_t *ncand, int32_t *cand_res,
                      int32_t *cand_wall, double *cand_d, int32_t *nrem, int32_t *rem,
                      uint8_t *is_conn, int32_t *num_conns);
+/* AFM tip statistics: per velocity tip (first n_tips entries) the moving
+ * average of the force on the tip and of its perpendicular part; per force
+ * tip (after them) the moving average of the pulled residue's velocity;
+ * has = the average has a value. avg [3 * tips], perp [tips], has [tips]. */
+int32_t ora_afm_stats(const ora_sys *s, double *avg, double *perp, uint8_t *has);
 /* wall connections, kind 0 harmonic, 1 lj (live ones in slot order) */
 int32_t ora_wall_conns(const ora_sys *s, int32_t kind, int32_t cap, int32_t *res,
                        int32_t *wall, double *off, double *sat);

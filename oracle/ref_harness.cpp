@@ -469,6 +469,57 @@ double ref_force_afm(int n, double t, int k, const int *res, const double *force
   return 0.0;
 }
 
+// The AFM tips' statistics after afm::vel::eval_forces and
+// afm::force::eval_forces ran over a sequence of samples: sample j has the
+// time t[j], positions r[j] and velocities v[j] ([ns][n][3] each). Outputs
+// per tip (velocity tips, then force tips) the average (force on the tip,
+// or residue velocity), the perpendicular force average (velocity tips) and
+// whether each average has a value.
+void ref_afm_stats(int n, int ns, const double *t, const double *r, const double *v, int kv,
+                   const int *vres, const double *vorig, const double *vvel, int kf,
+                   const int *fres, const double *fforce, double H1, double H2, double window,
+                   double *avg, double *perp, uint8_t *has) {
+  vect::vector<afm::vel::tip> vt;
+  for (int a = 0; a < kv; ++a)
+    vt.emplace_back(vres[a], vec3r(vorig[3 * a], vorig[3 * a + 1], vorig[3 * a + 2]),
+                    (real)0.0, vec3r(vvel[3 * a], vvel[3 * a + 1], vvel[3 * a + 2]),
+                    moving_avg<vec3r, real>(window), moving_avg<real, real>(window));
+  vect::vector<afm::force::tip> ft;
+  for (int a = 0; a < kf; ++a)
+    ft.emplace_back(fres[a], vec3r(fforce[3 * a], fforce[3 * a + 1], fforce[3 * a + 2]),
+                    moving_avg<vec3r, real>(window));
+  for (int j = 0; j < ns; ++j) {
+    auto R = to_vec(n, r + (size_t)3 * n * j), Vel = to_vec(n, v + (size_t)3 * n * j);
+    vect::vector<vec3r> Fv(n);
+    real V = 0, tt = t[j];
+    afm::vel::eval_forces ev;
+    ev.r = R;
+    ev.F = Fv;
+    ev.V = &V;
+    ev.t = &tt;
+    ev.afm_force.H1 = H1;
+    ev.afm_force.H2 = H2;
+    ev.afm_force.nat_r = 0;
+    ev.afm_tips = vt;
+    run(ev);
+    afm::force::eval_forces ef;
+    ef.F = Fv;
+    ef.afm_tips = ft;
+    ef.v = Vel;
+    ef.t = &tt;
+    run(ef);
+  }
+  for (int a = 0; a < kv + kf; ++a) {
+    const bool isv = a < kv;
+    auto const &opt = isv ? *vt[a].avg_force().operator->() : *ft[a - kv].avg_vel().operator->();
+    has[a] = opt.has_value() ? 1 : 0;
+    vec3r x = opt.has_value() ? opt.value() : vec3r::Zero();
+    avg[3 * a] = x.x(), avg[3 * a + 1] = x.y(), avg[3 * a + 2] = x.z();
+    perp[a] = 0.0;
+    if (isv && vt[a].avg_perp_force()->has_value()) perp[a] = vt[a].avg_perp_force()->value();
+  }
+}
+
 // wall::solid::eval_forces over the walls of the parameters
 double ref_solid_wall(int n, const double *r, const ora_params *p, double *F) {
   auto R = to_vec(n, r);

@@ -111,6 +111,33 @@ struct MAvg {
   double sum[3], avg[3], init_time, window;
   int num, has;
 };
+// An AFM tip's moving averages: a velocity tip's force on the tip (x, y, z)
+// and its perpendicular part (entry 3), which the reference keeps in two
+// moving_avg objects fed at the same times (so one count and start time
+// serve both); a force tip's residue velocity (x, y, z)
+struct TipStat {
+  double sum[4], avg[4], init_time, window;
+  int num, has;
+};
+
+// moving_avg::add (cg/include/cg/types/avg.h:14-29) over dim components
+template <typename M>
+__device__ __forceinline__ void mavg_add(M &a, double time, const double *sample, int dim) {
+#pragma clang fp contract(off)
+  if (a.num == 0) {
+    a.init_time = time;
+    for (int c = 0; c < dim; ++c) a.sum[c] = sample[c];
+    a.num = 1;
+  } else {
+    for (int c = 0; c < dim; ++c) a.sum[c] += sample[c];
+    ++a.num;
+  }
+  if (a.init_time + a.window <= time) {
+    for (int c = 0; c < dim; ++c) a.avg[c] = a.sum[c] / (double)a.num;
+    a.has = 1;
+    a.num = 0;
+  }
+}
 
 // Walls and the simulation box of a plain context (one system): the
 // reaction force of every plane in the last evaluation, the wall
@@ -211,6 +238,7 @@ struct DevBatch {
   const int *nc_i[2]; const double *nc_nat; const uint8_t *nc_type;
   uint8_t *nc_active; double *nc_change_t;
   const int *tip_res; const double *tip_orig[3], *tip_vel[3];
+  TipStat *tip_stat;          // per tip slot, as tip_res
   // dynamic lists
   int *nl_i[2]; double *nl_d; uint8_t *nl_cls;  // class bits of the partition
   int *ncur;                                      // current native contacts (index into all)

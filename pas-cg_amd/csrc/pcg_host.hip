@@ -291,6 +291,14 @@ int build_system(const pcg_params &p, const pcg_model &m, double temperature, in
   return PCG_OK;
 }
 
+// an AFM tip's moving averages before its first sample
+TipStat fresh_tip_stat(const pcg_params &p) {
+  TipStat t;
+  std::memset(&t, 0, sizeof(t));
+  t.window = p.afm_stats_window;
+  return t;
+}
+
 void fill_dev_params(const pcg_params &p, DevParams &d) {
   d.p = p;
   d.nl_eff_cutoff = std::max(p.nl_cutoff, p.dh_cutoff);
@@ -569,6 +577,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     A.add(&tip_orig[c], std::max(ntip, 1LL));
     A.add(&tip_vel[c], std::max(ntip, 1LL));
   }
+  A.add(&D.tip_stat, std::max(ntip, 1LL));
   for (int k = 0; k < 2; ++k) {
     A.add(&D.nl_i[k], nl);
     A.add(&D.pauli_i[k], nl);
@@ -808,6 +817,11 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   {
     std::vector<double> ct(std::max(nnc, 1LL), -1.0);
     if ((rc = up(D.nc_change_t, ct.data(), ct.size() * 8))) return rc;
+  }
+  {
+    // every tip's moving averages start empty, with the parameters' window
+    std::vector<TipStat> ts(std::max(ntip, 1LL), fresh_tip_stat(p));
+    if ((rc = up(D.tip_stat, ts.data(), ts.size() * sizeof(TipStat)))) return rc;
   }
 #undef CATI
 #undef CATD
@@ -1052,6 +1066,9 @@ int set_tips(DeviceBatch &B, int s, int32_t nv, const int32_t *vres, const doubl
   int rc = read_state(B, s, st);
   if (rc) return rc;
   if (nv + nf > 0) {
+    const std::vector<TipStat> ts(nv + nf, fresh_tip_stat(B.hp.p));
+    HIPCHK(hipMemcpyAsync(B.d.tip_stat + d.tip0, ts.data(), ts.size() * sizeof(TipStat),
+                          hipMemcpyHostToDevice, B.stream));
     // the device views are const for the kernels; the arena itself is writable
     HIPCHK(hipMemcpyAsync((void *)(B.d.tip_res + d.tip0), res.data(), res.size() * sizeof(int32_t),
                           hipMemcpyHostToDevice, B.stream));
@@ -1205,7 +1222,7 @@ void enqueue_step(DeviceBatch &B) {
   if (p.qa_on) launch_prepare_nh(d, B.stream);
   launch_fix_nl(d, B.max_nlcap, B.stream);
   launch_terms(d, B.prec, B.max_items, -1, 0, -1, B.stream);
-  if (p.qa_on) launch_qa_finish(d, B.stream);
+  if (p.qa_on) launch_qa_finish(d, B.max_n, B.stream);
   if (B.hp.ljw_terms) launch_ljw_process(d, B.stream);
   if (B.box_set) launch_box_log(d, B.stream);
   launch_lang(d, B.prec, B.noise, B.stream);
@@ -1507,7 +1524,7 @@ int pcg_qa_finish(pcg_ctx *c) {
   if (!c) return fail(PCG_E_ARG, "null ctx");
   int rc = switch_form(*c, FORM_STAGES);
   if (rc) return rc;
-  launch_qa_finish(c->B->d, c->B->stream);
+  launch_qa_finish(c->B->d, c->B->max_n, c->B->stream);
   HIPCHK(hipStreamSynchronize(c->B->stream));
   return check_overflow(*c->B);
 }
@@ -1613,6 +1630,27 @@ int pcg_set_tips(pcg_ctx *c, int32_t n_tips, const int32_t *tip_res, const doubl
   return set_tips(*c->B, 0, n_tips, tip_res, tip_orig, tip_vel, n_ftips, ftip_res, ftip_force);
 }
 
+int pcg_afm_stats(pcg_ctx *c, int32_t cap, int32_t *size, double *avg, double *perp,
+                  uint8_t *has) {
+  if (!c || !size || cap < 0) return fail(PCG_E_ARG, "bad arguments");
+  DeviceBatch &B = *c->B;
+  const SysDesc &d = B.sys[0];
+  const int nt = d.ntip + d.nftip, m = std::min(nt, (int)cap);
+  *size = nt;
+  if (m <= 0) return PCG_OK;
+  HIPCHK(hipSetDevice(B.device));
+  std::vector<TipStat> ts(m);
+  HIPCHK(hipStreamSynchronize(B.stream));
+  HIPCHK(hipMemcpy(ts.data(), B.d.tip_stat + d.tip0, m * sizeof(TipStat), hipMemcpyDeviceToHost));
+  for (int k = 0; k < m; ++k) {
+    for (int a = 0; a < 3; ++a)
+      if (avg) avg[3 * k + a] = ts[k].avg[a];
+    if (perp) perp[k] = k < d.ntip ? ts[k].avg[3] : 0.0;
+    if (has) has[k] = ts[k].has ? 1 : 0;
+  }
+  return PCG_OK;
+}
+
 int pcg_set_noise_scale(pcg_ctx *c, double scale) {
   if (!c || scale < 0) return fail(PCG_E_ARG, "bad arguments");
   DeviceBatch &B = *c->B;
@@ -1627,7 +1665,11 @@ int pcg_set_noise_scale(pcg_ctx *c, double scale) {
 
 int pcg_get_list(pcg_ctx *c, int32_t kind, int32_t cap, int32_t *size, int32_t *i1, int32_t *i2,
                  double *val, uint8_t *flag) {
-  if (!c || !size || kind < 0 || kind > 6) return fail(PCG_E_ARG, "bad arguments");
+  if (!c || !size || kind < 0 || kind > 7) return fail(PCG_E_ARG, "bad arguments");
+  // the persistent form keeps the per-term lists but not the NL itself
+  if (kind == 0 && c->last_form == FORM_PERSISTENT)
+    return fail(PCG_E_STATE, "the persistent step form keeps no neighbour list; "
+                             "pcg_fix_nl rebuilds it in the stage form");
   DeviceBatch &B = *c->B;
   SysState st;
   int rc = read_state(B, 0, st);
@@ -1646,6 +1688,7 @@ int pcg_get_list(pcg_ctx *c, int32_t kind, int32_t cap, int32_t *size, int32_t *
   case 4: n = st.pid_n, off = d.nl0, a = D.pid_i[0], b2 = D.pid_i[1], v = D.pid_d; break;
   case 5: n = st.qf_n, off = d.qf0, a = D.qf_i[0], b2 = D.qf_i[1], v = D.qf_d, f = D.qf_vac; break;
   case 6: n = st.qc_n, off = d.qc0, a = D.qc_i[0], b2 = D.qc_i[1], v = D.qc_sat, f = D.qc_vac; break;
+  case 7: n = st.qc_n, off = d.qc0, a = D.qc_i[0], b2 = D.qc_i[1], f = D.qc_vac; break;
   }
   *size = n;
   int m = std::min(n, cap);
@@ -1675,6 +1718,11 @@ int pcg_get_list(pcg_ctx *c, int32_t kind, int32_t cap, int32_t *size, int32_t *
     HIPCHK(hipMemcpy(tb.data(), b2 + off, m * 4, hipMemcpyDeviceToHost));
     if (v) HIPCHK(hipMemcpy(tv.data(), v + off, m * 8, hipMemcpyDeviceToHost));
     if (f) HIPCHK(hipMemcpy(tf.data(), f + off, m, hipMemcpyDeviceToHost));
+    if (kind == 7) {
+      std::vector<int16_t> ty(m);
+      HIPCHK(hipMemcpy(ty.data(), D.qc_t + off, m * sizeof(int16_t), hipMemcpyDeviceToHost));
+      for (int k = 0; k < m; ++k) tv[k] = (double)ty[k];
+    }
     for (int k = 0; k < m; ++k) {
       ta[k] -= d.res0;
       tb[k] -= d.res0;
@@ -1919,7 +1967,7 @@ void digest_params(Digest &d, const pcg_params &p) {
   PCG_D(ptype); PCG_D(base_sync); PCG_D(n_walls); PCG_D(wall_origin); PCG_D(wall_normal);
   PCG_D(wall_depth); PCG_D(wall_min_dist); PCG_D(wall_kind); PCG_D(wall_side); PCG_D(wall_limit);
   PCG_D(hw_on); PCG_D(ljw_on); PCG_D(wall_hh1); PCG_D(wall_lj_depth); PCG_D(ljw_breaking_dist);
-  PCG_D(ljw_saturation_diff);
+  PCG_D(ljw_saturation_diff); PCG_D(afm_stats_window);
 #undef PCG_D
 }
 }  // extern "C++"
@@ -1972,7 +2020,7 @@ int ckpt_write(DeviceBatch &B, const char magic[8], void *buf, int64_t bytes) {
   HIPCHK(hipSetDevice(B.device));
   CkptHeader h{};
   std::memcpy(h.magic, magic, 8);
-  h.version = 3;
+  h.version = 4;
   h.nsys = (int32_t)B.sys.size();
   h.ntot = B.d.ntot;
   h.prec = B.prec;
@@ -1993,7 +2041,7 @@ int ckpt_read(DeviceBatch &B, const char magic[8], const void *buf, int64_t byte
   CkptHeader h;
   if (bytes < (int64_t)sizeof h) return fail(PCG_E_ARG, "not a checkpoint");
   std::memcpy(&h, buf, sizeof h);
-  if (std::memcmp(h.magic, magic, 8) != 0 || h.version != 3) return fail(PCG_E_ARG, "not a checkpoint");
+  if (std::memcmp(h.magic, magic, 8) != 0 || h.version != 4) return fail(PCG_E_ARG, "not a checkpoint");
   if (h.nsys != (int32_t)B.sys.size() || h.ntot != B.d.ntot || h.prec != B.prec ||
       h.arena_bytes != B.arena.total || h.fingerprint != ckpt_fingerprint(B))
     return fail(PCG_E_ARG, "checkpoint of a different batch (models, parameters or precision)");

@@ -583,9 +583,12 @@ __device__ double pid_item(const DevBatch &b, SysDescC &s, const G &g, A &acc, i
 
 // ----------------------------------------------------------------- AFM ----
 // afm::vel::eval_forces::iter (cg/src/cg/afm/vel/eval_forces.cpp:10-31); t is
-// the time since the tips were placed (t - ref_t)
+// the time since the tips were placed (t - ref_t), now the current time.
+// One thread takes a tip, so its moving averages have one writer.
 template <typename R, typename G, typename A>
-__device__ double vel_afm_item(const DevBatch &b, const G &g, A &acc, int k, double t) {
+__device__ double vel_afm_item(const DevBatch &b, const G &g, A &acc, int k, double t,
+                               double now) {
+#pragma clang fp contract(off)
   ParamsC &p = b.P->p;
   int res = b.tip_res[k];
   v3<double> cur{b.tip_orig[0][k] + t * b.tip_vel[0][k], b.tip_orig[1][k] + t * b.tip_vel[1][k],
@@ -595,16 +598,37 @@ __device__ double vel_afm_item(const DevBatch &b, const G &g, A &acc, int k, dou
   if (!(n > (R)1e-10)) return 0.0;
   R V, dV;
   harmonicR<R>((R)p.afm_H1, (R)p.afm_H2, (R)0, n, V, dV);
-  acc.add(res, (dV / n) * ra);
+  const v3<R> f = (dV / n) * ra;
+  acc.add(res, f);
+  if (b.tip_stat) {
+    // the force on the tip, and its part perpendicular to the pulling
+    // direction in the x-z plane (eval_forces.cpp:23-29); a tip at rest has
+    // no direction and its perpendicular part is NaN, as in the reference
+    const double ww = 1.0 / sqrt(b.tip_vel[0][k] * b.tip_vel[0][k] +
+                                 b.tip_vel[1][k] * b.tip_vel[1][k] +
+                                 b.tip_vel[2][k] * b.tip_vel[2][k]);
+    const double wx = ww * b.tip_vel[0][k], wz = ww * b.tip_vel[2][k];
+    const double fx = (double)f.x, fz = (double)f.z;
+    const double smp[4] = {-fx, -(double)f.y, -fz,
+                           fx * wz / (wx + wz * wz / wx) - fz / (1.0 + (wz / wx) * (wz / wx))};
+    mavg_add(b.tip_stat[k], now, smp, 4);
+  }
   return (double)V;
 }
 
 // afm::force::eval_forces::iter (cg/src/cg/afm/force/eval_forces.cpp:10-13):
 // the tip's constant pulling force on its residue, no energy. Force tips sit
-// after the velocity tips, their force in tip_vel.
+// after the velocity tips, their force in tip_vel. The tip's average
+// velocity reads the state's velocities (y1 / dt as the last Langevin step
+// left them, zero before the first) at the current time.
 template <typename A>
-__device__ __forceinline__ double force_afm_item(const DevBatch &b, A &acc, int k) {
-  acc.add(b.tip_res[k], v3<double>{b.tip_vel[0][k], b.tip_vel[1][k], b.tip_vel[2][k]});
+__device__ __forceinline__ double force_afm_item(const DevBatch &b, A &acc, int k, double now) {
+  const int res = b.tip_res[k];
+  acc.add(res, v3<double>{b.tip_vel[0][k], b.tip_vel[1][k], b.tip_vel[2][k]});
+  if (b.tip_stat) {
+    const double smp[3] = {b.vel[0][res], b.vel[1][res], b.vel[2][res]};
+    mavg_add(b.tip_stat[k], now, smp, 3);
+  }
   return 0.0;
 }
 

@@ -45,34 +45,162 @@ def scalars_row(rep: dict, nat_cont: bool, qa: bool, pid: bool) -> dict:
 
 
 def _field(v) -> str:
-    if isinstance(v, bool):
+    """A cell as the SL4 writer prints it (sl4.cpp:15-44): the value goes
+    through its stream text and back, so a float prints as printf "%g" and
+    a non-finite one, which the stream does not read back, as 0; booleans
+    as T/F, integers as integers, text as is."""
+    import math
+    import numpy as np
+    if isinstance(v, (bool, np.bool_)):
         return "T" if v else "F"
-    if isinstance(v, int):
-        return "%d" % v
-    if isinstance(v, float):
-        return "%g" % v
+    if isinstance(v, (int, np.integer)):
+        return "%d" % int(v)
+    if isinstance(v, (float, np.floating)):
+        v = float(v)
+        return "%g" % v if math.isfinite(v) else "0"
     return str(v)
 
 
 def sl4_table(rows: list[dict]) -> str:
-    """A table in the SL4 layout (sl4.cpp:65-123); columns in the order the
-    first row gives them."""
-    if not rows:
-        return ""
-    cols = list(rows[0])
-    cells = [[_field(r.get(c, "")) for c in cols] for r in rows]
+    """A table in the SL4 layout (sl4.cpp:65-123): columns in the order they
+    first appear, a cell a row does not set left blank, every column right-
+    aligned to its widest entry; a table without rows prints its empty
+    header, "#"."""
+    cols = []
+    for r in rows:
+        for c in r:
+            if c not in cols:
+                cols.append(c)
+    cells = [[_field(r[c]) if c in r else "" for c in cols] for r in rows]
     width = [max([len(c)] + [len(row[k]) for row in cells]) for k, c in enumerate(cols)]
     head = "#" + "   ".join(c.upper().rjust(w) for c, w in zip(cols, width))
     body = [" " + "   ".join(f.rjust(w) for f, w in zip(row, width)) for row in cells]
     return "\n".join([head] + body)
 
 
-def out_file(trajectories: list[list[dict]]) -> str:
-    """Text of a `.out` file: per trajectory its comment and scalars table."""
+def _comment(*parts) -> str:
+    """An SL4 comment (sl4.h:75-83): its parts as stream text (floats as
+    "%g"), upper-cased, after '#'."""
+    text = "".join(("%g" % p) if isinstance(p, float) else str(p) for p in parts)
+    return "#" + text.upper()
+
+
+def out_file(trajectories: list[list[dict]], afm: list | None = None,
+             results: str | None = None) -> str:
+    """Text of a `.out` file (make_report::emit_out, make_report.cpp:355-359):
+    per trajectory its comment and scalars table, then, for a trajectory
+    that reported AFM tips, the force AFM and velocity AFM divs
+    (afm[idx] = (force rows, velocity rows) of afm_rows, accumulated over
+    the reports; None without AFM); after the last trajectory the results
+    div of at_simul_end when `results` (results_div) is given."""
     divs = []
     for idx, rows in enumerate(trajectories):
-        divs.append("#TRAJECTORY %d\n%s" % (idx, sl4_table(rows)))
+        parts = ["#TRAJECTORY %d" % idx, sl4_table(rows)]
+        tips = afm[idx] if afm is not None and idx < len(afm) else None
+        if tips is not None:
+            parts += ["#FORCE AFM TIPS", sl4_table(tips[0]),
+                      "#VELOCITY AFM TIPS", sl4_table(tips[1])]
+        divs.append("\n".join(parts))
+    if results is not None:
+        divs.append(results)
     return "\n".join(divs)
+
+
+def afm_rows(stats: dict, tips: dict) -> tuple[list[dict], list[dict]]:
+    """One report's rows of the force AFM and velocity AFM tables
+    (make_report::add_afm_stuff, make_report.cpp:264-311) from the tips'
+    statistics (pcg.Context.afm_stats: velocity tips, then force tips) and
+    the tips in place (tip_res, ftip_res): per force tip its residue and the
+    norm of the average velocity in A/tau, per velocity tip its residue, the
+    norm of the average force and the perpendicular force in eps/A; -1 for
+    an average that has no value yet."""
+    import numpy as np
+    vres = list(np.asarray(tips.get("tip_res", []), np.int64))
+    fres = list(np.asarray(tips.get("ftip_res", []), np.int64))
+    nv = len(vres)
+    vel_unit = A / P.TAU
+    force_unit = P.EPS / A
+    force_rows, vel_rows = [], []
+    for k, res in enumerate(fres):
+        j = nv + k
+        avg = float(np.linalg.norm(stats["avg"][j])) / vel_unit if stats["has"][j] else -1.0
+        force_rows.append({"res_idx": int(res), "avg_vel [A/tau]": avg})
+    for k, res in enumerate(vres):
+        has = bool(stats["has"][k])
+        vel_rows.append({"res_idx": int(res),
+                         "avg_force [eps/A]": (float(np.linalg.norm(stats["avg"][k])) / force_unit
+                                               if has else -1.0),
+                         "avg_perp_force [eps/A]": (float(stats["perp"][k]) / force_unit
+                                                    if has else -1.0)})
+    return force_rows, vel_rows
+
+
+NC_TYPES = ["any", "bb", "bs", "sb", "ss", "ssbond"]
+
+
+def nc_type_name(t: int) -> str:
+    """format_nc_type (make_report.cpp:58-75) of a nat_cont::type value
+    (ANY, BACK_BACK, BACK_SIDE, SIDE_BACK, SIDE_SIDE, SSBOND)."""
+    return NC_TYPES[t] if 0 <= t < len(NC_TYPES) else ""
+
+
+def qa_type_name(t: int) -> str:
+    """format_qa_type (make_report.cpp:473-482) of a qa::contact_type value
+    (1 back-back, 2 back-side, 3 side-back, side-side otherwise)."""
+    return {1: "bb", 2: "bs", 3: "sb"}.get(int(t), "ss")
+
+
+def results_div(contacts: dict | None = None, nc_times: list | None = None,
+                unfold_times: list | None = None, num_traj: int = 1) -> str:
+    """The results div the `.out` file ends with (make_report::at_simul_end,
+    make_report.cpp:77-128). With the unfolding study's time measurement on
+    (contacts given: i1, i2, type, nat_dist of every native contact), the
+    average breaking/forming time of each contact over the trajectories
+    that changed it (change_t >= 0 in nc_times, one array a trajectory;
+    -1 if none did), rows sorted by (time, contact index), and the median
+    unfolding time: entry num_traj // 2 of the sorted unfold_times (the
+    latest change time of each trajectory whose every contact changed),
+    N/A when there are not that many."""
+    out = ["#RESULTS"]
+    if contacts is None:
+        return "\n".join(out)
+    k = len(contacts["i1"])
+    rows = []
+    for idx in range(k):
+        s, num = 0.0, 0
+        for tr in nc_times or []:
+            if tr[idx] >= 0.0:
+                s += float(tr[idx])
+                num += 1
+        rows.append((s / num / P.TAU if num > 0 else -1.0, idx))
+    rows.sort()
+    tab = []
+    for avg, idx in rows:
+        tab.append({"i1": int(contacts["i1"][idx]), "i2": int(contacts["i2"][idx]),
+                    "type": nc_type_name(int(contacts["type"][idx])),
+                    "nat_dist[A]": float(contacts["nat_dist"][idx]) / A,
+                    "avg_time[tau]": float(avg)})
+    out += ["#AVERAGE BREAKING/FORMING TIMES", sl4_table(tab)]
+    times = sorted(float(x) for x in (unfold_times or []))
+    mid = num_traj // 2
+    out.append("#MEDIAN (UN)FOLDING TIME: N/A" if mid >= len(times)
+               else _comment("median (un)folding time: ", times[mid]))
+    return "\n".join(out)
+
+
+def unfolding_sample(active_change_t, num_changed: int):
+    """What traj_end_step records of one trajectory for the unfolding study
+    (thread_main.cpp:340-351): every native contact's change time, and the
+    unfolding time (the latest change time) when every contact changed,
+    else None. active_change_t = pcg.Context.nat_cont_state(k)[1]."""
+    import numpy as np
+    ct = np.asarray(active_change_t, np.float64).copy()
+    t = None
+    if num_changed == len(ct):
+        t = 0.0
+        for x in ct:
+            t = max(t, float(x))
+    return ct, t
 
 
 def raw_data_line(traj_idx: int, step_idx: int, t: float, V: float) -> str:
@@ -196,3 +324,189 @@ def pdb_snapshots(snapshots, chain, atype, cell=None) -> str:
             serial += 1
         out.append(_record("ENDMDL"))
     return "\n".join("".join(line) for line in out)
+
+
+# ---------------------------------------------------------------- map files --
+# make_report::add_map_data (make_report.cpp:499-656): per report a div with
+# the time, the native contacts, the QA contacts, the active PID bundles and
+# every angle and dihedral of the model at the current positions.
+SEQ_CHAIN_IDS = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz!\"#$%&\\'()*+,-./"
+
+
+def _c_round(x):
+    """C's round(): halves away from zero."""
+    import numpy as np
+    t = np.trunc(x)
+    return np.where(np.abs(x - t) >= 0.5, t + np.sign(x), t)
+
+
+def _wrapped_dist(pos, cell, i1, i2):
+    """|pbc.wrap(r[i1], r[i2])| (cg/include/cg/sbox/pbc.h:39-50): each axis
+    with a cell takes the nearest image."""
+    import numpy as np
+    c = np.asarray(cell, np.float64)
+    inv = np.where(c != 0.0, 1.0 / np.where(c != 0.0, c, 1.0), 0.0)
+    d = pos[np.asarray(i2)] - pos[np.asarray(i1)]
+    d = d - c * _c_round(d * inv)
+    return np.sqrt((d * d).sum(axis=-1))
+
+
+def _unit(v):
+    import numpy as np
+    return (1.0 / np.sqrt((v * v).sum(axis=-1)))[..., None] * v
+
+
+def map_angles(pos, ang):
+    """angle_value (make_report.cpp:484-487): the angle between the bond
+    vectors r2 - r1 and r3 - r2 of each triple, in radians."""
+    import numpy as np
+    ang = np.asarray(ang, np.int64).reshape(-1, 3)
+    u1 = _unit(pos[ang[:, 1]] - pos[ang[:, 0]])
+    u2 = _unit(pos[ang[:, 2]] - pos[ang[:, 1]])
+    with np.errstate(invalid="ignore"):
+        return np.arccos((u1 * u2).sum(axis=-1))
+
+
+def map_dihedrals(pos, dih):
+    """dihedral_value (make_report.cpp:489-497): the angle between the
+    planes' normals r12 x r23 and r23 x r34, negative when r12 x r23 points
+    against r34."""
+    import numpy as np
+    dih = np.asarray(dih, np.int64).reshape(-1, 4)
+    r12 = pos[dih[:, 1]] - pos[dih[:, 0]]
+    r23 = pos[dih[:, 2]] - pos[dih[:, 1]]
+    r34 = pos[dih[:, 3]] - pos[dih[:, 2]]
+    x1, x2 = np.cross(r12, r23), np.cross(r23, r34)
+    with np.errstate(invalid="ignore"):
+        phi = np.arccos((_unit(x1) * _unit(x2)).sum(axis=-1))
+    return np.where((x1 * r34).sum(axis=-1) < 0.0, -phi, phi)
+
+
+def pid_active(pos, cell, i1, i2, atype, prm: dict):
+    """pid::eval_forces::is_active (pid/eval_forces.cpp:204-212) of bundles
+    (i1, i2): distance <= counting factor * the side-chain pair's r_min *
+    2^(-1/6)."""
+    import numpy as np
+    i1, i2 = np.asarray(i1, np.int64), np.asarray(i2, np.int64)
+    at = np.asarray(atype, np.int64)
+    r_high = np.asarray(prm["pid_ss_r_high"], np.float64).reshape(-1)[at[i1] * 21 + at[i2]]
+    sigma = r_high * P.C216_INV
+    return _wrapped_dist(pos, cell, i1, i2) <= prm["counting_factor"] * sigma
+
+
+def _res_cells(m: dict, chain_ids: str, i: int, k: str) -> dict:
+    ch = int(m["chain"][i])
+    return {"chain" + k: chain_ids[ch], "seq" + k: int(m["seq"][i]) + 1,
+            "res" + k: P.AA_NAMES[int(m["atype"][i])]}
+
+
+def _pair_rows(m, chain_ids, pos, cell, i1, i2):
+    rows = []
+    dist = _wrapped_dist(pos, cell, i1, i2) if len(i1) else []
+    for a, b, d in zip(i1, i2, dist):
+        row = _res_cells(m, chain_ids, int(a), "1")
+        row.update(_res_cells(m, chain_ids, int(b), "2"))
+        # the reference labels a same-chain pair "inter" (make_report.cpp:531)
+        same = int(m["chain"][int(a)]) == int(m["chain"][int(b)])
+        row["inter/intra"] = "inter" if same else "intra"
+        rows.append((row, float(d) / A))
+    return rows
+
+
+def map_div(t: float, pos, m: dict, cell=None, chain_ids: str | None = None,
+            nat: dict | None = None, qa: dict | None = None, pid: dict | None = None) -> str:
+    """One report's div of the `.map` file. m is the model (chain, seq,
+    atype, ang/ang_has/ang_theta, dih/dih_has/dih_phi); chain ids default to
+    the model's (chain_ids) or the sequence-file order A, B, ... Sections
+    present only when their term is on: nat (all native contacts: i1, i2,
+    type, active, change_t), qa (live contacts in slot order: i1, i2,
+    type), pid (all bundles: i1, i2, active)."""
+    import numpy as np
+    pos = np.asarray(pos, np.float64)
+    cell = np.asarray(cell if cell is not None else m.get("cell", [0.0, 0.0, 0.0]), np.float64)
+    ids = chain_ids or m.get("chain_ids") or SEQ_CHAIN_IDS
+    out = [_comment("T = ", float(t))]
+    if nat is not None:
+        rows = []
+        for (row, d), ty, act, ct in zip(_pair_rows(m, ids, pos, cell, nat["i1"], nat["i2"]),
+                                         nat["type"], nat["active"], nat["change_t"]):
+            row["type"] = nc_type_name(int(ty))
+            row["dist[A]"] = d
+            row["active"] = bool(act)
+            row["time[tau]"] = float(ct) / P.TAU
+            rows.append(row)
+        out += ["#NATIVE CONTACTS", _comment("n = ", int(np.count_nonzero(nat["active"]))),
+                sl4_table(rows)]
+    if qa is not None:
+        rows = []
+        for (row, d), ty in zip(_pair_rows(m, ids, pos, cell, qa["i1"], qa["i2"]), qa["type"]):
+            row["type"] = qa_type_name(int(ty))
+            row["dist[A]"] = d
+            rows.append(row)
+        out += ["#QA", _comment("n = ", len(rows)), sl4_table(rows)]
+    if pid is not None:
+        act = np.asarray(pid["active"], bool)
+        i1 = np.asarray(pid["i1"])[act]
+        i2 = np.asarray(pid["i2"])[act]
+        rows = []
+        for row, d in _pair_rows(m, ids, pos, cell, i1, i2):
+            row["dist[A]"] = d
+            rows.append(row)
+        out += ["#PID", _comment("n = ", len(rows)), sl4_table(rows)]
+    rows = []
+    ang = np.asarray(m["ang"], np.int64).reshape(-1, 3)
+    theta = map_angles(pos, ang)
+    for k in range(len(ang)):
+        i = int(ang[k, 0])
+        row = {"i1": i}
+        row.update(_res_cells(m, ids, i, "1"))
+        row["theta"] = float(theta[k])
+        if m["ang_has"][k]:
+            row["native"] = float(m["ang_theta"][k])
+        rows.append(row)
+    out += ["#ANGLES", sl4_table(rows)]
+    rows = []
+    dih = np.asarray(m["dih"], np.int64).reshape(-1, 4)
+    phi = map_dihedrals(pos, dih)
+    for k in range(len(dih)):
+        i = int(dih[k, 0])
+        row = {"i1": i}
+        row.update(_res_cells(m, ids, i, "1"))
+        row["phi"] = float(phi[k])
+        if m["dih_has"][k]:
+            row["native"] = float(m["dih_phi"][k])
+        rows.append(row)
+    out += ["#DIHEDRALS", sl4_table(rows)]
+    return "\n".join(out)
+
+
+def map_file(trajectories: list[list[str]]) -> str:
+    """Text of a `.map` file (make_report::emit_map, make_report.cpp:658-662):
+    per trajectory its comment, then its reports' divs (map_div)."""
+    return "\n".join("\n".join(["#TRAJECTORY %d" % idx] + divs)
+                     for idx, divs in enumerate(trajectories))
+
+
+def map_snapshot(ctx, m: dict, prm: dict, cell=None, chain_ids: str | None = None) -> str:
+    """map_div of a pcg.Context's current state: positions and time, the
+    native contacts' state (pcg_nat_cont_state), the live QA contacts with
+    their types (pcg_get_list kind 7) and the PID bundles (kind 4), each
+    when its term is on."""
+    import numpy as np
+    t, _ = ctx.time()
+    pos = ctx.r()
+    nat = qa = pid = None
+    cont = np.asarray(m["cont"], np.int64).reshape(-1, 2)
+    if prm.get("nat_cont_on") and len(cont):
+        act, ct, _ = ctx.nat_cont_state(len(cont))
+        nat = {"i1": cont.min(axis=1), "i2": cont.max(axis=1), "type": m["cont_type"],
+               "active": act, "change_t": ct}
+    if prm.get("qa_on"):
+        i1, i2, ty, vac = ctx.list(7)
+        live = vac == 0
+        qa = {"i1": i1[live], "i2": i2[live], "type": ty[live].astype(np.int64)}
+    c = np.asarray(cell if cell is not None else m.get("cell", [0.0, 0.0, 0.0]), np.float64)
+    if prm.get("pid_on"):
+        i1, i2, _, _ = ctx.list(4)
+        pid = {"i1": i1, "i2": i2, "active": pid_active(pos, c, i1, i2, m["atype"], prm)}
+    return map_div(t, pos, m, c, chain_ids, nat, qa, pid)

@@ -16,6 +16,8 @@ import struct
 # ---------------------------------------------------------------- units ----
 # cg/include/cg/utils/units.h:8-59, evaluated in the same order.
 F77UNIT = 1.0
+# 2^(-1/6) as cg/include/cg/utils/math.h:6 writes it
+C216_INV = 0.89089871814033927
 ANGSTROM = F77UNIT / 5.0
 NANOMETER = 10.0 * ANGSTROM
 METER = NANOMETER * 1.0e9
@@ -229,6 +231,8 @@ def default_params() -> dict:
     p["afm_on"] = 0
     p["afm_H1"] = 30.0 / (A ** 2)
     p["afm_H2"] = 0.0
+    # afm.average stats over: the window of the tips' moving averages
+    p["afm_stats_window"] = 100.0 * TAU
     # simulation box walls (sbox.walls): none by default; depth of the "solid"
     # type and the wall threshold
     p["n_walls"] = 0

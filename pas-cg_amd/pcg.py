@@ -64,7 +64,7 @@ class PcgParams(C.Structure):
         ("wall_kind", I * 6), ("wall_side", I * 6), ("wall_limit", I * 6),
         ("hw_on", I), ("ljw_on", I),
         ("wall_hh1", D), ("wall_lj_depth", D), ("ljw_breaking_dist", D),
-        ("ljw_saturation_diff", D),
+        ("ljw_saturation_diff", D), ("afm_stats_window", D),
     ]
 
 
@@ -214,6 +214,7 @@ SIGNATURES = {
     "pcg_stream_copy": [I64, I, I, C.POINTER(D), C.POINTER(D)],
     "pcg_set_tips": [C.c_void_p, I, C.POINTER(I), C.POINTER(D), C.POINTER(D), I, C.POINTER(I),
                      C.POINTER(D)],
+    "pcg_afm_stats": [C.c_void_p, I, C.POINTER(I), C.POINTER(D), C.POINTER(D), C.POINTER(U8)],
     "pcg_batch_set_tips": [C.c_void_p, I, I, C.POINTER(I), C.POINTER(D), C.POINTER(D), I,
                            C.POINTER(I), C.POINTER(D)],
     "pcg_morph_saw": [C.POINTER(PcgSawOptions), I, C.POINTER(I), I, C.POINTER(D),
@@ -472,6 +473,18 @@ class Context:
         """Replaces the AFM tips (pcg_set_tips); an empty dict turns AFM off."""
         keep, args = _tip_args(tips)
         check(self.L.pcg_set_tips(self.h, *args))
+
+    def afm_stats(self) -> dict:
+        """The tips' moving averages (pcg_afm_stats): avg [tips, 3], perp
+        [tips], has [tips]; velocity tips first, then force tips."""
+        size = C.c_int32()
+        check(self.L.pcg_afm_stats(self.h, 0, C.byref(size), None, None, None))
+        k = size.value
+        avg, perp, has = np.zeros((k, 3)), np.zeros(k), np.zeros(k, np.uint8)
+        if k:
+            check(self.L.pcg_afm_stats(self.h, k, C.byref(size), _p(avg, D), _p(perp, D),
+                                       _p(has, U8)))
+        return {"avg": avg, "perp": perp, "has": has.astype(bool)}
 
     def list(self, kind: int):
         size = C.c_int32()

@@ -6,13 +6,14 @@ from oracle_ffi import Oracle
 from pas_cg_amd import pcg as G
 
 
-def qa_lockstep(prm, m, precision, steps, r0=None, t0=None, seed=31):
+def qa_lockstep(prm, m, precision, steps, r0=None, t0=None, seed=31, keep=False):
     """Runs the oracle's noise-free step piece by piece and evaluates the GPU
     context on the oracle's state at each piece: verify (and both rebuild),
     prepare_nh, forces, QA finish. Returns the worst relative force error of
     the whole force and of the two QA terms, and the worst relative energy
     error of the QA terms; asserts that QA contacts, saturations and sync
-    budgets stay identical (every QA decision is exact in both precisions)."""
+    budgets stay identical (every QA decision is exact in both precisions).
+    keep: also returns the oracle and the context, both on the final state."""
     o = Oracle(prm, m, seed=seed)
     o.set_noise_scale(0.0)
     if r0 is not None:
@@ -50,5 +51,10 @@ def qa_lockstep(prm, m, precision, steps, r0=None, t0=None, seed=31):
         assert np.allclose(ci[2], gi[2], rtol=0, atol=1e-12), k
         formed = max(formed, no)
         o.lang_step()
+    if keep:
+        # both on the oracle's final state
+        c.set(0, o.r())
+        c.set_time(o.t)
+        return worst, formed, o, c
     c.close()
     return worst, formed

@@ -57,7 +57,7 @@ class OraParams(C.Structure):
         ("wall_kind", I * 6), ("wall_side", I * 6), ("wall_limit", I * 6),
         ("hw_on", I), ("ljw_on", I),
         ("wall_hh1", D), ("wall_lj_depth", D), ("ljw_breaking_dist", D),
-        ("ljw_saturation_diff", D),
+        ("ljw_saturation_diff", D), ("afm_stats_window", D),
     ]
 
 
@@ -171,6 +171,8 @@ def lib():
         L.ora_qa_contacts.argtypes = [P, I, C.POINTER(I), C.POINTER(D), C.POINTER(I),
                                       C.POINTER(D), C.POINTER(C.c_int8),
                                       C.POINTER(C.c_uint8)]
+        L.ora_afm_stats.restype = I
+        L.ora_afm_stats.argtypes = [P, C.POINTER(D), C.POINTER(D), C.POINTER(C.c_uint8)]
         L.ora_box_setup.argtypes = [P, C.POINTER(OraBox)]
         L.ora_box_start.argtypes = [P]
         L.ora_box_state.argtypes = [P, C.POINTER(D)]
@@ -407,6 +409,15 @@ class Oracle:
         out = np.zeros(6, np.int32)
         self.L.ora_pid_counts(self.h, _ptr(out, I))
         return out
+
+    def afm_stats(self) -> dict:
+        """The tips' moving averages: avg [tips, 3], perp [tips], has [tips]
+        (velocity tips first, then force tips)."""
+        k = self.L.ora_afm_stats(self.h, None, None, None)
+        avg, perp, has = np.zeros((k, 3)), np.zeros(k), np.zeros(k, np.uint8)
+        if k:
+            self.L.ora_afm_stats(self.h, _ptr(avg, D), _ptr(perp, D), _ptr(has, C.c_uint8))
+        return {"avg": avg, "perp": perp, "has": has.astype(bool)}
 
     def box_setup(self, box: dict):
         self._box = fill_struct(OraBox(), box)

@@ -122,3 +122,208 @@ def test_pdb_snapshots_layout_and_round_trip():
     r = m["pos"][m["chain"] == 0]
     rg = np.sqrt(((r - r.mean(axis=0)) ** 2).sum() / 12.0) / P.ANGSTROM
     assert abs(float(vals[1]) - rg) <= 1e-5 * rg
+
+
+def test_sl4_cells_and_empty_tables():
+    """sl4.cpp:15-44 and table.cpp: a column appears where a row first sets
+    it, a row without it leaves a padded blank, a float that is not finite
+    does not survive the stream round trip and prints 0, booleans print T/F,
+    and a table without rows prints its empty header."""
+    import numpy as np
+    assert O.sl4_table([]) == "#"
+    rows = [{"a": 1, "b": True}, {"a": np.int32(22), "c": float("nan"), "b": np.bool_(False)},
+            {"a": 3, "c": -float("inf")}]
+    assert O.sl4_table(rows) == ("# A   B   C\n"
+                                 "  1   T    \n"
+                                 " 22   F   0\n"
+                                 "  3       0")
+    assert O._comment("T = ", 1e-05) == "#T = 1E-05"
+    assert O._comment("n = ", 7) == "#N = 7"
+
+
+def _map_model():
+    import numpy as np
+    A = P.ANGSTROM
+    pos = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [1, 1, 1], [3, 0, 0]], float) * A
+    m = {"chain": np.array([0, 0, 0, 0, 1]), "seq": np.array([0, 1, 2, 3, 0]),
+         "atype": np.array([0, 4, 7, 12, 19], np.uint8),
+         "ang": np.array([[0, 1, 2], [1, 2, 3]]), "ang_has": np.array([1, 0]),
+         "ang_theta": np.array([2.0, 0.0]), "dih": np.array([[0, 1, 2, 3]]),
+         "dih_has": np.array([0]), "dih_phi": np.array([0.0]), "cell": np.zeros(3)}
+    return pos, m
+
+
+def test_map_div_layout():
+    """make_report::add_map_data (make_report.cpp:499-656) on a hand-built
+    system with closed-form geometry: the bond vectors x, y, z are pairwise
+    perpendicular, so every angle between consecutive bonds and the
+    dihedral are pi/2 (positive: r12 x r23 points along r34); contact
+    distances sqrt(2), 2, sqrt(3) and sqrt(5) A; sequence numbers from 1;
+    a same-chain pair labelled "inter" as the reference does; only active
+    PID bundles listed; a native value only where the model has one."""
+    pos, m = _map_model()
+    nat = {"i1": [0, 1], "i2": [2, 4], "type": [1, 5], "active": [True, False],
+           "change_t": [-1.0, 3.5]}
+    qa = {"i1": [0, 2], "i2": [3, 4], "type": [1, 30]}
+    pid = {"i1": [0, 1], "i2": [3, 4], "active": [True, False]}
+    want = """#T = 12.5
+#NATIVE CONTACTS
+#N = 1
+#CHAIN1   SEQ1   RES1   CHAIN2   SEQ2   RES2   INTER/INTRA     TYPE   DIST[A]   ACTIVE   TIME[TAU]
+      A      1    ALA        A      3    GLY         inter       bb   1.41421        T          -1
+      A      2    CYS        B      1    VAL         intra   ssbond         2        F         3.5
+#QA
+#N = 2
+#CHAIN1   SEQ1   RES1   CHAIN2   SEQ2   RES2   INTER/INTRA   TYPE   DIST[A]
+      A      1    ALA        A      4    MET         inter     bb   1.73205
+      A      3    GLY        B      1    VAL         intra     ss   2.23607
+#PID
+#N = 1
+#CHAIN1   SEQ1   RES1   CHAIN2   SEQ2   RES2   INTER/INTRA   DIST[A]
+      A      1    ALA        A      4    MET         inter   1.73205
+#ANGLES
+#I1   CHAIN1   SEQ1   RES1    THETA   NATIVE
+  0        A      1    ALA   1.5708        2
+  1        A      2    CYS   1.5708         
+#DIHEDRALS
+#I1   CHAIN1   SEQ1   RES1      PHI
+  0        A      1    ALA   1.5708"""
+    assert O.map_div(12.5, pos, m, nat=nat, qa=qa, pid=pid) == want
+    # two trajectories, the second with a 2.5 A periodic x axis: the 2 A
+    # pair's nearest image is 0.5 A away
+    txt = O.map_file([[O.map_div(0.0, pos, m)],
+                      [O.map_div(1e-5, pos, m, cell=[2.5 * P.ANGSTROM, 0, 0], nat=nat)]])
+    lines = txt.split("\n")
+    assert lines[0] == "#TRAJECTORY 0" and lines[1] == "#T = 0"
+    k = lines.index("#TRAJECTORY 1")
+    assert lines[k + 1] == "#T = 1E-05" and lines[k + 2] == "#NATIVE CONTACTS"
+    assert lines[k + 6].split()[8] == "0.5"
+
+
+def test_map_geometry_closed_forms():
+    """angle_value is the angle between consecutive bonds (pi minus the bond
+    angle) and dihedral_value takes the sign of (r12 x r23) . r34."""
+    import math
+    import numpy as np
+    pos = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [1, 1, -1], [2, 1, 0]], float)
+    assert O.map_angles(pos, [[0, 1, 2]])[0] == pytest.approx(math.pi / 2, abs=1e-15)
+    assert O.map_angles(pos, [[0, 1, 0]])[0] == pytest.approx(math.pi, abs=1e-7)
+    assert O.map_dihedrals(pos, [[0, 1, 2, 3]])[0] == pytest.approx(-math.pi / 2, abs=1e-15)
+    # a planar trans chain: phi = pi; cis: 0
+    trans = np.array([[0, 1, 0], [0, 0, 0], [1, 0, 0], [1, -1, 0]], float)
+    cis = np.array([[0, 1, 0], [0, 0, 0], [1, 0, 0], [1, 1, 0]], float)
+    assert abs(O.map_dihedrals(trans, [[0, 1, 2, 3]])[0]) == pytest.approx(math.pi, abs=1e-7)
+    assert O.map_dihedrals(cis, [[0, 1, 2, 3]])[0] == pytest.approx(0.0, abs=1e-7)
+    # C round: a separation of exactly half the cell goes to the image away
+    # from zero (|d| unchanged), as round() does and numpy's even rounding
+    # would not for 2.5 cells
+    assert O._c_round(np.array([2.5, -2.5, 0.5, 1.5, 0.49]))[:5].tolist() == [3, -3, 1, 2, 0]
+
+
+def test_results_div_unfolding_times():
+    """make_report::at_simul_end (make_report.cpp:77-128): per contact the
+    mean change time over the trajectories that changed it (-1 if none),
+    rows sorted by (time, index); the median entry num_traj // 2 of the
+    sorted unfolding times, N/A when fewer trajectories unfolded."""
+    import numpy as np
+    A = P.ANGSTROM
+    contacts = {"i1": [0, 2, 5], "i2": [4, 9, 11], "type": [0, 5, 1],
+                "nat_dist": [5.0 * A, 6.5 * A, 4.25 * A]}
+    times = [np.array([2.0, -1.0, 1.0]), np.array([4.0, -1.0, 0.5]), np.array([3.0, -1.0, -1.0])]
+    ct0, u0 = O.unfolding_sample(times[0], 2)
+    assert u0 is None and ct0.tolist() == [2.0, -1.0, 1.0]
+    _, u1 = O.unfolding_sample(np.array([4.0, 7.0, 0.5]), 3)
+    assert u1 == 7.0
+    txt = O.results_div(contacts, times, [9.0, 7.0], num_traj=3)
+    want = """#RESULTS
+#AVERAGE BREAKING/FORMING TIMES
+#I1   I2     TYPE   NAT_DIST[A]   AVG_TIME[TAU]
+  2    9   ssbond           6.5              -1
+  5   11       bb          4.25            0.75
+  0    4      any             5               3
+#MEDIAN (UN)FOLDING TIME: 9"""
+    assert txt == want
+    assert O.results_div(contacts, times, [9.0], num_traj=3).endswith("#MEDIAN (UN)FOLDING TIME: N/A")
+    assert O.results_div() == "#RESULTS"
+
+
+def test_afm_tables():
+    """make_report::add_afm_stuff (make_report.cpp:264-311): the norms of the
+    averages in A/tau and eps/A, -1 before a window closed; a resting
+    velocity tip's perpendicular force is NaN and prints 0."""
+    import numpy as np
+    A = P.ANGSTROM
+    stats = {"avg": np.array([[3.0 / A, 4.0 / A, 0.0], [1.0, 0.0, 0.0], [0.0, 0.6 * A, 0.8 * A]]),
+             "perp": np.array([float("nan"), 0.5, 0.0]), "has": np.array([True, False, True])}
+    tips = {"tip_res": [0, 11], "ftip_res": [7]}
+    f, v = O.afm_rows(stats, tips)
+    assert f == [{"res_idx": 7, "avg_vel [A/tau]": pytest.approx(1.0, rel=1e-15)}]
+    assert v[0]["avg_force [eps/A]"] == pytest.approx(5.0, rel=1e-15)
+    assert v[1] == {"res_idx": 11, "avg_force [eps/A]": -1.0, "avg_perp_force [eps/A]": -1.0}
+    txt = O.out_file([[{"TIME[tau]": 1.0}]], afm=[(f, v)])
+    assert txt.split("\n") == ["#TRAJECTORY 0", "#TIME[TAU]", "         1",
+                               "#FORCE AFM TIPS", "#RES_IDX   AVG_VEL [A/TAU]",
+                               "       7                 1",
+                               "#VELOCITY AFM TIPS",
+                               "#RES_IDX   AVG_FORCE [EPS/A]   AVG_PERP_FORCE [EPS/A]",
+                               "       0                   5                        0",
+                               "      11                  -1                       -1"]
+    # no force tips: the force AFM div still comes, with its empty table
+    f, v = O.afm_rows(stats, {"tip_res": [0, 11]})
+    assert O.out_file([[{"TIME[tau]": 1.0}]], afm=[(f, v)]).split("\n")[3:5] == \
+        ["#FORCE AFM TIPS", "#"]
+
+
+@pytest.mark.gpu
+def test_map_of_context_equals_oracle_map():
+    """map_snapshot of the plain context, through pcg_nat_cont_state,
+    pcg_get_list kind 7 (QA contacts with their types) and kind 4 (PID
+    bundles), is the map the oracle's state gives: QA on, after a lockstep
+    run in which contacts formed; then PID on, on the oracle's positions."""
+    import numpy as np
+    from lockstep import qa_lockstep
+    from oracle_ffi import Oracle
+    from pas_cg_amd import model as M
+    from pas_cg_amd import pcg as G
+    prm = P.default_params()
+    prm.update(qa_on=1, pid_on=0, afm_on=0, temperature=0.3)
+    one = M.synthetic_protein(50, M.job_seed(31))
+    one["atype"][::5] = P.AA_INDEX["ARG"]
+    one["atype"][2::6] = P.AA_INDEX["ASP"]
+    m = M.replicate(one, 3, 45.0 * P.ANGSTROM, 31)
+    _, formed, o, c = qa_lockstep(prm, m, G.FP64, 60, keep=True)
+    assert formed > 0
+    k = len(m["cont"])
+    act, ct = o.nat_cont_state(k)
+    ij, _, ty, _, _, vac = o.qa_contacts()
+    live = vac == 0
+    gi1, gi2, gty, gvac = c.list(7)
+    assert np.array_equal(gvac != 0, ~live)
+    assert np.array_equal(gty[live].astype(np.int64), ty[live])
+    cont = np.asarray(m["cont"])
+    want = O.map_div(o.t, o.r(), m,
+                     nat={"i1": cont.min(axis=1), "i2": cont.max(axis=1), "type": m["cont_type"],
+                          "active": act, "change_t": ct},
+                     qa={"i1": ij[live, 0], "i2": ij[live, 1], "type": ty[live]})
+    got = O.map_snapshot(c, m, prm)
+    assert got == want
+    assert "#QA\n#N = %d\n" % int(live.sum()) in got
+    c.close()
+
+    prm = P.default_params()
+    prm.update(qa_on=0, pid_on=1)
+    m = M.synthetic_protein(80, M.job_seed(12))
+    o = Oracle(prm, m, seed=3)
+    o.step(40)
+    c = G.Context(prm, m, precision=G.FP64, noise=G.RAN2, seed=3)
+    c.set(0, o.r())
+    c.set_time(o.t)
+    o.fix_nl()
+    c.fix_nl()
+    i1, i2, _, _ = o.list(4)
+    active = O.pid_active(o.r(), m["cell"], i1, i2, m["atype"], prm)
+    assert active.any()
+    want = O.map_div(o.t, o.r(), m, pid={"i1": i1, "i2": i2, "active": active})
+    got = O.map_snapshot(c, m, dict(prm, nat_cont_on=0))
+    assert got == want
+    c.close()
