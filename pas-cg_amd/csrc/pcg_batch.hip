@@ -250,10 +250,108 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
   uint32_t chg = 0u;
   for (int t = 0; t < 21; ++t) chg |= (p.charge[t] != 0.0 ? 1u : 0u) << t;
   const int kbeg = rb_row_start(w, nw, n), kend = rb_row_start(w + 1, nw, n);
+  // When every pair fits the NL capacity, the first pass also stores the
+  // wave's NL pairs in order (k | j << 16 in nl_i[0], their classes in
+  // nl_cls, a native contact's index in nl_i[1]) from the index of its
+  // block's first pair, which no other wave's pairs reach, and the second
+  // pass walks those instead of every pair of the block.
+  const bool compact = (long long)n * (n - 1) / 2 <= (long long)sd.nlcap;
+  const int seg = sd.nl0 + (int)((long long)kbeg * (n - 1) - (long long)kbeg * (kbeg - 1) / 2);
+  int my_nl = 0;
+  // the second pass's writes of one pair: native contact c, QA free pair
+  // and near-list entry, PID bundle, DH pair, Pauli pair, each at the wave's
+  // running offset plus its rank among the chunk's earlier pairs of the list
+  auto emit = [&](int k, int j, int c, bool is_nc, bool near, bool is_pid, bool is_dh,
+                  bool is_pauli, unsigned long long m_nc, unsigned long long m_near,
+                  unsigned long long m_pid, unsigned long long m_dh, unsigned long long m_pauli,
+                  const int *off) {
+    const int i = r0 + k, jg = r0 + j;
+    const int ai = g.at[k] & 63, aj = g.at[j] & 63;
+    if (is_nc) b.ncur[sd.nc0 + off[RB_NC] + __popcll(m_nc & below)] = c;
+    double d64 = 0.0;
+    if (near | is_pid) {
+      v3<double> dd = xsub(v3<double>{g.x[j], g.y[j], g.z[j]}, v3<double>{g.x[k], g.y[k], g.z[k]});
+      if (cell) dd = xwrapd(dd, sd);
+      d64 = xnorm3(dd);
+    }
+    if (near) {
+      const int f = off[RB_NEAR] + __popcll(m_near & below);
+      if (f < sd.qfcap) {
+        const int q = sd.qf0 + f;
+        b.qf_i[0][q] = i;
+        b.qf_i[1][q] = jg;
+        b.qf_d[q] = d64;
+        b.qf_vac[q] = 0;
+        // near list entry f: the free pair in slot f
+        const double req = fmax(fmax(fmax(g.req_t[1], g.req_t[2]), g.req_t[3]),
+                                g.req_t[4 + ai + 21 * aj]);
+        const double thr = fmin(P.qa_max_req, fmax(p.rep_cutoff, req)) * (1.0 + 1e-5);
+        b.qh_slot[q] = f;
+        b.qh_i[0][q] = i;
+        b.qh_i[1][q] = jg;
+        b.qh_vac[q] = 0;
+        b.qh_req[q] = req;
+        b.qh_thr2[q] = (float)(thr * thr);
+        b.qh_ij[q] = (uint32_t)k | ((uint32_t)j << 16);
+      }
+    }
+    if (is_pid) {
+      const int pos = off[RB_PID] + __popcll(m_pid & below);
+      if (pos < sd.nlcap) {
+        const int q = sd.nl0 + pos;
+        b.pid_i[0][q] = i;
+        b.pid_i[1][q] = jg;
+        b.pid_d[q] = d64;
+        b.pid_t[q] = (int16_t)(ai * 21 + aj);
+      }
+    }
+    if (is_dh) {
+      const int pos = off[RB_DH] + __popcll(m_dh & below);
+      if (pos < sd.nlcap) {
+        const int q = sd.nl0 + pos;
+        b.dh_i[0][q] = i;
+        b.dh_i[1][q] = jg;
+        b.dh_q[q] = p.charge[ai] * p.charge[aj];
+      }
+    }
+    if (is_pauli) {
+      const int pos = off[RB_PAULI] + __popcll(m_pauli & below);
+      if (pos < sd.nlcap) {
+        const int q = sd.nl0 + pos;
+        b.pauli_i[0][q] = i;
+        b.pauli_i[1][q] = jg;
+      }
+    }
+  };
   __syncthreads();
   for (int pass = 0; pass < 2; ++pass) {
     int off[RB_N];
     for (int c = 0; c < RB_N; ++c) off[c] = pass ? wcnt[w][c] : 0;
+    if (pass && compact) {
+      // the wave's NL pairs, 64 at a time, in the first pass's order
+      for (int t0 = 0; t0 < my_nl; t0 += 64) {
+        const int t = t0 + lane;
+        const bool v = t < my_nl;
+        const uint32_t kw = v ? (uint32_t)b.nl_i[0][seg + t] : 0u;
+        const int cls = v ? (int)b.nl_cls[seg + t] : 0;
+        const int k = (int)(kw & 0xffffu), j = (int)(kw >> 16);
+        const bool is_nc = cls & CL_NC, near = cls & CL_NEAR, is_pid = cls & CL_PID,
+                   is_dh = cls & CL_DH, is_pauli = cls & CL_PAULI, is_qa = cls & CL_QA;
+        const int c = is_nc ? b.nl_i[1][seg + t] : 0;
+        const unsigned long long m_nc = __ballot(is_nc), m_qa = __ballot(is_qa),
+                                 m_near = __ballot(near), m_pid = __ballot(is_pid),
+                                 m_dh = __ballot(is_dh), m_pauli = __ballot(is_pauli);
+        if (v) emit(k, j, c, is_nc, near, is_pid, is_dh, is_pauli, m_nc, m_near, m_pid, m_dh,
+                    m_pauli, off);
+        off[RB_NC] += __popcll(m_nc);
+        off[RB_QA] += __popcll(m_qa);
+        off[RB_NEAR] += __popcll(m_near);
+        off[RB_PID] += __popcll(m_pid);
+        off[RB_DH] += __popcll(m_dh);
+        off[RB_PAULI] += __popcll(m_pauli);
+      }
+      continue;
+    }
     for (int k = kbeg; k < kend; ++k) {
       const double xi = g.x[k], yi = g.y[k], zi = g.z[k];
       const int ci = chn[k], si = sqi[k], ai = g.at[k] & 63;
@@ -311,62 +409,16 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
                                  m_pid = __ballot(is_pid), m_dh = __ballot(is_dh),
                                  m_pauli = __ballot(is_pauli);
         if (pass) {
-          const int i = r0 + k, jg = r0 + j;
-          if (is_nc) {
-            // the contact: the row's partners before column j; its fields
-            // are copied after the pass
-            const int c = sd.nc0 + tp0 + __popcll(natm & below);
-            b.ncur[sd.nc0 + off[RB_NC] + __popcll(m_nc & below)] = c;
-          }
-          if (near) {
-            const int f = off[RB_NEAR] + __popcll(m_near & below);
-            if (f < sd.qfcap) {
-              const double d64 = xnorm3(dd);
-              const int q = sd.qf0 + f;
-              b.qf_i[0][q] = i;
-              b.qf_i[1][q] = jg;
-              b.qf_d[q] = d64;
-              b.qf_vac[q] = 0;
-              // near list entry f: the free pair in slot f
-              const double req = fmax(fmax(fmax(g.req_t[1], g.req_t[2]), g.req_t[3]),
-                                      g.req_t[4 + ai + 21 * aj]);
-              const double thr = fmin(P.qa_max_req, fmax(p.rep_cutoff, req)) * (1.0 + 1e-5);
-              b.qh_slot[q] = f;
-              b.qh_i[0][q] = i;
-              b.qh_i[1][q] = jg;
-              b.qh_vac[q] = 0;
-              b.qh_req[q] = req;
-              b.qh_thr2[q] = (float)(thr * thr);
-              b.qh_ij[q] = (uint32_t)k | ((uint32_t)j << 16);
-            }
-          }
-          if (is_pid) {
-            const int pos = off[RB_PID] + __popcll(m_pid & below);
-            if (pos < sd.nlcap) {
-              const int q = sd.nl0 + pos;
-              b.pid_i[0][q] = i;
-              b.pid_i[1][q] = jg;
-              b.pid_d[q] = xnorm3(dd);
-              b.pid_t[q] = (int16_t)(ai * 21 + aj);
-            }
-          }
-          if (is_dh) {
-            const int pos = off[RB_DH] + __popcll(m_dh & below);
-            if (pos < sd.nlcap) {
-              const int q = sd.nl0 + pos;
-              b.dh_i[0][q] = i;
-              b.dh_i[1][q] = jg;
-              b.dh_q[q] = p.charge[ai] * p.charge[aj];
-            }
-          }
-          if (is_pauli) {
-            const int pos = off[RB_PAULI] + __popcll(m_pauli & below);
-            if (pos < sd.nlcap) {
-              const int q = sd.nl0 + pos;
-              b.pauli_i[0][q] = i;
-              b.pauli_i[1][q] = jg;
-            }
-          }
+          const int c = is_nc ? sd.nc0 + tp0 + __popcll(natm & below) : 0;
+          if (valid) emit(k, j, c, is_nc, near, is_pid, is_dh, is_pauli, m_nc, m_near, m_pid,
+                          m_dh, m_pauli, off);
+        } else if (compact & in) {
+          const int t = seg + off[RB_NL] + __popcll(m_nl & below);
+          b.nl_i[0][t] = (int)((uint32_t)k | ((uint32_t)j << 16));
+          b.nl_cls[t] = (uint8_t)((is_nc ? CL_NC : 0) | (is_qa ? CL_QA : 0) | (near ? CL_NEAR : 0) |
+                                  (is_pid ? CL_PID : 0) | (is_dh ? CL_DH : 0) |
+                                  (is_pauli ? CL_PAULI : 0));
+          if (is_nc) b.nl_i[1][t] = sd.nc0 + tp0 + __popcll(natm & below);
         }
         off[RB_NL] += __popcll(m_nl);
         off[RB_NC] += __popcll(m_nc);
@@ -378,6 +430,7 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
       }
     }
     if (!pass) {
+      my_nl = off[RB_NL];
       if (lane == 0)
         for (int c = 0; c < RB_N; ++c) wcnt[w][c] = off[c];
       __syncthreads();
@@ -954,6 +1007,10 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
                       return kk == 0 ? (c == 0 ? rx[k] : c == 1 ? ry[k] : rz[k])
                                      : ys[((kk - 1) * 3 + c) * n + k];
                     });
+          // a force tip averages its residue's velocity in the next
+          // evaluation, which reads it from HBM
+          if (sd.nftip > 0)
+            for (int c = 0; c < 3; ++c) b.vel[c][i] = vel[c];
         } else {
           gear_core(b, sd, st, minv_t[aa], sqminv_t[aa], Fx, Fy, Fz, nz, r, vel,
                     [&](int kk, int c) -> double & { return b.y[kk][c][i]; });

@@ -4,6 +4,8 @@
 // a batch; stages that the reference runs conditionally (NL rebuild and list
 // updates) exit on the device when not needed, so that a step never waits on
 // the host and can be captured in a hipGraph.
+#include <algorithm>
+
 #include "pcg_launch.hpp"
 #include "pcg_stages.hpp"
 
@@ -183,24 +185,31 @@ __device__ __forceinline__ void block_counts6(int cls, int valid, int counts[NCL
   __syncthreads();
 }
 
-// blockIdx.x: chunk of BLK pairs, blockIdx.y: system
-__global__ void k_part_count(DevBatch b) {
-  int s = blockIdx.y;
-  const SysState &st = b.st[s];
-  if (!st.active || !st.nl_invalid) return;
-  SysDescC &sd = b.sys[s];
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int nchunks = (sd.nlcap + BLK - 1) / BLK;
-  if ((int)blockIdx.x >= nchunks) return;
+// One chunk of BLK pairs of system s: each pair's classes, and the chunk's
+// count per class
+__device__ __forceinline__ void part_count_chunk(DevBatch &b, SysDescC &sd, const SysState &st,
+                                                 int ch) {
+  int k = ch * BLK + (int)threadIdx.x;
   int valid = k < st.nl_n;
   int cls = valid ? classify(b, sd, st, sd.nl0 + k) : 0;
   if (valid) b.nl_cls[sd.nl0 + k] = (uint8_t)cls;
   int counts[NCL], ranks[NCL];
   block_counts6(cls, valid, counts, ranks);
   if (threadIdx.x == 0) {
-    int base = (sd.nl0 / BLK + blockIdx.x) * NCL;
+    int base = (sd.nl0 / BLK + ch) * NCL;
     for (int c = 0; c < NCL; ++c) b.blk_cnt[base + c] = counts[c];
   }
+}
+
+// blockIdx.y: system; the blocks stride over the chunks the list fills, so
+// a step without a rebuild costs a small grid's launch
+__global__ void k_part_count(DevBatch b) {
+  int s = blockIdx.y;
+  const SysState &st = b.st[s];
+  if (!st.active || !st.nl_invalid) return;
+  SysDescC &sd = b.sys[s];
+  const int nchunks = (st.nl_n + BLK - 1) / BLK;
+  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) part_count_chunk(b, sd, st, ch);
 }
 
 __global__ void k_part_scan(DevBatch b) {
@@ -229,21 +238,17 @@ __global__ void k_part_scan(DevBatch b) {
   st.qh_n = tot[5] < sd.qfcap ? tot[5] : sd.qfcap;
 }
 
-__global__ void k_part_write(DevBatch b) {
-  int s = blockIdx.y;
-  SysState &st = b.st[s];
-  if (!st.active || !st.nl_invalid) return;
-  SysDescC &sd = b.sys[s];
-  int nchunks = (sd.nlcap + BLK - 1) / BLK;
-  if ((int)blockIdx.x >= nchunks) return;
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+// One chunk's pairs into their lists at the chunk's offsets (k_part_scan)
+__device__ __forceinline__ void part_write_chunk(DevBatch &b, SysDescC &sd, const SysState &st,
+                                                 int ch) {
+  int k = ch * BLK + (int)threadIdx.x;
   int valid = k < st.nl_n;
   int slot = sd.nl0 + k;
   int cls = valid ? b.nl_cls[slot] : 0;
   int counts[NCL], ranks[NCL];
   block_counts6(cls, valid, counts, ranks);
   if (!valid || !cls) return;
-  int base = (sd.nl0 / BLK + blockIdx.x) * NCL;
+  int base = (sd.nl0 / BLK + ch) * NCL;
   int i1 = b.nl_i[0][slot], i2 = b.nl_i[1][slot];
   double d = b.nl_d[slot];
   ParamsC &p = b.P->p;
@@ -292,6 +297,16 @@ __global__ void k_part_write(DevBatch b) {
   }
 }
 
+
+__global__ void k_part_write(DevBatch b) {
+  int s = blockIdx.y;
+  const SysState &st = b.st[s];
+  if (!st.active || !st.nl_invalid) return;
+  SysDescC &sd = b.sys[s];
+  const int nchunks = (st.nl_n + BLK - 1) / BLK;
+  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) part_write_chunk(b, sd, st, ch);
+}
+
 // The rebuild is complete: record it and clear nl_invalid.
 __global__ void k_nl_done(DevBatch b) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -331,11 +346,39 @@ __global__ void k_prepare_nh(DevBatch b) {
   prepare_nh_item(b, g, i);
 }
 
-__global__ void k_qa_finish(DevBatch b) {
+// The serial part of the finish (removals, then acceptance in order) reads
+// and writes the sync budgets of each candidate's residues: with
+// sync_lds the system's budgets sit in LDS for the kernel (one word a
+// residue, loaded and stored coalesced), and up to QF_CAND_LDS candidates
+// are staged there in acceptance order, so the serial loop waits on LDS
+// rather than on a chain of HBM loads.
+__global__ void k_qa_finish(DevBatch b, int sync_lds) {
+  extern __shared__ __align__(16) unsigned char qf_lds[];
   int s = blockIdx.x;
   SysState &st = b.st[s];
   if (!st.active || !b.P->p.qa_on) return;
-  qa_finish_block(b, b.sys[s], st, b.sync, 0);
+  SysDescC &sd = b.sys[s];
+  if (!sync_lds) {
+    qa_finish_block(b, sd, st, b.sync, 0);
+    return;
+  }
+  // nothing removed and no candidate: the block's own early return, before
+  // any staging
+  if (st.cand_n == 0 && st.rem_n == 0) return;
+  const int n = sd.n, r0 = sd.res0;
+  uint32_t *syl = (uint32_t *)qf_lds;
+  QfLds ql;
+  ql.key = (long long *)(qf_lds + 4 * (((size_t)sync_lds + 1) & ~(size_t)1));
+  ql.sd = (uint64_t *)(ql.key + QF_CAND_LDS);
+  ql.idx = (int *)(ql.sd + QF_CAND_LDS);
+  ql.sc = ql.idx + QF_CAND_LDS;
+  ql.i1 = ql.sc + QF_CAND_LDS;
+  ql.i2 = ql.i1 + QF_CAND_LDS;
+  ql.cap = QF_CAND_LDS;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) syl[k] = *(const uint32_t *)(b.sync + 4 * (r0 + k));
+  __syncthreads();
+  qa_finish_block(b, sd, st, (int8_t *)syl, r0, &ql);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) *(uint32_t *)(b.sync + 4 * (r0 + k)) = syl[k];
 }
 
 // The displacement maximum goes to the system's state with one atomic per
@@ -518,23 +561,6 @@ __global__ void __launch_bounds__(1024) k_ljw_process(DevBatch b) {
 }
 
 // ------------------------------------------------------------- box protocol ----
-__device__ __forceinline__ void mavg_add(MAvg &a, double time, const double *sample, int dim) {
-#pragma clang fp contract(off)
-  if (a.num == 0) {
-    a.init_time = time;
-    for (int c = 0; c < dim; ++c) a.sum[c] = sample[c];
-    a.num = 1;
-  } else {
-    for (int c = 0; c < dim; ++c) a.sum[c] += sample[c];
-    ++a.num;
-  }
-  if (a.init_time + a.window <= time) {
-    for (int c = 0; c < dim; ++c) a.avg[c] = a.sum[c] / (double)a.num;
-    a.has = 1;
-    a.num = 0;
-  }
-}
-
 // state::reinit_wall_values (cg/src/cg/simul/state.cpp:708-729): each plane
 // on a box side goes to the centre plus the projection on its normal of the
 // vector to its face; shift = new origin - old; periodic axes take the box
@@ -776,7 +802,10 @@ void launch_fix_nl(const DevBatch &b, int max_nlcap, hipStream_t s) {
   hipLaunchKernelGGL(k_nl_scan, dim3(b.nsys), dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_nl_rows<true>, rg, dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_nl_snapshot, grid1(b.ntot), dim3(BLK), 0, s, b);
-  dim3 pg((unsigned)((max_nlcap + BLK - 1) / BLK), (unsigned)b.nsys);
+  // the partition strides over the filled chunks: at most PART_GRID blocks
+  // a system (four per CU of a 256-CU device for one system)
+  const int part_grid = std::max(1, std::min((max_nlcap + BLK - 1) / BLK, PART_GRID));
+  dim3 pg((unsigned)part_grid, (unsigned)b.nsys);
   hipLaunchKernelGGL(k_part_count, pg, dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_part_scan, grid1(b.nsys, 64), dim3(64), 0, s, b);
   hipLaunchKernelGGL(k_part_write, pg, dim3(BLK), 0, s, b);
@@ -793,8 +822,14 @@ void launch_terms(const DevBatch &b, int prec, int max_items, int only_term, int
   else
     hipLaunchKernelGGL((k_terms<float, float>), g, dim3(BLK), 0, s, b, only_term, from, to);
 }
-void launch_qa_finish(const DevBatch &b, hipStream_t s) {
-  hipLaunchKernelGGL(k_qa_finish, dim3(b.nsys), dim3(BLK), 0, s, b);
+size_t qa_finish_lds(int max_n) {
+  return 4 * (((size_t)max_n + 1) & ~(size_t)1) + (size_t)QF_CAND_LDS * (8 + 8 + 4 * 4);
+}
+void launch_qa_finish(const DevBatch &b, int max_n, hipStream_t s) {
+  // the budgets in LDS while they and the candidates fit 96 KB
+  const size_t lds = qa_finish_lds(max_n);
+  const int sync_lds = lds <= QF_LDS_MAX ? max_n : 0;
+  hipLaunchKernelGGL(k_qa_finish, dim3(b.nsys), dim3(BLK), sync_lds ? lds : 0, s, b, sync_lds);
 }
 void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s) {
   if (prec == PCG_FP64)

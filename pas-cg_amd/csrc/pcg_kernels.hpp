@@ -8,6 +8,11 @@
 namespace pcg {
 
 constexpr int BLK = 256;
+// most blocks a system's partition kernels take (they stride over the chunks)
+constexpr int PART_GRID = 1024;
+// candidates the stage QA finish stages in LDS, and its LDS limit
+constexpr int QF_CAND_LDS = 1024;
+constexpr size_t QF_LDS_MAX = 96 * 1024;
 
 // Geometry and per-residue QA data from HBM.
 struct GeoG {
@@ -171,8 +176,8 @@ __device__ __forceinline__ double eval_item(const DevBatch &b, SysDescC &s, SysS
   case PCG_HEUR_DIH: return heur_dih_item<double>(b, g, acc, s.hdih0 + k);
   case PCG_CND: return nat_dih_item<double>(b, g, acc, s.ndih0 + k, 0);
   case PCG_SND: return nat_dih_item<double>(b, g, acc, s.ndih0 + k, 1);
-  case PCG_VEL_AFM: return vel_afm_item<double>(b, g, acc, s.tip0 + k, st.t - s.afm_t0);
-  case PCG_FORCE_AFM: return force_afm_item(b, acc, s.tip0 + s.ntip + k);
+  case PCG_VEL_AFM: return vel_afm_item<double>(b, g, acc, s.tip0 + k, st.t - s.afm_t0, st.t);
+  case PCG_FORCE_AFM: return force_afm_item(b, acc, s.tip0 + s.ntip + k, st.t);
   case PCG_SOLID_WALL: return solid_wall_item(b, g, acc, s.res0 + k);
   case PCG_HW_FREE: return hw_free_item(b, g, acc, s.res0 + k);
   case PCG_HW_CONN: return hw_conn_item(b, g, acc, k);

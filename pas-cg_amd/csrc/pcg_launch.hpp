@@ -10,7 +10,7 @@ void launch_prepare_nh(const DevBatch &b, hipStream_t s);
 void launch_fix_nl(const DevBatch &b, int max_nlcap, hipStream_t s);
 void launch_terms(const DevBatch &b, int prec, int max_items, int only_term, int from, int to,
                   hipStream_t s);
-void launch_qa_finish(const DevBatch &b, hipStream_t s);
+void launch_qa_finish(const DevBatch &b, int max_n, hipStream_t s);
 void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s);
 void launch_tick(const DevBatch &b, int advance, hipStream_t s);
 // walls and the box protocol of a plain context (b.wall set): LJ candidate

@@ -91,9 +91,19 @@ __device__ inline void qa_near_filters(const DevBatch &b, int i1, int i2, double
   thr2 = (float)(thr * thr);
 }
 
+// LDS staging of the candidates for a QA finish that runs alone in its
+// workgroup: their sort keys, and in acceptance order their slots,
+// residues and budget changes (8 bytes), for up to cap candidates
+struct QfLds {
+  long long *key;
+  int *idx, *sc, *i1, *i2;
+  uint64_t *sd;
+  int cap;
+};
+
 // sy[4 * (i - sy_off) + c] is sync budget c of residue i (HBM or LDS).
 __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState &st,
-                                       int8_t *sy, int sy_off) {
+                                       int8_t *sy, int sy_off, const QfLds *ql = nullptr) {
   // Nothing removed and no candidate: budgets, lists and counts stay as they
   // are (the compress condition changes only with removals or acceptances).
   // Every thread reads the same final counts, so the return is uniform.
@@ -101,17 +111,36 @@ __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState
   int rem_n = st.rem_n < sd.qccap ? st.rem_n : sd.qccap;
   int cand_n = st.cand_n < sd.nlcap ? st.cand_n : sd.nlcap;
   // rank of every candidate by (dist, free-pair index) -> permutation
+  const bool staged = ql != nullptr && cand_n <= ql->cap;
+  if (staged) {
+    for (int c = threadIdx.x; c < cand_n; c += blockDim.x) {
+      ql->key[c] = total_key(b.cand_dist[sd.nl0 + c]);
+      ql->idx[c] = b.cand_idx[sd.nl0 + c];
+    }
+    __syncthreads();
+  }
   for (int c = threadIdx.x; c < cand_n; c += blockDim.x) {
     int sc = sd.nl0 + c;
-    long long kc = total_key(b.cand_dist[sc]);
-    int ic = b.cand_idx[sc];
+    long long kc = staged ? ql->key[c] : total_key(b.cand_dist[sc]);
+    int ic = staged ? ql->idx[c] : b.cand_idx[sc];
     int rank = 0;
-    for (int o = 0; o < cand_n; ++o) {
-      int so = sd.nl0 + o;
-      long long kd = total_key(b.cand_dist[so]);
-      rank += (kd < kc || (kd == kc && b.cand_idx[so] < ic)) ? 1 : 0;
+    if (staged) {
+      for (int o = 0; o < cand_n; ++o) {
+        long long kd = ql->key[o];
+        rank += (kd < kc || (kd == kc && ql->idx[o] < ic)) ? 1 : 0;
+      }
+      ql->sc[rank] = sc;
+      ql->i1[rank] = b.cand_i[0][sc];
+      ql->i2[rank] = b.cand_i[1][sc];
+      ql->sd[rank] = *(const uint64_t *)(b.cand_sd + 8 * (size_t)sc);
+    } else {
+      for (int o = 0; o < cand_n; ++o) {
+        int so = sd.nl0 + o;
+        long long kd = total_key(b.cand_dist[so]);
+        rank += (kd < kc || (kd == kc && b.cand_idx[so] < ic)) ? 1 : 0;
+      }
+      b.cand_perm[sd.nl0 + rank] = c;
     }
-    b.cand_perm[sd.nl0 + rank] = c;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -158,13 +187,25 @@ __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState
     }
     // sequential acceptance
     for (int a = 0; a < cand_n; ++a) {
-      int sc = sd.nl0 + b.cand_perm[sd.nl0 + a];
-      int i1 = b.cand_i[0][sc], i2 = b.cand_i[1][sc];
+      int sc, i1, i2;
+      int8_t csd[8];
+      if (staged) {
+        sc = ql->sc[a];
+        i1 = ql->i1[a];
+        i2 = ql->i2[a];
+        const uint64_t v = ql->sd[a];
+        for (int c = 0; c < 8; ++c) csd[c] = (int8_t)(v >> (8 * c));
+      } else {
+        sc = sd.nl0 + b.cand_perm[sd.nl0 + a];
+        i1 = b.cand_i[0][sc];
+        i2 = b.cand_i[1][sc];
+        for (int c = 0; c < 8; ++c) csd[c] = b.cand_sd[8 * sc + c];
+      }
       bool ok = true;
       int8_t n1[4], n2[4];
       for (int c = 0; c < 4; ++c) {
-        n1[c] = (int8_t)(sy[4 * (i1 - sy_off) + c] - b.cand_sd[8 * sc + c]);
-        n2[c] = (int8_t)(sy[4 * (i2 - sy_off) + c] - b.cand_sd[8 * sc + 4 + c]);
+        n1[c] = (int8_t)(sy[4 * (i1 - sy_off) + c] - csd[c]);
+        n2[c] = (int8_t)(sy[4 * (i2 - sy_off) + c] - csd[4 + c]);
       }
       for (int c = 0; c < 4; ++c) ok = ok && n1[c] >= 0;
       if (!ok) continue;
@@ -180,7 +221,7 @@ __device__ inline void qa_finish_block(const DevBatch &b, SysDescC &sd, SysState
       b.qc_d[q] = b.cand_od[sc];
       b.qc_t[q] = b.cand_t[sc];
       b.qc_sat[q] = 0.0;
-      for (int c = 0; c < 8; ++c) b.qc_sd[8 * q + c] = b.cand_sd[8 * sc + c];
+      for (int c = 0; c < 8; ++c) b.qc_sd[8 * q + c] = csd[c];
       b.qc_vac[q] = 0;
       st.qc_n += 1;
       st.num_qa += 1;
