@@ -250,11 +250,13 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
   uint32_t chg = 0u;
   for (int t = 0; t < 21; ++t) chg |= (p.charge[t] != 0.0 ? 1u : 0u) << t;
   const int kbeg = rb_row_start(w, nw, n), kend = rb_row_start(w + 1, nw, n);
-  // When every pair fits the NL capacity, the first pass also stores the
-  // wave's NL pairs in order (k | j << 16 in nl_i[0], their classes in
-  // nl_cls, a native contact's index in nl_i[1]) from the index of its
-  // block's first pair, which no other wave's pairs reach, and the second
-  // pass walks those instead of every pair of the block.
+  // When every pair fits the NL capacity, the first pass also stores, in
+  // order, the wave's pairs that the second pass writes somewhere (native
+  // contacts, QA near pairs, PID, DH and Pauli pairs; QA free pairs beyond
+  // the near distance are only counted): k | j << 16 in nl_i[0], the
+  // classes in nl_cls, a native contact's index in nl_i[1], from the index
+  // of its block's first pair, which no other wave's pairs reach. The
+  // second pass walks those instead of every pair of the block.
   const bool compact = (long long)n * (n - 1) / 2 <= (long long)sd.nlcap;
   const int seg = sd.nl0 + (int)((long long)kbeg * (n - 1) - (long long)kbeg * (kbeg - 1) / 2);
   int my_nl = 0;
@@ -336,15 +338,14 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
         const int cls = v ? (int)b.nl_cls[seg + t] : 0;
         const int k = (int)(kw & 0xffffu), j = (int)(kw >> 16);
         const bool is_nc = cls & CL_NC, near = cls & CL_NEAR, is_pid = cls & CL_PID,
-                   is_dh = cls & CL_DH, is_pauli = cls & CL_PAULI, is_qa = cls & CL_QA;
+                   is_dh = cls & CL_DH, is_pauli = cls & CL_PAULI;
         const int c = is_nc ? b.nl_i[1][seg + t] : 0;
-        const unsigned long long m_nc = __ballot(is_nc), m_qa = __ballot(is_qa),
-                                 m_near = __ballot(near), m_pid = __ballot(is_pid),
-                                 m_dh = __ballot(is_dh), m_pauli = __ballot(is_pauli);
+        const unsigned long long m_nc = __ballot(is_nc), m_near = __ballot(near),
+                                 m_pid = __ballot(is_pid), m_dh = __ballot(is_dh),
+                                 m_pauli = __ballot(is_pauli);
         if (v) emit(k, j, c, is_nc, near, is_pid, is_dh, is_pauli, m_nc, m_near, m_pid, m_dh,
                     m_pauli, off);
         off[RB_NC] += __popcll(m_nc);
-        off[RB_QA] += __popcll(m_qa);
         off[RB_NEAR] += __popcll(m_near);
         off[RB_PID] += __popcll(m_pid);
         off[RB_DH] += __popcll(m_dh);
@@ -412,13 +413,18 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
           const int c = is_nc ? sd.nc0 + tp0 + __popcll(natm & below) : 0;
           if (valid) emit(k, j, c, is_nc, near, is_pid, is_dh, is_pauli, m_nc, m_near, m_pid,
                           m_dh, m_pauli, off);
-        } else if (compact & in) {
-          const int t = seg + off[RB_NL] + __popcll(m_nl & below);
-          b.nl_i[0][t] = (int)((uint32_t)k | ((uint32_t)j << 16));
-          b.nl_cls[t] = (uint8_t)((is_nc ? CL_NC : 0) | (is_qa ? CL_QA : 0) | (near ? CL_NEAR : 0) |
-                                  (is_pid ? CL_PID : 0) | (is_dh ? CL_DH : 0) |
-                                  (is_pauli ? CL_PAULI : 0));
-          if (is_nc) b.nl_i[1][t] = sd.nc0 + tp0 + __popcll(natm & below);
+        } else if (compact) {
+          const bool keep = is_nc | near | is_pid | is_dh | is_pauli;
+          const unsigned long long m_keep = __ballot(keep);
+          if (keep) {
+            const int t = seg + my_nl + __popcll(m_keep & below);
+            b.nl_i[0][t] = (int)((uint32_t)k | ((uint32_t)j << 16));
+            b.nl_cls[t] = (uint8_t)((is_nc ? CL_NC : 0) | (near ? CL_NEAR : 0) |
+                                    (is_pid ? CL_PID : 0) | (is_dh ? CL_DH : 0) |
+                                    (is_pauli ? CL_PAULI : 0));
+            if (is_nc) b.nl_i[1][t] = sd.nc0 + tp0 + __popcll(natm & below);
+          }
+          my_nl += __popcll(m_keep);
         }
         off[RB_NL] += __popcll(m_nl);
         off[RB_NC] += __popcll(m_nc);
@@ -430,7 +436,6 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
       }
     }
     if (!pass) {
-      my_nl = off[RB_NL];
       if (lane == 0)
         for (int c = 0; c < RB_N; ++c) wcnt[w][c] = off[c];
       __syncthreads();
