@@ -244,6 +244,7 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
   const float e_nl[2] = {s_nl * mlo, s_nl * mhi}, e_nc[2] = {s_nc * mlo, s_nc * mhi},
               e_dh[2] = {s_dh * mlo, s_dh * mhi}, e_rep[2] = {s_rep * mlo, s_rep * mhi},
               e_near[2] = {s_near * mlo, s_near * mhi};
+  const float e_far = fmaxf(fmaxf(fmaxf(e_nl[1], e_nc[1]), fmaxf(e_dh[1], e_rep[1])), e_near[1]);
   const bool qa_on = p.qa_on, pid_on = p.pid_on, dh_on = p.dh_on, pauli_on = P.pauli_on;
   const bool qa4 = p.qa_include4, pid4 = p.pid_include4;
   // residue types with a charge, as bits (q1 q2 != 0 iff both are set)
@@ -373,14 +374,18 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
         // past the row's end); the bitwise & and | keep it branch-free
         const bool valid = j < n;
         const int jj = valid ? j : n - 1;
-        const int dseq = abs(sqi[jj] - si);
-        const bool same = chn[jj] == ci;
-        const bool four = same & (dseq == 4);
-        const bool pair = valid & !(same & (dseq < 3));
         v3<double> dd = xsub(v3<double>{g.x[jj], g.y[jj], g.z[jj]}, v3<double>{xi, yi, zi});
         if (cell) dd = xwrapd(dd, sd);
         const float fx = (float)dd.x, fy = (float)dd.y, fz = (float)dd.z;
         const float d2 = fx * fx + fy * fy + fz * fz;
+        // columns beyond every threshold and its margin join no list: a
+        // chunk of only such columns adds nothing (its native partners are
+        // already passed)
+        if (!__ballot(valid & (d2 <= e_far))) continue;
+        const int dseq = abs(sqi[jj] - si);
+        const bool same = chn[jj] == ci;
+        const bool four = same & (dseq == 4);
+        const bool pair = valid & !(same & (dseq < 3));
         bool b_nl = d2 < s_nl, b_nc = d2 < s_nc, b_dh = d2 < s_dh, b_rep = d2 < s_rep;
         bool b_near = d2 < s_near;
         const bool edge = pair & ((d2 >= e_nl[0]) & (d2 <= e_nl[1]) | (d2 >= e_nc[0]) & (d2 <= e_nc[1]) |
