@@ -484,8 +484,9 @@ int pcg_server_destroy(pcg_server *s);
 int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature,
                       int32_t seed, int64_t nsteps, int64_t *job_id);
 /* one scheduling round: admit by budget, run chunk_steps for every resident
- * job in one launch, retire finished jobs. Returns the number of jobs still
- * queued or running in *active. */
+ * job in one launch per size class (while no job waits for admission, up to
+ * four chunks, never past the class's nearest job end), retire finished
+ * jobs. Returns the number of jobs still queued or running in *active. */
 int pcg_server_round(pcg_server *s, int32_t *active);
 int pcg_server_status(pcg_server *s, int64_t job, int32_t *status,
                       int64_t *steps_done);

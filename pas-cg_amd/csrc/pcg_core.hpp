@@ -35,6 +35,9 @@ struct SysDesc {
   int qf0, qfcap;   // QA free pairs
   int qc0, qccap;   // QA contacts and removed indices
   int bond_regular;  // every static bonded item is chain-local (see bond_item)
+  // cell grid of the stage-form NL rebuild (cg[0] == 0: no grid): cg[a]
+  // cells along axis a of the periodic cell, counts and starts at cell0
+  int cell0, cg[3];
   double cell[3], cell_inv[3];
   double temperature, const2; // const2 = sqrt(2 kB T gamma dt) dt (step.cpp:7-17)
   uint64_t philox_key;
@@ -51,6 +54,12 @@ struct SysDesc {
 // Dynamic LDS of one persistent workgroup, at most: 160 KiB per CU on
 // gfx950, less the kernel's static LDS (about 5 KiB).
 constexpr size_t LDS_DYN_MAX = 154 * 1024;
+// the stage rebuild's cell grid: smallest system that takes one, most cells
+// of one system, most 64-bit row-mask words (residues / 64) of one system
+// (the NL row kernel holds one mask a wave in LDS: 4 x 8 x words bytes)
+constexpr int CELL_MIN_N = 2048;
+constexpr long long CELL_MAX = 1 << 21;
+constexpr int CELL_WORDS_MAX = 2048;
 
 struct LdsLayout {
   // 32-bit words of the per-wave stacks of QA pairs past the first test
@@ -277,6 +286,10 @@ struct DevBatch {
   int *cand_perm;
   int *rem;
   int *row_cnt;               // NL rebuild scratch: per-residue row counts / offsets
+  // cell grid of the NL rebuild: each residue's cell, the residues by cell
+  // (system-local indices), each cell's first slot and its end (a count
+  // between the rebuild's kernels, zero between rebuilds)
+  int *cell_of, *cell_items, *cell_start, *cell_end;
   int *blk_cnt;               // partition scratch: per-chunk counts
   WallDev *wall;              // plain context: walls and the box protocol (else null)
 };

@@ -375,6 +375,7 @@ struct DeviceBatch {
   Arena arena;
   int max_nlcap = 0, max_n = 0, max_items = 0;
   int max_nnc = 0;  // most native contacts of one system
+  int cell_words = 0;  // row-mask words of the largest system on a cell grid
   size_t bytes = 0;
   std::vector<Ran2> ran2;  // per system, RAN2 noise mode
   std::vector<double> noise_h;
@@ -441,7 +442,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   }
   // offsets
   long long ntot = 0, nteth = 0, nlrep = 0, nchir = 0, nnang = 0, nhang = 0, nndih = 0,
-            nhdih = 0, nnc = 0, ntip = 0, nl = 0, qf = 0, qc = 0;
+            nhdih = 0, nnc = 0, ntip = 0, nl = 0, qf = 0, qc = 0, ncells = 0;
   B->sys.resize(nsys);
   for (int s = 0; s < nsys; ++s) {
     HostSys &h = B->hs[s];
@@ -465,6 +466,27 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     for (int c = 0; c < 3; ++c) {
       d.cell[c] = cell[c];
       d.cell_inv[c] = cell[c] != 0.0 ? 1.0 / cell[c] : 0.0;
+    }
+    // the stage rebuild's cell grid: a system of at least CELL_MIN_N
+    // residues in a cell periodic along every axis with room for three
+    // cells a side of at least the NL radius (with a relative 1e-6 to spare)
+    d.cell0 = (int)ncells;
+    d.cg[0] = d.cg[1] = d.cg[2] = 0;
+    {
+      const double req = (std::max(p.nl_cutoff, p.dh_cutoff) + p.nl_pad) * (1.0 + 1e-6);
+      long long cells = 1;
+      int gc[3];
+      bool ok = h.n >= CELL_MIN_N;
+      for (int c = 0; c < 3 && ok; ++c) {
+        gc[c] = cell[c] > 0.0 ? (int)std::min(std::floor(cell[c] / req), 1024.0) : 0;
+        ok = gc[c] >= 3;
+        cells *= ok ? gc[c] : 1;
+      }
+      if (ok && cells <= CELL_MAX && (h.n + 63) / 64 <= CELL_WORDS_MAX) {
+        for (int c = 0; c < 3; ++c) d.cg[c] = gc[c];
+        ncells += cells;
+        B->cell_words = std::max(B->cell_words, (h.n + 63) / 64);
+      }
     }
     d.temperature = h.temperature;
     d.const2 = const2_of(p, h.temperature);
@@ -513,6 +535,10 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   A.add(&local, ntot);
   A.add(&D.sync, 4 * ntot);
   A.add(&D.row_cnt, ntot + 1);
+  A.add(&D.cell_of, ntot);
+  A.add(&D.cell_items, ntot);
+  A.add(&D.cell_start, std::max(ncells, 1LL));
+  A.add(&D.cell_end, std::max(ncells, 1LL));
   int *teth_i[2], *lrep_i[2], *chir_i[4], *nang_i[3], *hang_i[3], *ndih_i[4], *hdih_i[4], *nc_i[2];
   double *teth_nat, *chir_f0, *chir_f1, *nang_th, *ndih_ph, *nc_nat;
   uint8_t *hang_t, *hdih_t, *nc_type;
@@ -1217,16 +1243,14 @@ void enqueue_step(DeviceBatch &B) {
   const DevBatch &d = B.d;
   const pcg_params &p = B.hp.p;
   if (B.box_set) launch_box_phase(d, B.dP, sys_rw(B), B.stream);
-  launch_begin(d, B.stream);
-  launch_zero_forces(d, B.prec, B.stream);
-  if (p.qa_on) launch_prepare_nh(d, B.stream);
-  launch_fix_nl(d, B.max_nlcap, B.stream);
+  launch_step_begin(d, B.prec, p.qa_on, B.stream);
+  launch_fix_nl(d, B.max_nlcap, B.cell_words, B.stream, false);
   launch_terms(d, B.prec, B.max_items, -1, 0, -1, B.stream);
   if (p.qa_on) launch_qa_finish(d, B.max_n, B.stream);
   if (B.hp.ljw_terms) launch_ljw_process(d, B.stream);
   if (B.box_set) launch_box_log(d, B.stream);
   launch_lang(d, B.prec, B.noise, B.stream);
-  launch_tick(d, 1, B.stream);
+  launch_tick(d, 1, B.stream, true);
 }
 
 // the reaction forces of the walls start a term evaluation at zero
@@ -1413,7 +1437,7 @@ int pcg_fix_nl(pcg_ctx *c) {
   st.nl_invalid = 1;
   st.verify_first = 0;
   if ((rc = write_state(B, 0, st))) return rc;
-  launch_fix_nl(B.d, B.max_nlcap, B.stream);
+  launch_fix_nl(B.d, B.max_nlcap, B.cell_words, B.stream);
   HIPCHK(hipStreamSynchronize(B.stream));
   return check_overflow(B);
 }
@@ -2020,7 +2044,7 @@ int ckpt_write(DeviceBatch &B, const char magic[8], void *buf, int64_t bytes) {
   HIPCHK(hipSetDevice(B.device));
   CkptHeader h{};
   std::memcpy(h.magic, magic, 8);
-  h.version = 4;
+  h.version = 5;
   h.nsys = (int32_t)B.sys.size();
   h.ntot = B.d.ntot;
   h.prec = B.prec;
@@ -2041,7 +2065,7 @@ int ckpt_read(DeviceBatch &B, const char magic[8], const void *buf, int64_t byte
   CkptHeader h;
   if (bytes < (int64_t)sizeof h) return fail(PCG_E_ARG, "not a checkpoint");
   std::memcpy(&h, buf, sizeof h);
-  if (std::memcmp(h.magic, magic, 8) != 0 || h.version != 4) return fail(PCG_E_ARG, "not a checkpoint");
+  if (std::memcmp(h.magic, magic, 8) != 0 || h.version != 5) return fail(PCG_E_ARG, "not a checkpoint");
   if (h.nsys != (int32_t)B.sys.size() || h.ntot != B.d.ntot || h.prec != B.prec ||
       h.arena_bytes != B.arena.total || h.fingerprint != ckpt_fingerprint(B))
     return fail(PCG_E_ARG, "checkpoint of a different batch (models, parameters or precision)");
@@ -2725,11 +2749,23 @@ int serve_launch(pcg_server *s, int k) {
     HIPCHK(hipHostMalloc(&bk.h_slots, bk.cap * sizeof(JobSlot)));
   }
   const size_t fsz = sizeof(double);  // the persistent kernel's LDS forces are FP64
+  // The launch's length: chunk_steps; while no job waits for admission,
+  // up to four chunks, but no longer than the class's nearest job end (so
+  // no workgroup idles to the launch's end and no finished job waits)
+  int64_t round = s->o.chunk_steps;
+  if (s->queue.empty()) {
+    int64_t rem = INT64_MAX;
+    for (int q = 0; q < nres; ++q) {
+      const Job &j = *s->jobs[bk.resident[q]];
+      rem = std::min<int64_t>(rem, j.nsteps - j.done);
+    }
+    round = std::max<int64_t>(round, std::min<int64_t>(rem, 4 * (int64_t)s->o.chunk_steps));
+  }
   size_t lds = 0;
   for (int q = 0; q < nres; ++q) {
     Job &j = *s->jobs[bk.resident[q]];
     DeviceBatch &B = *j.B;
-    bk.h_slots[q] = JobSlot{std::min<int64_t>(j.done + s->o.chunk_steps, j.nsteps), 0, 0.0, 0, 0};
+    bk.h_slots[q] = JobSlot{std::min<int64_t>(j.done + round, j.nsteps), 0, 0.0, 0, 0};
     // each job stages its bonded data and its Gear state in LDS when that
     // fits on its own
     const size_t nj = (size_t)B.max_n, cj = (size_t)B.max_nnc;
@@ -2744,7 +2780,7 @@ int serve_launch(pcg_server *s, int k) {
   HIPCHK(hipMemcpyAsync(bk.d_slots, bk.h_slots, nres * sizeof(JobSlot), hipMemcpyHostToDevice,
                         bk.stream));
   launch_persistent_jobs(bk.d_jobs, bk.d_slots, nres, s->o.precision, SERVE_BUCKET_BLOCK[k],
-                         s->o.chunk_steps, lds, bk.stream);
+                         (int)round, lds, bk.stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(bk.h_slots, bk.d_slots, nres * sizeof(JobSlot), hipMemcpyDeviceToHost,
                         bk.stream));

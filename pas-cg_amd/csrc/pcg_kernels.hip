@@ -14,10 +14,8 @@ namespace pcg {
 // ------------------------------------------------------------ begin step ----
 // st.V zeroed; NL verify decision (nl::verify, cg/src/cg/nl/verify.cpp:3-34)
 // from the displacement maximum the previous Langevin step left behind.
-__global__ void k_begin(DevBatch b) {
+__device__ __forceinline__ void begin_system(const DevBatch &b, int s) {
 #pragma clang fp contract(off)
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= b.nsys) return;
   SysState &st = b.st[s];
   SysDescC &d = b.sys[s];
   if (!st.active) return;
@@ -38,6 +36,12 @@ __global__ void k_begin(DevBatch b) {
     }
   }
   st.max_disp_bits = 0ull;
+}
+
+__global__ void k_begin(DevBatch b) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b.nsys) return;
+  begin_system(b, s);
 }
 
 template <typename RF> __global__ void k_zero_forces(DevBatch b) {
@@ -66,7 +70,103 @@ __device__ __forceinline__ bool nl_pred(const DevBatch &b, SysDescC &s, const Ge
   return dist < req;
 }
 
-template <bool WRITE> __global__ void k_nl_rows(DevBatch b) {
+// ------------------------------------------------------------- cell grid ----
+// A large periodic system can take its NL rows from a cell grid instead of
+// every later residue: the periodic cell cut into cg[a] >= 3 cells a side,
+// each at least the NL radius wide (with a relative 1e-6 to spare), so every
+// pair within the radius sits in neighbouring cells (the 27 of the stencil,
+// periodic). A row's candidates from those cells go into a bitmask over the
+// system's residues, which the row then walks in increasing column order:
+// the same pairs, tested by the same predicate, in the same order as the
+// scan over all columns. When the cell has shrunk below that (the box
+// protocol), the rebuild scans every column.
+__device__ __forceinline__ bool use_cells(const DevBatch &b, SysDescC &sd) {
+  if (sd.cg[0] <= 0) return false;
+  const double req = (b.P->nl_eff_cutoff + b.P->p.nl_pad) * (1.0 + 1e-6);
+  for (int a = 0; a < 3; ++a)
+    if (!(sd.cell[a] >= req * (double)sd.cg[a])) return false;
+  return true;
+}
+__device__ __forceinline__ int cell_coord(SysDescC &sd, int a, double x) {
+  // the position wrapped into [0, cell), then its cell along the axis
+  const double u = x - sd.cell[a] * floor(x * sd.cell_inv[a]);
+  const int k = (int)(u * sd.cell_inv[a] * (double)sd.cg[a]);
+  return k < 0 ? 0 : (k >= sd.cg[a] ? sd.cg[a] - 1 : k);
+}
+
+// each residue's cell; cell_end counts the residues of each cell
+__global__ void k_cell_count(DevBatch b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.ntot) return;
+  const int s = b.sysid[i];
+  const SysState &st = b.st[s];
+  if (!st.active || !st.nl_invalid) return;
+  SysDescC &sd = b.sys[s];
+  if (!use_cells(b, sd)) return;
+  const int c = (cell_coord(sd, 2, b.y[0][2][i]) * sd.cg[1] + cell_coord(sd, 1, b.y[0][1][i])) *
+                    sd.cg[0] + cell_coord(sd, 0, b.y[0][0][i]);
+  b.cell_of[i] = c;
+  atomicAdd(&b.cell_end[sd.cell0 + c], 1);
+}
+
+// one block a system: the cells' first slots (exclusive scan of the
+// counts); cell_end becomes the fill pointer
+__global__ void k_cell_scan(DevBatch b) {
+  const int s = blockIdx.x;
+  const SysState &st = b.st[s];
+  if (!st.active || !st.nl_invalid) return;
+  SysDescC &sd = b.sys[s];
+  if (!use_cells(b, sd)) return;
+  const int nc = sd.cg[0] * sd.cg[1] * sd.cg[2];
+  __shared__ int carry;
+  __shared__ int wsum[BLK / 64];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int base = 0; base < nc; base += blockDim.x) {
+    const int c = base + threadIdx.x;
+    const int v = c < nc ? b.cell_end[sd.cell0 + c] : 0;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int off = carry;
+    for (int k = 0; k < w; ++k) off += wsum[k];
+    if (c < nc) {
+      b.cell_start[sd.cell0 + c] = off + x - v;
+      b.cell_end[sd.cell0 + c] = off + x - v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) tot += wsum[k];
+      carry += tot;
+    }
+    __syncthreads();
+  }
+}
+
+// the residues into their cells' slots (any order within a cell: a row's
+// candidates go through a bitmask); cell_end ends at each cell's end
+__global__ void k_cell_fill(DevBatch b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.ntot) return;
+  const int s = b.sysid[i];
+  const SysState &st = b.st[s];
+  if (!st.active || !st.nl_invalid) return;
+  SysDescC &sd = b.sys[s];
+  if (!use_cells(b, sd)) return;
+  const int slot = atomicAdd(&b.cell_end[sd.cell0 + b.cell_of[i]], 1);
+  b.cell_items[sd.res0 + slot] = i - sd.res0;
+}
+
+// cell_words: 64-bit mask words of the largest system on the grid (the
+// dynamic LDS holds one row mask a wave), 0 without a grid
+template <bool WRITE> __global__ void k_nl_rows(DevBatch b, int cell_words) {
+  extern __shared__ unsigned long long row_mask[];
   int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   int lane = threadIdx.x & 63;
   if (row >= b.ntot) return;
@@ -75,10 +175,64 @@ template <bool WRITE> __global__ void k_nl_rows(DevBatch b) {
   if (!st.active || !st.nl_invalid) return;
   SysDescC &sd = b.sys[s];
   GeoG g{b.y[0][0], b.y[0][1], b.y[0][2]};
+  // the snapshot of the rebuild's positions (legacy_update.cpp:80-89)
+  if (WRITE && lane == 0)
+    for (int c = 0; c < 3; ++c) b.orig_r[c][row] = b.y[0][c][row];
   double req = b.P->nl_eff_cutoff + b.P->p.nl_pad;
   int end = sd.res0 + sd.n;
   int out = WRITE ? b.row_cnt[row] : 0;
   int cnt = 0;
+  if (cell_words > 0 && use_cells(b, sd)) {
+    unsigned long long *m = row_mask + (size_t)(threadIdx.x >> 6) * cell_words;
+    const int il = row - sd.res0, W = (sd.n + 63) >> 6, w0 = (il + 1) >> 6;
+    for (int w = w0 + lane; w < W; w += 64) m[w] = 0ull;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int c = b.cell_of[row], gx = sd.cg[0], gy = sd.cg[1], gz = sd.cg[2];
+    const int cx = c % gx, cy = (c / gx) % gy, cz = c / (gx * gy);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int nx = (cx + dx + gx) % gx, ny = (cy + dy + gy) % gy, nz = (cz + dz + gz) % gz;
+          const int nc = sd.cell0 + (nz * gy + ny) * gx + nx;
+          const int t1 = b.cell_end[nc];
+          for (int t = b.cell_start[nc] + lane; t < t1; t += 64) {
+            const int j = b.cell_items[sd.res0 + t];
+            if (j > il) atomicOr(&m[j >> 6], 1ull << (j & 63));
+          }
+        }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int wb = w0; wb < W; wb += 64) {
+      const int w = wb + lane;
+      const unsigned long long bits = w < W ? m[w] : 0ull;
+      unsigned long long nz = __ballot(bits != 0ull);
+      while (nz) {
+        const int k = __ffsll((long long)nz) - 1;
+        nz &= nz - 1ull;
+        const unsigned lo = __shfl((unsigned)bits, k, 64), hi = __shfl((unsigned)(bits >> 32), k, 64);
+        const unsigned long long bw = (unsigned long long)lo | ((unsigned long long)hi << 32);
+        const int j = sd.res0 + (wb + k) * 64 + lane;
+        double dist = 0.0;
+        const bool keep = ((bw >> lane) & 1ull) && nl_pred(b, sd, g, row, j, req, dist);
+        const unsigned long long mm = __ballot(keep);
+        if (WRITE && keep) {
+          int pos = out + cnt + __popcll(mm & ((1ull << lane) - 1ull));
+          if (pos < sd.nlcap) {
+            int slot = sd.nl0 + pos;
+            b.nl_i[0][slot] = row;
+            b.nl_i[1][slot] = j;
+            b.nl_d[slot] = dist;
+          }
+        }
+        cnt += __popcll(mm);
+      }
+    }
+    if (!WRITE && lane == 0) b.row_cnt[row] = cnt;
+    return;
+  }
   for (int j0 = row + 1; j0 < end; j0 += 64) {
     int j = j0 + lane;
     double dist = 0.0;
@@ -143,16 +297,6 @@ __global__ void k_nl_scan(DevBatch b) {
   }
 }
 
-// Snapshot after a rebuild: orig_r, orig_pad, orig_pbc (legacy_update.cpp:80-89)
-__global__ void k_nl_snapshot(DevBatch b) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.ntot) return;
-  int s = b.sysid[i];
-  const SysState &st = b.st[s];
-  if (!st.active || !st.nl_invalid) return;
-  for (int c = 0; c < 3; ++c) b.orig_r[c][i] = b.y[0][c][i];
-}
-
 // ------------------------------------------------------------ list updates ----
 // The ordered updates of thread::fix_nl_async (thread_main.cpp:409-423):
 // native contacts take first (nat_cont/update_contacts.cpp:7-37), then QA
@@ -208,6 +352,11 @@ __global__ void k_part_count(DevBatch b) {
   const SysState &st = b.st[s];
   if (!st.active || !st.nl_invalid) return;
   SysDescC &sd = b.sys[s];
+  // the cell counts start the next rebuild at zero (the rows are written)
+  if (sd.cg[0] > 0)
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x, nc = sd.cg[0] * sd.cg[1] * sd.cg[2]; c < nc;
+         c += gridDim.x * blockDim.x)
+      b.cell_end[sd.cell0 + c] = 0;
   const int nchunks = (st.nl_n + BLK - 1) / BLK;
   for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) part_count_chunk(b, sd, st, ch);
 }
@@ -346,6 +495,26 @@ __global__ void k_prepare_nh(DevBatch b) {
   prepare_nh_item(b, g, i);
 }
 
+// The first kernel of a full step, over residues: each system's first
+// residue runs the system's begin (energies and the walls' reaction forces
+// cleared, nl::verify), every residue clears its force and, with QA on,
+// computes its n and h (k_begin, k_zero_forces and k_prepare_nh in one
+// launch; none reads what another writes)
+template <typename RF> __global__ void k_step_begin(DevBatch b, int qa_on) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.ntot) return;
+  const int s = b.sysid[i];
+  if (b.local[i] == 0) begin_system(b, s);
+  if (!b.st[s].active) return;
+  ((RF *)b.F[0])[i] = 0;
+  ((RF *)b.F[1])[i] = 0;
+  ((RF *)b.F[2])[i] = 0;
+  if (qa_on) {
+    GeoG g{b.y[0][0], b.y[0][1], b.y[0][2]};
+    prepare_nh_item(b, g, i);
+  }
+}
+
 // The serial part of the finish (removals, then acceptance in order) reads
 // and writes the sync budgets of each candidate's residues: with
 // sync_lds the system's budgets sit in LDS for the kernel (one word a
@@ -422,11 +591,16 @@ template <typename RF> __global__ void k_lang(DevBatch b, int noise_mode) {
 }
 
 // time and energy bookkeeping at the end of a step (step.cpp:157-162)
-__global__ void k_tick(DevBatch b, int advance) {
+__global__ void k_tick(DevBatch b, int advance, int nl_done) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= b.nsys) return;
   SysState &st = b.st[s];
   if (!st.active) return;
+  // a full step's rebuild, if it made one, is complete (k_nl_done)
+  if (nl_done && st.nl_invalid) {
+    st.nl_invalid = 0;
+    st.rebuilds += 1;
+  }
   double v = 0.0;
   for (int k = 0; k < NT; ++k) v += st.V[k];
   st.Vtot = v;
@@ -795,13 +969,24 @@ void launch_zero_forces(const DevBatch &b, int prec, hipStream_t s) {
 void launch_prepare_nh(const DevBatch &b, hipStream_t s) {
   hipLaunchKernelGGL(k_prepare_nh, grid1(b.ntot), dim3(BLK), 0, s, b);
 }
-void launch_fix_nl(const DevBatch &b, int max_nlcap, hipStream_t s) {
+void launch_step_begin(const DevBatch &b, int prec, int qa_on, hipStream_t s) {
+  if (prec == PCG_FP64)
+    hipLaunchKernelGGL(k_step_begin<double>, grid1(b.ntot), dim3(BLK), 0, s, b, qa_on);
+  else
+    hipLaunchKernelGGL(k_step_begin<float>, grid1(b.ntot), dim3(BLK), 0, s, b, qa_on);
+}
+void launch_fix_nl(const DevBatch &b, int max_nlcap, int cell_words, hipStream_t s, bool done) {
   int rows_per_block = BLK / 64;
   dim3 rg((unsigned)((b.ntot + rows_per_block - 1) / rows_per_block));
-  hipLaunchKernelGGL(k_nl_rows<false>, rg, dim3(BLK), 0, s, b);
+  const size_t mask_lds = (size_t)rows_per_block * cell_words * sizeof(unsigned long long);
+  if (cell_words > 0) {
+    hipLaunchKernelGGL(k_cell_count, grid1(b.ntot), dim3(BLK), 0, s, b);
+    hipLaunchKernelGGL(k_cell_scan, dim3(b.nsys), dim3(BLK), 0, s, b);
+    hipLaunchKernelGGL(k_cell_fill, grid1(b.ntot), dim3(BLK), 0, s, b);
+  }
+  hipLaunchKernelGGL(k_nl_rows<false>, rg, dim3(BLK), mask_lds, s, b, cell_words);
   hipLaunchKernelGGL(k_nl_scan, dim3(b.nsys), dim3(BLK), 0, s, b);
-  hipLaunchKernelGGL(k_nl_rows<true>, rg, dim3(BLK), 0, s, b);
-  hipLaunchKernelGGL(k_nl_snapshot, grid1(b.ntot), dim3(BLK), 0, s, b);
+  hipLaunchKernelGGL(k_nl_rows<true>, rg, dim3(BLK), mask_lds, s, b, cell_words);
   // the partition strides over the filled chunks: at most PART_GRID blocks
   // a system (four per CU of a 256-CU device for one system)
   const int part_grid = std::max(1, std::min((max_nlcap + BLK - 1) / BLK, PART_GRID));
@@ -809,7 +994,7 @@ void launch_fix_nl(const DevBatch &b, int max_nlcap, hipStream_t s) {
   hipLaunchKernelGGL(k_part_count, pg, dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_part_scan, grid1(b.nsys, 64), dim3(64), 0, s, b);
   hipLaunchKernelGGL(k_part_write, pg, dim3(BLK), 0, s, b);
-  hipLaunchKernelGGL(k_nl_done, grid1(b.nsys, 64), dim3(64), 0, s, b);
+  if (done) hipLaunchKernelGGL(k_nl_done, grid1(b.nsys, 64), dim3(64), 0, s, b);
 }
 void launch_terms(const DevBatch &b, int prec, int max_items, int only_term, int from, int to,
                   hipStream_t s) {
@@ -837,8 +1022,8 @@ void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s) {
   else
     hipLaunchKernelGGL(k_lang<float>, grid1(b.ntot), dim3(BLK), 0, s, b, noise_mode);
 }
-void launch_tick(const DevBatch &b, int advance, hipStream_t s) {
-  hipLaunchKernelGGL(k_tick, grid1(b.nsys, 64), dim3(64), 0, s, b, advance);
+void launch_tick(const DevBatch &b, int advance, hipStream_t s, bool nl_done) {
+  hipLaunchKernelGGL(k_tick, grid1(b.nsys, 64), dim3(64), 0, s, b, advance, nl_done ? 1 : 0);
 }
 void launch_ljw_process(const DevBatch &b, hipStream_t s) {
   hipLaunchKernelGGL(k_ljw_process, dim3(1), dim3(1024), 0, s, b);
