@@ -731,8 +731,21 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
     st[s].active = 1;
   }
   if (plain) st[0].n_hw = (int)hw.size();
+  // Every array set from the host but the walls' sits before the first
+  // NL-sized array: those go into one host image of that prefix of the
+  // arena, zeros elsewhere as the arena starts, which goes up in one copy
+  // (a job's admission is then one copy and one synchronisation); writes
+  // past the prefix are copied on their own.
+  const size_t prefix = (size_t)((char *)D.nl_i[0] - (char *)A.base);
+  std::vector<char> image(prefix, 0);
   auto up = [&](void *dst_, const void *src, size_t bytes) -> int {
-    if (bytes) HIPCHK(hipMemcpyAsync(dst_, src, bytes, hipMemcpyHostToDevice, B->stream));
+    if (!bytes) return PCG_OK;
+    const size_t off = (size_t)((char *)dst_ - (char *)A.base);
+    if ((char *)dst_ >= (char *)A.base && off + bytes <= prefix) {
+      std::memcpy(image.data() + off, src, bytes);
+      return PCG_OK;
+    }
+    HIPCHK(hipMemcpyAsync(dst_, src, bytes, hipMemcpyHostToDevice, B->stream));
     return PCG_OK;
   };
   std::vector<double> col(ntot);
@@ -881,6 +894,7 @@ int make_batch(const pcg_params &p, int nsys, const pcg_model *models, const dou
   std::vector<char> zero(ntot * fsz, 0);
   for (int c = 0; c < 3; ++c)
     if ((rc = up(D.F[c], zero.data(), zero.size()))) return rc;
+  if (prefix) HIPCHK(hipMemcpyAsync(A.base, image.data(), prefix, hipMemcpyHostToDevice, B->stream));
   HIPCHK(hipStreamSynchronize(B->stream));
   out = std::move(B);
   return PCG_OK;
@@ -2835,8 +2849,10 @@ int serve_admit(pcg_server *s) {
     ServeBucket &bk = s->bucket[k];
     pcg_model v = j.model.view();
     uint64_t key = mix64(s->o.philox_seed ^ mix64((uint64_t)(uint32_t)j.seed));
+    // set up on the server's own stream: a class's stream may be busy with
+    // its launch, which the set-up's synchronisation would wait for
     int rc = make_batch(s->p, 1, &v, &j.T, &j.seed, s->o.precision, key, s->o.device, j.B,
-                        bk.stream, nullptr, 0, blk, blk_bytes);
+                        s->stream, nullptr, 0, blk, blk_bytes);
     if (rc) {
       s->pool.emplace(blk_bytes, blk);
       s->pooled_bytes += blk_bytes;
