@@ -361,30 +361,59 @@ __global__ void k_part_count(DevBatch b) {
   for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) part_count_chunk(b, sd, st, ch);
 }
 
+// One block a system: every class's per-chunk counts become their exclusive
+// prefix in chunk order, and the totals the lists' sizes. Each thread takes
+// a contiguous run of chunks; a scan over the threads gives each run its
+// offsets.
 __global__ void k_part_scan(DevBatch b) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= b.nsys) return;
+  const int s = blockIdx.x;
   SysState &st = b.st[s];
   if (!st.active || !st.nl_invalid) return;
   SysDescC &sd = b.sys[s];
-  int nchunks = (st.nl_n + BLK - 1) / BLK;
-  int tot[NCL] = {0, 0, 0, 0, 0, 0};
-  for (int ch = 0; ch < nchunks; ++ch) {
-    int base = (sd.nl0 / BLK + ch) * NCL;
-    for (int c = 0; c < NCL; ++c) {
-      int v = b.blk_cnt[base + c];
-      b.blk_cnt[base + c] = tot[c];
-      tot[c] += v;
+  const int nchunks = (st.nl_n + BLK - 1) / BLK, T = blockDim.x;
+  const int per = (nchunks + T - 1) / T;
+  const int c0 = min((int)threadIdx.x * per, nchunks), c1 = min(c0 + per, nchunks);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = T >> 6;
+  __shared__ int wsum[BLK / 64][NCL];
+  int run[NCL], tot[NCL];
+  for (int c = 0; c < NCL; ++c) run[c] = 0;
+  for (int ch = c0; ch < c1; ++ch)
+    for (int c = 0; c < NCL; ++c) run[c] += b.blk_cnt[(sd.nl0 / BLK + ch) * NCL + c];
+  for (int c = 0; c < NCL; ++c) {
+    int x = run[c];
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
     }
+    if (lane == 63) wsum[w][c] = x;
+    run[c] = x - run[c];  // exclusive within the wave
   }
-  st.nc_n = tot[0];
-  st.qf_n = tot[1] < sd.qfcap ? tot[1] : sd.qfcap;
-  st.qf_store = st.qf_n;
-  if (tot[1] > sd.qfcap) st.overflow = 1;
-  st.pid_n = tot[2];
-  st.dh_n = tot[3];
-  st.pauli_n = tot[4];
-  st.qh_n = tot[5] < sd.qfcap ? tot[5] : sd.qfcap;
+  __syncthreads();
+  for (int c = 0; c < NCL; ++c) {
+    int off = 0, t = 0;
+    for (int k = 0; k < nw; ++k) {
+      if (k < w) off += wsum[k][c];
+      t += wsum[k][c];
+    }
+    run[c] += off;
+    tot[c] = t;
+  }
+  for (int ch = c0; ch < c1; ++ch)
+    for (int c = 0; c < NCL; ++c) {
+      const int at = (sd.nl0 / BLK + ch) * NCL + c, v = b.blk_cnt[at];
+      b.blk_cnt[at] = run[c];
+      run[c] += v;
+    }
+  if (threadIdx.x == 0) {
+    st.nc_n = tot[0];
+    st.qf_n = tot[1] < sd.qfcap ? tot[1] : sd.qfcap;
+    st.qf_store = st.qf_n;
+    if (tot[1] > sd.qfcap) st.overflow = 1;
+    st.pid_n = tot[2];
+    st.dh_n = tot[3];
+    st.pauli_n = tot[4];
+    st.qh_n = tot[5] < sd.qfcap ? tot[5] : sd.qfcap;
+  }
 }
 
 // One chunk's pairs into their lists at the chunk's offsets (k_part_scan)
@@ -992,7 +1021,7 @@ void launch_fix_nl(const DevBatch &b, int max_nlcap, int cell_words, hipStream_t
   const int part_grid = std::max(1, std::min((max_nlcap + BLK - 1) / BLK, PART_GRID));
   dim3 pg((unsigned)part_grid, (unsigned)b.nsys);
   hipLaunchKernelGGL(k_part_count, pg, dim3(BLK), 0, s, b);
-  hipLaunchKernelGGL(k_part_scan, grid1(b.nsys, 64), dim3(64), 0, s, b);
+  hipLaunchKernelGGL(k_part_scan, dim3(b.nsys), dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_part_write, pg, dim3(BLK), 0, s, b);
   if (done) hipLaunchKernelGGL(k_nl_done, grid1(b.nsys, 64), dim3(64), 0, s, b);
 }
@@ -1014,7 +1043,7 @@ void launch_qa_finish(const DevBatch &b, int max_n, hipStream_t s) {
   // the budgets in LDS while they and the candidates fit 96 KB
   const size_t lds = qa_finish_lds(max_n);
   const int sync_lds = lds <= QF_LDS_MAX ? max_n : 0;
-  hipLaunchKernelGGL(k_qa_finish, dim3(b.nsys), dim3(BLK), sync_lds ? lds : 0, s, b, sync_lds);
+  hipLaunchKernelGGL(k_qa_finish, dim3(b.nsys), dim3(1024), sync_lds ? lds : 0, s, b, sync_lds);
 }
 void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s) {
   if (prec == PCG_FP64)
