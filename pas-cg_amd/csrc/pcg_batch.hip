@@ -207,13 +207,16 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ull << lane) - 1ull;
   const int nnc = p.nat_cont_on ? sd.nnc : 0;
-  int *chn = (int *)rb, *sqi = chn + n, *rs = sqi + n;  // rs[0..n]
+  // each residue's sequence index, chain and type in one word (seq | chain
+  // << 12 | type << 24: a persistent system has fewer than 4096 residues)
+  uint32_t *csq = (uint32_t *)rb;
+  int *rs = (int *)(csq + n);  // rs[0..n]
   uint16_t *pj = (uint16_t *)(rs + n + 1);               // [nnc]
   __shared__ int wcnt[16][RB_N];
   const bool cell = has_cell(sd);
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    chn[k] = b.chain[r0 + k];
-    sqi[k] = b.seq[r0 + k];
+    csq[k] = (uint32_t)b.seq[r0 + k] | ((uint32_t)b.chain[r0 + k] << 12) |
+             ((uint32_t)(g.at[k] & 63) << 24);
   }
   // contacts sd.nc0 + [rs[k], rs[k + 1]) of the list sorted by (i1, i2) are
   // those of residue r0 + k with later residues; pj holds their second
@@ -356,7 +359,9 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
     }
     for (int k = kbeg; k < kend; ++k) {
       const double xi = g.x[k], yi = g.y[k], zi = g.z[k];
-      const int ci = chn[k], si = sqi[k], ai = g.at[k] & 63;
+      const uint32_t wk = csq[k];
+      const int si = (int)(wk & 0xFFFu), ai = (int)(wk >> 24);
+      const uint32_t ci = (wk >> 12) & 0xFFFu;
       const int c0 = rs[k], ce = rs[k + 1];
       int tp = c0;  // the row's next native partner
       for (int j0 = k + 1; j0 < n; j0 += 64) {
@@ -382,8 +387,9 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
         // chunk of only such columns adds nothing (its native partners are
         // already passed)
         if (!__ballot(valid & (d2 <= e_far))) continue;
-        const int dseq = abs(sqi[jj] - si);
-        const bool same = chn[jj] == ci;
+        const uint32_t wj = csq[jj];
+        const int dseq = abs((int)(wj & 0xFFFu) - si);
+        const bool same = ((wj >> 12) & 0xFFFu) == ci;
         const bool four = same & (dseq == 4);
         const bool pair = valid & !(same & (dseq < 3));
         bool b_nl = d2 < s_nl, b_nc = d2 < s_nc, b_dh = d2 < s_dh, b_rep = d2 < s_rep;
@@ -407,7 +413,7 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
         const bool near = is_qa & b_near;
         const bool rest = in & !is_nc & !is_qa;
         const bool is_pid = rest & pid_on & (pid4 | !four) & b_nc;
-        const int aj = g.at[jj] & 63;
+        const int aj = (int)(wj >> 24);
         const bool is_dh = rest & dh_on & (bool)((chg >> ai) & (chg >> aj) & 1u) & b_dh;
         const bool is_pauli = rest & pauli_on & b_rep;
         const unsigned long long m_nl = __ballot(in), m_nc = __ballot(is_nc),

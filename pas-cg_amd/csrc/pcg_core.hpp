@@ -82,7 +82,7 @@ struct LdsLayout {
     total = q + (rb > 4 * QA_QUEUE ? rb : 4 * QA_QUEUE) + 64;
   }
   __host__ __device__ static size_t rebuild_bytes(size_t n, size_t nnc) {
-    return (12 * n + 4 + 2 * nnc + 15) / 16 * 16;
+    return (8 * n + 4 + 2 * nnc + 15) / 16 * 16;
   }
 };
 

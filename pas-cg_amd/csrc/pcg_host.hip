@@ -1213,7 +1213,8 @@ PersistentCfg persistent_cfg(DeviceBatch &B) {
   c.lds = LdsLayout((size_t)B.max_n, sizeof(double), staged, gear, (size_t)B.max_nnc).total;
   B.d.bond_lds = staged ? 1 : 0;
   B.d.gear_lds = gear ? 1 : 0;
-  c.ok = c.lds <= LDS_DYN_MAX;
+  // (the rebuild packs a residue's sequence index and chain in 12 bits each)
+  c.ok = c.lds <= LDS_DYN_MAX && B.max_n < 4096;
   int want = 64;
   while (want < 1024 && want < B.max_n * 2) want *= 2;
   c.block = std::min(std::max(want, 128), 512);
@@ -2690,7 +2691,8 @@ int pcg_server_submit(pcg_server *s, const pcg_model *m, double temperature, int
     return fail(PCG_E_NOMEM, "job needs " + std::to_string(block_size(bytes)) +
                                  " bytes, over the budget");
   const size_t nnc_bound = s->p.nat_cont_on ? (size_t)std::max(m->n_cont, 0) : 0;
-  if (LdsLayout((size_t)m->n, sizeof(double), false, false, nnc_bound).total > LDS_DYN_MAX)
+  if (LdsLayout((size_t)m->n, sizeof(double), false, false, nnc_bound).total > LDS_DYN_MAX ||
+      m->n >= 4096)
     return fail(PCG_E_ARG, "job of " + std::to_string(m->n) +
                                " residues does not fit the persistent kernel's LDS");
   j->bytes = bytes;
