@@ -510,3 +510,152 @@ def map_snapshot(ctx, m: dict, prm: dict, cell=None, chain_ids: str | None = Non
         i1, i2, _, _ = ctx.list(4)
         pid = {"i1": i1, "i2": i2, "active": pid_active(pos, c, i1, i2, m["atype"], prm)}
     return map_div(t, pos, m, c, chain_ids, nat, qa, pid)
+
+
+# ----------------------------------------------------------- report driver --
+class Reporter:
+    """The reference's report files for a run of a pcg.Context, as
+    make_report (cg/src/cg/output/make_report.cpp:29-56) writes them: every
+    `stats_every` of simulated time a row of the trajectory's scalars table,
+    its AFM rows (AFM on) and a `.map` div, then `<prefix>.out` and
+    `<prefix>.map` rewritten; every `struct_every` a snapshot, then
+    `<prefix>.pdb` rewritten with all of them. A trajectory's end adds one
+    of each (emit_all, thread_main.cpp:336-356) and the unfolding study's
+    sample; `simul_end` appends the results div (make_report.cpp:77-128).
+
+    The reference reports inside a step, after its force evaluation and
+    before the QA finish and the Langevin step (thread_main.cpp:503-520),
+    once t > 0. `run` advances the context to the first step whose time
+    passes the next report time and reports from the state between steps:
+    the same positions, time, velocities and QA contacts, and the energies
+    and native-contact flags of the evaluation one step earlier (the step
+    that produced those positions)."""
+
+    def __init__(self, prefix: str, model: dict, params: dict, stats_every: float | None = None,
+                 struct_every: float | None = None, chain_ids: str | None = None,
+                 measure_times: bool = False, num_traj: int = 1):
+        import numpy as np
+        self.prefix, self.m, self.p = prefix, model, params
+        self.stats_every, self.struct_every = stats_every, struct_every
+        self.chain_ids = chain_ids
+        self.measure_times, self.num_traj = measure_times, num_traj
+        cont = np.asarray(model["cont"], np.int64).reshape(-1, 2)
+        self.contacts = {"i1": cont.min(axis=1) if len(cont) else cont[:, 0],
+                         "i2": cont.max(axis=1) if len(cont) else cont[:, 0],
+                         "type": np.asarray(model["cont_type"]), "nat_dist": np.asarray(model["cont_len"])}
+        self.trajs: list[dict] = []
+        self.nc_times: list = []
+        self.unfold_times: list = []
+        self.results: str | None = None
+
+    # -- bookkeeping of make_report (report.cpp:6-16)
+    def traj_init(self):
+        import math
+        self.trajs.append({"rows": [], "afm": None, "maps": [], "snaps": []})
+        self.stats_last = self.struct_last = -math.inf
+
+    def _cur(self) -> dict:
+        if not self.trajs:
+            self.traj_init()
+        return self.trajs[-1]
+
+    def _flags(self):
+        p = self.p
+        return (bool(p.get("nat_cont_on")) and len(self.contacts["i1"]) > 0, bool(p.get("qa_on")),
+                bool(p.get("pid_on")))
+
+    def add_stats(self, ctx, tips: dict | None = None):
+        """add_cur_scalars, add_afm_stuff and add_map_data, then the files."""
+        tr = self._cur()
+        nat, qa, pid = self._flags()
+        tr["rows"].append(scalars_row(ctx.report(), nat, qa, pid))
+        if self.p.get("afm_on"):
+            f, v = afm_rows(ctx.afm_stats(), tips or {})
+            if tr["afm"] is None:
+                tr["afm"] = ([], [])
+            tr["afm"][0].extend(f)
+            tr["afm"][1].extend(v)
+        cell = self.m.get("cell")
+        tr["maps"].append(map_snapshot(ctx, self.m, self.p, cell, self.chain_ids))
+        self.emit_out()
+        self.emit_map()
+
+    def add_snapshot(self, ctx):
+        """add_cur_snapshot, then the `.pdb` file."""
+        import numpy as np
+        t, _ = ctx.time()
+        self._cur()["snaps"].append((t, np.asarray(ctx.r(), np.float64).copy()))
+        self.emit_pdb()
+
+    def emit_out(self):
+        with open(self.prefix + ".out", "w") as f:
+            f.write(out_file([tr["rows"] for tr in self.trajs], afm=[tr["afm"] for tr in self.trajs],
+                             results=self.results))
+
+    def emit_map(self):
+        with open(self.prefix + ".map", "w") as f:
+            f.write(map_file([tr["maps"] for tr in self.trajs]))
+
+    def emit_pdb(self):
+        snaps = [s for tr in self.trajs for s in tr["snaps"]]
+        cell = self.m.get("cell")
+        cell = cell if cell is not None and any(float(c) != 0.0 for c in cell) else None
+        with open(self.prefix + ".pdb", "w") as f:
+            f.write(pdb_snapshots(snaps, self.m["chain"], self.m["atype"], cell))
+
+    # -- make_report::operator() (make_report.cpp:29-46)
+    def __call__(self, ctx, tips: dict | None = None):
+        t, _ = ctx.time()
+        if not t > 0:
+            return
+        if self.stats_every is not None and t >= self.stats_last + self.stats_every:
+            self.add_stats(ctx, tips)
+            self.stats_last = t
+        if self.struct_every is not None and t >= self.struct_last + self.struct_every:
+            self.add_snapshot(ctx)
+            self.struct_last = t
+
+    def _next_steps(self, t: float, dt: float) -> int:
+        """Steps until the next report falls due (at least one); 0 with no
+        reports configured."""
+        from .protocol import steps_until
+        due = []
+        if self.stats_every is not None:
+            due.append(self.stats_last + self.stats_every)
+        if self.struct_every is not None:
+            due.append(self.struct_last + self.struct_every)
+        if not due:
+            return 0
+        return max(steps_until(t, min(due), dt), 1)
+
+    def run(self, ctx, steps: int, tips: dict | None = None):
+        """Advances the context by `steps` steps, reporting whenever a
+        report falls due on the way."""
+        self._cur()
+        dt = self.p["dt"]
+        self(ctx, tips)
+        done = 0
+        while done < steps:
+            t, _ = ctx.time()
+            k = self._next_steps(t, dt)
+            k = steps - done if k == 0 else min(k, steps - done)
+            ctx.step(k)
+            done += k
+            self(ctx, tips)
+
+    def traj_end(self, ctx, tips: dict | None = None):
+        """thread::traj_end_step: the unfolding study's sample, then emit_all."""
+        if self.measure_times and len(self.contacts["i1"]):
+            act, ct, changed = ctx.nat_cont_state(len(self.contacts["i1"]))
+            times, t = unfolding_sample(ct, changed)
+            self.nc_times.append(times)
+            if t is not None:
+                self.unfold_times.append(t)
+        self.add_stats(ctx, tips)
+        self.add_snapshot(ctx)
+
+    def simul_end(self):
+        """at_simul_end: the results div, then the `.out` file."""
+        self.results = results_div(self.contacts if self.measure_times else None, self.nc_times,
+                                   self.unfold_times, self.num_traj)
+        self.emit_out()

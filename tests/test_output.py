@@ -327,3 +327,122 @@ def test_map_of_context_equals_oracle_map():
     got = O.map_snapshot(c, m, dict(prm, nat_cont_on=0))
     assert got == want
     c.close()
+
+
+class _FakeCtx:
+    """A stand-in for pcg.Context with the calls the report driver makes:
+    every step adds dt to the time and moves residue 0."""
+
+    def __init__(self, m, dt):
+        import numpy as np
+        self.m, self.dt, self.t, self.k = m, dt, 0.0, 0
+        self.pos = np.asarray(m["pos"], np.float64).copy()
+        self.steps_taken = []
+
+    def step(self, k):
+        for _ in range(k):
+            self.t += self.dt
+            self.k += 1
+            self.pos[0, 0] += 0.01
+        self.steps_taken.append(k)
+
+    def time(self):
+        return self.t, self.k
+
+    def r(self):
+        return self.pos.copy()
+
+    def report(self):
+        rep = {k: 0.0 for k in ["TIME", "EPOT", "ETOT", "EKIN", "RG", "L", "RMSD", "W", "ICN",
+                                "ICNss", "INTEHC", "INTRHC"] + O.PID_COLS}
+        rep["TIME"] = self.t
+        return rep
+
+    def nat_cont_state(self, k):
+        import numpy as np
+        return np.zeros(k, np.uint8), np.full(k, -1.0), 0
+
+    def list(self, kind):
+        import numpy as np
+        z = np.zeros(0, np.int32)
+        return z, z, np.zeros(0), np.zeros(0, np.uint8)
+
+
+def test_reporter_timing_and_files(tmp_path):
+    """make_report's timing (make_report.cpp:29-46, called once t > 0): the
+    first report one step in, then whenever the time passes the last
+    report's plus the interval; the .out rows, .map divs and .pdb models
+    follow; a trajectory's end adds one of each, the simulation's end the
+    results div."""
+    import numpy as np
+    from pas_cg_amd import model as M
+    m = M.synthetic_protein(20, M.job_seed(3))
+    prm = P.default_params()
+    prm.update(qa_on=0, pid_on=0, nat_cont_on=1)
+    dt = prm["dt"]
+    ctx = _FakeCtx(m, dt)
+    rep = O.Reporter(str(tmp_path / "run"), m, prm, stats_every=10 * dt, struct_every=25 * dt,
+                     measure_times=True)
+    rep.run(ctx, 60)
+    out = (tmp_path / "run.out").read_text().split("\n")
+    rows = [ln for ln in out if ln.startswith(" ")]
+    times = [float(ln.split()[0]) for ln in rows]
+    # steps 1, 11, 21, 31, 41, 51 (the time passes last + 10 dt on the step
+    # that reaches it)
+    assert len(times) == 6
+    assert np.allclose(times, [dt * k for k in (1, 11, 21, 31, 41, 51)], rtol=1e-9)
+    mp = (tmp_path / "run.map").read_text()
+    assert mp.startswith("#TRAJECTORY 0\n#T = ") and mp.count("#ANGLES") == 6
+    pdb = (tmp_path / "run.pdb").read_text()
+    assert pdb.count("MODEL ") == 3  # steps 1, 26, 51
+    rep.traj_end(ctx)
+    rep.simul_end()
+    out = (tmp_path / "run.out").read_text()
+    assert out.split("#RESULTS")[0].count("\n ") == 7
+    assert (tmp_path / "run.pdb").read_text().count("MODEL ") == 4
+    assert out.rstrip().split("\n")[-1] == "#MEDIAN (UN)FOLDING TIME: N/A"
+    assert "#AVERAGE BREAKING/FORMING TIMES" in out
+    assert sum(ctx.steps_taken) == 60
+
+
+@pytest.mark.gpu
+def test_reporter_on_a_context(tmp_path):
+    """The report files of a live context with QA and a pulled protein: each
+    `.map` div's QA count equals the INTEHC + INTRHC of the `.out` row of
+    the same report (both count the live QA contacts of one state), the AFM
+    tables get a row a tip a report, and the `.pdb` holds every snapshot."""
+    from pas_cg_amd import model as M
+    from pas_cg_amd import pcg as G
+    m = M.add_pulling(M.synthetic_protein(80, M.job_seed(5)), vel_A_per_tau=0.5)
+    prm = P.default_params()
+    prm.update(qa_on=1, pid_on=0, afm_on=1, afm_stats_window=20 * prm["dt"])
+    c = G.Context(prm, m, precision=G.MIXED, noise=G.PHILOX, seed=5)
+    tips = {k: m[k] for k in ("tip_res", "tip_orig", "tip_vel")}
+    rep = O.Reporter(str(tmp_path / "p"), m, prm, stats_every=100 * prm["dt"],
+                     struct_every=200 * prm["dt"])
+    rep.run(c, 400, tips)
+    out = (tmp_path / "p.out").read_text().split("\n")
+    head = out[1][1:].split()
+    rows = []
+    for ln in out[2:]:
+        if not ln.startswith(" "):
+            break
+        rows.append(dict(zip(head, ln.split())))
+    # the reference's timing: t accumulated step by step, a report once
+    # t >= last + interval (the fourth lands a step late by rounding)
+    t, last, due = 0.0, -float("inf"), []
+    for k in range(1, 401):
+        t += prm["dt"]
+        if t >= last + 100 * prm["dt"]:
+            due.append(k)
+            last = t
+    assert len(rows) == len(due) and len(due) >= 4
+    assert [float(r["TIME[TAU]"]) for r in rows] == pytest.approx([k * prm["dt"] for k in due],
+                                                                   rel=1e-5)
+    mp = (tmp_path / "p.map").read_text().split("\n")
+    qa_counts = [int(mp[k + 1].split("=")[1]) for k, ln in enumerate(mp) if ln == "#QA"]
+    assert qa_counts == [int(r["INTEHC"]) + int(r["INTRHC"]) for r in rows]
+    k = out.index("#VELOCITY AFM TIPS")
+    assert len([ln for ln in out[k + 2:] if ln.startswith(" ")]) == 2 * len(rows)
+    assert (tmp_path / "p.pdb").read_text().count("MODEL ") == 2
+    c.close()
