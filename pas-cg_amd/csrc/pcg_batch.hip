@@ -392,11 +392,13 @@ __device__ __forceinline__ void wg_fix_nl(const DevBatch &b, SysDescC &sd, SysSt
         const bool same = ((wj >> 12) & 0xFFFu) == ci;
         const bool four = same & (dseq == 4);
         const bool pair = valid & !(same & (dseq < 3));
-        bool b_nl = d2 < s_nl, b_nc = d2 < s_nc, b_dh = d2 < s_dh, b_rep = d2 < s_rep;
-        bool b_near = d2 < s_near;
-        const bool edge = pair & ((d2 >= e_nl[0]) & (d2 <= e_nl[1]) | (d2 >= e_nc[0]) & (d2 <= e_nc[1]) |
-                                  (d2 >= e_dh[0]) & (d2 <= e_dh[1]) | (d2 >= e_rep[0]) & (d2 <= e_rep[1]) |
-                                  (d2 >= e_near[0]) & (d2 <= e_near[1]));
+        // below a threshold's lower margin the pair is in, above its upper
+        // margin out; between the two (lo != hi) the exact distance decides
+        bool b_nl = d2 < e_nl[0], b_nc = d2 < e_nc[0], b_dh = d2 < e_dh[0], b_rep = d2 < e_rep[0];
+        bool b_near = d2 < e_near[0];
+        const bool edge = pair & ((b_nl != (d2 <= e_nl[1])) | (b_nc != (d2 <= e_nc[1])) |
+                                  (b_dh != (d2 <= e_dh[1])) | (b_rep != (d2 <= e_rep[1])) |
+                                  (b_near != (d2 <= e_near[1])));
         if (__ballot(edge)) {  // rare: the exact FP64 decisions of the pairs in a band
           if (edge) {
             const double dist = xnorm3(dd);
