@@ -18,7 +18,12 @@
  *     gradient (tests/test_oracle_math.py);
  *   - ran2, initial velocities, Gear-5 + noise, tether/lrep/angles/Pauli/PID/
  *     DH/heuristic dihedrals/SAW: the golden V traces of the reference's
- *     parity fixtures (tests/test_golden.py).
+ *     parity fixtures (tests/test_golden.py);
+ *   - walls and the box protocol: the reference's wall kernels through
+ *     oracle/_ref and closed forms of the phases (tests/test_walls.py);
+ *   - the AFM tips' moving averages: the reference's afm::vel and
+ *     afm::force eval_forces fed the same samples, and a zero window's last
+ *     sample (tests/test_afm_stats.py).
  */
 #include "oracle.h"
 
