@@ -107,6 +107,10 @@ struct SysState {
   int qf_store;
   int n_hw, n_lj;             // harmonic wall connections, LJ wall connection slots
   int form;                   // plain context: the form (stages / persistent) whose lists these are
+  // stage kernels' completion counts (rows, residues, blocks, residues):
+  // the one that completes a system runs its follow-up (a scan, the tick)
+  // and sets the count back to zero, so it is zero between kernels
+  int done_rows, done_cells, done_part, done_lang;
   int64_t step;
   double t, true_t, orig_pad;
   double orig_cell[3];

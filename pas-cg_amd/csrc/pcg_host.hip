@@ -1264,8 +1264,7 @@ void enqueue_step(DeviceBatch &B) {
   if (p.qa_on) launch_qa_finish(d, B.max_n, B.stream);
   if (B.hp.ljw_terms) launch_ljw_process(d, B.stream);
   if (B.box_set) launch_box_log(d, B.stream);
-  launch_lang(d, B.prec, B.noise, B.stream);
-  launch_tick(d, 1, B.stream, true);
+  launch_lang(d, B.prec, B.noise, B.stream, true);
 }
 
 // the reaction forces of the walls start a term evaluation at zero
@@ -2059,7 +2058,7 @@ int ckpt_write(DeviceBatch &B, const char magic[8], void *buf, int64_t bytes) {
   HIPCHK(hipSetDevice(B.device));
   CkptHeader h{};
   std::memcpy(h.magic, magic, 8);
-  h.version = 5;
+  h.version = 6;
   h.nsys = (int32_t)B.sys.size();
   h.ntot = B.d.ntot;
   h.prec = B.prec;
@@ -2080,7 +2079,7 @@ int ckpt_read(DeviceBatch &B, const char magic[8], const void *buf, int64_t byte
   CkptHeader h;
   if (bytes < (int64_t)sizeof h) return fail(PCG_E_ARG, "not a checkpoint");
   std::memcpy(&h, buf, sizeof h);
-  if (std::memcmp(h.magic, magic, 8) != 0 || h.version != 5) return fail(PCG_E_ARG, "not a checkpoint");
+  if (std::memcmp(h.magic, magic, 8) != 0 || h.version != 6) return fail(PCG_E_ARG, "not a checkpoint");
   if (h.nsys != (int32_t)B.sys.size() || h.ntot != B.d.ntot || h.prec != B.prec ||
       h.arena_bytes != B.arena.total || h.fingerprint != ckpt_fingerprint(B))
     return fail(PCG_E_ARG, "checkpoint of a different batch (models, parameters or precision)");

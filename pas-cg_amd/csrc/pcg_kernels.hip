@@ -70,6 +70,56 @@ __device__ __forceinline__ bool nl_pred(const DevBatch &b, SysDescC &s, const Ge
   return dist < req;
 }
 
+// Completion count of a per-item stage kernel: every lane with `mine` (an
+// item of system `sys`) counts itself toward the system's residue count in
+// the SysState field `done`; a wave whose counted lanes share one system
+// adds them with one atomic. Returns, in the lane whose add completed a
+// system, that system (-1 elsewhere); what every lane wrote before the call
+// is visible to it then.
+__device__ __forceinline__ int count_done(const DevBatch &b, int SysState::*done, bool mine,
+                                          int sys) {
+  __threadfence();
+  const int lane = threadIdx.x & 63;
+  const unsigned long long m = __ballot(mine);
+  const int sys0 = __shfl(sys, 0, 64);
+  const bool uni = __all(!mine || sys == sys0) && ((m & 1ull) != 0ull);
+  int who = -1, add = 0;
+  if (uni) {
+    if (lane == 0) who = sys0, add = __popcll(m);
+  } else if (mine) {
+    who = sys, add = 1;
+  }
+  if (who < 0) return -1;
+  const int old = atomicAdd(&(b.st[who].*done), add);
+  if (old + add != b.sys[who].n) return -1;
+  __threadfence();
+  return who;
+}
+
+// One wave's exclusive scan of v[0..n) in place (each lane a contiguous run,
+// then a scan over the lanes), also into also[0..n) when given; returns the
+// total in every lane.
+__device__ __forceinline__ int wave_scan(int *v, int n, int *also) {
+  const int lane = threadIdx.x & 63, per = (n + 63) / 64;
+  const int a0 = min(lane * per, n), a1 = min(a0 + per, n);
+  int sum = 0;
+  for (int k = a0; k < a1; ++k) sum += v[k];
+  int x = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  const int total = __shfl(x, 63, 64);
+  int run = x - sum;
+  for (int k = a0; k < a1; ++k) {
+    const int t = v[k];
+    v[k] = run;
+    if (also) also[k] = run;
+    run += t;
+  }
+  return total;
+}
+
 // ------------------------------------------------------------- cell grid ----
 // A large periodic system can take its NL rows from a cell grid instead of
 // every later residue: the periodic cell cut into cg[a] >= 3 cells a side,
@@ -94,58 +144,34 @@ __device__ __forceinline__ int cell_coord(SysDescC &sd, int a, double x) {
   return k < 0 ? 0 : (k >= sd.cg[a] ? sd.cg[a] - 1 : k);
 }
 
-// each residue's cell; cell_end counts the residues of each cell
+// each residue's cell; cell_end counts the residues of each cell; the wave
+// whose residue completes a system turns its counts into the cells' first
+// slots (cell_start) and fill pointers (cell_end)
 __global__ void k_cell_count(DevBatch b) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.ntot) return;
-  const int s = b.sysid[i];
-  const SysState &st = b.st[s];
-  if (!st.active || !st.nl_invalid) return;
-  SysDescC &sd = b.sys[s];
-  if (!use_cells(b, sd)) return;
-  const int c = (cell_coord(sd, 2, b.y[0][2][i]) * sd.cg[1] + cell_coord(sd, 1, b.y[0][1][i])) *
-                    sd.cg[0] + cell_coord(sd, 0, b.y[0][0][i]);
-  b.cell_of[i] = c;
-  atomicAdd(&b.cell_end[sd.cell0 + c], 1);
-}
-
-// one block a system: the cells' first slots (exclusive scan of the
-// counts); cell_end becomes the fill pointer
-__global__ void k_cell_scan(DevBatch b) {
-  const int s = blockIdx.x;
-  const SysState &st = b.st[s];
-  if (!st.active || !st.nl_invalid) return;
-  SysDescC &sd = b.sys[s];
-  if (!use_cells(b, sd)) return;
-  const int nc = sd.cg[0] * sd.cg[1] * sd.cg[2];
-  __shared__ int carry;
-  __shared__ int wsum[BLK / 64];
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int base = 0; base < nc; base += blockDim.x) {
-    const int c = base + threadIdx.x;
-    const int v = c < nc ? b.cell_end[sd.cell0 + c] : 0;
-    int x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool mine = false;
+  int sys = -1;
+  if (i < b.ntot) {
+    sys = b.sysid[i];
+    const SysState &st = b.st[sys];
+    SysDescC &sd = b.sys[sys];
+    if (st.active && st.nl_invalid && use_cells(b, sd)) {
+      mine = true;
+      const int c = (cell_coord(sd, 2, b.y[0][2][i]) * sd.cg[1] + cell_coord(sd, 1, b.y[0][1][i])) *
+                        sd.cg[0] + cell_coord(sd, 0, b.y[0][0][i]);
+      b.cell_of[i] = c;
+      atomicAdd(&b.cell_end[sd.cell0 + c], 1);
     }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    int off = carry;
-    for (int k = 0; k < w; ++k) off += wsum[k];
-    if (c < nc) {
-      b.cell_start[sd.cell0 + c] = off + x - v;
-      b.cell_end[sd.cell0 + c] = off + x - v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int tot = 0;
-      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) tot += wsum[k];
-      carry += tot;
-    }
-    __syncthreads();
+  }
+  const int done = count_done(b, &SysState::done_cells, mine, sys);
+  unsigned long long dm = __ballot(done >= 0);
+  while (dm) {
+    const int src = __ffsll((long long)dm) - 1;
+    dm &= dm - 1ull;
+    const int ds = __shfl(done, src, 64);
+    SysDescC &sd = b.sys[ds];
+    wave_scan(b.cell_end + sd.cell0, sd.cg[0] * sd.cg[1] * sd.cg[2], b.cell_start + sd.cell0);
+    if ((threadIdx.x & 63) == 0) b.st[ds].done_cells = 0;
   }
 }
 
@@ -230,9 +256,7 @@ template <bool WRITE> __global__ void k_nl_rows(DevBatch b, int cell_words) {
         cnt += __popcll(mm);
       }
     }
-    if (!WRITE && lane == 0) b.row_cnt[row] = cnt;
-    return;
-  }
+  } else {
   for (int j0 = row + 1; j0 < end; j0 += 64) {
     int j = j0 + lane;
     double dist = 0.0;
@@ -249,51 +273,28 @@ template <bool WRITE> __global__ void k_nl_rows(DevBatch b, int cell_words) {
     }
     cnt += __popcll(m);
   }
-  if (!WRITE && lane == 0) b.row_cnt[row] = cnt;
-}
-
-// Exclusive scan of the row counts of one system (one block per system).
-__global__ void k_nl_scan(DevBatch b) {
-  int s = blockIdx.x;
-  SysState &st = b.st[s];
-  if (!st.active || !st.nl_invalid) return;
-  SysDescC &sd = b.sys[s];
-  __shared__ int carry;
-  __shared__ int wsum[BLK / 64];
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < sd.n; base += blockDim.x) {
-    int i = base + threadIdx.x;
-    int v = i < sd.n ? b.row_cnt[sd.res0 + i] : 0;
-    // inclusive wave scan
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    int off = carry;
-    for (int k = 0; k < w; ++k) off += wsum[k];
-    if (i < sd.n) b.row_cnt[sd.res0 + i] = off + x - v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int tot = 0;
-      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) tot += wsum[k];
-      carry += tot;
-    }
-    __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    st.nl_n = carry;
-    if (carry > sd.nlcap) {
-      st.overflow = 1;
-      st.nl_n = sd.nlcap;
+  if (!WRITE) {
+    // the row counts; the wave of the system's last row to finish scans them
+    // into the rows' first slots
+    int fin = 0;
+    if (lane == 0) {
+      b.row_cnt[row] = cnt;
+      __threadfence();
+      fin = atomicAdd(&st.done_rows, 1) == sd.n - 1;
     }
-    // the list updates filter with the new pad (legacy_update.cpp:84)
-    st.orig_pad = b.P->p.nl_pad;
-    for (int c = 0; c < 3; ++c) st.orig_cell[c] = sd.cell[c];
+    if (__shfl(fin, 0, 64)) {
+      __threadfence();
+      const int tot = wave_scan(b.row_cnt + sd.res0, sd.n, nullptr);
+      if (lane == 0) {
+        st.nl_n = tot > sd.nlcap ? sd.nlcap : tot;
+        if (tot > sd.nlcap) st.overflow = 1;
+        // the list updates filter with the new pad (legacy_update.cpp:84)
+        st.orig_pad = b.P->p.nl_pad;
+        for (int c = 0; c < 3; ++c) st.orig_cell[c] = sd.cell[c];
+        st.done_rows = 0;
+      }
+    }
   }
 }
 
@@ -329,46 +330,12 @@ __device__ __forceinline__ void block_counts6(int cls, int valid, int counts[NCL
   __syncthreads();
 }
 
-// One chunk of BLK pairs of system s: each pair's classes, and the chunk's
-// count per class
-__device__ __forceinline__ void part_count_chunk(DevBatch &b, SysDescC &sd, const SysState &st,
-                                                 int ch) {
-  int k = ch * BLK + (int)threadIdx.x;
-  int valid = k < st.nl_n;
-  int cls = valid ? classify(b, sd, st, sd.nl0 + k) : 0;
-  if (valid) b.nl_cls[sd.nl0 + k] = (uint8_t)cls;
-  int counts[NCL], ranks[NCL];
-  block_counts6(cls, valid, counts, ranks);
-  if (threadIdx.x == 0) {
-    int base = (sd.nl0 / BLK + ch) * NCL;
-    for (int c = 0; c < NCL; ++c) b.blk_cnt[base + c] = counts[c];
-  }
-}
-
-// blockIdx.y: system; the blocks stride over the chunks the list fills, so
-// a step without a rebuild costs a small grid's launch
-__global__ void k_part_count(DevBatch b) {
-  int s = blockIdx.y;
-  const SysState &st = b.st[s];
-  if (!st.active || !st.nl_invalid) return;
-  SysDescC &sd = b.sys[s];
-  // the cell counts start the next rebuild at zero (the rows are written)
-  if (sd.cg[0] > 0)
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x, nc = sd.cg[0] * sd.cg[1] * sd.cg[2]; c < nc;
-         c += gridDim.x * blockDim.x)
-      b.cell_end[sd.cell0 + c] = 0;
-  const int nchunks = (st.nl_n + BLK - 1) / BLK;
-  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) part_count_chunk(b, sd, st, ch);
-}
-
-// One block a system: every class's per-chunk counts become their exclusive
+// One block of system s: every class's per-chunk counts become their exclusive
 // prefix in chunk order, and the totals the lists' sizes. Each thread takes
 // a contiguous run of chunks; a scan over the threads gives each run its
 // offsets.
-__global__ void k_part_scan(DevBatch b) {
-  const int s = blockIdx.x;
+__device__ __forceinline__ void part_scan_block(const DevBatch &b, int s) {
   SysState &st = b.st[s];
-  if (!st.active || !st.nl_invalid) return;
   SysDescC &sd = b.sys[s];
   const int nchunks = (st.nl_n + BLK - 1) / BLK, T = blockDim.x;
   const int per = (nchunks + T - 1) / T;
@@ -416,7 +383,52 @@ __global__ void k_part_scan(DevBatch b) {
   }
 }
 
-// One chunk's pairs into their lists at the chunk's offsets (k_part_scan)
+// One chunk of BLK pairs of system s: each pair's classes, and the chunk's
+// count per class
+__device__ __forceinline__ void part_count_chunk(DevBatch &b, SysDescC &sd, const SysState &st,
+                                                 int ch) {
+  int k = ch * BLK + (int)threadIdx.x;
+  int valid = k < st.nl_n;
+  int cls = valid ? classify(b, sd, st, sd.nl0 + k) : 0;
+  if (valid) b.nl_cls[sd.nl0 + k] = (uint8_t)cls;
+  int counts[NCL], ranks[NCL];
+  block_counts6(cls, valid, counts, ranks);
+  if (threadIdx.x == 0) {
+    int base = (sd.nl0 / BLK + ch) * NCL;
+    for (int c = 0; c < NCL; ++c) b.blk_cnt[base + c] = counts[c];
+  }
+}
+
+// blockIdx.y: system; the blocks stride over the chunks the list fills, so
+// a step without a rebuild costs a small grid's launch
+__global__ void k_part_count(DevBatch b) {
+  int s = blockIdx.y;
+  SysState &st = b.st[s];
+  if (!st.active || !st.nl_invalid) return;
+  SysDescC &sd = b.sys[s];
+  // the cell counts start the next rebuild at zero (the rows are written)
+  if (sd.cg[0] > 0)
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x, nc = sd.cg[0] * sd.cg[1] * sd.cg[2]; c < nc;
+         c += gridDim.x * blockDim.x)
+      b.cell_end[sd.cell0 + c] = 0;
+  const int nchunks = (st.nl_n + BLK - 1) / BLK;
+  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) part_count_chunk(b, sd, st, ch);
+  // the system's last block to finish scans the chunk counts
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(&st.done_part, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  part_scan_block(b, s);
+  if (threadIdx.x == 0) st.done_part = 0;
+}
+
+// One chunk's pairs into their lists at the chunk's offsets (the scan of
+// k_part_count)
 __device__ __forceinline__ void part_write_chunk(DevBatch &b, SysDescC &sd, const SysState &st,
                                                  int ch) {
   int k = ch * BLK + (int)threadIdx.x;
@@ -579,17 +591,45 @@ __global__ void k_qa_finish(DevBatch b, int sync_lds) {
   for (int k = threadIdx.x; k < n; k += blockDim.x) *(uint32_t *)(b.sync + 4 * (r0 + k)) = syl[k];
 }
 
+// time and energy bookkeeping at the end of a step (step.cpp:157-162)
+__device__ __forceinline__ void tick_system(const DevBatch &b, int s, int advance, int nl_done) {
+  SysState &st = b.st[s];
+  // a full step's rebuild, if it made one, is complete (k_nl_done)
+  if (nl_done && st.nl_invalid) {
+    st.nl_invalid = 0;
+    st.rebuilds += 1;
+  }
+  double v = 0.0;
+  for (int k = 0; k < NT; ++k) v += st.V[k];
+  st.Vtot = v;
+  if (advance && b.P->p.lang_on) {
+    st.true_t += b.P->dt;
+    st.t = st.true_t;
+    st.step += 1;
+  }
+}
+
+__global__ void k_tick(DevBatch b, int advance, int nl_done) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b.nsys) return;
+  if (!b.st[s].active) return;
+  tick_system(b, s, advance, nl_done);
+}
+
+
 // The displacement maximum goes to the system's state with one atomic per
 // wave when the wave's residues belong to one system (a system of the plain
 // context has thousands of residues: one atomic per residue on one address
 // serialised the whole kernel), else one per residue.
-template <typename RF> __global__ void k_lang(DevBatch b, int noise_mode) {
+template <typename RF> __global__ void k_lang(DevBatch b, int noise_mode, int tick) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int s = -1;
+  int s = -1, sys = -1;
+  bool mine = false;   // a residue of an active system
   double disp = -1.0;  // below every displacement: lanes without a residue
   if (i < b.ntot) {
-    s = b.sysid[i];
+    s = sys = b.sysid[i];
     SysState &st = b.st[s];
+    mine = st.active != 0;
     if (st.active && b.P->p.lang_on) {
       SysDescC &sd = b.sys[s];
       double nz[3];
@@ -617,28 +657,18 @@ template <typename RF> __global__ void k_lang(DevBatch b, int noise_mode) {
   } else if (s >= 0) {
     atomicMax(&b.st[s].max_disp_bits, (unsigned long long)__double_as_longlong(disp));
   }
+  // the step's tick (k_tick) by the residue that completes its system, once
+  // every residue's Langevin work, and with it every read of the system's
+  // time, is done
+  if (tick) {
+    const int done = count_done(b, &SysState::done_lang, mine, sys);
+    if (done >= 0) {
+      tick_system(b, done, 1, 1);
+      b.st[done].done_lang = 0;
+    }
+  }
 }
 
-// time and energy bookkeeping at the end of a step (step.cpp:157-162)
-__global__ void k_tick(DevBatch b, int advance, int nl_done) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= b.nsys) return;
-  SysState &st = b.st[s];
-  if (!st.active) return;
-  // a full step's rebuild, if it made one, is complete (k_nl_done)
-  if (nl_done && st.nl_invalid) {
-    st.nl_invalid = 0;
-    st.rebuilds += 1;
-  }
-  double v = 0.0;
-  for (int k = 0; k < NT; ++k) v += st.V[k];
-  st.Vtot = v;
-  if (advance && b.P->p.lang_on) {
-    st.true_t += b.P->dt;
-    st.t = st.true_t;
-    st.step += 1;
-  }
-}
 
 // ------------------------------------------------------------------ walls ----
 // wall::lj::process_candidates (cg/src/cg/wall/lj/process_candidates.cpp:5-31)
@@ -1008,20 +1038,19 @@ void launch_fix_nl(const DevBatch &b, int max_nlcap, int cell_words, hipStream_t
   int rows_per_block = BLK / 64;
   dim3 rg((unsigned)((b.ntot + rows_per_block - 1) / rows_per_block));
   const size_t mask_lds = (size_t)rows_per_block * cell_words * sizeof(unsigned long long);
+  // each count kernel ends with the scan of its counts, by the wave or
+  // block that completes a system
   if (cell_words > 0) {
     hipLaunchKernelGGL(k_cell_count, grid1(b.ntot), dim3(BLK), 0, s, b);
-    hipLaunchKernelGGL(k_cell_scan, dim3(b.nsys), dim3(BLK), 0, s, b);
     hipLaunchKernelGGL(k_cell_fill, grid1(b.ntot), dim3(BLK), 0, s, b);
   }
   hipLaunchKernelGGL(k_nl_rows<false>, rg, dim3(BLK), mask_lds, s, b, cell_words);
-  hipLaunchKernelGGL(k_nl_scan, dim3(b.nsys), dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_nl_rows<true>, rg, dim3(BLK), mask_lds, s, b, cell_words);
   // the partition strides over the filled chunks: at most PART_GRID blocks
   // a system (four per CU of a 256-CU device for one system)
   const int part_grid = std::max(1, std::min((max_nlcap + BLK - 1) / BLK, PART_GRID));
   dim3 pg((unsigned)part_grid, (unsigned)b.nsys);
   hipLaunchKernelGGL(k_part_count, pg, dim3(BLK), 0, s, b);
-  hipLaunchKernelGGL(k_part_scan, dim3(b.nsys), dim3(BLK), 0, s, b);
   hipLaunchKernelGGL(k_part_write, pg, dim3(BLK), 0, s, b);
   if (done) hipLaunchKernelGGL(k_nl_done, grid1(b.nsys, 64), dim3(64), 0, s, b);
 }
@@ -1045,11 +1074,12 @@ void launch_qa_finish(const DevBatch &b, int max_n, hipStream_t s) {
   const int sync_lds = lds <= QF_LDS_MAX ? max_n : 0;
   hipLaunchKernelGGL(k_qa_finish, dim3(b.nsys), dim3(1024), sync_lds ? lds : 0, s, b, sync_lds);
 }
-void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s) {
+void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s, bool tick) {
+  const int t = tick ? 1 : 0;
   if (prec == PCG_FP64)
-    hipLaunchKernelGGL(k_lang<double>, grid1(b.ntot), dim3(BLK), 0, s, b, noise_mode);
+    hipLaunchKernelGGL(k_lang<double>, grid1(b.ntot), dim3(BLK), 0, s, b, noise_mode, t);
   else
-    hipLaunchKernelGGL(k_lang<float>, grid1(b.ntot), dim3(BLK), 0, s, b, noise_mode);
+    hipLaunchKernelGGL(k_lang<float>, grid1(b.ntot), dim3(BLK), 0, s, b, noise_mode, t);
 }
 void launch_tick(const DevBatch &b, int advance, hipStream_t s, bool nl_done) {
   hipLaunchKernelGGL(k_tick, grid1(b.nsys, 64), dim3(64), 0, s, b, advance, nl_done ? 1 : 0);

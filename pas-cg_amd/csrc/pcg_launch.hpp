@@ -14,7 +14,9 @@ void launch_fix_nl(const DevBatch &b, int max_nlcap, int cell_words, hipStream_t
 void launch_terms(const DevBatch &b, int prec, int max_items, int only_term, int from, int to,
                   hipStream_t s);
 void launch_qa_finish(const DevBatch &b, int max_n, hipStream_t s);
-void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s);
+// tick: the step's k_tick (advance, close a rebuild) by the residue that
+// completes each system
+void launch_lang(const DevBatch &b, int prec, int noise_mode, hipStream_t s, bool tick = false);
 void launch_tick(const DevBatch &b, int advance, hipStream_t s, bool nl_done = false);
 // walls and the box protocol of a plain context (b.wall set): LJ candidate
 // processing; box set-up, the phase logic before a step, the force log after
