@@ -663,7 +663,10 @@ __device__ __forceinline__ void bonded_by_base(const DevBatch &b, SysDescC &sd, 
 
 // Native contacts, NK items per thread per pass, with every item's list
 // fields loaded before any is evaluated (their chains of list load -> LDS
-// positions -> decisions overlap).
+// positions -> decisions overlap). The step takes NK = 1: on the ragged
+// 256-protein workload NK = 1, 2, 3, 4 and 6 measured 34.9, 35.1, 35.6,
+// 36.6 and 37.7 us a step (the larger NK keep more registers live across
+// the loop, in a kernel already limited to 128 VGPRs).
 template <int NK, typename R>
 __device__ __forceinline__ void nat_cont_loop(const DevBatch &b, SysDescC &sd, SysState &st,
                                               const GeoL &g, AccL<double> &acc, const int *sizes,
@@ -981,7 +984,7 @@ __device__ __forceinline__ void persistent_body(const DevBatch &b, int s, int ns
     }
     // three slots a thread: the largest proteins (some 2600 contacts on 1024
     // threads) fill them in one pass
-    nat_cont_loop<3, R>(b, sd, st, g, acc, sizes, vacc, wv);
+    nat_cont_loop<1, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_PAULI, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_CONST_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);
     term_loop<PCG_REL_DH, R>(b, sd, st, g, acc, sizes, vacc, wv);

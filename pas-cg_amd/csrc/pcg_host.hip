@@ -1259,7 +1259,7 @@ void enqueue_step(DeviceBatch &B) {
   const pcg_params &p = B.hp.p;
   if (B.box_set) launch_box_phase(d, B.dP, sys_rw(B), B.stream);
   launch_step_begin(d, B.prec, p.qa_on, B.stream);
-  launch_fix_nl(d, B.max_nlcap, B.cell_words, B.stream, false);
+  launch_fix_nl(d, B.max_n, B.max_nlcap, B.cell_words, B.stream, false);
   launch_terms(d, B.prec, B.max_items, -1, 0, -1, B.stream);
   if (p.qa_on) launch_qa_finish(d, B.max_n, B.stream);
   if (B.hp.ljw_terms) launch_ljw_process(d, B.stream);
@@ -1451,7 +1451,7 @@ int pcg_fix_nl(pcg_ctx *c) {
   st.nl_invalid = 1;
   st.verify_first = 0;
   if ((rc = write_state(B, 0, st))) return rc;
-  launch_fix_nl(B.d, B.max_nlcap, B.cell_words, B.stream);
+  launch_fix_nl(B.d, B.max_n, B.max_nlcap, B.cell_words, B.stream);
   HIPCHK(hipStreamSynchronize(B.stream));
   return check_overflow(B);
 }

@@ -9,7 +9,10 @@ namespace pcg {
 
 constexpr int BLK = 256;
 // most blocks a system's partition kernels take (they stride over the chunks)
-constexpr int PART_GRID = 1024;
+constexpr int PART_GRID = 256;
+// systems of this many residues or more take the NL row scan as a launch of
+// its own (k_nl_scan) instead of in the count pass's last row
+constexpr int ROWS_FUSE_MAX = 2048;
 // candidates the stage QA finish stages in LDS, and its LDS limit
 constexpr int QF_CAND_LDS = 1024;
 constexpr size_t QF_LDS_MAX = 96 * 1024;

@@ -10,7 +10,8 @@ void launch_prepare_nh(const DevBatch &b, hipStream_t s);
 // begin, force clearing and (qa_on) prepare_nh of a full step in one launch
 void launch_step_begin(const DevBatch &b, int prec, int qa_on, hipStream_t s);
 // done: close the rebuild here (else the step's k_tick does, nl_done)
-void launch_fix_nl(const DevBatch &b, int max_nlcap, int cell_words, hipStream_t s, bool done = true);
+void launch_fix_nl(const DevBatch &b, int max_n, int max_nlcap, int cell_words, hipStream_t s,
+                   bool done = true);
 void launch_terms(const DevBatch &b, int prec, int max_items, int only_term, int from, int to,
                   hipStream_t s);
 void launch_qa_finish(const DevBatch &b, int max_n, hipStream_t s);
