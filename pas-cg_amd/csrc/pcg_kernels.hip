@@ -78,6 +78,10 @@ __device__ __forceinline__ bool nl_pred(const DevBatch &b, SysDescC &s, const Ge
 // is visible to it then.
 __device__ __forceinline__ int count_done(const DevBatch &b, int SysState::*done, bool mine,
                                           int sys) {
+  // a wave with nothing to count skips the fence: an agent-scope release
+  // writes back the L2, and on a step without a rebuild every wave of the
+  // cell count has nothing to count
+  if (!__any(mine)) return -1;
   __threadfence();
   const int lane = threadIdx.x & 63;
   const unsigned long long m = __ballot(mine);
